@@ -1,0 +1,296 @@
+/*
+ * rgrt.h — C ABI of the MI355X-native ray-tracing core for Raygun.
+ *
+ * Plain C: no exceptions cross this boundary, every entry point returns an
+ * rgrt_status (0 = OK), handles are opaque, host arrays are owned by the
+ * caller, device memory is owned by the library. A context is not
+ * thread-safe: one host thread drives it, as in the reference where the main
+ * thread records and submits all GPU work.
+ *
+ * Record layouts are byte-identical to the reference's X-macro `.def` files
+ * that are shared by its C++ and GLSL code, so a Raygun scene packs into the
+ * same buffers it packs into today.
+ */
+#ifndef RGRT_H
+#define RGRT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Status codes                                                              */
+/* ------------------------------------------------------------------------ */
+typedef int32_t rgrt_status;
+#define RGRT_OK 0
+#define RGRT_ERR_INVALID_ARG 1
+#define RGRT_ERR_OUT_OF_MEMORY 2
+#define RGRT_ERR_HIP 3
+#define RGRT_ERR_NOT_READY 4   /* e.g. trace before TLAS / frame constants */
+#define RGRT_ERR_NO_DEVICE 5
+
+/* ------------------------------------------------------------------------ */
+/* Records shared with the engine (layout contracts)                          */
+/* ------------------------------------------------------------------------ */
+
+/* Vertex, 32 bytes. Replaces resources/shaders/vertex.def:3-7 and
+ * raygun/render/vertex.hpp:27-29 (struct Vertex). */
+typedef struct rgrt_vertex {
+    float position[3];
+    uint32_t matIndex;
+    float normal[3];
+    float pad1;
+} rgrt_vertex;
+
+/* Material, 64 bytes. Replaces resources/shaders/gpu_material.def:11-26 and
+ * raygun/gpu/gpu_material.hpp:27-31 (gpu::Material). Defaults of the
+ * reference: diffuse (1,0,1), transparency 0, specular (1,1,1),
+ * reflectivity 0, roughness 0, ior 1, effectId 0, rayConsumption 1,
+ * emission 0. */
+typedef struct rgrt_material {
+    float diffuse[3];
+    float transparency;
+    float specular[3];
+    float reflectivity;
+    float roughness;
+    float ior;
+    uint32_t effectId;
+    uint32_t rayConsumption;
+    float emission;
+    float pad0, pad1, pad2;
+} rgrt_material;
+
+/* Per-frame constants, 192 bytes, std140. Replaces
+ * resources/shaders/uniform_buffer_object.def:3-17 and
+ * raygun/gpu/uniform_buffer.hpp:29-31. Matrices are column-major 4x4
+ * (glm). `showAlpha` is a 4-byte boolean slot here; the reference's C++
+ * `bool` occupies the first byte of the same slot and the rest is zeroed by
+ * memset (raygun/render/render_system.cpp:239), so the bytes agree. */
+typedef struct rgrt_ubo {
+    float viewInverse[16];
+    float projInverse[16];
+    float clearColor[3];
+    int32_t numSamples;
+    float lightDir[3];
+    int32_t maxRecursions;
+    float time;
+    uint32_t showAlpha;
+    float pad0;
+    float pad1;
+    float fadeColor[4];
+} rgrt_ubo;
+
+/* Per-instance offsets into the shared buffers, 12 bytes, in elements.
+ * Replaces resources/shaders/instance_offset_table.def:1-3 and
+ * raygun/render/acceleration_structure.hpp:75-78. */
+typedef struct rgrt_instance_offsets {
+    uint32_t vertexBufferOffset;
+    uint32_t indexBufferOffset;
+    uint32_t materialBufferOffset;
+} rgrt_instance_offsets;
+
+/* BLAS handle: index of a bottom-level structure inside a context. */
+typedef uint32_t rgrt_blas;
+
+/* Geometry range of one BLAS, in the packed vertex/index buffers.
+ * Mirrors vk::AccelerationStructureBuildRangeInfoKHR as filled by
+ * raygun/render/acceleration_structure.cpp:167-170: indices are mesh-local,
+ * vertex k of a triangle is vertices[firstVertex + indices[...]]. */
+typedef struct rgrt_blas_desc {
+    uint32_t firstVertex;      /* vertexBufferRef.offsetInElements()   */
+    uint32_t indexOffsetBytes; /* indexBufferRef.offsetInBytes          */
+    uint32_t numFaces;         /* mesh.numFaces()                       */
+    uint32_t maxVertex;        /* mesh.vertices.size()                  */
+} rgrt_blas_desc;
+
+/* One TLAS instance. Replaces vk::AccelerationStructureInstanceKHR as filled
+ * by instanceFromEntity (raygun/render/acceleration_structure.cpp:34-51)
+ * plus the matching InstanceOffsetTableEntry (:79-82). `transform` is the
+ * object-to-world affine map as a 3x4 row-major matrix
+ * (transpose(globalTransform().toMat4()) rows 0..2). */
+typedef struct rgrt_instance {
+    float transform[12];
+    uint32_t customIndex; /* gl_InstanceCustomIndexEXT: DFS order         */
+    uint32_t mask;        /* low 8 bits used, reference uses 0xff          */
+    uint32_t flags;       /* RGRT_INSTANCE_* (cull-disable is the only mode) */
+    rgrt_blas blas;
+    rgrt_instance_offsets offsets;
+    uint32_t pad;
+} rgrt_instance; /* 80 bytes */
+
+#define RGRT_INSTANCE_TRIANGLE_CULL_DISABLE 0x1u
+
+/* ------------------------------------------------------------------------ */
+/* Ray queries (closest hit) — the traversal on its own                       */
+/* ------------------------------------------------------------------------ */
+typedef struct rgrt_ray {
+    float origin[3];
+    float tmin;
+    float dir[3];
+    float tmax;
+} rgrt_ray; /* 32 bytes */
+
+#define RGRT_NO_HIT 0xffffffffu
+
+typedef struct rgrt_hit {
+    float t;
+    float u; /* barycentric weight of vertex 1 (attribs.x) */
+    float v; /* barycentric weight of vertex 2 (attribs.y) */
+    uint32_t instance; /* customIndex, RGRT_NO_HIT on a miss */
+    uint32_t primitive; /* gl_PrimitiveID                    */
+} rgrt_hit; /* 20 bytes */
+
+/* ------------------------------------------------------------------------ */
+/* Canonical BVH2 export (used to compare builds node by node)               */
+/* ------------------------------------------------------------------------ */
+/* Child reference: >= 0 internal node index; < 0 leaf, ~child = position of
+ * the primitive in Morton order. An empty child has child == RGRT_EMPTY_CHILD
+ * and an inverted box. */
+#define RGRT_EMPTY_CHILD ((int32_t)0x80000000)
+
+typedef struct rgrt_bvh_node {
+    float lo[2][3];
+    float hi[2][3];
+    int32_t child[2];
+    uint32_t parent; /* 0xffffffff for the root */
+    uint32_t pad;
+} rgrt_bvh_node; /* 64 bytes */
+
+/* ------------------------------------------------------------------------ */
+/* Context                                                                    */
+/* ------------------------------------------------------------------------ */
+typedef struct rgrt_ctx rgrt_ctx;
+
+typedef struct rgrt_config {
+    int32_t device;       /* HIP device ordinal                           */
+    uint32_t width;       /* render target size (may be changed later)    */
+    uint32_t height;
+    uint32_t flags;       /* RGRT_CFG_*                                    */
+} rgrt_config;
+
+#define RGRT_CFG_FP32_RADIANCE 0x1u /* also keep fp32 copies of the 3 raygen outputs */
+#define RGRT_CFG_PRIMARY_AOV 0x2u   /* keep primary-ray (t,u,v,instance,primitive)    */
+#define RGRT_CFG_TIMING 0x4u        /* record hipEvents around every stage and kernel */
+
+/* Replaces the ray-tracing parts of VulkanContext() and Raytracer()
+ * (raygun/vulkan_context.cpp:48-69, raygun/render/raytracer.cpp:38-52). */
+rgrt_status rgrt_create(rgrt_ctx** out, const rgrt_config* cfg);
+void rgrt_destroy(rgrt_ctx* ctx);
+const char* rgrt_last_error(const rgrt_ctx* ctx);
+/* Stream all work of the context is ordered on (hipStream_t as void*). */
+void* rgrt_stream(rgrt_ctx* ctx);
+rgrt_status rgrt_synchronize(rgrt_ctx* ctx);
+
+/* Replaces RenderSystem::setupModelBuffers / updateModelBuffers
+ * (raygun/render/render_system.cpp:192-230, 270-330): the caller passes the
+ * already concatenated vertex / index / material buffers. Invalidates every
+ * BLAS built on earlier geometry. */
+rgrt_status rgrt_upload_geometry(rgrt_ctx* ctx, const rgrt_vertex* vertices, uint64_t numVertices, const uint32_t* indices,
+                                 uint64_t numIndices);
+rgrt_status rgrt_upload_materials(rgrt_ctx* ctx, const rgrt_material* materials, uint32_t numMaterials);
+/* Rewrites vertex positions/normals of a range in place (deforming meshes);
+ * follow with rgrt_refit_blas or rgrt_build_blas. */
+rgrt_status rgrt_update_vertices(rgrt_ctx* ctx, uint64_t firstVertex, const rgrt_vertex* vertices, uint64_t count);
+
+/* Replaces BottomLevelAS(cmd, mesh) (raygun/render/acceleration_structure.cpp:140-194).
+ * LBVH build on the device; blocks until the build is complete, as the
+ * reference waits on a fence (raygun/render/raytracer.cpp:71-73). */
+rgrt_status rgrt_build_blas(rgrt_ctx* ctx, const rgrt_blas_desc* desc, rgrt_blas* out);
+/* Refit of an existing BLAS after rgrt_update_vertices (no counterpart in
+ * the reference, which never updates a BLAS): same topology, new boxes. */
+rgrt_status rgrt_refit_blas(rgrt_ctx* ctx, rgrt_blas blas);
+/* Number of internal nodes / leaves and the root box of a BLAS. */
+rgrt_status rgrt_blas_info(rgrt_ctx* ctx, rgrt_blas blas, uint32_t* numNodes, uint32_t* numLeaves, float rootLo[3],
+                           float rootHi[3]);
+/* Canonical BVH2 of a BLAS: numFaces-1 nodes (1 if numFaces <= 1) and the
+ * original primitive index of every leaf in Morton order. */
+rgrt_status rgrt_export_blas(rgrt_ctx* ctx, rgrt_blas blas, rgrt_bvh_node* nodes, uint32_t nodeCapacity,
+                             uint32_t* leafPrimitive, uint32_t leafCapacity);
+
+/* Replaces TopLevelAS(cmd, scene) + instanceOffsetTable()
+ * (raygun/render/acceleration_structure.cpp:55-138). The caller keeps the
+ * scene DFS (:63-85) so customIndex is the DFS order. Asynchronous on the
+ * context stream. */
+rgrt_status rgrt_build_tlas(rgrt_ctx* ctx, const rgrt_instance* instances, uint32_t numInstances);
+rgrt_status rgrt_export_tlas(rgrt_ctx* ctx, rgrt_bvh_node* nodes, uint32_t nodeCapacity, uint32_t* leafInstance,
+                             uint32_t leafCapacity, uint32_t* numNodes);
+
+/* Replaces RenderSystem::updateUniformBuffer (raygun/render/render_system.cpp:246-268). */
+rgrt_status rgrt_set_frame(rgrt_ctx* ctx, const rgrt_ubo* ubo);
+rgrt_status rgrt_resize(rgrt_ctx* ctx, uint32_t width, uint32_t height);
+
+/* Closest-hit queries over the current TLAS, host arrays in and out
+ * (synchronous). cullMask follows traceRayEXT's (instance.mask & cullMask). */
+rgrt_status rgrt_trace_rays(rgrt_ctx* ctx, const rgrt_ray* rays, uint32_t numRays, uint32_t cullMask, rgrt_hit* hits);
+/* Same on device pointers, asynchronous on the context stream. */
+rgrt_status rgrt_trace_rays_device(rgrt_ctx* ctx, const rgrt_ray* rays, uint32_t numRays, uint32_t cullMask, rgrt_hit* hits);
+
+/* Replaces Raytracer::doRaytracing (raygun/render/raytracer.cpp:87-147):
+ * raygen + Whitted recursion (traceRaysKHR), rough prepare, 10x (H,V) blur,
+ * postprocess, and FXAA followed by swap(base, final) when useFxaa != 0.
+ * Asynchronous on the context stream. */
+rgrt_status rgrt_render(rgrt_ctx* ctx, int32_t useFxaa);
+/* The traceRaysKHR part alone (writes base, normal, rough). */
+rgrt_status rgrt_trace(rgrt_ctx* ctx);
+/* The compute post part alone (reads base, normal, rough). */
+rgrt_status rgrt_postprocess(rgrt_ctx* ctx, int32_t useFxaa);
+
+/* Images, as in raygun/render/raytracer.cpp:173-195. FXAA writes the base
+ * image and the reference then swaps base and final (:137-140); the same
+ * swap happens here, so after a render with FXAA, FINAL holds the
+ * anti-aliased result and BASE the composite it was computed from. */
+typedef enum rgrt_image {
+    RGRT_IMAGE_FINAL = 0,        /* rgba16f */
+    RGRT_IMAGE_BASE = 1,         /* rgba16f */
+    RGRT_IMAGE_NORMAL = 2,       /* rgba16f */
+    RGRT_IMAGE_ROUGH = 3,        /* rgba16f */
+    RGRT_IMAGE_ROUGH_TRANSITIONS = 4, /* r8 snorm, 1 byte per pixel */
+    RGRT_IMAGE_ROUGH_COLORS_A = 5,   /* rgba16f */
+    RGRT_IMAGE_ROUGH_COLORS_B = 6,   /* rgba16f */
+    RGRT_IMAGE_BASE_F32 = 7,     /* rgba32f, RGRT_CFG_FP32_RADIANCE */
+    RGRT_IMAGE_NORMAL_F32 = 8,   /* rgba32f, RGRT_CFG_FP32_RADIANCE */
+    RGRT_IMAGE_ROUGH_F32 = 9,    /* rgba32f, RGRT_CFG_FP32_RADIANCE */
+    RGRT_IMAGE_PRIMARY_HIT = 10  /* rgrt_hit per pixel, RGRT_CFG_PRIMARY_AOV */
+} rgrt_image;
+
+/* Copies an image to host memory (synchronous); `bytes` must equal its size. */
+rgrt_status rgrt_read_image(rgrt_ctx* ctx, rgrt_image which, void* dst, uint64_t bytes);
+/* Device pointer of an image (valid until the next resize / render swap). */
+rgrt_status rgrt_image_device_ptr(rgrt_ctx* ctx, rgrt_image which, void** dptr, uint64_t* bytes);
+/* Uploads base/normal/rough images (rgba16f) so the post passes can run on
+ * given inputs. */
+rgrt_status rgrt_write_image(rgrt_ctx* ctx, rgrt_image which, const void* src, uint64_t bytes);
+
+/* ------------------------------------------------------------------------ */
+/* Profiling: the reference profiler's stage names (raygun/profiler.def:6-10) */
+/* ------------------------------------------------------------------------ */
+typedef struct rgrt_stats {
+    /* last frame, milliseconds, hipEvent pairs (RGRT_CFG_TIMING) */
+    float asBuildMs;   /* TLAS build                     */
+    float rtTotalMs;   /* trace + post                   */
+    float rtOnlyMs;    /* trace (all wavefront passes)   */
+    float postprocMs;  /* rough + composite + FXAA       */
+    float roughMs;     /* rough prepare + blur           */
+    /* counters of the last rgrt_trace / rgrt_render */
+    uint64_t rays;        /* traversal queries with a non-zero cull mask   */
+    uint64_t skyLookups;  /* cull-mask-0 rays (never traverse)             */
+    uint64_t zeroDirRays; /* refraction rays at total internal reflection  */
+    uint32_t waves;       /* wavefront iterations                          */
+    uint32_t traceLaunches; /* launches of the traversal kernel            */
+    float traceKernelMs;  /* summed duration of those launches (hipEvents) */
+    float shadeKernelMs;  /* summed duration of the shading launches       */
+} rgrt_stats;
+
+rgrt_status rgrt_get_stats(rgrt_ctx* ctx, rgrt_stats* out);
+
+/* Library version string. */
+const char* rgrt_version(void);
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif
+
+#endif /* RGRT_H */

@@ -1,0 +1,839 @@
+// api.hip — the C ABI (include/rgrt.h) over the HIP kernels.
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "rgrt_internal.h"
+
+namespace rgrt {
+uint32_t trace_spill_words_per_thread();
+uint32_t trace_block_size();
+void launch_trace_queue_impl(hipStream_t s, const TraceScene& sc, const WavefrontState& st, int cur, uint32_t gridBlocks,
+                             int32_t* spill, uint32_t* hostSlot);
+void launch_trace_rays_impl(hipStream_t s, const TraceScene& sc, const rgrt_ray* rays, uint32_t n, uint32_t cullMask,
+                            rgrt_hit* hits, const InstanceShade* inst, int32_t* spill, uint32_t gridBlocks);
+} // namespace rgrt
+
+using namespace rgrt;
+
+namespace {
+
+struct BlasRec {
+    rgrt_blas_desc desc{};
+    Node* nodes = nullptr;
+    Tri* tris = nullptr;
+    uint32_t* leafPrim = nullptr;
+    bool valid = false;
+};
+
+template<typename T>
+void dev_free(T*& p)
+{
+    if(p) (void)hipFree((void*)p);
+    p = nullptr;
+}
+
+template<typename T>
+void dev_alloc(T*& p, size_t count)
+{
+    dev_free(p);
+    check(hipMalloc((void**)&p, std::max<size_t>(count, 1) * sizeof(T)));
+}
+
+// Bound on the rays one pixel can trace one after another for one sample
+// when every material reflects and refracts with rayConsumption 1:
+// B(r) = (R - r) shadow-chain rays + 2 * (1 + B(r + 1)), B(R) = 0.
+uint32_t max_sequential_rays(int maxRec)
+{
+    uint64_t b = 0;
+    for(int r = maxRec - 1; r >= 0; --r) b = (uint64_t)(maxRec - r) + 2 * (1 + b);
+    return (uint32_t)std::min<uint64_t>(b + 1, 1u << 20);
+}
+
+} // namespace
+
+struct rgrt_ctx {
+    int device = 0;
+    uint32_t flags = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    uint32_t numCUs = 256;
+
+    rgrt_vertex* vb = nullptr;
+    uint64_t nv = 0;
+    uint32_t* ib = nullptr;
+    uint64_t ni = 0;
+    rgrt_material* mats = nullptr;
+    uint32_t nm = 0;
+
+    std::vector<BlasRec> blas;
+    DevBlas* dBlasTable = nullptr;
+    float4* dBlasRoot = nullptr;
+    uint32_t blasCap = 0;
+
+    uint32_t instCap = 0;
+    uint32_t numInstances = 0;
+    bool haveTlas = false;
+    rgrt_instance* hInstStage = nullptr; // pinned
+    hipEvent_t stageDone = nullptr;
+    rgrt_instance* dInstIn = nullptr;
+    Node* tlasNodes = nullptr;
+    uint32_t* tlasLeaf = nullptr;
+    DevInstance* dInst = nullptr;
+    InstanceShade* dShade = nullptr;
+    float* dW2O = nullptr;
+    float4* tlasRoot = nullptr;
+
+    LbvhBuilder builder;
+
+    rgrt_ubo ubo{};
+    bool haveUbo = false;
+
+    uint32_t w = 0, h = 0;
+    Images img;
+    WavefrontState st;
+    int32_t* spill = nullptr;
+    uint32_t traceGrid = 0, shadeGrid = 0;
+
+    rgrt_ray* dRays = nullptr;
+    rgrt_hit* dHits = nullptr;
+    uint32_t rayCap = 0;
+
+    uint32_t* hostCounts = nullptr; // pinned, device-visible
+    uint32_t* hostCountsDev = nullptr;
+    uint32_t hostCountsCap = 0;
+    std::vector<hipEvent_t> iterEvents;
+
+    hipEvent_t evAsStart = nullptr, evAsEnd = nullptr, evRtStart = nullptr, evRtOnlyEnd = nullptr, evRoughEnd = nullptr,
+               evPostEnd = nullptr;
+    std::vector<hipEvent_t> traceEvents; // pairs per trace launch (timing mode)
+    std::vector<hipEvent_t> shadeEvents;
+    uint32_t lastTraceLaunches = 0;
+    uint32_t lastWaves = 0;
+    bool timedAs = false, timedRt = false, timedPost = false;
+
+    TraceScene scene() const
+    {
+        TraceScene sc;
+        sc.tlasNodes = tlasNodes;
+        sc.instances = dInst;
+        sc.blas = dBlasTable;
+        sc.numInstances = haveTlas ? numInstances : 0;
+        return sc;
+    }
+};
+
+namespace {
+
+rgrt_status fail(rgrt_ctx* c, rgrt_status code, const std::string& msg)
+{
+    if(c) c->err = msg;
+    return code;
+}
+
+template<typename F>
+rgrt_status guarded(rgrt_ctx* c, F f)
+{
+    if(!c) return RGRT_ERR_INVALID_ARG;
+    try {
+        return f();
+    }
+    catch(const HipError& e) {
+        return fail(c, e.code == hipErrorOutOfMemory ? RGRT_ERR_OUT_OF_MEMORY : RGRT_ERR_HIP, std::string("HIP: ") + e.what());
+    }
+    catch(const std::exception& e) {
+        return fail(c, RGRT_ERR_HIP, e.what());
+    }
+}
+
+void free_images(rgrt_ctx* c)
+{
+    Images& m = c->img;
+    dev_free(m.base);
+    dev_free(m.normal);
+    dev_free(m.rough);
+    dev_free(m.final_);
+    dev_free(m.transitions);
+    dev_free(m.colorsA);
+    dev_free(m.colorsB);
+    dev_free(m.base32);
+    dev_free(m.normal32);
+    dev_free(m.rough32);
+    dev_free(m.primary);
+    WavefrontState& s = c->st;
+    dev_free(s.payload);
+    dev_free(s.accum);
+    dev_free(s.meta);
+    dev_free(s.frames);
+    dev_free(s.rays[0]);
+    dev_free(s.rays[1]);
+    dev_free(s.hits);
+    dev_free(s.counters);
+    s.npix = 0;
+}
+
+void alloc_images(rgrt_ctx* c, uint32_t w, uint32_t h)
+{
+    free_images(c);
+    const size_t npix = (size_t)w * h;
+    Images& m = c->img;
+    m.w = w;
+    m.h = h;
+    dev_alloc(m.base, npix * 4);
+    dev_alloc(m.normal, npix * 4);
+    dev_alloc(m.rough, npix * 4);
+    dev_alloc(m.final_, npix * 4);
+    dev_alloc(m.transitions, npix);
+    dev_alloc(m.colorsA, npix * 4);
+    dev_alloc(m.colorsB, npix * 4);
+    if(c->flags & RGRT_CFG_FP32_RADIANCE) {
+        dev_alloc(m.base32, npix * 4);
+        dev_alloc(m.normal32, npix * 4);
+        dev_alloc(m.rough32, npix * 4);
+    }
+    if(c->flags & RGRT_CFG_PRIMARY_AOV) dev_alloc(m.primary, npix);
+    check(hipMemsetAsync(m.base, 0, npix * 8, c->stream));
+    check(hipMemsetAsync(m.normal, 0, npix * 8, c->stream));
+    check(hipMemsetAsync(m.rough, 0, npix * 8, c->stream));
+    check(hipMemsetAsync(m.final_, 0, npix * 8, c->stream));
+    WavefrontState& s = c->st;
+    s.npix = (uint32_t)npix;
+    dev_alloc(s.payload, npix * kPayloadWords);
+    dev_alloc(s.accum, npix * kAccumWords);
+    dev_alloc(s.meta, npix);
+    dev_alloc(s.frames, npix * kFrameWords * kMaxFrames);
+    dev_alloc(s.rays[0], npix * kRayWords);
+    dev_alloc(s.rays[1], npix * kRayWords);
+    dev_alloc(s.hits, npix * kHitWords);
+    dev_alloc(s.counters, 64);
+    check(hipMemsetAsync(s.counters, 0, 64 * 4, c->stream));
+    c->w = w;
+    c->h = h;
+}
+
+void ensure_blas_cap(rgrt_ctx* c, uint32_t n)
+{
+    if(n <= c->blasCap) return;
+    const uint32_t cap = std::max<uint32_t>(n, std::max<uint32_t>(16, c->blasCap * 2));
+    DevBlas* t = nullptr;
+    float4* r = nullptr;
+    check(hipMalloc((void**)&t, cap * sizeof(DevBlas)));
+    check(hipMalloc((void**)&r, cap * 2 * sizeof(float4)));
+    if(c->blasCap) {
+        check(hipMemcpyAsync(t, c->dBlasTable, c->blasCap * sizeof(DevBlas), hipMemcpyDeviceToDevice, c->stream));
+        check(hipMemcpyAsync(r, c->dBlasRoot, c->blasCap * 2 * sizeof(float4), hipMemcpyDeviceToDevice, c->stream));
+        check(hipStreamSynchronize(c->stream));
+    }
+    dev_free(c->dBlasTable);
+    dev_free(c->dBlasRoot);
+    c->dBlasTable = t;
+    c->dBlasRoot = r;
+    c->blasCap = cap;
+}
+
+void ensure_inst_cap(rgrt_ctx* c, uint32_t n)
+{
+    if(n <= c->instCap) return;
+    const uint32_t cap = std::max<uint32_t>(n, 64);
+    if(c->hInstStage) {
+        check(hipEventSynchronize(c->stageDone));
+        (void)hipHostFree(c->hInstStage);
+        c->hInstStage = nullptr;
+    }
+    check(hipHostMalloc((void**)&c->hInstStage, cap * sizeof(rgrt_instance), hipHostMallocDefault));
+    dev_alloc(c->dInstIn, cap);
+    dev_alloc(c->tlasNodes, cap);
+    dev_alloc(c->tlasLeaf, cap);
+    dev_alloc(c->dInst, cap);
+    dev_alloc(c->dShade, cap);
+    dev_alloc(c->dW2O, (size_t)cap * 12);
+    if(!c->tlasRoot) dev_alloc(c->tlasRoot, 2);
+    c->instCap = cap;
+}
+
+void ensure_host_counts(rgrt_ctx* c, uint32_t n)
+{
+    if(n <= c->hostCountsCap) return;
+    if(c->hostCounts) {
+        check(hipStreamSynchronize(c->stream));
+        (void)hipHostFree(c->hostCounts);
+    }
+    const uint32_t cap = std::max<uint32_t>(n, 1024);
+    check(hipHostMalloc((void**)&c->hostCounts, cap * 4, hipHostMallocMapped));
+    check(hipHostGetDevicePointer((void**)&c->hostCountsDev, c->hostCounts, 0));
+    c->hostCountsCap = cap;
+}
+
+float elapsed(hipEvent_t a, hipEvent_t b)
+{
+    float ms = 0.0f;
+    if(hipEventElapsedTime(&ms, a, b) != hipSuccess) return -1.0f;
+    return ms;
+}
+
+void make_event(hipEvent_t& e)
+{
+    if(!e) check(hipEventCreate(&e));
+}
+
+rgrt_status do_trace(rgrt_ctx* c)
+{
+    if(!c->haveTlas) return fail(c, RGRT_ERR_NOT_READY, "rgrt_trace: no TLAS built");
+    if(!c->haveUbo) return fail(c, RGRT_ERR_NOT_READY, "rgrt_trace: no frame constants (rgrt_set_frame)");
+    if(!c->w || !c->h) return fail(c, RGRT_ERR_NOT_READY, "rgrt_trace: zero-sized render target");
+    hipStream_t s = c->stream;
+    const bool timing = (c->flags & RGRT_CFG_TIMING) != 0;
+
+    ShadeParams p;
+    p.vb = c->vb;
+    p.ib = c->ib;
+    p.mats = c->mats;
+    p.inst = c->dShade;
+    p.numInstances = c->numInstances;
+    p.numVertices = c->nv;
+    p.numIndices = c->ni;
+    p.numMaterials = c->nm;
+    p.ubo = c->ubo;
+    p.width = c->w;
+    p.height = c->h;
+    p.st = c->st;
+    p.img = c->img;
+    p.keepF32 = (c->flags & RGRT_CFG_FP32_RADIANCE) ? 1 : 0;
+    p.keepAov = (c->flags & RGRT_CFG_PRIMARY_AOV) ? 1 : 0;
+
+    const uint32_t perSample = max_sequential_rays(c->ubo.maxRecursions);
+    const uint32_t maxIters = perSample * (uint32_t)c->ubo.numSamples + 2;
+    ensure_host_counts(c, maxIters + 2);
+    const int ring = 8;
+    while((int)c->iterEvents.size() < ring) {
+        hipEvent_t e;
+        check(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        c->iterEvents.push_back(e);
+    }
+    if(timing) {
+        make_event(c->evRtStart);
+        make_event(c->evRtOnlyEnd);
+        check(hipEventRecord(c->evRtStart, s));
+    }
+
+    const TraceScene sc = c->scene();
+    launch_raygen(s, p);
+    int cur = 0;
+    uint32_t it = 0;
+    uint32_t traceLaunches = 0;
+    for(; it < maxIters; ++it) {
+        // The count of iteration it-2's input queue is known by now (the GPU
+        // runs ahead by at most two iterations); zero means every pixel is done.
+        if(it >= 2) {
+            check(hipEventSynchronize(c->iterEvents[(it - 2) % ring]));
+            if(__atomic_load_n(&c->hostCounts[it - 2], __ATOMIC_ACQUIRE) == 0u) break;
+        }
+        if(timing) {
+            while(c->traceEvents.size() < 2 * (traceLaunches + 1)) {
+                hipEvent_t e;
+                check(hipEventCreate(&e));
+                c->traceEvents.push_back(e);
+            }
+            while(c->shadeEvents.size() < 2 * (traceLaunches + 1)) {
+                hipEvent_t e;
+                check(hipEventCreate(&e));
+                c->shadeEvents.push_back(e);
+            }
+            check(hipEventRecord(c->traceEvents[2 * traceLaunches], s));
+        }
+        launch_trace_queue_impl(s, sc, c->st, cur, c->traceGrid, c->spill, c->hostCountsDev + it);
+        check(hipEventRecord(c->iterEvents[it % ring], s));
+        if(timing) {
+            check(hipEventRecord(c->traceEvents[2 * traceLaunches + 1], s));
+            check(hipEventRecord(c->shadeEvents[2 * traceLaunches], s));
+        }
+        launch_shade(s, p, cur, c->shadeGrid);
+        if(timing) check(hipEventRecord(c->shadeEvents[2 * traceLaunches + 1], s));
+        ++traceLaunches;
+        cur ^= 1;
+    }
+    if(it == maxIters) {
+        // all pixels must be done after the bound; check the last count
+        check(hipStreamSynchronize(s));
+        uint32_t remaining = 0;
+        check(hipMemcpy(&remaining, c->st.counters + cur, 4, hipMemcpyDeviceToHost));
+        if(remaining) return fail(c, RGRT_ERR_HIP, "rgrt_trace: ray queue did not drain within the recursion bound");
+    }
+    if(timing) check(hipEventRecord(c->evRtOnlyEnd, s));
+    c->lastTraceLaunches = traceLaunches;
+    c->lastWaves = it;
+    c->timedRt = timing;
+    return RGRT_OK;
+}
+
+rgrt_status do_post(rgrt_ctx* c, int32_t useFxaa)
+{
+    if(!c->haveUbo) return fail(c, RGRT_ERR_NOT_READY, "rgrt_postprocess: no frame constants");
+    if(!c->w || !c->h) return fail(c, RGRT_ERR_NOT_READY, "rgrt_postprocess: zero-sized render target");
+    const bool timing = (c->flags & RGRT_CFG_TIMING) != 0;
+    if(timing) {
+        make_event(c->evRoughEnd);
+        make_event(c->evPostEnd);
+        make_event(c->evRtOnlyEnd);
+        make_event(c->evRtStart);
+        if(!c->timedRt) {
+            check(hipEventRecord(c->evRtStart, c->stream));
+            check(hipEventRecord(c->evRtOnlyEnd, c->stream));
+        }
+    }
+    launch_post(c->stream, c->img, c->ubo, useFxaa, timing ? c->evRoughEnd : nullptr);
+    if(timing) check(hipEventRecord(c->evPostEnd, c->stream));
+    if(useFxaa) std::swap(c->img.base, c->img.final_); // raygun/render/raytracer.cpp:139
+    c->timedPost = timing;
+    return RGRT_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+const char* rgrt_version(void) { return "rgrt 0.1 (gfx950)"; }
+
+rgrt_status rgrt_create(rgrt_ctx** out, const rgrt_config* cfg)
+{
+    if(!out || !cfg) return RGRT_ERR_INVALID_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if(hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return RGRT_ERR_NO_DEVICE;
+    if(cfg->device < 0 || cfg->device >= ndev) return RGRT_ERR_INVALID_ARG;
+    auto* c = new rgrt_ctx();
+    c->device = cfg->device;
+    c->flags = cfg->flags;
+    rgrt_status st = guarded(c, [&]() {
+        check(hipSetDevice(c->device));
+        check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        hipDeviceProp_t prop;
+        check(hipGetDeviceProperties(&prop, c->device));
+        c->numCUs = (uint32_t)std::max(1, prop.multiProcessorCount);
+        c->traceGrid = c->numCUs * 8;
+        c->shadeGrid = c->numCUs * 8;
+        check(hipMalloc((void**)&c->spill, (size_t)c->traceGrid * trace_block_size() * trace_spill_words_per_thread() * 4));
+        check(hipEventCreateWithFlags(&c->stageDone, hipEventDisableTiming));
+        if(cfg->width && cfg->height) alloc_images(c, cfg->width, cfg->height);
+        return RGRT_OK;
+    });
+    if(st != RGRT_OK) {
+        rgrt_destroy(c);
+        return st;
+    }
+    *out = c;
+    return RGRT_OK;
+}
+
+void rgrt_destroy(rgrt_ctx* c)
+{
+    if(!c) return;
+    (void)hipSetDevice(c->device);
+    if(c->stream) (void)hipStreamSynchronize(c->stream);
+    free_images(c);
+    for(auto& b: c->blas) {
+        dev_free(b.nodes);
+        dev_free(b.tris);
+        dev_free(b.leafPrim);
+    }
+    dev_free(c->vb);
+    dev_free(c->ib);
+    dev_free(c->mats);
+    dev_free(c->dBlasTable);
+    dev_free(c->dBlasRoot);
+    dev_free(c->dInstIn);
+    dev_free(c->tlasNodes);
+    dev_free(c->tlasLeaf);
+    dev_free(c->dInst);
+    dev_free(c->dShade);
+    dev_free(c->dW2O);
+    dev_free(c->tlasRoot);
+    dev_free(c->spill);
+    dev_free(c->dRays);
+    dev_free(c->dHits);
+    c->builder.release();
+    if(c->hInstStage) (void)hipHostFree(c->hInstStage);
+    if(c->hostCounts) (void)hipHostFree(c->hostCounts);
+    for(hipEvent_t e: c->iterEvents) (void)hipEventDestroy(e);
+    for(hipEvent_t e: c->traceEvents) (void)hipEventDestroy(e);
+    for(hipEvent_t e: c->shadeEvents) (void)hipEventDestroy(e);
+    for(hipEvent_t e: {c->evAsStart, c->evAsEnd, c->evRtStart, c->evRtOnlyEnd, c->evRoughEnd, c->evPostEnd, c->stageDone})
+        if(e) (void)hipEventDestroy(e);
+    if(c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char* rgrt_last_error(const rgrt_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+void* rgrt_stream(rgrt_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+rgrt_status rgrt_synchronize(rgrt_ctx* c)
+{
+    return guarded(c, [&]() {
+        check(hipStreamSynchronize(c->stream));
+        return RGRT_OK;
+    });
+}
+
+rgrt_status rgrt_upload_geometry(rgrt_ctx* c, const rgrt_vertex* vertices, uint64_t nv, const uint32_t* indices, uint64_t ni)
+{
+    return guarded(c, [&]() {
+        if((nv && !vertices) || (ni && !indices) || ni % 3) return fail(c, RGRT_ERR_INVALID_ARG, "upload_geometry: bad arrays");
+        check(hipSetDevice(c->device));
+        check(hipStreamSynchronize(c->stream));
+        dev_alloc(c->vb, nv);
+        dev_alloc(c->ib, ni);
+        if(nv) check(hipMemcpy(c->vb, vertices, nv * sizeof(rgrt_vertex), hipMemcpyHostToDevice));
+        if(ni) check(hipMemcpy(c->ib, indices, ni * 4, hipMemcpyHostToDevice));
+        c->nv = nv;
+        c->ni = ni;
+        for(auto& b: c->blas) b.valid = false; // geometry changed: every BLAS must be rebuilt
+        c->haveTlas = false;
+        return RGRT_OK;
+    });
+}
+
+rgrt_status rgrt_update_vertices(rgrt_ctx* c, uint64_t first, const rgrt_vertex* vertices, uint64_t count)
+{
+    return guarded(c, [&]() {
+        if(first + count > c->nv || (count && !vertices)) return fail(c, RGRT_ERR_INVALID_ARG, "update_vertices: range");
+        if(count)
+            check(hipMemcpyAsync(c->vb + first, vertices, count * sizeof(rgrt_vertex), hipMemcpyHostToDevice, c->stream));
+        check(hipStreamSynchronize(c->stream));
+        return RGRT_OK;
+    });
+}
+
+rgrt_status rgrt_upload_materials(rgrt_ctx* c, const rgrt_material* materials, uint32_t n)
+{
+    return guarded(c, [&]() {
+        if(n && !materials) return fail(c, RGRT_ERR_INVALID_ARG, "upload_materials: null");
+        check(hipStreamSynchronize(c->stream));
+        dev_alloc(c->mats, n);
+        if(n) check(hipMemcpy(c->mats, materials, n * sizeof(rgrt_material), hipMemcpyHostToDevice));
+        c->nm = n;
+        return RGRT_OK;
+    });
+}
+
+rgrt_status rgrt_build_blas(rgrt_ctx* c, const rgrt_blas_desc* d, rgrt_blas* out)
+{
+    return guarded(c, [&]() {
+        if(!d || !out) return fail(c, RGRT_ERR_INVALID_ARG, "build_blas: null");
+        if(d->indexOffsetBytes % 4) return fail(c, RGRT_ERR_INVALID_ARG, "build_blas: index offset not 4-aligned");
+        const uint64_t io = d->indexOffsetBytes / 4;
+        if(io + 3ull * d->numFaces > c->ni) return fail(c, RGRT_ERR_INVALID_ARG, "build_blas: index range outside the buffer");
+        if((uint64_t)d->firstVertex + d->maxVertex > c->nv)
+            return fail(c, RGRT_ERR_INVALID_ARG, "build_blas: vertex range outside the buffer");
+        if(d->numFaces && d->maxVertex == 0) return fail(c, RGRT_ERR_INVALID_ARG, "build_blas: maxVertex is 0");
+        // every index must address a vertex of the range
+        std::vector<uint32_t> idx(3ull * d->numFaces);
+        if(!idx.empty()) check(hipMemcpy(idx.data(), c->ib + io, idx.size() * 4, hipMemcpyDeviceToHost));
+        for(uint32_t v: idx)
+            if(v >= d->maxVertex) return fail(c, RGRT_ERR_INVALID_ARG, "build_blas: index >= maxVertex");
+
+        hipStream_t s = c->stream;
+        const uint32_t id = (uint32_t)c->blas.size();
+        ensure_blas_cap(c, id + 1);
+        BlasRec b;
+        b.desc = *d;
+        const uint32_t n = d->numFaces;
+        dev_alloc(b.nodes, std::max<uint32_t>(n, 2) - 1);
+        dev_alloc(b.tris, n);
+        dev_alloc(b.leafPrim, n);
+        c->builder.reserve(std::max<uint32_t>(n, 1));
+        launch_tri_boxes(s, c->vb, c->ib, d->firstVertex, (uint32_t)io, n, c->builder.boxLo, c->builder.boxHi);
+        c->builder.build(s, n, b.nodes, b.leafPrim, c->dBlasRoot + 2 * id);
+        launch_tri_gather(s, c->vb, c->ib, d->firstVertex, (uint32_t)io, b.leafPrim, n, b.tris);
+        DevBlas db{b.nodes, b.tris, n, 0};
+        check(hipMemcpyAsync(c->dBlasTable + id, &db, sizeof(DevBlas), hipMemcpyHostToDevice, s));
+        check(hipStreamSynchronize(s)); // the reference blocks on a fence (raytracer.cpp:71-73)
+        b.valid = true;
+        c->blas.push_back(b);
+        *out = id;
+        return RGRT_OK;
+    });
+}
+
+rgrt_status rgrt_refit_blas(rgrt_ctx* c, rgrt_blas id)
+{
+    return guarded(c, [&]() {
+        if(id >= c->blas.size() || !c->blas[id].valid) return fail(c, RGRT_ERR_INVALID_ARG, "refit_blas: bad handle");
+        BlasRec& b = c->blas[id];
+        hipStream_t s = c->stream;
+        const uint32_t n = b.desc.numFaces;
+        const uint32_t io = b.desc.indexOffsetBytes / 4;
+        c->builder.reserve(std::max<uint32_t>(n, 1));
+        launch_tri_boxes(s, c->vb, c->ib, b.desc.firstVertex, io, n, c->builder.boxLo, c->builder.boxHi);
+        c->builder.refit(s, n, b.nodes, b.leafPrim, c->dBlasRoot + 2 * id);
+        launch_tri_gather(s, c->vb, c->ib, b.desc.firstVertex, io, b.leafPrim, n, b.tris);
+        check(hipStreamSynchronize(s));
+        return RGRT_OK;
+    });
+}
+
+rgrt_status rgrt_blas_info(rgrt_ctx* c, rgrt_blas id, uint32_t* numNodes, uint32_t* numLeaves, float rootLo[3], float rootHi[3])
+{
+    return guarded(c, [&]() {
+        if(id >= c->blas.size() || !c->blas[id].valid) return fail(c, RGRT_ERR_INVALID_ARG, "blas_info: bad handle");
+        const uint32_t n = c->blas[id].desc.numFaces;
+        if(numNodes) *numNodes = std::max<uint32_t>(n, 2) - 1;
+        if(numLeaves) *numLeaves = n;
+        float4 r[2];
+        check(hipStreamSynchronize(c->stream));
+        check(hipMemcpy(r, c->dBlasRoot + 2 * id, sizeof(r), hipMemcpyDeviceToHost));
+        if(rootLo) {
+            rootLo[0] = r[0].x;
+            rootLo[1] = r[0].y;
+            rootLo[2] = r[0].z;
+        }
+        if(rootHi) {
+            rootHi[0] = r[1].x;
+            rootHi[1] = r[1].y;
+            rootHi[2] = r[1].z;
+        }
+        return RGRT_OK;
+    });
+}
+
+rgrt_status rgrt_export_blas(rgrt_ctx* c, rgrt_blas id, rgrt_bvh_node* nodes, uint32_t nodeCap, uint32_t* leafPrim, uint32_t leafCap)
+{
+    return guarded(c, [&]() {
+        if(id >= c->blas.size() || !c->blas[id].valid) return fail(c, RGRT_ERR_INVALID_ARG, "export_blas: bad handle");
+        const BlasRec& b = c->blas[id];
+        const uint32_t nn = std::max<uint32_t>(b.desc.numFaces, 2) - 1;
+        if((nodes && nodeCap < nn) || (leafPrim && leafCap < b.desc.numFaces))
+            return fail(c, RGRT_ERR_INVALID_ARG, "export_blas: capacity");
+        check(hipStreamSynchronize(c->stream));
+        if(nodes) check(hipMemcpy(nodes, b.nodes, nn * sizeof(Node), hipMemcpyDeviceToHost));
+        if(leafPrim && b.desc.numFaces) check(hipMemcpy(leafPrim, b.leafPrim, b.desc.numFaces * 4ull, hipMemcpyDeviceToHost));
+        return RGRT_OK;
+    });
+}
+
+rgrt_status rgrt_build_tlas(rgrt_ctx* c, const rgrt_instance* in, uint32_t n)
+{
+    return guarded(c, [&]() {
+        if(n && !in) return fail(c, RGRT_ERR_INVALID_ARG, "build_tlas: null instances");
+        for(uint32_t i = 0; i < n; ++i) {
+            if(in[i].blas >= c->blas.size() || !c->blas[in[i].blas].valid)
+                return fail(c, RGRT_ERR_INVALID_ARG, "build_tlas: instance references an invalid BLAS");
+            if(in[i].customIndex >= n)
+                return fail(c, RGRT_ERR_INVALID_ARG, "build_tlas: customIndex must index the offset table");
+        }
+        hipStream_t s = c->stream;
+        const bool timing = (c->flags & RGRT_CFG_TIMING) != 0;
+        if(timing) {
+            make_event(c->evAsStart);
+            make_event(c->evAsEnd);
+            check(hipEventRecord(c->evAsStart, s));
+        }
+        ensure_inst_cap(c, std::max<uint32_t>(n, 1));
+        check(hipEventSynchronize(c->stageDone)); // previous frame's upload has left the staging buffer
+        if(n) std::memcpy(c->hInstStage, in, n * sizeof(rgrt_instance));
+        if(n) check(hipMemcpyAsync(c->dInstIn, c->hInstStage, n * sizeof(rgrt_instance), hipMemcpyHostToDevice, s));
+        check(hipEventRecord(c->stageDone, s));
+        c->builder.reserve(std::max<uint32_t>(n, 1));
+        launch_inst_prep(s, c->dInstIn, n, c->dBlasRoot, c->builder.boxLo, c->builder.boxHi, c->dW2O, c->dShade);
+        c->builder.build(s, n, c->tlasNodes, c->tlasLeaf, c->tlasRoot);
+        launch_inst_gather(s, c->dW2O, c->dInstIn, c->tlasLeaf, n, c->dInst);
+        if(timing) check(hipEventRecord(c->evAsEnd, s));
+        c->timedAs = timing;
+        c->numInstances = n;
+        c->haveTlas = true;
+        return RGRT_OK;
+    });
+}
+
+rgrt_status rgrt_export_tlas(rgrt_ctx* c, rgrt_bvh_node* nodes, uint32_t nodeCap, uint32_t* leafInstance, uint32_t leafCap,
+                             uint32_t* numNodes)
+{
+    return guarded(c, [&]() {
+        if(!c->haveTlas) return fail(c, RGRT_ERR_NOT_READY, "export_tlas: no TLAS");
+        const uint32_t n = c->numInstances;
+        const uint32_t nn = std::max<uint32_t>(n, 2) - 1;
+        if(numNodes) *numNodes = nn;
+        if((nodes && nodeCap < nn) || (leafInstance && leafCap < n)) return fail(c, RGRT_ERR_INVALID_ARG, "export_tlas: capacity");
+        check(hipStreamSynchronize(c->stream));
+        if(nodes) check(hipMemcpy(nodes, c->tlasNodes, nn * sizeof(Node), hipMemcpyDeviceToHost));
+        if(leafInstance && n) check(hipMemcpy(leafInstance, c->tlasLeaf, n * 4ull, hipMemcpyDeviceToHost));
+        return RGRT_OK;
+    });
+}
+
+rgrt_status rgrt_set_frame(rgrt_ctx* c, const rgrt_ubo* ubo)
+{
+    return guarded(c, [&]() {
+        if(!ubo) return fail(c, RGRT_ERR_INVALID_ARG, "set_frame: null");
+        if(ubo->numSamples < 1 || ubo->numSamples > 32) return fail(c, RGRT_ERR_INVALID_ARG, "set_frame: numSamples outside 1..32");
+        if(ubo->maxRecursions < 0 || ubo->maxRecursions > 7)
+            return fail(c, RGRT_ERR_INVALID_ARG, "set_frame: maxRecursions outside 0..7 (pipeline maximum 7)");
+        c->ubo = *ubo;
+        c->haveUbo = true;
+        return RGRT_OK;
+    });
+}
+
+rgrt_status rgrt_resize(rgrt_ctx* c, uint32_t w, uint32_t h)
+{
+    return guarded(c, [&]() {
+        if(!w || !h || (uint64_t)w * h > (1ull << 26)) return fail(c, RGRT_ERR_INVALID_ARG, "resize: size");
+        if(w == c->w && h == c->h) return RGRT_OK;
+        check(hipStreamSynchronize(c->stream));
+        alloc_images(c, w, h);
+        return RGRT_OK;
+    });
+}
+
+rgrt_status rgrt_trace_rays_device(rgrt_ctx* c, const rgrt_ray* rays, uint32_t n, uint32_t cullMask, rgrt_hit* hits)
+{
+    return guarded(c, [&]() {
+        if(!c->haveTlas) return fail(c, RGRT_ERR_NOT_READY, "trace_rays: no TLAS");
+        if(n && (!rays || !hits)) return fail(c, RGRT_ERR_INVALID_ARG, "trace_rays: null");
+        launch_trace_rays_impl(c->stream, c->scene(), rays, n, cullMask, hits, c->dShade, c->spill, c->traceGrid);
+        return RGRT_OK;
+    });
+}
+
+rgrt_status rgrt_trace_rays(rgrt_ctx* c, const rgrt_ray* rays, uint32_t n, uint32_t cullMask, rgrt_hit* hits)
+{
+    return guarded(c, [&]() {
+        if(!c->haveTlas) return fail(c, RGRT_ERR_NOT_READY, "trace_rays: no TLAS");
+        if(n && (!rays || !hits)) return fail(c, RGRT_ERR_INVALID_ARG, "trace_rays: null");
+        if(n > c->rayCap) {
+            dev_alloc(c->dRays, n);
+            dev_alloc(c->dHits, n);
+            c->rayCap = n;
+        }
+        if(!n) return RGRT_OK;
+        check(hipMemcpyAsync(c->dRays, rays, n * sizeof(rgrt_ray), hipMemcpyHostToDevice, c->stream));
+        launch_trace_rays_impl(c->stream, c->scene(), c->dRays, n, cullMask, c->dHits, c->dShade, c->spill, c->traceGrid);
+        check(hipMemcpyAsync(hits, c->dHits, n * sizeof(rgrt_hit), hipMemcpyDeviceToHost, c->stream));
+        check(hipStreamSynchronize(c->stream));
+        return RGRT_OK;
+    });
+}
+
+rgrt_status rgrt_trace(rgrt_ctx* c)
+{
+    return guarded(c, [&]() {
+        c->timedRt = false;
+        c->timedPost = false;
+        return do_trace(c);
+    });
+}
+
+rgrt_status rgrt_postprocess(rgrt_ctx* c, int32_t useFxaa)
+{
+    return guarded(c, [&]() {
+        c->timedRt = false;
+        return do_post(c, useFxaa);
+    });
+}
+
+rgrt_status rgrt_render(rgrt_ctx* c, int32_t useFxaa)
+{
+    return guarded(c, [&]() {
+        c->timedRt = false;
+        c->timedPost = false;
+        rgrt_status st = do_trace(c);
+        if(st != RGRT_OK) return st;
+        return do_post(c, useFxaa);
+    });
+}
+
+static bool image_info(rgrt_ctx* c, rgrt_image which, void** ptr, uint64_t* bytes)
+{
+    const uint64_t npix = (uint64_t)c->w * c->h;
+    Images& m = c->img;
+    switch(which) {
+    case RGRT_IMAGE_FINAL: *ptr = m.final_; *bytes = npix * 8; break;
+    case RGRT_IMAGE_BASE: *ptr = m.base; *bytes = npix * 8; break;
+    case RGRT_IMAGE_NORMAL: *ptr = m.normal; *bytes = npix * 8; break;
+    case RGRT_IMAGE_ROUGH: *ptr = m.rough; *bytes = npix * 8; break;
+    case RGRT_IMAGE_ROUGH_TRANSITIONS: *ptr = m.transitions; *bytes = npix; break;
+    case RGRT_IMAGE_ROUGH_COLORS_A: *ptr = m.colorsA; *bytes = npix * 8; break;
+    case RGRT_IMAGE_ROUGH_COLORS_B: *ptr = m.colorsB; *bytes = npix * 8; break;
+    case RGRT_IMAGE_BASE_F32: *ptr = m.base32; *bytes = npix * 16; break;
+    case RGRT_IMAGE_NORMAL_F32: *ptr = m.normal32; *bytes = npix * 16; break;
+    case RGRT_IMAGE_ROUGH_F32: *ptr = m.rough32; *bytes = npix * 16; break;
+    case RGRT_IMAGE_PRIMARY_HIT: *ptr = m.primary; *bytes = npix * sizeof(rgrt_hit); break;
+    default: return false;
+    }
+    return *ptr != nullptr;
+}
+
+rgrt_status rgrt_read_image(rgrt_ctx* c, rgrt_image which, void* dst, uint64_t bytes)
+{
+    return guarded(c, [&]() {
+        void* p = nullptr;
+        uint64_t sz = 0;
+        if(!image_info(c, which, &p, &sz)) return fail(c, RGRT_ERR_INVALID_ARG, "read_image: image not available");
+        if(bytes != sz || !dst) return fail(c, RGRT_ERR_INVALID_ARG, "read_image: size mismatch");
+        check(hipStreamSynchronize(c->stream));
+        check(hipMemcpy(dst, p, sz, hipMemcpyDeviceToHost));
+        return RGRT_OK;
+    });
+}
+
+rgrt_status rgrt_write_image(rgrt_ctx* c, rgrt_image which, const void* src, uint64_t bytes)
+{
+    return guarded(c, [&]() {
+        void* p = nullptr;
+        uint64_t sz = 0;
+        if(!image_info(c, which, &p, &sz)) return fail(c, RGRT_ERR_INVALID_ARG, "write_image: image not available");
+        if(bytes != sz || !src) return fail(c, RGRT_ERR_INVALID_ARG, "write_image: size mismatch");
+        check(hipStreamSynchronize(c->stream));
+        check(hipMemcpy(p, src, sz, hipMemcpyHostToDevice));
+        return RGRT_OK;
+    });
+}
+
+rgrt_status rgrt_image_device_ptr(rgrt_ctx* c, rgrt_image which, void** dptr, uint64_t* bytes)
+{
+    return guarded(c, [&]() {
+        void* p = nullptr;
+        uint64_t sz = 0;
+        if(!dptr || !image_info(c, which, &p, &sz)) return fail(c, RGRT_ERR_INVALID_ARG, "image_device_ptr: not available");
+        *dptr = p;
+        if(bytes) *bytes = sz;
+        return RGRT_OK;
+    });
+}
+
+rgrt_status rgrt_get_stats(rgrt_ctx* c, rgrt_stats* out)
+{
+    return guarded(c, [&]() {
+        if(!out) return fail(c, RGRT_ERR_INVALID_ARG, "get_stats: null");
+        check(hipStreamSynchronize(c->stream));
+        std::memset(out, 0, sizeof(*out));
+        if(c->st.counters) {
+            uint64_t cnt[6];
+            check(hipMemcpy(cnt, c->st.counters + 4, sizeof(cnt), hipMemcpyDeviceToHost));
+            out->rays = cnt[0];
+            out->skyLookups = cnt[1];
+            out->zeroDirRays = cnt[2];
+        }
+        out->waves = c->lastWaves;
+        out->traceLaunches = c->lastTraceLaunches;
+        if(c->timedAs) out->asBuildMs = elapsed(c->evAsStart, c->evAsEnd);
+        if(c->timedRt) {
+            out->rtOnlyMs = elapsed(c->evRtStart, c->evRtOnlyEnd);
+            float tk = 0.0f, sk = 0.0f;
+            for(uint32_t i = 0; i < c->lastTraceLaunches; ++i) {
+                tk += elapsed(c->traceEvents[2 * i], c->traceEvents[2 * i + 1]);
+                sk += elapsed(c->shadeEvents[2 * i], c->shadeEvents[2 * i + 1]);
+            }
+            out->traceKernelMs = tk;
+            out->shadeKernelMs = sk;
+        }
+        if(c->timedPost) {
+            out->roughMs = elapsed(c->evRtOnlyEnd, c->evRoughEnd);
+            out->postprocMs = elapsed(c->evRtOnlyEnd, c->evPostEnd);
+            out->rtTotalMs = elapsed(c->evRtStart, c->evPostEnd);
+        }
+        return RGRT_OK;
+    });
+}
+
+} // extern "C"

@@ -1,0 +1,572 @@
+// bvh_build.hip — LBVH construction on the device (BLAS and TLAS).
+//
+// Replaces the driver builds behind vkCmdBuildAccelerationStructuresKHR
+// (raygun/render/acceleration_structure.cpp:134 TLAS, :193 BLAS). Steps:
+//   1. primitive boxes (triangles, or transformed instance boxes)
+//   2. scene box (atomic min/max on order-preserving integer encodings)
+//   3. 30-bit Morton codes of box centroids normalised to the scene box
+//   4. stable LSD radix sort (4 x 8-bit digits), ranks from wave64 ballots
+//   5. Karras 2012 hierarchy, ties between equal codes broken by index
+//   6. bottom-up box refit, second arrival at a node continues upwards
+// Every step is deterministic, so the tree equals the CPU oracle's node for
+// node.
+#pragma clang fp contract(off)
+
+#include "rgrt_device.h"
+#include "rgrt_internal.h"
+
+namespace rgrt {
+
+namespace {
+
+constexpr int kSortThreads = 256;
+constexpr int kSortItems = 8;
+constexpr int kSortTile = kSortThreads * kSortItems;
+constexpr int kWaves = kSortThreads / 64;
+
+__device__ __forceinline__ uint32_t float_to_ordered(float f)
+{
+    uint32_t b = __float_as_uint(f);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__device__ __forceinline__ float ordered_to_float(uint32_t k)
+{
+    uint32_t b = (k & 0x80000000u) ? (k & 0x7fffffffu) : ~k;
+    return __uint_as_float(b);
+}
+
+__device__ __forceinline__ uint32_t expand_bits10(uint32_t v)
+{
+    v = (v * 0x00010001u) & 0xFF0000FFu;
+    v = (v * 0x00000101u) & 0x0F00F00Fu;
+    v = (v * 0x00000011u) & 0xC30C30C3u;
+    v = (v * 0x00000005u) & 0x49249249u;
+    return v;
+}
+__device__ __forceinline__ uint32_t quantize10(float x) { return (uint32_t)fminf(fmaxf(x * 1024.0f, 0.0f), 1023.0f); }
+
+// ---------------------------------------------------------------------------
+// 1. primitive boxes
+// ---------------------------------------------------------------------------
+__global__ void k_tri_boxes(const rgrt_vertex* __restrict__ vb, const uint32_t* __restrict__ ib, uint32_t firstVertex,
+                            uint32_t indexOffset, uint32_t n, float4* __restrict__ boxLo, float4* __restrict__ boxHi)
+{
+    const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+    if(f >= n) return;
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for(int k = 0; k < 3; ++k) {
+        const rgrt_vertex& v = vb[firstVertex + ib[indexOffset + 3 * f + k]];
+        for(int c = 0; c < 3; ++c) {
+            lo[c] = fminf(lo[c], v.position[c]);
+            hi[c] = fmaxf(hi[c], v.position[c]);
+        }
+    }
+    boxLo[f] = make_float4(lo[0], lo[1], lo[2], 0.0f);
+    boxHi[f] = make_float4(hi[0], hi[1], hi[2], 0.0f);
+}
+
+// World box of each instance (Arvo's method on the BLAS root box), the
+// world-to-object inverse in double, and the per-position shading record.
+__global__ void k_inst_prep(const rgrt_instance* __restrict__ in, uint32_t n, const float4* __restrict__ blasRoot,
+                            float4* __restrict__ boxLo, float4* __restrict__ boxHi, float* __restrict__ w2oOut,
+                            InstanceShade* __restrict__ shade)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if(i >= n) return;
+    const rgrt_instance r = in[i];
+    const float* m = r.transform;
+    const float4 rlo = blasRoot[2 * r.blas], rhi = blasRoot[2 * r.blas + 1];
+    const float blo[3] = {rlo.x, rlo.y, rlo.z}, bhi[3] = {rhi.x, rhi.y, rhi.z};
+    float wlo[3], whi[3];
+    if(!(blo[0] <= bhi[0])) {
+        for(int c = 0; c < 3; ++c) {
+            wlo[c] = blo[c];
+            whi[c] = bhi[c];
+        }
+    }
+    else {
+        for(int row = 0; row < 3; ++row) {
+            float lo = m[row * 4 + 3], hi = m[row * 4 + 3];
+            for(int j = 0; j < 3; ++j) {
+                const float p = m[row * 4 + j] * blo[j];
+                const float q = m[row * 4 + j] * bhi[j];
+                lo = lo + fminf(p, q);
+                hi = hi + fmaxf(p, q);
+            }
+            wlo[row] = lo;
+            whi[row] = hi;
+        }
+    }
+    boxLo[i] = make_float4(wlo[0], wlo[1], wlo[2], 0.0f);
+    boxHi[i] = make_float4(whi[0], whi[1], whi[2], 0.0f);
+
+    // inverse of the affine map: adjugate of the 3x3 part in double
+    const double a = m[0], b = m[1], c = m[2], d = m[4], e = m[5], f = m[6], g = m[8], h = m[9], k = m[10];
+    const double A = e * k - f * h, B = -(d * k - f * g), C = d * h - e * g;
+    const double det = a * A + b * B + c * C;
+    const double rr = 1.0 / det;
+    const double inv[9] = {A * rr, -(b * k - c * h) * rr, (b * f - c * e) * rr, B * rr, (a * k - c * g) * rr,
+                           -(a * f - c * d) * rr, C * rr, -(a * h - b * g) * rr, (a * e - b * d) * rr};
+    const double tx = m[3], ty = m[7], tz = m[11];
+    float* w = w2oOut + 12 * (size_t)i;
+    for(int row = 0; row < 3; ++row) {
+        w[row * 4 + 0] = (float)inv[row * 3 + 0];
+        w[row * 4 + 1] = (float)inv[row * 3 + 1];
+        w[row * 4 + 2] = (float)inv[row * 3 + 2];
+        w[row * 4 + 3] = (float)(-(inv[row * 3 + 0] * tx + inv[row * 3 + 1] * ty + inv[row * 3 + 2] * tz));
+    }
+    InstanceShade s;
+    for(int q = 0; q < 12; ++q) s.o2w[q] = m[q];
+    s.offsets = r.offsets;
+    s.customIndex = r.customIndex;
+    shade[i] = s;
+}
+
+// ---------------------------------------------------------------------------
+// 2. scene box  3. Morton codes
+// ---------------------------------------------------------------------------
+__global__ void k_scene_box_init(uint32_t* sb)
+{
+    if(threadIdx.x < 3) sb[threadIdx.x] = float_to_ordered(INFINITY);
+    else if(threadIdx.x < 6) sb[threadIdx.x] = float_to_ordered(-INFINITY);
+}
+
+__global__ void k_scene_box(const float4* __restrict__ boxLo, const float4* __restrict__ boxHi, uint32_t n, uint32_t* sb)
+{
+    __shared__ uint32_t red[6];
+    if(threadIdx.x < 3) red[threadIdx.x] = float_to_ordered(INFINITY);
+    else if(threadIdx.x < 6) red[threadIdx.x] = float_to_ordered(-INFINITY);
+    __syncthreads();
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for(uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const float4 a = boxLo[i], b = boxHi[i];
+        lo[0] = fminf(lo[0], a.x);
+        lo[1] = fminf(lo[1], a.y);
+        lo[2] = fminf(lo[2], a.z);
+        hi[0] = fmaxf(hi[0], b.x);
+        hi[1] = fmaxf(hi[1], b.y);
+        hi[2] = fmaxf(hi[2], b.z);
+    }
+    for(int c = 0; c < 3; ++c) {
+        atomicMin(&red[c], float_to_ordered(lo[c]));
+        atomicMax(&red[3 + c], float_to_ordered(hi[c]));
+    }
+    __syncthreads();
+    if(threadIdx.x < 3) atomicMin(&sb[threadIdx.x], red[threadIdx.x]);
+    else if(threadIdx.x < 6) atomicMax(&sb[threadIdx.x], red[threadIdx.x]);
+}
+
+__global__ void k_morton(const float4* __restrict__ boxLo, const float4* __restrict__ boxHi, uint32_t n,
+                         const uint32_t* __restrict__ sb, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if(i >= n) return;
+    const float slo[3] = {ordered_to_float(sb[0]), ordered_to_float(sb[1]), ordered_to_float(sb[2])};
+    const float shi[3] = {ordered_to_float(sb[3]), ordered_to_float(sb[4]), ordered_to_float(sb[5])};
+    const float4 a = boxLo[i], b = boxHi[i];
+    const float lo[3] = {a.x, a.y, a.z}, hi[3] = {b.x, b.y, b.z};
+    uint32_t q[3];
+    for(int c = 0; c < 3; ++c) {
+        const float ext = shi[c] - slo[c];
+        const float centroid = (lo[c] + hi[c]) * 0.5f;
+        const float x = ext > 0.0f ? (centroid - slo[c]) / ext : 0.0f;
+        q[c] = quantize10(x);
+    }
+    keys[i] = (expand_bits10(q[0]) << 2) | (expand_bits10(q[1]) << 1) | expand_bits10(q[2]);
+    vals[i] = i;
+}
+
+// ---------------------------------------------------------------------------
+// 4. stable LSD radix sort
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kSortThreads) void k_sort_hist(const uint32_t* __restrict__ keys, uint32_t n, int shift,
+                                                             uint32_t* __restrict__ hist, uint32_t numBlocks)
+{
+    __shared__ uint32_t cnt[256];
+    cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const uint32_t base = blockIdx.x * kSortTile;
+    for(int r = 0; r < kSortItems; ++r) {
+        const uint32_t i = base + r * kSortThreads + threadIdx.x;
+        if(i < n) atomicAdd(&cnt[(keys[i] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    hist[threadIdx.x * numBlocks + blockIdx.x] = cnt[threadIdx.x];
+}
+
+// Exclusive scan of `total` counters in one workgroup of 1024 threads.
+__global__ __launch_bounds__(1024) void k_sort_scan(uint32_t* __restrict__ hist, uint32_t total)
+{
+    __shared__ uint32_t part[1024];
+    const uint32_t per = (total + 1023) / 1024;
+    const uint32_t b = threadIdx.x * per;
+    const uint32_t e = min(total, b + per);
+    uint32_t s = 0;
+    for(uint32_t i = b; i < e; ++i) s += hist[i];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for(uint32_t off = 1; off < 1024; off <<= 1) {
+        const uint32_t v = threadIdx.x >= off ? part[threadIdx.x - off] : 0u;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    uint32_t run = part[threadIdx.x] - s; // exclusive prefix of this thread's chunk
+    for(uint32_t i = b; i < e; ++i) {
+        const uint32_t c = hist[i];
+        hist[i] = run;
+        run += c;
+    }
+}
+
+// Scatter of one tile, in tile order: round r holds keys [r*256, r*256+256),
+// wave w its 64 consecutive keys. A key's rank is the count of equal digits
+// before it: earlier rounds (running), earlier waves of the round (LDS
+// counts) and lower lanes of its wave (ballot match mask).
+__global__ __launch_bounds__(kSortThreads) void k_sort_scatter(const uint32_t* __restrict__ keysIn,
+                                                                const uint32_t* __restrict__ valsIn,
+                                                                uint32_t* __restrict__ keysOut,
+                                                                uint32_t* __restrict__ valsOut, uint32_t n, int shift,
+                                                                const uint32_t* __restrict__ hist, uint32_t numBlocks)
+{
+    __shared__ uint32_t gofs[256];
+    __shared__ uint32_t waveCnt[kWaves][256];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    gofs[tid] = hist[tid * numBlocks + blockIdx.x];
+    for(int w = 0; w < kWaves; ++w) waveCnt[w][tid] = 0;
+    __syncthreads();
+    const uint64_t ltMask = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+    const uint32_t base = blockIdx.x * kSortTile;
+    for(int r = 0; r < kSortItems; ++r) {
+        const uint32_t i = base + r * kSortThreads + tid;
+        const bool valid = i < n;
+        const uint32_t key = valid ? keysIn[i] : 0u;
+        const uint32_t val = valid ? valsIn[i] : 0u;
+        const uint32_t d = (key >> shift) & 255u;
+        uint64_t m = __ballot(valid);
+        for(int bit = 0; bit < 8; ++bit) {
+            const bool set = (d >> bit) & 1u;
+            const uint64_t bb = __ballot(valid && set);
+            m &= set ? bb : ~bb;
+        }
+        const uint32_t rankInWave = (uint32_t)__popcll(m & ltMask);
+        const bool leader = valid && ((m & ltMask) == 0ull);
+        if(leader) waveCnt[wave][d] = (uint32_t)__popcll(m);
+        __syncthreads();
+        if(valid) {
+            uint32_t pre = 0;
+            for(int w = 0; w < wave; ++w) pre += waveCnt[w][d];
+            const uint32_t pos = gofs[d] + pre + rankInWave;
+            keysOut[pos] = key;
+            valsOut[pos] = val;
+        }
+        __syncthreads();
+        {
+            uint32_t add = 0;
+            for(int w = 0; w < kWaves; ++w) {
+                add += waveCnt[w][tid];
+                waveCnt[w][tid] = 0;
+            }
+            gofs[tid] += add;
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 5. Karras hierarchy
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int karras_delta(const uint32_t* __restrict__ k, int n, int i, int j)
+{
+    if(j < 0 || j >= n) return -1;
+    const uint32_t a = k[i], b = k[j];
+    if(a == b) return 32 + __clz((int)((uint32_t)i ^ (uint32_t)j));
+    return __clz((int)(a ^ b));
+}
+
+__global__ void k_karras(const uint32_t* __restrict__ keys, int n, Node* __restrict__ nodes, uint32_t* __restrict__ leafParent)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if(i >= n - 1) return;
+    const int d = (karras_delta(keys, n, i, i + 1) - karras_delta(keys, n, i, i - 1)) >= 0 ? 1 : -1;
+    const int dmin = karras_delta(keys, n, i, i - d);
+    int lmax = 2;
+    while(karras_delta(keys, n, i, i + lmax * d) > dmin) lmax *= 2;
+    int l = 0;
+    for(int t = lmax / 2; t >= 1; t /= 2)
+        if(karras_delta(keys, n, i, i + (l + t) * d) > dmin) l += t;
+    const int j = i + l * d;
+    const int dnode = karras_delta(keys, n, i, j);
+    int s = 0;
+    for(int div = 2;; div *= 2) {
+        const int t = (l + div - 1) / div;
+        if(karras_delta(keys, n, i, i + (s + t) * d) > dnode) s += t;
+        if(t <= 1) break;
+    }
+    const int gamma = i + s * d + min(d, 0);
+    const int lo = min(i, j), hi = max(i, j);
+    const int left = (lo == gamma) ? ~gamma : gamma;
+    const int right = (hi == gamma + 1) ? ~(gamma + 1) : gamma + 1;
+    Node& nd = nodes[i];
+    nd.child0 = left;
+    nd.child1 = right;
+    nd.pad = 0;
+    if(i == 0) nd.parent = 0xffffffffu;
+    if(left >= 0) nodes[left].parent = (uint32_t)i;
+    else leafParent[gamma] = (uint32_t)i;
+    if(right >= 0) nodes[right].parent = (uint32_t)i;
+    else leafParent[gamma + 1] = (uint32_t)i;
+}
+
+__global__ void k_fill_u32(uint32_t* p, uint32_t n, uint32_t v)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if(i < n) p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// 6. bottom-up refit
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void write_slot(Node* nd, int slot, const float lo[3], const float hi[3])
+{
+    volatile float* f = reinterpret_cast<volatile float*>(nd);
+    // lo[slot] at float offset 3*slot, hi[slot] at 6 + 3*slot
+    for(int c = 0; c < 3; ++c) {
+        f[3 * slot + c] = lo[c];
+        f[6 + 3 * slot + c] = hi[c];
+    }
+}
+
+__global__ void k_refit(const float4* __restrict__ boxLo, const float4* __restrict__ boxHi,
+                        const uint32_t* __restrict__ leafPrim, const uint32_t* __restrict__ leafParent, Node* nodes,
+                        uint32_t* counters, int n, float4* rootBox)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if(k >= n) return;
+    const uint32_t prim = leafPrim[k];
+    const float4 a = boxLo[prim], b = boxHi[prim];
+    float lo[3] = {a.x, a.y, a.z}, hi[3] = {b.x, b.y, b.z};
+    int child = ~k;
+    uint32_t p = leafParent[k];
+    for(;;) {
+        Node* nd = nodes + p;
+        const int slot = (reinterpret_cast<volatile int*>(nd)[12] == child) ? 0 : 1;
+        write_slot(nd, slot, lo, hi);
+        __threadfence();
+        const uint32_t prev = atomicAdd(&counters[p], 1u);
+        if(prev == 0) return; // the sibling's thread finishes this node
+        __threadfence();
+        const volatile float* f = reinterpret_cast<const volatile float*>(nd);
+        for(int c = 0; c < 3; ++c) {
+            lo[c] = fminf(f[c], f[3 + c]);
+            hi[c] = fmaxf(f[6 + c], f[9 + c]);
+        }
+        if(p == 0) {
+            rootBox[0] = make_float4(lo[0], lo[1], lo[2], 0.0f);
+            rootBox[1] = make_float4(hi[0], hi[1], hi[2], 0.0f);
+            return;
+        }
+        child = (int)p;
+        p = reinterpret_cast<volatile uint32_t*>(nd)[14];
+    }
+}
+
+// Trees of 0 or 1 primitives: one node with an empty second child.
+__global__ void k_small_tree(const float4* __restrict__ boxLo, const float4* __restrict__ boxHi, int n, Node* nodes,
+                             float4* rootBox)
+{
+    if(threadIdx.x != 0) return;
+    Node nd;
+    const float inf = INFINITY;
+    float lo0[3] = {inf, inf, inf}, hi0[3] = {-inf, -inf, -inf};
+    if(n == 1) {
+        const float4 a = boxLo[0], b = boxHi[0];
+        lo0[0] = a.x;
+        lo0[1] = a.y;
+        lo0[2] = a.z;
+        hi0[0] = b.x;
+        hi0[1] = b.y;
+        hi0[2] = b.z;
+    }
+    nd.lo0x = lo0[0];
+    nd.lo0y = lo0[1];
+    nd.lo0z = lo0[2];
+    nd.hi0x = hi0[0];
+    nd.hi0y = hi0[1];
+    nd.hi0z = hi0[2];
+    nd.lo1x = nd.lo1y = nd.lo1z = inf;
+    nd.hi1x = nd.hi1y = nd.hi1z = -inf;
+    nd.child0 = n == 1 ? ~0 : kEmptyChild;
+    nd.child1 = kEmptyChild;
+    nd.parent = 0xffffffffu;
+    nd.pad = 0;
+    nodes[0] = nd;
+    rootBox[0] = make_float4(lo0[0], lo0[1], lo0[2], 0.0f);
+    rootBox[1] = make_float4(hi0[0], hi0[1], hi0[2], 0.0f);
+}
+
+// ---------------------------------------------------------------------------
+// leaf payloads
+// ---------------------------------------------------------------------------
+__global__ void k_tri_gather(const rgrt_vertex* __restrict__ vb, const uint32_t* __restrict__ ib, uint32_t firstVertex,
+                             uint32_t indexOffset, const uint32_t* __restrict__ leafPrim, uint32_t n, Tri* __restrict__ tris)
+{
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if(k >= n) return;
+    const uint32_t prim = leafPrim[k];
+    const uint32_t* ix = ib + indexOffset + 3 * prim;
+    const rgrt_vertex& a = vb[firstVertex + ix[0]];
+    const rgrt_vertex& b = vb[firstVertex + ix[1]];
+    const rgrt_vertex& c = vb[firstVertex + ix[2]];
+    Tri t;
+    t.v0x = a.position[0];
+    t.v0y = a.position[1];
+    t.v0z = a.position[2];
+    t.prim = prim;
+    t.v1x = b.position[0];
+    t.v1y = b.position[1];
+    t.v1z = b.position[2];
+    t.pad0 = 0;
+    t.v2x = c.position[0];
+    t.v2y = c.position[1];
+    t.v2z = c.position[2];
+    t.pad1 = 0;
+    tris[k] = t;
+}
+
+__global__ void k_inst_gather(const float* __restrict__ w2o, const rgrt_instance* __restrict__ in,
+                              const uint32_t* __restrict__ leafInst, uint32_t n, DevInstance* __restrict__ out)
+{
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if(k >= n) return;
+    const uint32_t i = leafInst[k];
+    DevInstance d;
+    for(int q = 0; q < 12; ++q) d.w2o[q] = w2o[12 * (size_t)i + q];
+    d.blas = in[i].blas;
+    d.mask = in[i].mask & 0xffu;
+    d.position = i;
+    d.customIndex = in[i].customIndex;
+    out[k] = d;
+}
+
+inline uint32_t blocks_for(uint64_t n, uint32_t t) { return (uint32_t)((n + t - 1) / t); }
+
+} // namespace
+
+// ---------------------------------------------------------------------------
+// Host-side drivers
+// ---------------------------------------------------------------------------
+void LbvhBuilder::reserve(uint32_t n)
+{
+    if(n <= capacity) return;
+    release();
+    const uint32_t cap = n < 1024 ? 1024 : n;
+    const uint32_t nb = blocks_for(cap, kSortTile);
+    check(hipMalloc(&keysA, cap * 4ull));
+    check(hipMalloc(&keysB, cap * 4ull));
+    check(hipMalloc(&valsA, cap * 4ull));
+    check(hipMalloc(&valsB, cap * 4ull));
+    check(hipMalloc(&hist, 256ull * nb * 4ull));
+    check(hipMalloc(&boxLo, cap * 16ull));
+    check(hipMalloc(&boxHi, cap * 16ull));
+    check(hipMalloc(&leafParent, cap * 4ull));
+    check(hipMalloc(&counters, cap * 4ull));
+    check(hipMalloc(&sceneBox, 64));
+    capacity = cap;
+}
+
+void LbvhBuilder::release()
+{
+    for(void* p: {(void*)keysA, (void*)keysB, (void*)valsA, (void*)valsB, (void*)hist, (void*)boxLo, (void*)boxHi,
+                  (void*)leafParent, (void*)counters, (void*)sceneBox})
+        if(p) (void)hipFree(p);
+    keysA = keysB = valsA = valsB = hist = leafParent = counters = sceneBox = nullptr;
+    boxLo = boxHi = nullptr;
+    capacity = 0;
+}
+
+// Builds the hierarchy over boxLo/boxHi[0..n) (already filled). Writes
+// max(n-1,1) nodes, the Morton-ordered primitive ids and the root box.
+void LbvhBuilder::build(hipStream_t s, uint32_t n, Node* nodes, uint32_t* leafPrimOut, float4* rootBox)
+{
+    if(n <= 1) {
+        k_small_tree<<<1, 64, 0, s>>>(boxLo, boxHi, (int)n, nodes, rootBox);
+        if(n == 1) k_fill_u32<<<1, 64, 0, s>>>(leafPrimOut, 1, 0u);
+        return;
+    }
+    k_scene_box_init<<<1, 64, 0, s>>>(sceneBox);
+    const uint32_t sbBlocks = std::min<uint32_t>(blocks_for(n, 256), 1024);
+    k_scene_box<<<sbBlocks, 256, 0, s>>>(boxLo, boxHi, n, sceneBox);
+    k_morton<<<blocks_for(n, 256), 256, 0, s>>>(boxLo, boxHi, n, sceneBox, keysA, valsA);
+    const uint32_t nb = blocks_for(n, kSortTile);
+    uint32_t *kin = keysA, *vin = valsA, *kout = keysB, *vout = valsB;
+    for(int shift = 0; shift < 32; shift += 8) {
+        k_sort_hist<<<nb, kSortThreads, 0, s>>>(kin, n, shift, hist, nb);
+        k_sort_scan<<<1, 1024, 0, s>>>(hist, 256u * nb);
+        k_sort_scatter<<<nb, kSortThreads, 0, s>>>(kin, vin, kout, vout, n, shift, hist, nb);
+        std::swap(kin, kout);
+        std::swap(vin, vout);
+    }
+    // after 4 passes the sorted data is back in keysA / valsA
+    k_karras<<<blocks_for(n - 1, 256), 256, 0, s>>>(kin, (int)n, nodes, leafParent);
+    k_fill_u32<<<blocks_for(n - 1, 256), 256, 0, s>>>(counters, n - 1, 0u);
+    k_refit<<<blocks_for(n, 256), 256, 0, s>>>(boxLo, boxHi, vin, leafParent, nodes, counters, (int)n, rootBox);
+    check(hipMemcpyAsync(leafPrimOut, vin, n * 4ull, hipMemcpyDeviceToDevice, s));
+}
+
+// Refit with the existing topology: new primitive boxes, same leaf order.
+void LbvhBuilder::refit(hipStream_t s, uint32_t n, Node* nodes, const uint32_t* leafPrim, float4* rootBox)
+{
+    if(n <= 1) {
+        k_small_tree<<<1, 64, 0, s>>>(boxLo, boxHi, (int)n, nodes, rootBox);
+        return;
+    }
+    // leafParent from the existing nodes
+    recompute_leaf_parents(s, n, nodes);
+    k_fill_u32<<<blocks_for(n - 1, 256), 256, 0, s>>>(counters, n - 1, 0u);
+    k_refit<<<blocks_for(n, 256), 256, 0, s>>>(boxLo, boxHi, leafPrim, leafParent, nodes, counters, (int)n, rootBox);
+}
+
+namespace {
+__global__ void k_leaf_parents(const Node* __restrict__ nodes, uint32_t numInternal, uint32_t* __restrict__ leafParent)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if(i >= numInternal) return;
+    const Node nd = nodes[i];
+    if(nd.child0 < 0 && nd.child0 != kEmptyChild) leafParent[~nd.child0] = i;
+    if(nd.child1 < 0 && nd.child1 != kEmptyChild) leafParent[~nd.child1] = i;
+}
+} // namespace
+
+void LbvhBuilder::recompute_leaf_parents(hipStream_t s, uint32_t n, const Node* nodes)
+{
+    k_leaf_parents<<<blocks_for(n - 1, 256), 256, 0, s>>>(nodes, n - 1, leafParent);
+}
+
+void launch_tri_boxes(hipStream_t s, const rgrt_vertex* vb, const uint32_t* ib, uint32_t firstVertex, uint32_t indexOffset,
+                      uint32_t n, float4* boxLo, float4* boxHi)
+{
+    if(n) k_tri_boxes<<<blocks_for(n, 256), 256, 0, s>>>(vb, ib, firstVertex, indexOffset, n, boxLo, boxHi);
+}
+
+void launch_tri_gather(hipStream_t s, const rgrt_vertex* vb, const uint32_t* ib, uint32_t firstVertex, uint32_t indexOffset,
+                       const uint32_t* leafPrim, uint32_t n, Tri* tris)
+{
+    if(n) k_tri_gather<<<blocks_for(n, 256), 256, 0, s>>>(vb, ib, firstVertex, indexOffset, leafPrim, n, tris);
+}
+
+void launch_inst_prep(hipStream_t s, const rgrt_instance* in, uint32_t n, const float4* blasRoot, float4* boxLo,
+                      float4* boxHi, float* w2o, InstanceShade* shade)
+{
+    if(n) k_inst_prep<<<blocks_for(n, 128), 128, 0, s>>>(in, n, blasRoot, boxLo, boxHi, w2o, shade);
+}
+
+void launch_inst_gather(hipStream_t s, const float* w2o, const rgrt_instance* in, const uint32_t* leafInst, uint32_t n,
+                        DevInstance* out)
+{
+    if(n) k_inst_gather<<<blocks_for(n, 128), 128, 0, s>>>(w2o, in, leafInst, n, out);
+}
+
+} // namespace rgrt

@@ -1,0 +1,370 @@
+// post.hip — the compute post passes of Raytracer::doRaytracing
+// (raygun/render/raytracer.cpp:106-140) on the HIP core.
+//
+// k_rough_post fuses rough_prepare (resources/shaders/rough_prepare.comp:28-54),
+// the ten (rough_blur_h, rough_blur_v) pairs (rough_blur.h:23-40,
+// raytracer.cpp:116-121) and postprocess (postprocess.comp:28-50) into one
+// launch: a 32x32 output tile is computed from a 52x52 region (10-pixel
+// halo, one pixel per H or V pass), the two colour ping-pong images live in
+// LDS as fp16 exactly as the reference's rgba16f images hold them between
+// passes, and transitions are stored R8_SNORM. k_fxaa is FXAA 3.11 quality,
+// preset 39 (fxaa.comp:29-36) with an emulated bilinear clamp-to-edge
+// sampler (8-bit sub-texel weights).
+#pragma clang fp contract(off)
+
+#include "rgrt_internal.h"
+
+namespace rgrt {
+
+namespace {
+
+constexpr int kTile = 32;
+constexpr int kHalo = 10;
+constexpr int kRegion = kTile + 2 * kHalo; // 52
+constexpr int kRegionPix = kRegion * kRegion;
+constexpr int kPostThreads = 256;
+
+__device__ __forceinline__ float h2f(uint16_t b) { return (float)__builtin_bit_cast(_Float16, b); }
+__device__ __forceinline__ uint16_t f2h(float x) { return __builtin_bit_cast(uint16_t, (_Float16)x); }
+
+struct Px4 {
+    float x, y, z, w;
+};
+
+__device__ __forceinline__ Px4 load16(const uint16_t* img, int W, int H, int x, int y)
+{
+    if(x < 0 || y < 0 || x >= W || y >= H) return {0.0f, 0.0f, 0.0f, 0.0f};
+    const ushort4 v = reinterpret_cast<const ushort4*>(img)[(size_t)y * W + x];
+    return {h2f(v.x), h2f(v.y), h2f(v.z), h2f(v.w)};
+}
+
+__device__ __forceinline__ float dist4(Px4 a, Px4 b)
+{
+    const float d0 = a.x - b.x, d1 = a.y - b.y, d2 = a.z - b.z, d3 = a.w - b.w;
+    return sqrtf(((d0 * d0 + d1 * d1) + d2 * d2) + d3 * d3);
+}
+
+__device__ __forceinline__ int8_t to_snorm8(float f)
+{
+    const float c = clampf(f, -1.0f, 1.0f);
+    return (int8_t)__builtin_rintf(c * 127.0f);
+}
+__device__ __forceinline__ float from_snorm8(int8_t v) { return fmaxf((float)v / 127.0f, -1.0f); }
+
+__global__ __launch_bounds__(kPostThreads) void k_rough_post(Images img, float4 fade)
+{
+    __shared__ ushort4 A[kRegionPix];
+    __shared__ ushort4 B[kRegionPix];
+    __shared__ float T[kRegionPix];
+    const int W = (int)img.w, H = (int)img.h;
+    const int tx0 = blockIdx.x * kTile, ty0 = blockIdx.y * kTile;
+    const int rx0 = tx0 - kHalo, ry0 = ty0 - kHalo;
+
+    // rough_prepare over the region
+    for(int q = threadIdx.x; q < kRegionPix; q += kPostThreads) {
+        const int x = rx0 + q % kRegion, y = ry0 + q / kRegion;
+        if(x < 0 || y < 0 || x >= W || y >= H) {
+            A[q] = make_ushort4(0, 0, 0, 0);
+            B[q] = make_ushort4(0, 0, 0, 0);
+            T[q] = 0.0f;
+            continue;
+        }
+        const Px4 r = load16(img.rough, W, H, x, y);
+        const Px4 ru = load16(img.rough, W, H, x, y + 1);
+        const Px4 rd = load16(img.rough, W, H, x, y - 1);
+        const Px4 rl = load16(img.rough, W, H, x + 1, y);
+        const Px4 rr = load16(img.rough, W, H, x - 1, y);
+        const Px4 n = load16(img.normal, W, H, x, y);
+        const Px4 nu = load16(img.normal, W, H, x, y + 1);
+        const Px4 nd = load16(img.normal, W, H, x, y - 1);
+        const Px4 nl = load16(img.normal, W, H, x + 1, y);
+        const Px4 nr = load16(img.normal, W, H, x - 1, y);
+        const float uf = fminf(ru.w, r.w) * clampf(1.0f - dist4(nu, n) * 10.0f, 0.0f, 1.0f);
+        const float df = fminf(rd.w, r.w) * clampf(1.0f - dist4(nd, n) * 10.0f, 0.0f, 1.0f);
+        const float lf = fminf(rl.w, r.w) * clampf(1.0f - dist4(nl, n) * 10.0f, 0.0f, 1.0f);
+        const float rf = fminf(rr.w, r.w) * clampf(1.0f - dist4(nr, n) * 10.0f, 0.0f, 1.0f);
+        const int8_t tq = to_snorm8(fminf(fminf(fminf(uf, df), lf), rf));
+        T[q] = from_snorm8(tq);
+        const ushort4 rb = reinterpret_cast<const ushort4*>(img.rough)[(size_t)y * W + x];
+        A[q] = rb;
+        B[q] = rb;
+        const int lx = x - tx0, ly = y - ty0;
+        if(lx >= 0 && ly >= 0 && lx < kTile && ly < kTile) img.transitions[(size_t)y * W + x] = tq;
+    }
+    __syncthreads();
+
+    // 10 x (H: A -> B, V: B -> A). Region-border pixels read a missing
+    // neighbour as 0; that error stays inside the halo (one pixel per pass).
+    for(int pass = 0; pass < 20; ++pass) {
+        const bool horiz = (pass & 1) == 0;
+        const ushort4* in = horiz ? A : B;
+        ushort4* out = horiz ? B : A;
+        const int ox = horiz ? 1 : 0, oy = horiz ? 0 : 1;
+        for(int q = threadIdx.x; q < kRegionPix; q += kPostThreads) {
+            const float t = T[q];
+            if(t < 0.001f) continue;
+            const int lx = q % kRegion, ly = q / kRegion;
+            const ushort4 c = in[q];
+            const int q1x = lx + ox, q1y = ly + oy, q2x = lx - ox, q2y = ly - oy;
+            const ushort4 z = make_ushort4(0, 0, 0, 0);
+            const ushort4 c1 = (q1x < kRegion && q1y < kRegion) ? in[q1y * kRegion + q1x] : z;
+            const ushort4 c2 = (q2x >= 0 && q2y >= 0) ? in[q2y * kRegion + q2x] : z;
+            const float w0 = (1.0f - t) - t;
+            ushort4 o;
+            o.x = f2h(((h2f(c.x) * w0) + (h2f(c1.x) * t)) + (h2f(c2.x) * t));
+            o.y = f2h(((h2f(c.y) * w0) + (h2f(c1.y) * t)) + (h2f(c2.y) * t));
+            o.z = f2h(((h2f(c.z) * w0) + (h2f(c1.z) * t)) + (h2f(c2.z) * t));
+            o.w = c.w;
+            out[q] = o;
+        }
+        __syncthreads();
+    }
+
+    // write colorsA / colorsB and the composite (postprocess.comp:30-38)
+    for(int q = threadIdx.x; q < kTile * kTile; q += kPostThreads) {
+        const int x = tx0 + q % kTile, y = ty0 + q / kTile;
+        if(x >= W || y >= H) continue;
+        const int rq = (y - ry0) * kRegion + (x - rx0);
+        const size_t pix = (size_t)y * W + x;
+        const ushort4 a = A[rq];
+        reinterpret_cast<ushort4*>(img.colorsA)[pix] = a;
+        reinterpret_cast<ushort4*>(img.colorsB)[pix] = B[rq];
+        const ushort4 bb = reinterpret_cast<const ushort4*>(img.base)[pix];
+        const float ba = h2f(bb.w);
+        const float c0 = clampf(mixf(h2f(bb.x), h2f(a.x), ba), 0.0f, 1.0f);
+        const float c1 = clampf(mixf(h2f(bb.y), h2f(a.y), ba), 0.0f, 1.0f);
+        const float c2 = clampf(mixf(h2f(bb.z), h2f(a.z), ba), 0.0f, 1.0f);
+        const float luma = ((c0 * 0.299f) + (c1 * 0.587f)) + (c2 * 0.114f);
+        ushort4 f;
+        f.x = f2h(mixf(c0, fade.x, fade.w));
+        f.y = f2h(mixf(c1, fade.y, fade.w));
+        f.z = f2h(mixf(c2, fade.z, fade.w));
+        f.w = f2h(luma);
+        reinterpret_cast<ushort4*>(img.final_)[pix] = f;
+    }
+}
+
+// postprocess.comp:41-49 (ubo.showAlpha): every image shows its alpha.
+__global__ void k_show_alpha(Images img)
+{
+    const size_t npix = (size_t)img.w * img.h;
+    const size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if(p >= npix) return;
+    uint16_t* imgs[6] = {img.final_, img.normal, img.rough, img.base, img.colorsA, img.colorsB};
+    for(int k = 0; k < 6; ++k) {
+        ushort4 v = reinterpret_cast<ushort4*>(imgs[k])[p];
+        v.x = v.y = v.z = v.w;
+        reinterpret_cast<ushort4*>(imgs[k])[p] = v;
+    }
+    img.transitions[p] = to_snorm8(1.0f); // .aaaa of a one-channel image reads alpha = 1
+}
+
+// ---------------------------------------------------------------------------
+// FXAA
+// ---------------------------------------------------------------------------
+struct Tex {
+    const uint16_t* p;
+    int W, H;
+};
+
+// textureLod / textureLodOffset on the linear clamp-to-edge sampler
+// (raygun/compute/compute_system.cpp:76-85), fp32 filtering of the fp16
+// texels with weights rounded to 1/256.
+__device__ __forceinline__ Px4 tex_sample(const Tex& t, float px, float py, int ox, int oy)
+{
+    const float u = (px * (float)t.W - 0.5f) + (float)ox;
+    const float v = (py * (float)t.H - 0.5f) + (float)oy;
+    const float fu = floorf(u), fv = floorf(v);
+    const float au = __builtin_rintf((u - fu) * 256.0f) / 256.0f;
+    const float av = __builtin_rintf((v - fv) * 256.0f) / 256.0f;
+    int x0 = (int)fu, y0 = (int)fv;
+    int x1 = x0 + 1, y1 = y0 + 1;
+    x0 = min(max(x0, 0), t.W - 1);
+    x1 = min(max(x1, 0), t.W - 1);
+    y0 = min(max(y0, 0), t.H - 1);
+    y1 = min(max(y1, 0), t.H - 1);
+    const Px4 t00 = load16(t.p, t.W, t.H, x0, y0), t10 = load16(t.p, t.W, t.H, x1, y0);
+    const Px4 t01 = load16(t.p, t.W, t.H, x0, y1), t11 = load16(t.p, t.W, t.H, x1, y1);
+    Px4 r;
+    r.x = mixf(mixf(t00.x, t10.x, au), mixf(t01.x, t11.x, au), av);
+    r.y = mixf(mixf(t00.y, t10.y, au), mixf(t01.y, t11.y, au), av);
+    r.z = mixf(mixf(t00.z, t10.z, au), mixf(t01.z, t11.z, au), av);
+    r.w = mixf(mixf(t00.w, t10.w, au), mixf(t01.w, t11.w, au), av);
+    return r;
+}
+
+// FXAA_GREEN_AS_LUMA = 1: luma is the green channel.
+__device__ __forceinline__ float luma_at(const Tex& t, float px, float py, int ox = 0, int oy = 0)
+{
+    return tex_sample(t, px, py, ox, oy).y;
+}
+
+__constant__ float kFxaaSteps[12] = {1.0f, 1.0f, 1.0f, 1.0f, 1.0f, 1.5f, 2.0f, 2.0f, 2.0f, 2.0f, 4.0f, 8.0f};
+
+__global__ __launch_bounds__(256) void k_fxaa(const uint16_t* __restrict__ src, uint16_t* __restrict__ out, int W, int H)
+{
+    const int x = blockIdx.x * 16 + (threadIdx.x & 15);
+    const int y = blockIdx.y * 16 + (threadIdx.x >> 4);
+    if(x >= W || y >= H) return;
+    const Tex tex{src, W, H};
+    const float subpix = 1.0f, edgeThreshold = 0.063f, edgeThresholdMin = 0.0312f;
+    const float rcpX = 1.0f / (float)W, rcpY = 1.0f / (float)H;
+    float posMx = ((float)x + 0.5f) / (float)W;
+    float posMy = ((float)y + 0.5f) / (float)H;
+
+    const Px4 rgbyM = tex_sample(tex, posMx, posMy, 0, 0);
+    const float lumaM = rgbyM.y;
+    float lumaS = luma_at(tex, posMx, posMy, 0, 1);
+    float lumaE = luma_at(tex, posMx, posMy, 1, 0);
+    float lumaN = luma_at(tex, posMx, posMy, 0, -1);
+    float lumaW = luma_at(tex, posMx, posMy, -1, 0);
+    const float maxSM = fmaxf(lumaS, lumaM), minSM = fminf(lumaS, lumaM);
+    const float maxESM = fmaxf(lumaE, maxSM), minESM = fminf(lumaE, minSM);
+    const float maxWN = fmaxf(lumaN, lumaW), minWN = fminf(lumaN, lumaW);
+    const float rangeMax = fmaxf(maxWN, maxESM), rangeMin = fminf(minWN, minESM);
+    const float rangeMaxScaled = rangeMax * edgeThreshold;
+    const float range = rangeMax - rangeMin;
+    const float rangeMaxClamped = fmaxf(edgeThresholdMin, rangeMaxScaled);
+    ushort4 o;
+    if(range < rangeMaxClamped) {
+        o.x = f2h(rgbyM.x);
+        o.y = f2h(rgbyM.y);
+        o.z = f2h(rgbyM.z);
+        o.w = f2h(rgbyM.w);
+        reinterpret_cast<ushort4*>(out)[(size_t)y * W + x] = o;
+        return;
+    }
+    const float lumaNW = luma_at(tex, posMx, posMy, -1, -1);
+    const float lumaSE = luma_at(tex, posMx, posMy, 1, 1);
+    const float lumaNE = luma_at(tex, posMx, posMy, 1, -1);
+    const float lumaSW = luma_at(tex, posMx, posMy, -1, 1);
+
+    const float lumaNS = lumaN + lumaS, lumaWE = lumaW + lumaE;
+    const float subpixRcpRange = 1.0f / range;
+    const float subpixNSWE = lumaNS + lumaWE;
+    const float edgeHorz1 = (-2.0f * lumaM) + lumaNS;
+    const float edgeVert1 = (-2.0f * lumaM) + lumaWE;
+    const float lumaNESE = lumaNE + lumaSE, lumaNWNE = lumaNW + lumaNE;
+    const float edgeHorz2 = (-2.0f * lumaE) + lumaNESE;
+    const float edgeVert2 = (-2.0f * lumaN) + lumaNWNE;
+    const float lumaNWSW = lumaNW + lumaSW, lumaSWSE = lumaSW + lumaSE;
+    const float edgeHorz4 = (fabsf(edgeHorz1) * 2.0f) + fabsf(edgeHorz2);
+    const float edgeVert4 = (fabsf(edgeVert1) * 2.0f) + fabsf(edgeVert2);
+    const float edgeHorz3 = (-2.0f * lumaW) + lumaNWSW;
+    const float edgeVert3 = (-2.0f * lumaS) + lumaSWSE;
+    const float edgeHorz = fabsf(edgeHorz3) + edgeHorz4;
+    const float edgeVert = fabsf(edgeVert3) + edgeVert4;
+    const float subpixNWSWNESE = lumaNWSW + lumaNESE;
+    float lengthSign = rcpX;
+    const bool horzSpan = edgeHorz >= edgeVert;
+    const float subpixA = subpixNSWE * 2.0f + subpixNWSWNESE;
+    if(!horzSpan) {
+        lumaN = lumaW;
+        lumaS = lumaE;
+    }
+    if(horzSpan) lengthSign = rcpY;
+    const float subpixB = (subpixA * (float)(1.0 / 12.0)) - lumaM;
+    const float gradientN = lumaN - lumaM, gradientS = lumaS - lumaM;
+    float lumaNN = lumaN + lumaM;
+    const float lumaSS = lumaS + lumaM;
+    const bool pairN = fabsf(gradientN) >= fabsf(gradientS);
+    const float gradient = fmaxf(fabsf(gradientN), fabsf(gradientS));
+    if(pairN) lengthSign = -lengthSign;
+    const float subpixC = clampf(fabsf(subpixB) * subpixRcpRange, 0.0f, 1.0f);
+
+    float posBx = posMx, posBy = posMy;
+    const float offNPx = (!horzSpan) ? 0.0f : rcpX;
+    const float offNPy = (horzSpan) ? 0.0f : rcpY;
+    if(!horzSpan) posBx += lengthSign * 0.5f;
+    if(horzSpan) posBy += lengthSign * 0.5f;
+
+    float posNx = posBx - offNPx * kFxaaSteps[0], posNy = posBy - offNPy * kFxaaSteps[0];
+    float posPx = posBx + offNPx * kFxaaSteps[0], posPy = posBy + offNPy * kFxaaSteps[0];
+    const float subpixD = ((-2.0f) * subpixC) + 3.0f;
+    float lumaEndN = luma_at(tex, posNx, posNy);
+    const float subpixE = subpixC * subpixC;
+    float lumaEndP = luma_at(tex, posPx, posPy);
+    if(!pairN) lumaNN = lumaSS;
+    const float gradientScaled = (gradient * 1.0f) / 4.0f;
+    const float lumaMM = lumaM - lumaNN * 0.5f;
+    const float subpixF = subpixD * subpixE;
+    const bool lumaMLTZero = lumaMM < 0.0f;
+    lumaEndN -= lumaNN * 0.5f;
+    lumaEndP -= lumaNN * 0.5f;
+    bool doneN = fabsf(lumaEndN) >= gradientScaled;
+    bool doneP = fabsf(lumaEndP) >= gradientScaled;
+    if(!doneN) {
+        posNx -= offNPx * kFxaaSteps[1];
+        posNy -= offNPy * kFxaaSteps[1];
+    }
+    bool doneNP = (!doneN) || (!doneP);
+    if(!doneP) {
+        posPx += offNPx * kFxaaSteps[1];
+        posPy += offNPy * kFxaaSteps[1];
+    }
+    for(int step = 2; step < 12 && doneNP; ++step) {
+        if(!doneN) lumaEndN = luma_at(tex, posNx, posNy);
+        if(!doneP) lumaEndP = luma_at(tex, posPx, posPy);
+        if(!doneN) lumaEndN = lumaEndN - lumaNN * 0.5f;
+        if(!doneP) lumaEndP = lumaEndP - lumaNN * 0.5f;
+        doneN = fabsf(lumaEndN) >= gradientScaled;
+        doneP = fabsf(lumaEndP) >= gradientScaled;
+        if(!doneN) {
+            posNx -= offNPx * kFxaaSteps[step];
+            posNy -= offNPy * kFxaaSteps[step];
+        }
+        doneNP = (!doneN) || (!doneP);
+        if(!doneP) {
+            posPx += offNPx * kFxaaSteps[step];
+            posPy += offNPy * kFxaaSteps[step];
+        }
+    }
+    float dstN = posMx - posNx, dstP = posPx - posMx;
+    if(!horzSpan) {
+        dstN = posMy - posNy;
+        dstP = posPy - posMy;
+    }
+    const bool goodSpanN = (lumaEndN < 0.0f) != lumaMLTZero;
+    const float spanLength = dstP + dstN;
+    const bool goodSpanP = (lumaEndP < 0.0f) != lumaMLTZero;
+    const float spanLengthRcp = 1.0f / spanLength;
+    const bool directionN = dstN < dstP;
+    const float dst = fminf(dstN, dstP);
+    const bool goodSpan = directionN ? goodSpanN : goodSpanP;
+    const float subpixG = subpixF * subpixF;
+    const float pixelOffset = (dst * (-spanLengthRcp)) + 0.5f;
+    const float subpixH = subpixG * subpix;
+    const float pixelOffsetGood = goodSpan ? pixelOffset : 0.0f;
+    const float pixelOffsetSubpix = fmaxf(pixelOffsetGood, subpixH);
+    if(!horzSpan) posMx += pixelOffsetSubpix * lengthSign;
+    if(horzSpan) posMy += pixelOffsetSubpix * lengthSign;
+    const Px4 c = tex_sample(tex, posMx, posMy, 0, 0);
+    o.x = f2h(c.x);
+    o.y = f2h(c.y);
+    o.z = f2h(c.z);
+    o.w = f2h(lumaM);
+    reinterpret_cast<ushort4*>(out)[(size_t)y * W + x] = o;
+}
+
+} // namespace
+
+// Post passes: rough + composite into `final_`, optional showAlpha, and
+// FXAA from `final_` into `base`. `roughEnd` (may be null) is recorded
+// after the fused rough/composite kernel.
+void launch_post(hipStream_t s, const Images& img, const rgrt_ubo& ubo, int useFxaa, hipEvent_t roughEnd)
+{
+    const dim3 grid((img.w + kTile - 1) / kTile, (img.h + kTile - 1) / kTile);
+    const float4 fade = make_float4(ubo.fadeColor[0], ubo.fadeColor[1], ubo.fadeColor[2], ubo.fadeColor[3]);
+    k_rough_post<<<grid, kPostThreads, 0, s>>>(img, fade);
+    if(roughEnd) (void)hipEventRecord(roughEnd, s);
+    if(ubo.showAlpha) {
+        const size_t npix = (size_t)img.w * img.h;
+        k_show_alpha<<<(unsigned)((npix + 255) / 256), 256, 0, s>>>(img);
+    }
+    if(useFxaa) {
+        const dim3 g2((img.w + 15) / 16, (img.h + 15) / 16);
+        k_fxaa<<<g2, 256, 0, s>>>(img.final_, img.base, (int)img.w, (int)img.h);
+    }
+}
+
+} // namespace rgrt

@@ -1,0 +1,121 @@
+// rgrt_device.h — device-side records and GLSL built-ins for the HIP core.
+//
+// Layouts are the C ABI's (include/rgrt.h). The built-ins follow the GLSL
+// 4.60 definitions, evaluated in the order written; files that include this
+// header compile with floating-point contraction off (see the pragma at the
+// top of each .hip file) so results do not depend on fused multiply-add
+// choices of the compiler.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/rgrt.h"
+
+namespace rgrt {
+
+constexpr int32_t kEmptyChild = (int32_t)0x80000000;
+constexpr int32_t kSentinel = (int32_t)0x80000000; // stack marker: return to the TLAS
+constexpr uint32_t kNoHit = 0xffffffffu;
+
+// ---------------------------------------------------------------------------
+// BVH records in HBM
+// ---------------------------------------------------------------------------
+// Node: 64 bytes, the canonical rgrt_bvh_node layout; the traversal reads it
+// as four 16-byte vectors.
+struct __align__(16) Node {
+    float lo0x, lo0y, lo0z, lo1x; // lo[0][0..2], lo[1][0]
+    float lo1y, lo1z, hi0x, hi0y; // lo[1][1..2], hi[0][0..1]
+    float hi0z, hi1x, hi1y, hi1z; // hi[0][2], hi[1][0..2]
+    int32_t child0, child1;
+    uint32_t parent, pad;
+};
+static_assert(sizeof(Node) == 64, "node is 64 bytes");
+static_assert(sizeof(Node) == sizeof(rgrt_bvh_node), "canonical node layout");
+
+// Leaf triangle in Morton order: the three vertices and the original
+// primitive index (gl_PrimitiveID), 48 bytes.
+struct __align__(16) Tri {
+    float v0x, v0y, v0z;
+    uint32_t prim;
+    float v1x, v1y, v1z;
+    uint32_t pad0;
+    float v2x, v2y, v2z;
+    uint32_t pad1;
+};
+static_assert(sizeof(Tri) == 48, "triangle is 48 bytes");
+
+// TLAS leaf record in Morton order, 64 bytes.
+struct __align__(16) DevInstance {
+    float w2o[12];     // world -> object, row-major 3x4
+    uint32_t blas;     // BLAS handle
+    uint32_t mask;     // instance mask (low 8 bits)
+    uint32_t position; // index in the caller's instance array
+    uint32_t customIndex;
+};
+static_assert(sizeof(DevInstance) == 64, "instance leaf is 64 bytes");
+
+struct DevBlas {
+    const Node* nodes;
+    const Tri* tris;
+    uint32_t numFaces;
+    uint32_t pad;
+};
+
+// Per-instance data the shading reads, indexed by instance position.
+struct __align__(16) InstanceShade {
+    float o2w[12];
+    rgrt_instance_offsets offsets; // the offset table entry at this position
+    uint32_t customIndex;
+};
+static_assert(sizeof(InstanceShade) == 64, "shade record is 64 bytes");
+
+// Scene seen by the traversal.
+struct TraceScene {
+    const Node* tlasNodes;
+    const DevInstance* instances; // Morton order
+    const DevBlas* blas;
+    uint32_t numInstances;
+};
+
+// ---------------------------------------------------------------------------
+// GLSL built-ins (fp32, evaluated as written)
+// ---------------------------------------------------------------------------
+struct F3 {
+    float x, y, z;
+};
+__device__ __forceinline__ F3 f3(float x, float y, float z) { return {x, y, z}; }
+__device__ __forceinline__ F3 f3(float s) { return {s, s, s}; }
+__device__ __forceinline__ F3 operator+(F3 a, F3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ F3 operator-(F3 a, F3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ F3 operator*(F3 a, F3 b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
+__device__ __forceinline__ F3 operator*(F3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ F3 operator*(float s, F3 a) { return {s * a.x, s * a.y, s * a.z}; }
+__device__ __forceinline__ F3 operator-(F3 a) { return {-a.x, -a.y, -a.z}; }
+__device__ __forceinline__ float dot(F3 a, F3 b) { return ((a.x * b.x) + (a.y * b.y)) + (a.z * b.z); }
+__device__ __forceinline__ float length(F3 a) { return __builtin_sqrtf(dot(a, a)); }
+__device__ __forceinline__ F3 normalize(F3 a)
+{
+    const float l = length(a);
+    return {a.x / l, a.y / l, a.z / l};
+}
+__device__ __forceinline__ float distance(F3 a, F3 b) { return length(a - b); }
+__device__ __forceinline__ float mixf(float x, float y, float a) { return (x * (1.0f - a)) + (y * a); }
+__device__ __forceinline__ F3 mix(F3 x, F3 y, float a) { return {mixf(x.x, y.x, a), mixf(x.y, y.y, a), mixf(x.z, y.z, a)}; }
+__device__ __forceinline__ float clampf(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }
+__device__ __forceinline__ float glsl_mod(float x, float y) { return x - y * floorf(x / y); }
+__device__ __forceinline__ F3 reflect(F3 I, F3 N)
+{
+    const float d2 = 2.0f * dot(N, I);
+    return {I.x - d2 * N.x, I.y - d2 * N.y, I.z - d2 * N.z};
+}
+__device__ __forceinline__ F3 refract(F3 I, F3 N, float eta)
+{
+    const float d = dot(N, I);
+    const float k = 1.0f - (eta * eta) * (1.0f - d * d);
+    if(k < 0.0f) return f3(0.0f);
+    const float s = eta * d + __builtin_sqrtf(k);
+    return {eta * I.x - s * N.x, eta * I.y - s * N.y, eta * I.z - s * N.z};
+}
+
+} // namespace rgrt

@@ -1,0 +1,115 @@
+// rgrt_internal.h — host-side structures shared by the HIP core's sources.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "rgrt_device.h"
+
+namespace rgrt {
+
+struct HipError : std::runtime_error {
+    hipError_t code;
+    HipError(hipError_t c, const char* what) : std::runtime_error(what), code(c) {}
+};
+
+inline void check(hipError_t e)
+{
+    if(e != hipSuccess) throw HipError(e, hipGetErrorString(e));
+}
+
+// Reusable scratch of the LBVH builder (device memory).
+struct LbvhBuilder {
+    uint32_t capacity = 0;
+    uint32_t *keysA = nullptr, *keysB = nullptr, *valsA = nullptr, *valsB = nullptr;
+    uint32_t* hist = nullptr;
+    float4 *boxLo = nullptr, *boxHi = nullptr;
+    uint32_t* leafParent = nullptr;
+    uint32_t* counters = nullptr;
+    uint32_t* sceneBox = nullptr;
+
+    void reserve(uint32_t n);
+    void release();
+    void build(hipStream_t s, uint32_t n, Node* nodes, uint32_t* leafPrimOut, float4* rootBox);
+    void refit(hipStream_t s, uint32_t n, Node* nodes, const uint32_t* leafPrim, float4* rootBox);
+    void recompute_leaf_parents(hipStream_t s, uint32_t n, const Node* nodes);
+};
+
+void launch_tri_boxes(hipStream_t s, const rgrt_vertex* vb, const uint32_t* ib, uint32_t firstVertex, uint32_t indexOffset,
+                      uint32_t n, float4* boxLo, float4* boxHi);
+void launch_tri_gather(hipStream_t s, const rgrt_vertex* vb, const uint32_t* ib, uint32_t firstVertex, uint32_t indexOffset,
+                       const uint32_t* leafPrim, uint32_t n, Tri* tris);
+void launch_inst_prep(hipStream_t s, const rgrt_instance* in, uint32_t n, const float4* blasRoot, float4* boxLo,
+                      float4* boxHi, float* w2o, InstanceShade* shade);
+void launch_inst_gather(hipStream_t s, const float* w2o, const rgrt_instance* in, const uint32_t* leafInst, uint32_t n,
+                        DevInstance* out);
+
+// ---------------------------------------------------------------------------
+// Wavefront state (device memory, structure of arrays, one slot per pixel)
+// ---------------------------------------------------------------------------
+constexpr int kPayloadWords = 16;
+constexpr int kAccumWords = 12;
+constexpr int kFrameWords = 34;
+constexpr int kMaxFrames = 8; // maxRecursions <= 7 keeps at most 7 frames waiting
+constexpr int kRayWords = 10; // origin, tmin, dir, tmax, pixel, flags
+constexpr int kHitWords = 5;
+
+// Ray flags in a queue entry
+constexpr uint32_t kFlagMiss1 = 1u;     // miss shader index 1 (shadow miss)
+constexpr uint32_t kFlagPrimary = 2u;   // primary ray of sample 0 (AOV record)
+constexpr uint32_t kMaskShift = 8u;     // cull mask in bits 8..15
+
+struct WavefrontState {
+    uint32_t npix = 0;
+    float* payload = nullptr; // [kPayloadWords][npix]
+    float* accum = nullptr;   // [kAccumWords][npix]
+    uint32_t* meta = nullptr; // [npix]: sp | sample << 8
+    float* frames = nullptr;  // [kMaxFrames][kFrameWords][npix]
+    float* rays[2] = {nullptr, nullptr}; // [kRayWords][npix]
+    float* hits = nullptr;    // [kHitWords][npix]
+    uint32_t* counters = nullptr; // [0..1] queue counts, [2..3] fetch, [4..] stats (u64 pairs)
+};
+
+struct Images {
+    uint32_t w = 0, h = 0;
+    uint16_t* base = nullptr;
+    uint16_t* normal = nullptr;
+    uint16_t* rough = nullptr;
+    uint16_t* final_ = nullptr;
+    int8_t* transitions = nullptr;
+    uint16_t* colorsA = nullptr;
+    uint16_t* colorsB = nullptr;
+    float* base32 = nullptr;
+    float* normal32 = nullptr;
+    float* rough32 = nullptr;
+    rgrt_hit* primary = nullptr;
+};
+
+// Arguments of the shading kernels
+struct ShadeParams {
+    const rgrt_vertex* vb;
+    const uint32_t* ib;
+    const rgrt_material* mats;
+    const InstanceShade* inst; // by instance position
+    uint32_t numInstances;
+    uint64_t numVertices, numIndices;
+    uint32_t numMaterials;
+    rgrt_ubo ubo;
+    uint32_t width, height;
+    WavefrontState st;
+    Images img;
+    int32_t keepF32;
+    int32_t keepAov;
+};
+
+void launch_raygen(hipStream_t s, const ShadeParams& p);
+void launch_trace_queue(hipStream_t s, const TraceScene& sc, const WavefrontState& st, int cur, uint32_t gridBlocks);
+void launch_shade(hipStream_t s, const ShadeParams& p, int cur, uint32_t gridBlocks);
+void launch_trace_rays(hipStream_t s, const TraceScene& sc, const rgrt_ray* rays, uint32_t n, uint32_t cullMask, rgrt_hit* hits);
+void launch_post(hipStream_t s, const Images& img, const rgrt_ubo& ubo, int useFxaa, hipEvent_t roughEnd);
+
+} // namespace rgrt

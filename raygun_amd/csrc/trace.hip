@@ -1,0 +1,114 @@
+// trace.hip — traversal kernels: the wavefront queue pass and plain
+// closest-hit queries (rgrt_trace_rays).
+#pragma clang fp contract(off)
+
+#include "rgrt_internal.h"
+#include "traverse.h"
+
+namespace rgrt {
+
+namespace {
+
+constexpr int kTraceBlock = 256;
+constexpr int kLdsStack = 16;
+constexpr int kSpill = 112; // LBVH depth <= 62 per level, two levels + sentinel < 128
+
+// Wavefront trace pass: rays [0, count) of queue `cur` -> hit records.
+// Waves fetch 64 rays at a time from a global counter (persistent grid).
+__global__ __launch_bounds__(kTraceBlock) void k_trace_queue(TraceScene sc, const float* __restrict__ rays,
+                                                              float* __restrict__ hits, uint32_t npix,
+                                                              uint32_t* __restrict__ counters, int cur,
+                                                              int32_t* __restrict__ spillBase, uint32_t* hostSlot)
+{
+    __shared__ int32_t ldsStack[kLdsStack * kTraceBlock];
+    const uint32_t count = counters[cur];
+    if(blockIdx.x == 0 && threadIdx.x == 0) {
+        *hostSlot = count;         // host-visible: lets the host stop once the queues drain
+        counters[1 - cur] = 0;     // the output queue of the shading pass that follows
+        counters[2 + (1 - cur)] = 0; // the fetch counter of the next traversal pass
+        // ray statistics (64-bit) at counters[4..5]
+        atomicAdd(reinterpret_cast<unsigned long long*>(counters + 4), (unsigned long long)count);
+    }
+    const int lane = threadIdx.x & 63;
+    const uint32_t gtid = blockIdx.x * kTraceBlock + threadIdx.x;
+    TraversalStack<kLdsStack, kTraceBlock, kSpill> stack{ldsStack, spillBase + (size_t)gtid * kSpill, (int)threadIdx.x};
+    uint32_t* fetch = counters + 2 + cur;
+    for(;;) {
+        uint32_t base = 0;
+        if(lane == 0) base = atomicAdd(fetch, 64u);
+        base = __shfl(base, 0);
+        if(base >= count) break;
+        const uint32_t i = base + lane;
+        if(i < count) {
+            RayIn r;
+            r.ox = rays[0 * (size_t)npix + i];
+            r.oy = rays[1 * (size_t)npix + i];
+            r.oz = rays[2 * (size_t)npix + i];
+            r.tmin = rays[3 * (size_t)npix + i];
+            r.dx = rays[4 * (size_t)npix + i];
+            r.dy = rays[5 * (size_t)npix + i];
+            r.dz = rays[6 * (size_t)npix + i];
+            r.tmax = rays[7 * (size_t)npix + i];
+            const uint32_t flags = __float_as_uint(rays[9 * (size_t)npix + i]);
+            const HitRec h = traverse(sc, r, (flags >> kMaskShift) & 0xffu, stack);
+            hits[0 * (size_t)npix + i] = h.t;
+            hits[1 * (size_t)npix + i] = h.u;
+            hits[2 * (size_t)npix + i] = h.v;
+            hits[3 * (size_t)npix + i] = __uint_as_float(h.inst);
+            hits[4 * (size_t)npix + i] = __uint_as_float(h.prim);
+        }
+    }
+}
+
+// Closest-hit queries, grid-stride over a persistent grid (the spill
+// region is sized per launched thread).
+__global__ __launch_bounds__(kTraceBlock) void k_trace_rays(TraceScene sc, const rgrt_ray* __restrict__ rays, uint32_t n,
+                                                             uint32_t cullMask, rgrt_hit* __restrict__ hits,
+                                                             int32_t* __restrict__ spillBase)
+{
+    __shared__ int32_t ldsStack[kLdsStack * kTraceBlock];
+    const uint32_t gtid = blockIdx.x * kTraceBlock + threadIdx.x;
+    TraversalStack<kLdsStack, kTraceBlock, kSpill> stack{ldsStack, spillBase + (size_t)gtid * kSpill, (int)threadIdx.x};
+    for(uint32_t i = gtid; i < n; i += gridDim.x * kTraceBlock) {
+        const rgrt_ray rr = rays[i];
+        RayIn r{rr.origin[0], rr.origin[1], rr.origin[2], rr.tmin, rr.dir[0], rr.dir[1], rr.dir[2], rr.tmax};
+        const HitRec h = traverse(sc, r, cullMask & 0xffu, stack);
+        rgrt_hit o;
+        o.t = h.t;
+        o.u = h.u;
+        o.v = h.v;
+        o.instance = h.inst; // position; mapped to customIndex below
+        o.primitive = h.prim;
+        hits[i] = o;
+    }
+}
+
+__global__ void k_hit_custom_index(rgrt_hit* hits, uint32_t n, const InstanceShade* __restrict__ inst)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if(i >= n) return;
+    const uint32_t p = hits[i].instance;
+    if(p != kNoHit) hits[i].instance = inst[p].customIndex;
+}
+
+} // namespace
+
+uint32_t trace_spill_words_per_thread() { return kSpill; }
+uint32_t trace_block_size() { return kTraceBlock; }
+
+void launch_trace_queue_impl(hipStream_t s, const TraceScene& sc, const WavefrontState& st, int cur, uint32_t gridBlocks,
+                             int32_t* spill, uint32_t* hostSlot)
+{
+    k_trace_queue<<<gridBlocks, kTraceBlock, 0, s>>>(sc, st.rays[cur], st.hits, st.npix, st.counters, cur, spill, hostSlot);
+}
+
+void launch_trace_rays_impl(hipStream_t s, const TraceScene& sc, const rgrt_ray* rays, uint32_t n, uint32_t cullMask,
+                            rgrt_hit* hits, const InstanceShade* inst, int32_t* spill, uint32_t gridBlocks)
+{
+    if(!n) return;
+    const uint32_t blocks = std::min<uint32_t>(gridBlocks, (n + kTraceBlock - 1) / kTraceBlock);
+    k_trace_rays<<<blocks, kTraceBlock, 0, s>>>(sc, rays, n, cullMask, hits, spill);
+    k_hit_custom_index<<<(n + 255) / 256, 256, 0, s>>>(hits, n, inst);
+}
+
+} // namespace rgrt

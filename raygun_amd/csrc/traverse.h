@@ -1,0 +1,296 @@
+// traverse.h — closest-hit traversal of the two-level LBVH.
+//
+// Stands in for the driver's traversal behind traceRayEXT
+// (resources/shaders/raygen.h:102, closesthit.rchit:133,144,158,193,215,232,244):
+// opaque geometry, no face culling (the instance flag set by
+// raygun/render/acceleration_structure.cpp:41), instance mask test
+// (mask & cullMask), t measured along the unnormalised world ray.
+//
+// Depth-first over child references (>= 0 internal node, < 0 leaf ~k). At a
+// node whose two children are hit, the nearer is visited next (child 0 on an
+// equal entry distance) and the farther waits on the stack. An instance leaf
+// maps the ray to object space, pushes a sentinel and continues at the BLAS
+// root; popping the sentinel restores the world ray. A candidate is accepted
+// when tmin <= t < tbest, so the first triangle found wins an exact tie.
+//
+// The stack lives in LDS (kLdsStack entries per thread, lane-interleaved so
+// a wave's pushes hit distinct banks) and spills to a per-thread global
+// region beyond that depth.
+#pragma once
+
+#include "rgrt_device.h"
+
+namespace rgrt {
+
+struct RayIn {
+    float ox, oy, oz, tmin, dx, dy, dz, tmax;
+};
+
+struct HitRec {
+    float t, u, v;
+    uint32_t inst; // instance position, kNoHit on a miss
+    uint32_t prim;
+};
+
+// Reciprocal for the slab test: a zero component becomes +-1e-20 so the
+// fused form lo*inv - o*inv stays finite.
+__device__ __forceinline__ float safe_rcp(float d)
+{
+    const float a = fabsf(d) > 1e-20f ? d : copysignf(1e-20f, d);
+    return 1.0f / a;
+}
+
+struct BoxRay {
+    float ix, iy, iz;    // 1/d
+    float nox, noy, noz; // -o/d
+};
+
+__device__ __forceinline__ void box_ray_setup(BoxRay& b, float ox, float oy, float oz, float dx, float dy, float dz)
+{
+    b.ix = safe_rcp(dx);
+    b.iy = safe_rcp(dy);
+    b.iz = safe_rcp(dz);
+    b.nox = -ox * b.ix;
+    b.noy = -oy * b.iy;
+    b.noz = -oz * b.iz;
+}
+
+// Slab test of both children of a node. Only culls, so it uses fused
+// multiply-adds; the far distance is widened by 1 + 2*gamma(3) (Ize 2013)
+// to stay conservative.
+__device__ __forceinline__ void box_test2(const BoxRay& r, const float4 a, const float4 b, const float4 c, float tmin,
+                                          float tmax, bool& h0, bool& h1, float& tn0, float& tn1)
+{
+    // a = (lo0x lo0y lo0z lo1x), b = (lo1y lo1z hi0x hi0y), c = (hi0z hi1x hi1y hi1z)
+    const float x00 = __builtin_fmaf(a.x, r.ix, r.nox), x01 = __builtin_fmaf(b.z, r.ix, r.nox);
+    const float y00 = __builtin_fmaf(a.y, r.iy, r.noy), y01 = __builtin_fmaf(b.w, r.iy, r.noy);
+    const float z00 = __builtin_fmaf(a.z, r.iz, r.noz), z01 = __builtin_fmaf(c.x, r.iz, r.noz);
+    const float x10 = __builtin_fmaf(a.w, r.ix, r.nox), x11 = __builtin_fmaf(c.y, r.ix, r.nox);
+    const float y10 = __builtin_fmaf(b.x, r.iy, r.noy), y11 = __builtin_fmaf(c.z, r.iy, r.noy);
+    const float z10 = __builtin_fmaf(b.y, r.iz, r.noz), z11 = __builtin_fmaf(c.w, r.iz, r.noz);
+    const float n0 = fmaxf(fmaxf(fminf(x00, x01), fminf(y00, y01)), fmaxf(fminf(z00, z01), tmin));
+    const float f0 = fminf(fminf(fmaxf(x00, x01), fmaxf(y00, y01)), fminf(fmaxf(z00, z01), tmax));
+    const float n1 = fmaxf(fmaxf(fminf(x10, x11), fminf(y10, y11)), fmaxf(fminf(z10, z11), tmin));
+    const float f1 = fminf(fminf(fmaxf(x10, x11), fmaxf(y10, y11)), fminf(fmaxf(z10, z11), tmax));
+    h0 = n0 <= f0 * 1.0000004f;
+    h1 = n1 <= f1 * 1.0000004f;
+    tn0 = n0;
+    tn1 = n1;
+}
+
+// Per-ray setup of the watertight test (Woop, Benthin, Wald 2013).
+struct WtRay {
+    int kx, ky, kz;
+    float Sx, Sy, Sz;
+    float okx, oky, okz; // origin permuted
+};
+
+__device__ __forceinline__ float sel3(float x, float y, float z, int k) { return k == 0 ? x : (k == 1 ? y : z); }
+
+__device__ __forceinline__ void wt_setup(WtRay& w, float ox, float oy, float oz, float dx, float dy, float dz)
+{
+    int kz = 0;
+    float m = fabsf(dx);
+    if(fabsf(dy) > m) {
+        kz = 1;
+        m = fabsf(dy);
+    }
+    if(fabsf(dz) > m) kz = 2;
+    int kx = kz + 1 == 3 ? 0 : kz + 1;
+    int ky = kx + 1 == 3 ? 0 : kx + 1;
+    const float dkz = sel3(dx, dy, dz, kz);
+    if(dkz < 0.0f) {
+        const int t = kx;
+        kx = ky;
+        ky = t;
+    }
+    w.kx = kx;
+    w.ky = ky;
+    w.kz = kz;
+    const float dkx = sel3(dx, dy, dz, kx), dky = sel3(dx, dy, dz, ky);
+    w.Sx = dkx / dkz;
+    w.Sy = dky / dkz;
+    w.Sz = 1.0f / dkz;
+    w.okx = sel3(ox, oy, oz, kx);
+    w.oky = sel3(ox, oy, oz, ky);
+    w.okz = sel3(ox, oy, oz, kz);
+}
+
+// Watertight ray/triangle test; on a hit with tmin <= t < tbest returns
+// true with t and the weights u (vertex 1), v (vertex 2).
+__device__ __forceinline__ bool wt_intersect(const WtRay& w, const Tri& tri, float tmin, float tbest, float& t, float& u,
+                                             float& v)
+{
+    const float Akx = sel3(tri.v0x, tri.v0y, tri.v0z, w.kx) - w.okx;
+    const float Aky = sel3(tri.v0x, tri.v0y, tri.v0z, w.ky) - w.oky;
+    const float Akz = sel3(tri.v0x, tri.v0y, tri.v0z, w.kz) - w.okz;
+    const float Bkx = sel3(tri.v1x, tri.v1y, tri.v1z, w.kx) - w.okx;
+    const float Bky = sel3(tri.v1x, tri.v1y, tri.v1z, w.ky) - w.oky;
+    const float Bkz = sel3(tri.v1x, tri.v1y, tri.v1z, w.kz) - w.okz;
+    const float Ckx = sel3(tri.v2x, tri.v2y, tri.v2z, w.kx) - w.okx;
+    const float Cky = sel3(tri.v2x, tri.v2y, tri.v2z, w.ky) - w.oky;
+    const float Ckz = sel3(tri.v2x, tri.v2y, tri.v2z, w.kz) - w.okz;
+    const float Ax = Akx - w.Sx * Akz;
+    const float Ay = Aky - w.Sy * Akz;
+    const float Bx = Bkx - w.Sx * Bkz;
+    const float By = Bky - w.Sy * Bkz;
+    const float Cx = Ckx - w.Sx * Ckz;
+    const float Cy = Cky - w.Sy * Ckz;
+    float U = Cx * By - Cy * Bx;
+    float V = Ax * Cy - Ay * Cx;
+    float W = Bx * Ay - By * Ax;
+    if(U == 0.0f || V == 0.0f || W == 0.0f) {
+        U = (float)((double)Cx * (double)By - (double)Cy * (double)Bx);
+        V = (float)((double)Ax * (double)Cy - (double)Ay * (double)Cx);
+        W = (float)((double)Bx * (double)Ay - (double)By * (double)Ax);
+    }
+    if((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
+    const float det = U + V + W;
+    if(det == 0.0f) return false;
+    const float Az = w.Sz * Akz;
+    const float Bz = w.Sz * Bkz;
+    const float Cz = w.Sz * Ckz;
+    const float T = U * Az + V * Bz + W * Cz;
+    const float rcp = 1.0f / det;
+    const float tt = T * rcp;
+    if(!(tt >= tmin && tt < tbest)) return false;
+    t = tt;
+    u = V * rcp;
+    v = W * rcp;
+    return true;
+}
+
+template<int kLdsStack, int kBlock, int kSpillWords>
+struct TraversalStack {
+    int32_t* lds;   // base of this block's [kLdsStack][kBlock] array
+    int32_t* spill; // this thread's global overflow region (kSpillWords)
+    int tid;
+    int sp = 0;
+    // A two-level LBVH is at most 2 x 63 levels deep plus the sentinel, so
+    // kLdsStack + kSpillWords >= 128 never drops an entry; the bound check
+    // only keeps every access inside the allocation.
+    __device__ __forceinline__ void push(int32_t v)
+    {
+        if(sp < kLdsStack) lds[sp * kBlock + tid] = v;
+        else if(sp < kLdsStack + kSpillWords) spill[sp - kLdsStack] = v;
+        ++sp;
+    }
+    __device__ __forceinline__ int32_t pop()
+    {
+        --sp;
+        if(sp < kLdsStack) return lds[sp * kBlock + tid];
+        if(sp < kLdsStack + kSpillWords) return spill[sp - kLdsStack];
+        return kSentinel;
+    }
+};
+
+// Object-space mapping of a world ray by a 3x4 row-major matrix, summed
+// left to right without fused operations.
+__device__ __forceinline__ void xform_ray(const float* w, float ox, float oy, float oz, float dx, float dy, float dz, float& rox,
+                                          float& roy, float& roz, float& rdx, float& rdy, float& rdz)
+{
+    rox = ((w[0] * ox + w[1] * oy) + w[2] * oz) + w[3];
+    roy = ((w[4] * ox + w[5] * oy) + w[6] * oz) + w[7];
+    roz = ((w[8] * ox + w[9] * oy) + w[10] * oz) + w[11];
+    rdx = (w[0] * dx + w[1] * dy) + w[2] * dz;
+    rdy = (w[4] * dx + w[5] * dy) + w[6] * dz;
+    rdz = (w[8] * dx + w[9] * dy) + w[10] * dz;
+}
+
+template<typename Stack>
+__device__ HitRec traverse(const TraceScene& sc, const RayIn& ray, uint32_t cullMask, Stack& stack)
+{
+    HitRec h;
+    h.t = 0.0f;
+    h.u = 0.0f;
+    h.v = 0.0f;
+    h.inst = kNoHit;
+    h.prim = 0;
+    if(sc.numInstances == 0) return h;
+
+    BoxRay br;
+    box_ray_setup(br, ray.ox, ray.oy, ray.oz, ray.dx, ray.dy, ray.dz);
+    WtRay wt;
+    const Node* nodes = sc.tlasNodes;
+    const Tri* tris = nullptr;
+    bool inBlas = false;
+    uint32_t curInst = 0;
+    float tbest = ray.tmax;
+    const float tmin = ray.tmin;
+    int32_t ref = 0;
+    stack.sp = 0;
+
+    for(;;) {
+        if(ref >= 0) {
+            const float4* np = reinterpret_cast<const float4*>(nodes + ref);
+            const float4 a = np[0], b = np[1], c = np[2];
+            const int4 d = reinterpret_cast<const int4*>(np)[3];
+            bool h0, h1;
+            float tn0, tn1;
+            box_test2(br, a, b, c, tmin, tbest, h0, h1, tn0, tn1);
+            h0 = h0 && d.x != kEmptyChild;
+            h1 = h1 && d.y != kEmptyChild;
+            if(h0 && h1) {
+                const bool near1 = tn1 < tn0;
+                stack.push(near1 ? d.x : d.y);
+                ref = near1 ? d.y : d.x;
+            }
+            else if(h0) {
+                ref = d.x;
+            }
+            else if(h1) {
+                ref = d.y;
+            }
+            else {
+                if(stack.sp == 0) break;
+                ref = stack.pop();
+            }
+            continue;
+        }
+        if(ref == kSentinel) {
+            inBlas = false;
+            nodes = sc.tlasNodes;
+            box_ray_setup(br, ray.ox, ray.oy, ray.oz, ray.dx, ray.dy, ray.dz);
+            if(stack.sp == 0) break;
+            ref = stack.pop();
+            continue;
+        }
+        const int32_t leaf = ~ref;
+        if(!inBlas) {
+            const DevInstance& di = sc.instances[leaf];
+            const DevBlas bl = sc.blas[di.blas];
+            if((di.mask & cullMask) != 0u && bl.numFaces > 0u) {
+                float w[12];
+                for(int q = 0; q < 12; ++q) w[q] = di.w2o[q];
+                float rox, roy, roz, rdx, rdy, rdz;
+                xform_ray(w, ray.ox, ray.oy, ray.oz, ray.dx, ray.dy, ray.dz, rox, roy, roz, rdx, rdy, rdz);
+                box_ray_setup(br, rox, roy, roz, rdx, rdy, rdz);
+                wt_setup(wt, rox, roy, roz, rdx, rdy, rdz);
+                stack.push(kSentinel);
+                nodes = bl.nodes;
+                tris = bl.tris;
+                inBlas = true;
+                curInst = di.position;
+                ref = 0;
+                continue;
+            }
+        }
+        else {
+            const Tri tri = tris[leaf];
+            float t, u, v;
+            if(wt_intersect(wt, tri, tmin, tbest, t, u, v)) {
+                tbest = t;
+                h.t = t;
+                h.u = u;
+                h.v = v;
+                h.inst = curInst;
+                h.prim = tri.prim;
+            }
+        }
+        if(stack.sp == 0) break;
+        ref = stack.pop();
+    }
+    return h;
+}
+
+} // namespace rgrt

@@ -1,0 +1,711 @@
+// wavefront.hip — raygen and the closest-hit / miss stages as a wavefront.
+//
+// The reference traces one recursive ray tree per pixel inside
+// traceRaysKHR: raygen (resources/shaders/raygen.h:69-115) calls
+// traceRayEXT, the closest-hit shader (closesthit.rchit:93-268) calls it
+// again for shadow, reflection and refraction rays, and every call shares
+// one payload (payload.h:29-39) by reference. Here each pixel keeps that
+// payload, its sample accumulators and a stack of suspended closest-hit
+// frames in HBM. A pass of the shading kernel takes one traced ray per
+// pixel, runs the closest-hit or miss stage for it, resumes suspended
+// frames until one of them issues the next ray (or the pixel's samples are
+// done) and appends that ray to the next queue. Appends are compacted per
+// wave with a ballot and one atomic. The traversal pass in between is
+// trace.hip.
+//
+// Frame stages (where a suspended closest-hit continues):
+//   G_SHADOW   after the direct shadow ray          closesthit.rchit:193-197
+//   G_REFL     after the reflection ray             :215-220
+//   G_REFR     after the refraction ray             :232-236 / :244-249
+//   SI_TRACE   shadow-internal, after SHADOW_TRACE  :133-138
+//   SI_SKY     shadow-internal, after the sky ray   :144-145
+//   ST_INTERNAL shadow-trace, after SHADOW_INTERNAL :158-159
+// A ray with cull mask 0 (:144) cannot hit and a refraction ray with a zero
+// direction (refract() at total internal reflection) is defined here not to
+// hit either; both go straight to their miss shader without a queue entry.
+#pragma clang fp contract(off)
+
+#include "rgrt_internal.h"
+
+namespace rgrt {
+
+namespace {
+
+constexpr int RT_GENERIC = 0, RT_SHADOW_TRACE = 1, RT_SHADOW_INTERNAL = 2;
+constexpr int kShadeBlock = 256;
+
+enum Stage : int { G_SHADOW = 1, G_REFL = 2, G_REFR = 3, SI_TRACE = 4, SI_SKY = 5, ST_INTERNAL = 6 };
+enum Step : int { RET = 0, EMIT_FRAME = 1, EMIT = 2, DONE = 3 };
+
+__constant__ float kAAOffsets[9][8][2] = { // raygen.h:37-67
+    {{0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}},
+    {{0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}},
+    {{0.25f, 0.25f}, {-0.25f, -0.25f}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}},
+    {{-0.125f, -0.375f}, {0.375f, -0.125f}, {-0.375f, 0.125f}, {0.125f, 0.375f}, {0, 0}, {0, 0}, {0, 0}, {0, 0}},
+    {{-0.125f, -0.375f}, {0.375f, -0.125f}, {-0.375f, 0.125f}, {0.125f, 0.375f}, {0, 0}, {0, 0}, {0, 0}, {0, 0}},
+    {{0.0625f, -0.1875f}, {-0.0625f, 0.1875f}, {0.3125f, 0.0625f}, {-0.1875f, -0.3125f}, {-0.3125f, 0.3125f},
+     {-0.4375f, -0.0625f}, {0.1875f, 0.4375f}, {0.4375f, -0.4375f}},
+    {{0.0625f, -0.1875f}, {-0.0625f, 0.1875f}, {0.3125f, 0.0625f}, {-0.1875f, -0.3125f}, {-0.3125f, 0.3125f},
+     {-0.4375f, -0.0625f}, {0.1875f, 0.4375f}, {0.4375f, -0.4375f}},
+    {{0.0625f, -0.1875f}, {-0.0625f, 0.1875f}, {0.3125f, 0.0625f}, {-0.1875f, -0.3125f}, {-0.3125f, 0.3125f},
+     {-0.4375f, -0.0625f}, {0.1875f, 0.4375f}, {0.4375f, -0.4375f}},
+    {{0.0625f, -0.1875f}, {-0.0625f, 0.1875f}, {0.3125f, 0.0625f}, {-0.1875f, -0.3125f}, {-0.3125f, 0.3125f},
+     {-0.4375f, -0.0625f}, {0.1875f, 0.4375f}, {0.4375f, -0.4375f}},
+};
+
+struct Payload {
+    F3 hitValue;
+    float reflectContribution;
+    F3 normal;
+    float rv[4];
+    float depth, curIOR, refDepth;
+    int rayType, recDepth;
+};
+
+struct Frame {
+    int stage, frontFacing;
+    F3 origin, dir, vn;
+    float t;
+    F3 diffuse;
+    float transparency;
+    F3 specular;
+    float reflectivity, roughness, ior;
+    int rayConsumption;
+    float emission;
+    F3 baseColor, shadowColor, reflectColor;
+    float reflectDepth;
+};
+static_assert(sizeof(Frame) == kFrameWords * 4, "frame words");
+
+struct OutRay {
+    F3 o;
+    float tmin;
+    F3 d;
+    float tmax;
+    uint32_t mask;
+    uint32_t miss1;
+};
+
+struct Ctx {
+    const ShadeParams& p;
+    F3 L;
+    int maxRec;
+};
+
+// ---------------------------------------------------------------------------
+// State in HBM
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void load_payload(const ShadeParams& p, uint32_t pix, Payload& P)
+{
+    const float* b = p.st.payload + pix;
+    const size_t n = p.st.npix;
+    P.hitValue = f3(b[0 * n], b[1 * n], b[2 * n]);
+    P.reflectContribution = b[3 * n];
+    P.normal = f3(b[4 * n], b[5 * n], b[6 * n]);
+    P.rv[0] = b[7 * n];
+    P.rv[1] = b[8 * n];
+    P.rv[2] = b[9 * n];
+    P.rv[3] = b[10 * n];
+    P.depth = b[11 * n];
+    P.curIOR = b[12 * n];
+    P.refDepth = b[13 * n];
+    P.rayType = __float_as_int(b[14 * n]);
+    P.recDepth = __float_as_int(b[15 * n]);
+}
+
+__device__ __forceinline__ void store_payload(const ShadeParams& p, uint32_t pix, const Payload& P)
+{
+    float* b = p.st.payload + pix;
+    const size_t n = p.st.npix;
+    b[0 * n] = P.hitValue.x;
+    b[1 * n] = P.hitValue.y;
+    b[2 * n] = P.hitValue.z;
+    b[3 * n] = P.reflectContribution;
+    b[4 * n] = P.normal.x;
+    b[5 * n] = P.normal.y;
+    b[6 * n] = P.normal.z;
+    b[7 * n] = P.rv[0];
+    b[8 * n] = P.rv[1];
+    b[9 * n] = P.rv[2];
+    b[10 * n] = P.rv[3];
+    b[11 * n] = P.depth;
+    b[12 * n] = P.curIOR;
+    b[13 * n] = P.refDepth;
+    b[14 * n] = __int_as_float(P.rayType);
+    b[15 * n] = __int_as_float(P.recDepth);
+}
+
+__device__ __forceinline__ void store_frame(const ShadeParams& p, uint32_t pix, int level, const Frame& F)
+{
+    const size_t n = p.st.npix;
+    float* b = p.st.frames + (size_t)level * kFrameWords * n + pix;
+    const float* w = reinterpret_cast<const float*>(&F);
+#pragma unroll
+    for(int k = 0; k < kFrameWords; ++k) b[k * n] = w[k];
+}
+
+__device__ __forceinline__ void load_frame(const ShadeParams& p, uint32_t pix, int level, Frame& F)
+{
+    const size_t n = p.st.npix;
+    const float* b = p.st.frames + (size_t)level * kFrameWords * n + pix;
+    float* w = reinterpret_cast<float*>(&F);
+#pragma unroll
+    for(int k = 0; k < kFrameWords; ++k) w[k] = b[k * n];
+}
+
+__device__ __forceinline__ void init_payload(Payload& P) // raygen.h:92-100
+{
+    P.hitValue = f3(0.0f);
+    P.normal = f3(0.0f);
+    P.rv[0] = P.rv[1] = P.rv[2] = P.rv[3] = 0.0f;
+    P.depth = 0.0f;
+    P.refDepth = 0.0f;
+    P.curIOR = 1.0f;
+    P.rayType = RT_GENERIC;
+    P.recDepth = 0;
+    P.reflectContribution = 0.0f;
+}
+
+// raygen.h:78-90: origin = viewInverse * (0,0,0,1); direction =
+// viewInverse * (normalize((projInverse * (d, 1, 1)).xyz), 0).
+__device__ __forceinline__ void primary_ray(const rgrt_ubo& u, uint32_t px, uint32_t py, uint32_t W, uint32_t H, int sample,
+                                            OutRay& r)
+{
+    const float* vi = u.viewInverse;
+    const float* pi = u.projInverse;
+    const int n = u.numSamples;
+    const int row = n < 8 ? n : 8;
+    const float offx = kAAOffsets[row][sample & 7][0];
+    const float offy = kAAOffsets[row][sample & 7][1];
+    const float pcx = ((float)px + 0.5f) + offx;
+    const float pcy = ((float)py + 0.5f) + offy;
+    const float ux = pcx / (float)W, uy = pcy / (float)H;
+    const float dx = ux * 2.0f - 1.0f, dy = uy * 2.0f - 1.0f;
+    float tg[3];
+    for(int k = 0; k < 3; ++k) tg[k] = ((pi[k] * dx + pi[4 + k] * dy) + pi[8 + k] * 1.0f) + pi[12 + k] * 1.0f;
+    const F3 dn = normalize(f3(tg[0], tg[1], tg[2]));
+    r.d = f3(((vi[0] * dn.x + vi[4] * dn.y) + vi[8] * dn.z) + vi[12] * 0.0f,
+             ((vi[1] * dn.x + vi[5] * dn.y) + vi[9] * dn.z) + vi[13] * 0.0f,
+             ((vi[2] * dn.x + vi[6] * dn.y) + vi[10] * dn.z) + vi[14] * 0.0f);
+    r.o = f3(((vi[0] * 0.0f + vi[4] * 0.0f) + vi[8] * 0.0f) + vi[12] * 1.0f,
+             ((vi[1] * 0.0f + vi[5] * 0.0f) + vi[9] * 0.0f) + vi[13] * 1.0f,
+             ((vi[2] * 0.0f + vi[6] * 0.0f) + vi[10] * 0.0f) + vi[14] * 1.0f);
+    r.tmin = 0.001f;
+    r.tmax = 10000.0f;
+    r.mask = 0xffu;
+    r.miss1 = 0u;
+}
+
+// ---------------------------------------------------------------------------
+// Miss stages
+// ---------------------------------------------------------------------------
+__device__ F3 sky(F3 worldDir, F3 L) // miss.rmiss:38-74 with the constants of :78
+{
+    const F3 sunTone = f3(1.0f, 0.6f, 0.05f), skyTone = f3(0.2f, 0.4f, 0.8f), scatterTone = f3(1.0f, 0.3f, 0.0f);
+    const float scatterFactor = 1.3f, powFactor = 80.0f;
+    const bool zero = worldDir.x == 0.0f && worldDir.y == 0.0f && worldDir.z == 0.0f;
+    const F3 rayDir = zero ? f3(0.0f) : normalize(worldDir);
+    const float y = fabsf(worldDir.y + 1.5f) / 3.0f;
+    float sun = 1.0f - distance(rayDir, normalize(-L));
+    sun = clampf(sun, 0.0f, 2.0f);
+    float glow = sun;
+    glow = clampf(glow, 0.0f, 1.0f);
+    sun = powf(sun, powFactor);
+    sun *= 1000.0f;
+    sun = clampf(sun, 0.0f, 16.0f);
+    glow = powf(glow, 6.0f) * 1.0f;
+    glow = powf(glow, y);
+    glow = clampf(glow, 0.0f, 1.0f);
+    sun *= powf(y * y, (float)(1.0 / 1.65));
+    glow *= powf(y * y, (float)(1.0 / 2.0));
+    sun += glow;
+    const F3 sunColor = sunTone * sun;
+    const float atmosphere = sqrtf(1.0f - y);
+    float scatter = powf(4.0f - L.y, (float)(1.0 / 15.0));
+    scatter = 1.0f - clampf(scatter, 0.8f, 1.0f);
+    const F3 scatterColor = mix(f3(1.0f), scatterTone * 1.5f, scatter);
+    const F3 skyScatter = mix(skyTone, scatterColor, atmosphere / scatterFactor);
+    return sunColor + skyScatter;
+}
+
+__device__ __forceinline__ void run_miss(const Ctx& c, Payload& P, F3 dir, uint32_t miss1)
+{
+    if(miss1) { // shadowMiss.rmiss:30-34
+        P.hitValue = f3(1.0f);
+        return;
+    }
+    const F3 res = sky(dir, c.L); // miss.rmiss:76-83
+    P.hitValue = res;
+    P.rv[0] = res.x;
+    P.rv[1] = res.y;
+    P.rv[2] = res.z;
+    P.rv[3] = 0.0f;
+    P.depth = 10000.0f;
+}
+
+// ---------------------------------------------------------------------------
+// Closest-hit stage
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void set_ray(OutRay& r, F3 o, float tmin, F3 d, float tmax, uint32_t mask, uint32_t miss1)
+{
+    r.o = o;
+    r.tmin = tmin;
+    r.d = d;
+    r.tmax = tmax;
+    r.mask = mask;
+    r.miss1 = miss1;
+}
+
+// Final colour of a generic hit (closesthit.rchit:253-267). The 0/0 weight
+// of a material that neither reflects nor refracts is taken as 0.
+__device__ void generic_finish(const Frame& F, Payload& P, F3 refractColor)
+{
+    F3 baseColor = F.baseColor * F.shadowColor;
+    baseColor = baseColor + F.emission * F.diffuse;
+    const float totalContrib = fmaxf(F.transparency, F.reflectivity);
+    const float wsum = F.transparency + F.reflectivity;
+    const float w = wsum == 0.0f ? 0.0f : F.reflectivity / wsum;
+    const F3 roughCol = mix(refractColor, F.reflectColor, w);
+    P.hitValue = mix(baseColor, roughCol, totalContrib);
+    if(P.recDepth == 0) {
+        P.hitValue = baseColor;
+        P.normal = F.vn;
+        P.rv[0] = roughCol.x;
+        P.rv[1] = roughCol.y;
+        P.rv[2] = roughCol.z;
+        P.rv[3] = fminf((F.reflectDepth / 50.0f) * F.roughness, F.roughness / 2.1f);
+        P.reflectContribution = totalContrib;
+    }
+    P.depth = F.t;
+}
+
+// Refraction part (closesthit.rchit:223-251).
+__device__ int generic_refraction(const Ctx& c, Frame& F, Payload& P, OutRay& out)
+{
+    if(P.recDepth < c.maxRec && F.transparency > 0.0f) {
+        F3 dir;
+        if(F.frontFacing) {
+            const float eta = P.curIOR / F.ior;
+            dir = refract(F.dir, F.vn, eta);
+            P.recDepth++;
+            P.curIOR = F.ior;
+        }
+        else {
+            const float eta = F.ior / 1.0f;
+            dir = refract(F.dir, F.vn, eta);
+            P.recDepth++;
+            P.curIOR = 1.0f;
+        }
+        set_ray(out, F.origin, 0.01f, dir, 1000.0f, 0xffu, 0u);
+        F.stage = G_REFR;
+        return EMIT_FRAME;
+    }
+    generic_finish(F, P, f3(1.0f));
+    return RET;
+}
+
+// Reflection part (closesthit.rchit:206-221).
+__device__ int generic_reflection(const Ctx& c, Frame& F, Payload& P, OutRay& out)
+{
+    F.reflectColor = f3(1.0f);
+    F.reflectDepth = 0.0f;
+    if(P.recDepth < c.maxRec && F.reflectivity > 0.0f) {
+        const F3 dir = reflect(F.dir, F.vn);
+        P.recDepth += F.rayConsumption;
+        P.refDepth += F.t;
+        set_ray(out, F.origin, 0.01f, dir, 1000.0f, 0xffu, 0u);
+        F.stage = G_REFL;
+        return EMIT_FRAME;
+    }
+    return generic_refraction(c, F, P, out);
+}
+
+__device__ void grid_effect(float refDepth, float hitT, Frame& F, F3 pos) // closesthit.rchit:74-91
+{
+    const float aa = ((refDepth + hitT) + 8.0f) / 30.0f;
+    const float aa2 = aa / 2.0f;
+    float minmod = fminf(fabsf(glsl_mod((pos.x + 1000.0f) * 10.0f + aa2, 20.0f) - aa2),
+                         fabsf(glsl_mod((pos.z + 1000.0f) * 10.0f + aa2, 20.0f) - aa2));
+    if(minmod < aa2) {
+        minmod -= aa2 - powf(aa, 2.0f) / 3.0f;
+        minmod *= 3.0f / powf(aa, 2.0f);
+        const float m = mixf(aa / 10.0f, 1.0f, minmod);
+        F.diffuse = F.diffuse * m;
+        F.specular = F.specular * m;
+        F.reflectivity *= m;
+    }
+    if(glsl_mod((pos.x + 1000.0f) * 5.0f, 20.0f) < 10.0f && glsl_mod((pos.z + 1000.0f) * 5.0f, 20.0f) < 10.0f) {
+        F.reflectivity *= 1.5f;
+    }
+}
+
+// closesthit.rchit:93-204 up to the first nested trace.
+__device__ int closest_hit(const Ctx& c, Payload& P, uint32_t instPos, uint32_t prim, float hitT, float u, float v, F3 O, F3 D,
+                           Frame& F, OutRay& out)
+{
+    const ShadeParams& p = c.p;
+    // Gathers are clamped to the buffers: an inconsistent offset table reads
+    // wrong data (as the reference would) but never outside an allocation.
+    const InstanceShade& is = p.inst[min(instPos, p.numInstances - 1)];
+    const rgrt_instance_offsets table = p.inst[min(is.customIndex, p.numInstances - 1)].offsets; // offset table[customIndex]
+
+    const float b0 = (1.0f - u) - v, b1 = u, b2 = v;
+    const uint64_t lastIndex = p.numIndices ? p.numIndices - 1 : 0;
+    const uint64_t lastVertex = p.numVertices ? p.numVertices - 1 : 0;
+    const uint64_t ib0 = (uint64_t)table.indexBufferOffset + 3ull * prim;
+    const uint32_t i0 = p.ib[min(ib0 + 0, lastIndex)];
+    const uint32_t i1 = p.ib[min(ib0 + 1, lastIndex)];
+    const uint32_t i2 = p.ib[min(ib0 + 2, lastIndex)];
+    const rgrt_vertex& v0 = p.vb[min((uint64_t)table.vertexBufferOffset + i0, lastVertex)];
+    const rgrt_vertex& v1 = p.vb[min((uint64_t)table.vertexBufferOffset + i1, lastVertex)];
+    const rgrt_vertex& v2 = p.vb[min((uint64_t)table.vertexBufferOffset + i2, lastVertex)];
+    const uint32_t mi = table.materialBufferOffset + v0.matIndex;
+    const rgrt_material& m = p.mats[p.numMaterials ? min(mi, p.numMaterials - 1) : 0u];
+
+    F.diffuse = f3(m.diffuse[0], m.diffuse[1], m.diffuse[2]);
+    F.transparency = m.transparency;
+    F.specular = f3(m.specular[0], m.specular[1], m.specular[2]);
+    F.reflectivity = m.reflectivity;
+    F.roughness = m.roughness;
+    F.ior = m.ior;
+    F.rayConsumption = (int)m.rayConsumption;
+    F.emission = m.emission;
+
+    const float tmin = 0.01f, tmax = 1000.0f;
+    const F3 origin = O + D * hitT;
+    const F3 vn = (f3(v0.normal[0], v0.normal[1], v0.normal[2]) * b0 + f3(v1.normal[0], v1.normal[1], v1.normal[2]) * b1) +
+                  f3(v2.normal[0], v2.normal[1], v2.normal[2]) * b2;
+    const float* w = is.o2w;
+    const F3 wn = f3((w[0] * vn.x + w[1] * vn.y) + w[2] * vn.z, (w[4] * vn.x + w[5] * vn.y) + w[6] * vn.z,
+                     (w[8] * vn.x + w[9] * vn.y) + w[10] * vn.z);
+    F3 vnW = normalize(wn);
+
+    if(m.effectId == 1u) grid_effect(P.refDepth, hitT, F, origin);
+
+    F.origin = origin;
+    F.dir = D;
+    F.t = hitT;
+
+    if(P.rayType == RT_SHADOW_INTERNAL) { // :125-152
+        const float thicknessModulation = clampf((hitT * (1.0f - F.transparency)) * 10.0f, 0.0f, 1.0f);
+        const F3 shadowCol =
+            P.hitValue - mix(f3(0.0f), normalize(f3(1.1f) - F.diffuse) + f3(0.1f), thicknessModulation);
+        if(P.recDepth < c.maxRec) {
+            P.rayType = RT_SHADOW_TRACE;
+            P.recDepth++;
+            set_ray(out, origin, tmin, D, tmax, 0xffu, 0u);
+            F.shadowColor = shadowCol;
+            F.vn = vnW;
+            F.stage = SI_TRACE;
+            return EMIT_FRAME;
+        }
+        P.hitValue = shadowCol * f3(0.4f);
+        return RET;
+    }
+    if(P.rayType == RT_SHADOW_TRACE) { // :153-166
+        if(F.transparency > 0.0f) {
+            if(P.recDepth < c.maxRec) {
+                P.rayType = RT_SHADOW_INTERNAL;
+                P.recDepth++;
+                set_ray(out, origin, tmin, D, tmax, 0xffu, 1u);
+                F.stage = ST_INTERNAL;
+                return EMIT_FRAME;
+            }
+        }
+        else {
+            P.hitValue = P.hitValue * mix(f3(0.4f), f3(0.8f), clampf(logf(hitT) / 8.0f, 0.0f, 1.0f));
+        }
+        return RET;
+    }
+
+    const bool frontFacing = dot(-D, vnW) > 0.0f; // :170-171
+    if(!frontFacing) vnW = normalize(-vnW);
+    F.frontFacing = frontFacing ? 1 : 0;
+    F.vn = vnW;
+
+    const float dp = fmaxf(dot(-c.L, vnW), 0.2f); // :174-175
+    F.baseColor = dp * F.diffuse;
+
+    F.shadowColor = f3(1.0f); // :187-204
+    if(dot(-c.L, vnW) > 0.07f) {
+        if(P.recDepth < c.maxRec) {
+            P.hitValue = f3(1.0f);
+            P.rayType = RT_SHADOW_TRACE;
+            P.recDepth++;
+            set_ray(out, origin, tmin * 10.0f, -c.L, tmax, 0xffu, 1u);
+            F.stage = G_SHADOW;
+            return EMIT_FRAME;
+        }
+    }
+    else {
+        const float shadowModulation = powf(F.transparency, 2.0f);
+        F.shadowColor = F.transparency < 1.0f ? mix(f3(1.0f), F.diffuse * shadowModulation, F.transparency) : f3(0.4f);
+    }
+    return generic_reflection(c, F, P, out);
+}
+
+// Continuation of a suspended frame once its nested trace returned.
+__device__ int resume(const Ctx& c, Frame& F, Payload& P, OutRay& out)
+{
+    switch(F.stage) {
+    case G_SHADOW:
+        P.recDepth--;
+        F.shadowColor = P.hitValue;
+        P.rayType = RT_GENERIC;
+        return generic_reflection(c, F, P, out);
+    case G_REFL:
+        P.recDepth -= F.rayConsumption;
+        F.reflectColor = P.hitValue * F.specular;
+        F.reflectDepth = P.depth;
+        return generic_refraction(c, F, P, out);
+    case G_REFR: {
+        P.recDepth--;
+        F3 refractColor;
+        if(F.frontFacing) {
+            refractColor = P.hitValue;
+        }
+        else {
+            const F3 transmittanceModulation = mix(f3(1.0f), F.diffuse, logf(1.0f + F.t));
+            refractColor = transmittanceModulation * P.hitValue;
+        }
+        generic_finish(F, P, refractColor);
+        return RET;
+    }
+    case SI_TRACE:
+        P.recDepth--;
+        if(P.depth < 1000.0f) {
+            P.hitValue = P.hitValue * F.shadowColor;
+            return RET;
+        }
+        else {
+            const float eta = F.ior / 1.0f;
+            const F3 dir = refract(F.dir, F.vn, eta);
+            const float dp = powf(dot(c.L, dir), 5.0f) + 0.75f;
+            F.shadowColor = F.shadowColor * dp;
+            set_ray(out, F.origin, 0.01f, -dir, 1000.0f, 0x0u, 0u);
+            F.stage = SI_SKY;
+            return EMIT_FRAME;
+        }
+    case SI_SKY:
+        P.hitValue = F.shadowColor + 0.1f * P.hitValue;
+        return RET;
+    case ST_INTERNAL:
+    default:
+        P.recDepth--;
+        return RET;
+    }
+}
+
+__device__ __forceinline__ uint16_t to_half_bits(float x)
+{
+    const _Float16 h = (_Float16)x;
+    return __builtin_bit_cast(uint16_t, h);
+}
+
+__device__ __forceinline__ void write_rgba16(uint16_t* img, uint32_t pix, float a, float b, float c, float d)
+{
+    ushort4 v;
+    v.x = to_half_bits(a);
+    v.y = to_half_bits(b);
+    v.z = to_half_bits(c);
+    v.w = to_half_bits(d);
+    reinterpret_cast<ushort4*>(img)[pix] = v;
+}
+
+// raygen.h:105-114 accumulation and raygen.rgen:35-38 stores.
+__device__ void accumulate(const ShadeParams& p, uint32_t pix, const Payload& P)
+{
+    float* a = p.st.accum + pix;
+    const size_t n = p.st.npix;
+    a[0 * n] = a[0 * n] + P.hitValue.x;
+    a[1 * n] = a[1 * n] + P.hitValue.y;
+    a[2 * n] = a[2 * n] + P.hitValue.z;
+    a[3 * n] = a[3 * n] + P.normal.x;
+    a[4 * n] = a[4 * n] + P.normal.y;
+    a[5 * n] = a[5 * n] + P.normal.z;
+    a[6 * n] = a[6 * n] + P.rv[0];
+    a[7 * n] = a[7 * n] + P.rv[1];
+    a[8 * n] = a[8 * n] + P.rv[2];
+    a[9 * n] = a[9 * n] + P.rv[3];
+    a[10 * n] = a[10 * n] + P.reflectContribution;
+    a[11 * n] = a[11 * n] + P.depth;
+}
+
+__device__ void finalize(const ShadeParams& p, uint32_t pix)
+{
+    const float* a = p.st.accum + pix;
+    const size_t n = p.st.npix;
+    const float ns = (float)p.ubo.numSamples;
+    const float r0[4] = {a[0 * n] / ns, a[1 * n] / ns, a[2 * n] / ns, a[10 * n] / ns};
+    const float r1[4] = {a[3 * n] / ns, a[4 * n] / ns, a[5 * n] / ns, (logf(a[11 * n]) * 0.25f) / ns};
+    const float r2[4] = {a[6 * n] / ns, a[7 * n] / ns, a[8 * n] / ns, a[9 * n] / ns};
+    write_rgba16(p.img.base, pix, r0[0], r0[1], r0[2], r0[3]);
+    write_rgba16(p.img.normal, pix, r1[0], r1[1], r1[2], r1[3]);
+    write_rgba16(p.img.rough, pix, r2[0], r2[1], r2[2], r2[3]);
+    if(p.keepF32) {
+        reinterpret_cast<float4*>(p.img.base32)[pix] = make_float4(r0[0], r0[1], r0[2], r0[3]);
+        reinterpret_cast<float4*>(p.img.normal32)[pix] = make_float4(r1[0], r1[1], r1[2], r1[3]);
+        reinterpret_cast<float4*>(p.img.rough32)[pix] = make_float4(r2[0], r2[1], r2[2], r2[3]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Kernels
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void write_ray(float* rays, size_t n, uint32_t slot, const OutRay& r, uint32_t pix, uint32_t flags)
+{
+    rays[0 * n + slot] = r.o.x;
+    rays[1 * n + slot] = r.o.y;
+    rays[2 * n + slot] = r.o.z;
+    rays[3 * n + slot] = r.tmin;
+    rays[4 * n + slot] = r.d.x;
+    rays[5 * n + slot] = r.d.y;
+    rays[6 * n + slot] = r.d.z;
+    rays[7 * n + slot] = r.tmax;
+    rays[8 * n + slot] = __uint_as_float(pix);
+    rays[9 * n + slot] = __uint_as_float(flags);
+}
+
+__global__ __launch_bounds__(kShadeBlock) void k_raygen(ShadeParams p)
+{
+    const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
+    if(pix == 0) {
+        uint32_t* c = p.st.counters;
+        c[0] = p.st.npix; // queue 0 holds every pixel's primary ray
+        c[1] = 0;
+        c[2] = 0;
+        c[3] = 0;
+        for(int k = 4; k < 12; ++k) c[k] = 0; // ray / sky / zero-dir counters (u64)
+    }
+    if(pix >= p.st.npix) return;
+    const uint32_t px = pix % p.width, py = pix / p.width;
+    OutRay r;
+    primary_ray(p.ubo, px, py, p.width, p.height, 0, r);
+    Payload P;
+    init_payload(P);
+    store_payload(p, pix, P);
+    float* a = p.st.accum + pix;
+    for(int k = 0; k < kAccumWords; ++k) a[(size_t)k * p.st.npix] = 0.0f;
+    p.st.meta[pix] = 0u;
+    write_ray(p.st.rays[0], p.st.npix, pix, r, pix, kFlagPrimary | (0xffu << kMaskShift));
+}
+
+__global__ __launch_bounds__(kShadeBlock) void k_shade(ShadeParams p, int cur)
+{
+    const uint32_t count = p.st.counters[cur];
+    const size_t n = p.st.npix;
+    const float* __restrict__ rays = p.st.rays[cur];
+    float* __restrict__ outRays = p.st.rays[1 - cur];
+    uint32_t* outCount = p.st.counters + (1 - cur);
+    unsigned long long* skyCount = reinterpret_cast<unsigned long long*>(p.st.counters + 6);
+    unsigned long long* zeroCount = reinterpret_cast<unsigned long long*>(p.st.counters + 8);
+    const int lane = threadIdx.x & 63;
+    const uint64_t ltMask = lane == 0 ? 0ull : ((~0ull) >> (64 - lane));
+    const Ctx c{p, f3(p.ubo.lightDir[0], p.ubo.lightDir[1], p.ubo.lightDir[2]), p.ubo.maxRecursions};
+    const uint32_t stride = gridDim.x * blockDim.x;
+
+    for(uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < count; base += stride) {
+        const uint32_t i = base + lane;
+        bool emit = false;
+        OutRay out;
+        uint32_t pix = 0;
+        if(i < count) {
+            pix = __float_as_uint(rays[8 * n + i]);
+            const uint32_t flags = __float_as_uint(rays[9 * n + i]);
+            const F3 O = f3(rays[0 * n + i], rays[1 * n + i], rays[2 * n + i]);
+            const F3 D = f3(rays[4 * n + i], rays[5 * n + i], rays[6 * n + i]);
+            const float ht = p.st.hits[0 * n + i];
+            const float hu = p.st.hits[1 * n + i];
+            const float hv = p.st.hits[2 * n + i];
+            const uint32_t hinst = __float_as_uint(p.st.hits[3 * n + i]);
+            const uint32_t hprim = __float_as_uint(p.st.hits[4 * n + i]);
+            if((flags & kFlagPrimary) && p.keepAov) {
+                rgrt_hit a;
+                a.t = hinst != kNoHit ? ht : 0.0f;
+                a.u = hinst != kNoHit ? hu : 0.0f;
+                a.v = hinst != kNoHit ? hv : 0.0f;
+                a.instance = hinst != kNoHit ? p.inst[min(hinst, p.numInstances - 1)].customIndex : kNoHit;
+                a.primitive = hinst != kNoHit ? hprim : 0u;
+                p.img.primary[pix] = a;
+            }
+            Payload P;
+            load_payload(p, pix, P);
+            uint32_t meta = p.st.meta[pix];
+            int sp = (int)(meta & 0xffu);
+            int sample = (int)(meta >> 8);
+            Frame F;
+            int step;
+            if(hinst != kNoHit) {
+                step = closest_hit(c, P, hinst, hprim, ht, hu, hv, O, D, F, out);
+            }
+            else {
+                run_miss(c, P, D, flags & kFlagMiss1);
+                step = RET;
+            }
+            for(;;) {
+                if(step == EMIT_FRAME) {
+                    if(sp < kMaxFrames) store_frame(p, pix, sp, F);
+                    sp++;
+                    step = EMIT;
+                }
+                if(step == EMIT) {
+                    if(out.mask == 0u) {
+                        atomicAdd(skyCount, 1ull);
+                        run_miss(c, P, out.d, out.miss1);
+                        step = RET;
+                    }
+                    else if(out.d.x == 0.0f && out.d.y == 0.0f && out.d.z == 0.0f) {
+                        atomicAdd(zeroCount, 1ull);
+                        run_miss(c, P, out.d, out.miss1);
+                        step = RET;
+                    }
+                    else {
+                        emit = true;
+                        break;
+                    }
+                }
+                // step == RET: the innermost call returned
+                if(sp == 0) {
+                    accumulate(p, pix, P);
+                    sample++;
+                    if(sample < p.ubo.numSamples) {
+                        init_payload(P);
+                        primary_ray(p.ubo, pix % p.width, pix / p.width, p.width, p.height, sample, out);
+                        step = EMIT;
+                        continue;
+                    }
+                    finalize(p, pix);
+                    break;
+                }
+                sp--;
+                if(sp < kMaxFrames) load_frame(p, pix, sp, F);
+                step = resume(c, F, P, out);
+            }
+            store_payload(p, pix, P);
+            p.st.meta[pix] = (uint32_t)sp | ((uint32_t)sample << 8);
+        }
+        // append the emitted rays to the next queue, one atomic per wave
+        const uint64_t m = __ballot(emit);
+        if(m) {
+            uint32_t wbase = 0;
+            if(lane == 0) wbase = atomicAdd(outCount, (uint32_t)__popcll(m));
+            wbase = __shfl(wbase, 0);
+            if(emit) write_ray(outRays, n, wbase + (uint32_t)__popcll(m & ltMask), out, pix, (out.mask << kMaskShift) | out.miss1);
+        }
+    }
+}
+
+} // namespace
+
+void launch_raygen(hipStream_t s, const ShadeParams& p)
+{
+    const uint32_t blocks = (p.st.npix + kShadeBlock - 1) / kShadeBlock;
+    k_raygen<<<blocks, kShadeBlock, 0, s>>>(p);
+}
+
+void launch_shade(hipStream_t s, const ShadeParams& p, int cur, uint32_t gridBlocks)
+{
+    k_shade<<<gridBlocks, kShadeBlock, 0, s>>>(p, cur);
+}
+
+} // namespace rgrt

@@ -1,0 +1,87 @@
+"""Host-side mirror of Raygun's renderer classes over the C ABI.
+
+Same method names and call order as the reference's host code:
+
+- ``RenderSystem.setupModelBuffers``  raygun/render/render_system.cpp:192-223
+- ``Raytracer.setupBottomLevelAS``     raygun/render/raytracer.cpp:54-74
+- ``Raytracer.setupTopLevelAS``        raygun/render/raytracer.cpp:76-85
+- ``RenderSystem.updateUniformBuffer`` raygun/render/render_system.cpp:246-268
+- ``Raytracer.doRaytracing``           raygun/render/raytracer.cpp:87-147
+- ``RenderSystem.render``              raygun/render/render_system.cpp:88-162
+  (minus swapchain blit, ImGui and present, which are out of scope)
+
+One BLAS is built per distinct mesh (the reference builds one per model;
+models sharing a mesh get identical trees, so this only saves work).
+"""
+import numpy as np
+
+from . import rgrt
+from .scene import camera_ubo, pack_models
+
+
+class Raytracer:
+    def __init__(self, ctx):
+        self.ctx = ctx
+        self.use_fxaa = True  # m_useFXAA (raytracer.hpp:80)
+
+    def setupBottomLevelAS(self, models):
+        built = {}
+        for model in models:
+            mesh = model.mesh
+            key = id(mesh)
+            if key not in built:
+                if getattr(mesh, "blas", None) is None:
+                    mesh.blas = self.ctx.build_blas(mesh.first_vertex, mesh.index_offset_bytes, mesh.num_faces(),
+                                                    len(mesh.vertices))
+                built[key] = mesh.blas
+            model.blas = built[key]
+
+    def setupTopLevelAS(self, scene):
+        self.ctx.build_tlas(scene.collect_instances())
+
+    def updateRenderTarget(self, ubo):
+        self.ctx.set_frame(ubo)
+
+    def doRaytracing(self):
+        self.ctx.render(self.use_fxaa)
+        return rgrt.IMAGE_FINAL
+
+
+class RenderSystem:
+    def __init__(self, width, height, flags=0, device=0):
+        self.ctx = rgrt.Context(width, height, flags, device)
+        self.raytracer = Raytracer(self.ctx)
+        self.width, self.height = width, height
+
+    def setupModelBuffers(self, scene):
+        models = scene.models()
+        for m in models:
+            m.mesh.blas = None
+        vertices, indices, materials = pack_models(models)
+        self.ctx.upload_geometry(vertices, indices)
+        self.ctx.upload_materials(materials)
+        return models
+
+    def load_scene(self, scene):
+        """Raygun::finalizeLoadScene's render part (raygun.cpp:276-295)."""
+        models = self.setupModelBuffers(scene)
+        self.raytracer.setupBottomLevelAS(models)
+
+    def updateUniformBuffer(self, camera, max_recursions=5, num_samples=1, light_dir=(0.4, -0.6, -0.8), fade=(0, 0, 0, 0)):
+        ubo = camera_ubo(camera, self.width, self.height, light_dir=light_dir, num_samples=num_samples,
+                         max_recursions=max_recursions, fade=fade)
+        self.raytracer.updateRenderTarget(ubo)
+        return ubo
+
+    def render(self, scene, ubo=None):
+        self.raytracer.setupTopLevelAS(scene)
+        if ubo is not None:
+            self.raytracer.updateRenderTarget(ubo)
+        return self.raytracer.doRaytracing()
+
+    def read(self, which=rgrt.IMAGE_FINAL):
+        return self.ctx.read_image(which)
+
+
+def half_to_float(img16):
+    return np.asarray(img16, np.uint16).view(np.float16).astype(np.float32)

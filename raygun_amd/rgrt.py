@@ -1,0 +1,251 @@
+"""ctypes binding of librgrt.so (the C ABI declared in include/rgrt.h).
+
+The product path: every call goes to the HIP library; if the library is not
+built or no GPU is present, the calls raise instead of falling back.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from .records import BVH_NODE, HIT, INSTANCE, MATERIAL, RAY, STATS_FIELDS, UBO, VERTEX
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "librgrt.so")
+
+CFG_FP32_RADIANCE = 0x1
+CFG_PRIMARY_AOV = 0x2
+CFG_TIMING = 0x4
+
+IMAGE_FINAL, IMAGE_BASE, IMAGE_NORMAL, IMAGE_ROUGH, IMAGE_ROUGH_TRANSITIONS, IMAGE_ROUGH_COLORS_A, IMAGE_ROUGH_COLORS_B, \
+    IMAGE_BASE_F32, IMAGE_NORMAL_F32, IMAGE_ROUGH_F32, IMAGE_PRIMARY_HIT = range(11)
+
+STATUS = {0: "OK", 1: "INVALID_ARG", 2: "OUT_OF_MEMORY", 3: "HIP", 4: "NOT_READY", 5: "NO_DEVICE"}
+
+# Every symbol include/rgrt.h declares.
+EXPORTS = [
+    "rgrt_create", "rgrt_destroy", "rgrt_last_error", "rgrt_stream", "rgrt_synchronize",
+    "rgrt_upload_geometry", "rgrt_upload_materials", "rgrt_update_vertices",
+    "rgrt_build_blas", "rgrt_refit_blas", "rgrt_blas_info", "rgrt_export_blas",
+    "rgrt_build_tlas", "rgrt_export_tlas", "rgrt_set_frame", "rgrt_resize",
+    "rgrt_trace_rays", "rgrt_trace_rays_device", "rgrt_render", "rgrt_trace", "rgrt_postprocess",
+    "rgrt_read_image", "rgrt_image_device_ptr", "rgrt_write_image", "rgrt_get_stats", "rgrt_version",
+]
+
+
+class RgrtError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"rgrt status {status} ({STATUS.get(status, '?')}): {msg}")
+        self.status = status
+
+
+class _Config(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int32), ("width", ctypes.c_uint32), ("height", ctypes.c_uint32), ("flags", ctypes.c_uint32)]
+
+
+class _BlasDesc(ctypes.Structure):
+    _fields_ = [("firstVertex", ctypes.c_uint32), ("indexOffsetBytes", ctypes.c_uint32), ("numFaces", ctypes.c_uint32),
+                ("maxVertex", ctypes.c_uint32)]
+
+
+STATS = np.dtype(STATS_FIELDS)
+assert STATS.itemsize == 64
+
+_lib = None
+
+
+def lib():
+    """Loads librgrt.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RgrtError(5, f"{LIB_PATH} is not built; run raygun_amd/build.py or __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        P, u32, u64, i32 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int32
+        sig = {
+            "rgrt_create": (i32, [ctypes.POINTER(P), ctypes.POINTER(_Config)]),
+            "rgrt_destroy": (None, [P]),
+            "rgrt_last_error": (ctypes.c_char_p, [P]),
+            "rgrt_stream": (P, [P]),
+            "rgrt_synchronize": (i32, [P]),
+            "rgrt_upload_geometry": (i32, [P, P, u64, P, u64]),
+            "rgrt_upload_materials": (i32, [P, P, u32]),
+            "rgrt_update_vertices": (i32, [P, u64, P, u64]),
+            "rgrt_build_blas": (i32, [P, ctypes.POINTER(_BlasDesc), ctypes.POINTER(u32)]),
+            "rgrt_refit_blas": (i32, [P, u32]),
+            "rgrt_blas_info": (i32, [P, u32, ctypes.POINTER(u32), ctypes.POINTER(u32), P, P]),
+            "rgrt_export_blas": (i32, [P, u32, P, u32, P, u32]),
+            "rgrt_build_tlas": (i32, [P, P, u32]),
+            "rgrt_export_tlas": (i32, [P, P, u32, P, u32, ctypes.POINTER(u32)]),
+            "rgrt_set_frame": (i32, [P, P]),
+            "rgrt_resize": (i32, [P, u32, u32]),
+            "rgrt_trace_rays": (i32, [P, P, u32, u32, P]),
+            "rgrt_trace_rays_device": (i32, [P, P, u32, u32, P]),
+            "rgrt_render": (i32, [P, i32]),
+            "rgrt_trace": (i32, [P]),
+            "rgrt_postprocess": (i32, [P, i32]),
+            "rgrt_read_image": (i32, [P, i32, P, u64]),
+            "rgrt_image_device_ptr": (i32, [P, i32, ctypes.POINTER(P), ctypes.POINTER(u64)]),
+            "rgrt_write_image": (i32, [P, i32, P, u64]),
+            "rgrt_get_stats": (i32, [P, P]),
+            "rgrt_version": (ctypes.c_char_p, []),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+class Context:
+    """One rgrt context: device buffers, BLASes, the TLAS and the images."""
+
+    def __init__(self, width=0, height=0, flags=0, device=0):
+        self._L = lib()
+        self.h = ctypes.c_void_p()
+        cfg = _Config(device, width, height, flags)
+        st = self._L.rgrt_create(ctypes.byref(self.h), ctypes.byref(cfg))
+        if st != 0:
+            raise RgrtError(st, "rgrt_create failed")
+        self.width, self.height, self.flags = width, height, flags
+        self._blas_faces = []
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            self._L.rgrt_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _ok(self, st):
+        if st != 0:
+            raise RgrtError(st, self._L.rgrt_last_error(self.h).decode())
+
+    # -- scene data ---------------------------------------------------------
+    def upload_geometry(self, vertices, indices):
+        self._v = np.ascontiguousarray(vertices, VERTEX)
+        self._i = np.ascontiguousarray(indices, np.uint32)
+        self._ok(self._L.rgrt_upload_geometry(self.h, _p(self._v), len(self._v), _p(self._i), len(self._i)))
+
+    def upload_materials(self, materials):
+        m = np.ascontiguousarray(materials, MATERIAL)
+        self._ok(self._L.rgrt_upload_materials(self.h, _p(m), len(m)))
+
+    def update_vertices(self, first, vertices):
+        v = np.ascontiguousarray(vertices, VERTEX)
+        self._ok(self._L.rgrt_update_vertices(self.h, first, _p(v), len(v)))
+
+    def build_blas(self, first_vertex, index_offset_bytes, num_faces, max_vertex):
+        d = _BlasDesc(first_vertex, index_offset_bytes, num_faces, max_vertex)
+        out = ctypes.c_uint32()
+        self._ok(self._L.rgrt_build_blas(self.h, ctypes.byref(d), ctypes.byref(out)))
+        while len(self._blas_faces) <= out.value:
+            self._blas_faces.append(0)
+        self._blas_faces[out.value] = num_faces
+        return out.value
+
+    def refit_blas(self, b):
+        self._ok(self._L.rgrt_refit_blas(self.h, b))
+
+    def blas_info(self, b):
+        nn, nl = ctypes.c_uint32(), ctypes.c_uint32()
+        lo, hi = np.zeros(3, np.float32), np.zeros(3, np.float32)
+        self._ok(self._L.rgrt_blas_info(self.h, b, ctypes.byref(nn), ctypes.byref(nl), _p(lo), _p(hi)))
+        return nn.value, nl.value, lo, hi
+
+    def export_blas(self, b):
+        nn, nl, _, _ = self.blas_info(b)
+        nodes = np.zeros(nn, BVH_NODE)
+        leaf = np.zeros(nl, np.uint32)
+        self._ok(self._L.rgrt_export_blas(self.h, b, _p(nodes), nn, _p(leaf), nl))
+        return nodes, leaf
+
+    def build_tlas(self, instances):
+        inst = np.ascontiguousarray(instances, INSTANCE)
+        self._ok(self._L.rgrt_build_tlas(self.h, _p(inst), len(inst)))
+        self._ninst = len(inst)
+
+    def export_tlas(self):
+        nn = ctypes.c_uint32()
+        self._ok(self._L.rgrt_export_tlas(self.h, None, 0, None, 0, ctypes.byref(nn)))
+        nodes = np.zeros(nn.value, BVH_NODE)
+        leaf = np.zeros(self._ninst, np.uint32)
+        self._ok(self._L.rgrt_export_tlas(self.h, _p(nodes), nn.value, _p(leaf), len(leaf), ctypes.byref(nn)))
+        return nodes, leaf
+
+    def set_frame(self, ubo):
+        u = np.ascontiguousarray(np.asarray(ubo, UBO).reshape(()))
+        self._ok(self._L.rgrt_set_frame(self.h, _p(u)))
+
+    def resize(self, width, height):
+        self._ok(self._L.rgrt_resize(self.h, width, height))
+        self.width, self.height = width, height
+
+    # -- work ---------------------------------------------------------------
+    def trace_rays(self, rays, cull_mask=0xFF):
+        r = np.ascontiguousarray(rays, RAY)
+        hits = np.zeros(len(r), HIT)
+        self._ok(self._L.rgrt_trace_rays(self.h, _p(r), len(r), cull_mask, _p(hits)))
+        return hits
+
+    def trace_rays_device(self, rays_ptr, n, cull_mask, hits_ptr):
+        self._ok(self._L.rgrt_trace_rays_device(self.h, rays_ptr, n, cull_mask, hits_ptr))
+
+    def render(self, use_fxaa=True):
+        self._ok(self._L.rgrt_render(self.h, int(bool(use_fxaa))))
+
+    def trace(self):
+        self._ok(self._L.rgrt_trace(self.h))
+
+    def postprocess(self, use_fxaa=True):
+        self._ok(self._L.rgrt_postprocess(self.h, int(bool(use_fxaa))))
+
+    def synchronize(self):
+        self._ok(self._L.rgrt_synchronize(self.h))
+
+    # -- images -------------------------------------------------------------
+    def read_image(self, which):
+        W, H = self.width, self.height
+        if which == IMAGE_ROUGH_TRANSITIONS:
+            out = np.zeros((H, W), np.int8)
+        elif which in (IMAGE_BASE_F32, IMAGE_NORMAL_F32, IMAGE_ROUGH_F32):
+            out = np.zeros((H, W, 4), np.float32)
+        elif which == IMAGE_PRIMARY_HIT:
+            out = np.zeros((H, W), HIT)
+        else:
+            out = np.zeros((H, W, 4), np.uint16)
+        self._ok(self._L.rgrt_read_image(self.h, which, _p(out), out.nbytes))
+        return out
+
+    def write_image(self, which, data):
+        d = np.ascontiguousarray(data)
+        self._ok(self._L.rgrt_write_image(self.h, which, _p(d), d.nbytes))
+
+    def image_device_ptr(self, which):
+        p, n = ctypes.c_void_p(), ctypes.c_uint64()
+        self._ok(self._L.rgrt_image_device_ptr(self.h, which, ctypes.byref(p), ctypes.byref(n)))
+        return p.value, n.value
+
+    def stats(self):
+        s = np.zeros((), STATS)
+        self._ok(self._L.rgrt_get_stats(self.h, _p(s)))
+        return {k: s[k].item() for k in STATS.names if not k.startswith("pad")}
+
+
+def version():
+    return lib().rgrt_version().decode()

@@ -1,0 +1,78 @@
+"""Shared test helpers: one scene set up in the HIP core and in the oracle."""
+import numpy as np
+
+import oracle_py
+from raygun_amd import procgen
+from raygun_amd.records import HIT, RAY, VERTEX
+from raygun_amd.scene import pack_models
+
+NO_HIT = 0xFFFFFFFF
+
+
+def half(img16):
+    return np.asarray(img16, np.uint16).view(np.float16).astype(np.float32)
+
+
+def oracle_scene(setup):
+    """OracleScene + instances for a procgen.SceneSetup."""
+    models = setup.scene.models()
+    v, i, m = pack_models(models)
+    osc = oracle_py.OracleScene(v, i, m)
+    blas_of = {}
+    for mod in models:
+        k = id(mod.mesh)
+        if k not in blas_of:
+            blas_of[k] = osc.build_blas(mod.mesh.first_vertex, mod.mesh.index_offset_bytes, mod.mesh.num_faces())
+        mod.blas = blas_of[k]
+    inst = setup.scene.collect_instances()
+    osc.build_tlas(inst)
+    return osc, inst, (v, i, m)
+
+
+def gpu_context(setup, width=None, height=None, flags=0):
+    """rgrt.Context with the same scene (same BLAS ids as oracle_scene)."""
+    from raygun_amd import rgrt
+    W, H = width or setup.width, height or setup.height
+    ctx = rgrt.Context(W, H, flags)
+    models = setup.scene.models()
+    v, i, m = pack_models(models)
+    ctx.upload_geometry(v, i)
+    ctx.upload_materials(m)
+    blas_of = {}
+    for mod in models:
+        k = id(mod.mesh)
+        if k not in blas_of:
+            blas_of[k] = ctx.build_blas(mod.mesh.first_vertex, mod.mesh.index_offset_bytes, mod.mesh.num_faces(),
+                                        len(mod.mesh.vertices))
+        mod.blas = blas_of[k]
+    inst = setup.scene.collect_instances()
+    ctx.build_tlas(inst)
+    ctx.set_frame(setup.ubo(W, H))
+    return ctx, inst
+
+
+def random_rays(n, seed, center=(0, 0, 0), radius=3.0, tmin=0.001, tmax=1e4):
+    """Seeded rays from a sphere shell towards jittered points near `center`."""
+    rng = np.random.default_rng(seed)
+    c = np.asarray(center, np.float64)
+    o = rng.normal(size=(n, 3))
+    o = c + radius * o / np.linalg.norm(o, axis=1, keepdims=True)
+    tgt = c + rng.uniform(-0.6, 0.6, size=(n, 3)) * radius * 0.5
+    d = tgt - o
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    r = np.zeros(n, RAY)
+    r["origin"] = o.astype(np.float32)
+    r["dir"] = d.astype(np.float32)
+    r["tmin"] = tmin
+    r["tmax"] = tmax
+    return r
+
+
+def small_config2(width=320, height=180, tris_per_mesh=20_000, seed=2):
+    return procgen.config2(width, height, seed=seed, tris_per_mesh=tris_per_mesh)
+
+
+def compare_hits(h_gpu, h_ref, tie):
+    """Indices where the closest-hit identity differs and the oracle saw no tie."""
+    same_id = (h_gpu["instance"] == h_ref["instance"]) & ((h_gpu["primitive"] == h_ref["primitive"]) | (h_ref["instance"] == NO_HIT))
+    return np.nonzero(~same_id & ~tie)[0]
