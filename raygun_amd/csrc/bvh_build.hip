@@ -259,8 +259,10 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(const uint32_t* _
             uint32_t pre = 0;
             for(int w = 0; w < wave; ++w) pre += waveCnt[w][d];
             const uint32_t pos = gofs[d] + pre + rankInWave;
-            keysOut[pos] = key;
-            valsOut[pos] = val;
+            if(pos < n) { // always true: the histogram counted the same keys
+                keysOut[pos] = key;
+                valsOut[pos] = val;
+            }
         }
         __syncthreads();
         {
@@ -350,7 +352,10 @@ __global__ void k_refit(const float4* __restrict__ boxLo, const float4* __restri
     float lo[3] = {a.x, a.y, a.z}, hi[3] = {b.x, b.y, b.z};
     int child = ~k;
     uint32_t p = leafParent[k];
-    for(;;) {
+    // A Karras tree is at most 63 levels deep; the cap only guarantees that
+    // the walk ends even on a malformed tree.
+    for(int level = 0; level < 128; ++level) {
+        if(p >= (uint32_t)(n - 1)) return;
         Node* nd = nodes + p;
         const int slot = (reinterpret_cast<volatile int*>(nd)[12] == child) ? 0 : 1;
         write_slot(nd, slot, lo, hi);

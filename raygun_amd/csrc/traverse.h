@@ -220,7 +220,9 @@ __device__ HitRec traverse(const TraceScene& sc, const RayIn& ray, uint32_t cull
     int32_t ref = 0;
     stack.sp = 0;
 
-    for(;;) {
+    // Each node and leaf is visited at most once per level, so a well-formed
+    // two-level tree ends far below this cap; it only bounds the loop.
+    for(uint32_t guard = 0; guard < (1u << 26); ++guard) {
         if(ref >= 0) {
             const float4* np = reinterpret_cast<const float4*>(nodes + ref);
             const float4 a = np[0], b = np[1], c = np[2];
