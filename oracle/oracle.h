@@ -81,6 +81,9 @@ uint16_t orc_f32_to_f16(float f);
 float orc_f16_to_f32(uint16_t h);
 void orc_sky(const float* dir, const float* lightDir, float* rgb);
 uint32_t orc_morton3(float x, float y, float z); /* inputs in [0,1] */
+/* The primary ray of raygen.h:78-90 for pixel (x, y) and SSAA sample. */
+void orc_primary_ray(const void* ubo, uint32_t x, uint32_t y, uint32_t W, uint32_t H, int32_t sample, float* origin,
+                     float* dir);
 
 #ifdef __cplusplus
 }

@@ -64,6 +64,8 @@ def lib():
         L.orc_sky.argtypes = [P, P, P]
         L.orc_morton3.restype = u32
         L.orc_morton3.argtypes = [ctypes.c_float, ctypes.c_float, ctypes.c_float]
+        L.orc_primary_ray.restype = None
+        L.orc_primary_ray.argtypes = [P, u32, u32, u32, u32, i32, P, P]
         _lib = L
     return _lib
 
@@ -225,3 +227,11 @@ def sky(direction, light_dir):
 
 def morton3(x, y, z):
     return lib().orc_morton3(float(x), float(y), float(z))
+
+
+def primary_ray(ubo, x, y, width, height, sample=0):
+    ubo = np.ascontiguousarray(ubo)
+    o = np.zeros(3, np.float32)
+    d = np.zeros(3, np.float32)
+    lib().orc_primary_ray(_p(ubo), x, y, width, height, sample, _p(o), _p(d))
+    return o, d

@@ -1695,4 +1695,30 @@ void orc_sky(const float* dir, const float* lightDir, float* rgb)
 
 uint32_t orc_morton3(float x, float y, float z) { return morton3(x, y, z); }
 
+void orc_primary_ray(const void* uboPtr, uint32_t x, uint32_t y, uint32_t W, uint32_t H, int32_t sample, float* origin, float* dir)
+{
+    UBO ubo;
+    std::memcpy(&ubo, uboPtr, sizeof(UBO));
+    const int numSamples = ubo.numSamples;
+    const float* off = kAAOffsets[std::min(numSamples, 8)][sample % 8];
+    float pcx = ((float)x + 0.5f) + off[0];
+    float pcy = ((float)y + 0.5f) + off[1];
+    float ux = pcx / (float)W, uy = pcy / (float)H;
+    float dx = ux * 2.0f - 1.0f, dy = uy * 2.0f - 1.0f;
+    const float t4[4] = {dx, dy, 1.0f, 1.0f};
+    float target[4];
+    mat4_mul(ubo.projInverse, t4, target);
+    V3 dn = normalize(v3(target[0], target[1], target[2]));
+    V3 d = mat4_mul_dir(ubo.viewInverse, dn);
+    const float zero_one[4] = {0.0f, 0.0f, 0.0f, 1.0f};
+    float org4[4];
+    mat4_mul(ubo.viewInverse, zero_one, org4);
+    origin[0] = org4[0];
+    origin[1] = org4[1];
+    origin[2] = org4[2];
+    dir[0] = d.x;
+    dir[1] = d.y;
+    dir[2] = d.z;
+}
+
 } // extern "C"

@@ -1,0 +1,476 @@
+"""CPU tests that pin the oracle (oracle/rt_oracle.cpp) to what the
+mathematics and the reference fix, so that it can serve as the parity
+reference for the HIP core.
+
+The reference ships no tests, golden images or known-answer vectors for this
+path (its only check is an interactive UI window), so each function of the
+oracle is pinned here by a closed form, an invariant, an independent
+double-precision computation or brute force on small inputs.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle_py
+from helpers import NO_HIT, half, oracle_scene, random_rays
+from raygun_amd import procgen
+from raygun_amd.records import RAY, VERTEX, default_material
+from raygun_amd.scene import Camera, Entity, Mesh, Model, Scene, Transform, camera_ubo
+
+
+# ---------------------------------------------------------------------------
+# storage conversions
+# ---------------------------------------------------------------------------
+def test_f16_rounding_matches_ieee_round_to_nearest_even():
+    rng = np.random.default_rng(0)
+    vals = np.concatenate([
+        rng.normal(size=20000).astype(np.float32) * np.float32(10.0) ** rng.integers(-9, 6, 20000),
+        np.array([0.0, -0.0, 65504.0, 65519.99, 65520.0, 1e9, -1e9, 2.0 ** -24, 2.0 ** -25, 2.0 ** -26,
+                  3 * 2.0 ** -26, 6.1e-5, 2.0 ** -14, 1.0 + 2.0 ** -11, 1.0 + 3 * 2.0 ** -11], np.float32),
+    ]).astype(np.float32)
+    # exact halfway cases between consecutive halves
+    h = rng.integers(0, 0x7BFF, 5000).astype(np.uint16)
+    a = h.view(np.float16).astype(np.float64)
+    b = (h + 1).view(np.float16).astype(np.float64)
+    vals = np.concatenate([vals, ((a + b) / 2).astype(np.float32)])
+    ours = np.array([oracle_py.f32_to_f16(v) for v in vals], np.uint16)
+    with np.errstate(over="ignore"):
+        ref = vals.astype(np.float16).view(np.uint16)
+    assert np.array_equal(ours, ref)
+
+
+# ---------------------------------------------------------------------------
+# Morton codes and the LBVH
+# ---------------------------------------------------------------------------
+def _spread(v):
+    out = 0
+    for i in range(10):
+        out |= ((v >> i) & 1) << (3 * i)
+    return out
+
+
+def test_morton_interleaves_x_y_z():
+    assert oracle_py.morton3(0, 0, 0) == 0
+    assert oracle_py.morton3(1, 1, 1) == (1 << 30) - 1
+    rng = np.random.default_rng(1)
+    for _ in range(200):
+        x, y, z = rng.uniform(0, 1, 3).astype(np.float32)
+        q = [min(max(int(np.float32(c) * np.float32(1024.0)), 0), 1023) for c in (x, y, z)]
+        want = (_spread(q[0]) << 2) | (_spread(q[1]) << 1) | _spread(q[2])
+        assert oracle_py.morton3(x, y, z) == want
+
+
+def _mesh_from_tris(tris):
+    tris = np.asarray(tris, np.float64).reshape(-1, 3, 3)
+    v = np.zeros(len(tris) * 3, VERTEX)
+    v["position"] = tris.reshape(-1, 3)
+    v["normal"] = (0, 1, 0)
+    return Mesh(v, np.arange(len(v), dtype=np.uint32))
+
+
+def _check_lbvh(nodes, leaf, prim_boxes, codes):
+    n = len(leaf)
+    assert sorted(leaf.tolist()) == list(range(n))  # every primitive exactly once
+    assert np.all(np.diff(codes[leaf].astype(np.int64)) >= 0)  # Morton order
+    if n <= 1:
+        return
+    assert len(nodes) == n - 1
+    assert nodes["parent"][0] == 0xFFFFFFFF
+    seen_leaf = np.zeros(n, int)
+    seen_node = np.zeros(n - 1, int)
+    for i, nd in enumerate(nodes):
+        for c in range(2):
+            ch = int(nd["child"][c])
+            if ch >= 0:
+                seen_node[ch] += 1
+                assert nodes["parent"][ch] == i
+                lo = np.minimum(nodes["lo"][ch][0], nodes["lo"][ch][1])
+                hi = np.maximum(nodes["hi"][ch][0], nodes["hi"][ch][1])
+            else:
+                seen_leaf[~ch] += 1
+                lo, hi = prim_boxes[leaf[~ch]]
+            # the child slot holds exactly the union of what is below it
+            assert np.array_equal(nd["lo"][c], lo.astype(np.float32))
+            assert np.array_equal(nd["hi"][c], hi.astype(np.float32))
+    assert np.all(seen_leaf == 1)
+    assert seen_node[0] == 0 and np.all(seen_node[1:] == 1)
+    # Karras: each internal node covers a contiguous leaf range
+    def rng_of(ref):
+        if ref < 0:
+            return (~ref, ~ref)
+        a = rng_of(int(nodes["child"][ref][0]))
+        b = rng_of(int(nodes["child"][ref][1]))
+        assert a[1] + 1 == b[0]
+        return (a[0], b[1])
+    import sys
+    sys.setrecursionlimit(10000)
+    assert rng_of(0) == (0, n - 1)
+
+
+@pytest.mark.parametrize("n,dup", [(1, False), (2, False), (7, False), (500, False), (300, True)])
+def test_lbvh_invariants(n, dup):
+    rng = np.random.default_rng(n)
+    centers = rng.uniform(-5, 5, (n, 3))
+    if dup:  # many primitives with one centroid: Morton ties broken by index
+        centers[: n // 2] = centers[0]
+    tris = centers[:, None, :] + rng.uniform(-0.3, 0.3, (n, 3, 3))
+    mesh = _mesh_from_tris(tris)
+    osc = oracle_py.OracleScene(mesh.vertices, mesh.indices, np.array([default_material()]))
+    b = osc.build_blas(0, 0, n)
+    nodes, leaf = osc.export_blas(b)
+    t32 = tris.astype(np.float32)
+    boxes = np.stack([t32.min(axis=1), t32.max(axis=1)], axis=1)
+    _check_lbvh(nodes, leaf, boxes, osc.blas_morton(b))
+
+
+def test_lbvh_empty_mesh_has_one_empty_node():
+    mesh = _mesh_from_tris(np.zeros((1, 3, 3)))
+    osc = oracle_py.OracleScene(mesh.vertices, mesh.indices, np.array([default_material()]))
+    b = osc.build_blas(0, 0, 0)
+    nodes, leaf = osc.export_blas(b)
+    assert len(nodes) == 1 and len(leaf) == 0
+    assert nodes["child"][0].tolist() == [-(2 ** 31), -(2 ** 31)]
+
+
+# ---------------------------------------------------------------------------
+# intersection
+# ---------------------------------------------------------------------------
+@pytest.fixture(scope="module")
+def c1():
+    setup = procgen.config1()
+    osc, inst, bufs = oracle_scene(setup)
+    return setup, osc
+
+
+def test_fp32_traversal_agrees_with_fp64_brute_force(c1):
+    setup, osc = c1
+    rays = random_rays(3000, seed=5, radius=3.0)
+    h, tie = osc.trace_rays(rays)
+    t64, inst64, prim64, uv64 = osc.trace_rays_fp64(rays)
+    hit32 = h["instance"] != NO_HIT
+    hit64 = inst64 != NO_HIT
+    # same hit/miss decision except rays grazing an edge of the mesh
+    assert np.mean(hit32 == hit64) > 0.999
+    both = hit32 & hit64 & ~tie
+    same = h["primitive"][both] == prim64[both]
+    assert np.mean(same) > 0.995  # differences only at shared edges (both triangles are hit)
+    # Stated FP32 error against FP64: the absolute t error scales with the
+    # magnitude of the coordinates involved (|origin| + t), so a hit a few
+    # millimetres from an origin 3 units out has a large *relative* error.
+    t32, t_64 = h["t"][both][same], t64[both][same]
+    scale = np.linalg.norm(rays["origin"][both][same], axis=1) + t_64
+    assert (np.abs(t32 - t_64) / scale).max() < 1e-5
+    rel = np.abs(t32 - t_64) / t_64
+    assert np.median(rel) < 1e-6 and np.percentile(rel, 99) < 1e-4
+    du = np.abs(h["u"][both][same] - uv64[both][same, 0])
+    assert du.max() < 1e-4
+
+
+def test_hits_on_analytic_sphere_and_plane(c1):
+    setup, osc = c1
+    rng = np.random.default_rng(9)
+    n = 2000
+    o = np.tile([0.0, 0.5, 4.0], (n, 1))
+    tgt = rng.uniform(-0.7, 0.7, (n, 3))
+    d = tgt - o
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.zeros(n, RAY)
+    rays["origin"], rays["dir"], rays["tmin"], rays["tmax"] = o, d, 0.001, 1e4
+    h, tie = osc.trace_rays(rays)
+    sphere = (h["instance"] != NO_HIT) & (h["primitive"] < 16128)
+    # analytic sphere |o + t d| = 1
+    b = np.sum(o * d, axis=1)
+    c = np.sum(o * o, axis=1) - 1.0
+    disc = b * b - c
+    ta = -b - np.sqrt(np.maximum(disc, 0))
+    assert np.all(disc[sphere] > -1e-3)
+    # a 128x64 tessellation lies inside the sphere by at most 1 - cos(pi/64) ~ 1.2e-3
+    assert np.all(np.abs(h["t"][sphere] - ta[sphere]) < 0.02)
+    assert np.all(h["t"][sphere] >= ta[sphere] - 1e-4)  # chords are inside: hit no earlier than the sphere
+    # plane y = -1: t = (-1 - oy) / dy
+    down = (~sphere) & (h["instance"] != NO_HIT)
+    tp = (-1.0 - o[down, 1]) / d[down, 1]
+    assert np.allclose(h["t"][down], tp, rtol=1e-5)
+
+
+def test_watertight_through_shared_vertices_and_edges():
+    """Rays aimed exactly at mesh vertices and edge midpoints of a closed
+    sphere must hit it (no leak between adjacent triangles)."""
+    mesh = procgen.uv_sphere(32, 16, 1.0)
+    osc = oracle_py.OracleScene(mesh.vertices, mesh.indices, np.array([default_material()]))
+    b = osc.build_blas(0, 0, mesh.num_faces())
+    from raygun_amd.records import INSTANCE
+    inst = np.zeros(1, INSTANCE)
+    inst["transform"] = np.eye(4)[:3].reshape(12)
+    inst["mask"] = 0xFF
+    osc.build_tlas(inst)
+    P = mesh.vertices["position"].astype(np.float64)
+    idx = mesh.indices.reshape(-1, 3)
+    targets = np.concatenate([P, 0.5 * (P[idx[:, 0]] + P[idx[:, 1]]), 0.5 * (P[idx[:, 1]] + P[idx[:, 2]])])
+    rng = np.random.default_rng(2)
+    o = rng.normal(size=(len(targets), 3))
+    o = 4.0 * o / np.linalg.norm(o, axis=1, keepdims=True)
+    d = targets - o
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    # keep rays that cross the surface transversally at the target (a
+    # grazing ray may touch a limb vertex only, and fp32 rounding of its
+    # direction can move it off the mesh)
+    nrm = targets / np.linalg.norm(targets, axis=1, keepdims=True)
+    keep = np.abs(np.sum(d * nrm, axis=1)) > 0.2
+    rays = np.zeros(int(keep.sum()), RAY)
+    rays["origin"], rays["dir"], rays["tmin"], rays["tmax"] = o[keep], d[keep], 0.0, 100.0
+    assert len(rays) > 1500
+    h, _ = osc.trace_rays(rays)
+    # a ray through a shared vertex or edge of a closed mesh must hit it:
+    # no ray slips between adjacent triangles
+    assert np.all(h["instance"] != NO_HIT)
+
+
+# ---------------------------------------------------------------------------
+# ray generation
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("fov", [45.0, 60.0])
+def test_primary_rays_follow_the_pinhole_projection(fov):
+    W, H = 64, 36
+    cam = Camera(fov_deg=fov)
+    cam.look_at((1.0, 2.0, 3.0), (1.0, 2.0, 0.0))  # looks down -z, no rotation
+    ubo = camera_ubo(cam, W, H)
+    tanh = math.tan(math.radians(fov) / 2)
+    aspect = W / H
+    for (x, y) in [(0, 0), (W - 1, H - 1), (W // 2, H // 2), (5, 30)]:
+        o, d = oracle_py.primary_ray(ubo, x, y, W, H)
+        assert np.allclose(o, [1.0, 2.0, 3.0], atol=1e-6)
+        sx = (2 * (x + 0.5) / W - 1) * tanh * aspect
+        sy = -(2 * (y + 0.5) / H - 1) * tanh  # row 0 is the top (camera.cpp:46 flips Y)
+        want = np.array([sx, sy, -1.0])
+        want /= np.linalg.norm(want)
+        assert np.allclose(d, want, atol=2e-6)
+
+
+def test_ssaa_offsets_are_the_reference_table():
+    W, H = 8, 8
+    cam = Camera(fov_deg=60.0)
+    cam.look_at((0, 0, 0), (0, 0, -1))
+    ubo = camera_ubo(cam, W, H, num_samples=2)
+    o, d0 = oracle_py.primary_ray(ubo, 3, 3, W, H, 0)
+    o, d1 = oracle_py.primary_ray(ubo, 3, 3, W, H, 1)
+    tanh = math.tan(math.radians(60) / 2)
+    for d, off in ((d0, 0.25), (d1, -0.25)):  # raygen.h:45 (two samples: +-0.25)
+        sx = (2 * (3 + 0.5 + off) / W - 1) * tanh
+        sy = -(2 * (3 + 0.5 + off) / H - 1) * tanh
+        want = np.array([sx, sy, -1.0])
+        assert np.allclose(d, want / np.linalg.norm(want), atol=2e-6)
+
+
+# ---------------------------------------------------------------------------
+# shading closed forms
+# ---------------------------------------------------------------------------
+def _sky64(d, L):
+    """miss.rmiss:38-74 evaluated in float64 (a second, independent reading)."""
+    d = np.asarray(d, np.float64)
+    L = np.asarray(L, np.float64)
+    rd = d / np.linalg.norm(d)
+    y = abs(d[1] + 1.5) / 3.0
+    sun = 1.0 - np.linalg.norm(rd - (-L / np.linalg.norm(L)))
+    sun = min(max(sun, 0.0), 2.0)
+    glow = min(max(sun, 0.0), 1.0)
+    sun = min(max(sun ** 80 * 1000.0, 0.0), 16.0)
+    glow = min(max((glow ** 6) ** y, 0.0), 1.0)
+    sun *= (y * y) ** (1 / 1.65)
+    glow *= (y * y) ** 0.5
+    sun += glow
+    sun_color = np.array([1.0, 0.6, 0.05]) * sun
+    atmosphere = math.sqrt(1.0 - y)
+    scatter = 1.0 - min(max((4.0 - L[1]) ** (1 / 15), 0.8), 1.0)
+    scatter_color = np.ones(3) * (1 - scatter) + np.array([1.0, 0.3, 0.0]) * 1.5 * scatter
+    a = atmosphere / 1.3
+    sky = np.array([0.2, 0.4, 0.8]) * (1 - a) + scatter_color * a
+    return sun_color + sky
+
+
+def test_sky_matches_an_independent_float64_reading():
+    L = np.array([0.4, -0.6, -0.8]) / np.linalg.norm([0.4, -0.6, -0.8])
+    rng = np.random.default_rng(4)
+    g = rng.normal(size=(200, 3))
+    g /= np.linalg.norm(g, axis=1, keepdims=True)  # ray directions are unit length here
+    dirs = list(g) + [-L, -L * 0.999 + 0.001, np.array([0, -1.0, 0]), np.array([0, 1.0, 0])]
+    for d in dirs:
+        ours = oracle_py.sky(d, L)
+        assert np.allclose(ours, _sky64(d, L), rtol=2e-5, atol=2e-5), d
+    # at the sun's direction the sun term saturates at 16 (miss.rmiss:51)
+    y = abs(-L[1] + 1.5) / 3
+    assert oracle_py.sky(-L, L)[0] > 16 * (y * y) ** (1 / 1.65)
+
+
+def _plane_scene(material, light=(0.4, -0.6, -0.8), max_rec=1, W=48, H=32):
+    mesh = procgen.plane(50.0, -1.0)
+    sc = Scene()
+    sc.camera = Camera(fov_deg=60.0)
+    sc.camera.look_at((0.0, 1.0, 3.0), (0.0, -1.0, -3.0))
+    sc.root.add_child(Entity("plane", model=Model(mesh, np.array([material]))))
+    setup = procgen.SceneSetup(sc, W, H, max_recursions=max_rec)
+    osc, inst, _ = oracle_scene(setup)
+    ubo = camera_ubo(sc.camera, W, H, light_dir=light, max_recursions=max_rec)
+    return osc, ubo, W, H
+
+
+def test_lit_diffuse_plane_is_lambert_clamped_at_0_2():
+    mat = default_material(diffuse=(0.5, 0.25, 1.0))
+    L = np.array([0.4, -0.6, -0.8]) / np.linalg.norm([0.4, -0.6, -0.8])
+    osc, ubo, W, H = _plane_scene(mat)
+    out = osc.render(ubo, W, H)
+    hit = out["primary"]["instance"] != NO_HIT
+    assert hit.sum() > W * H // 4
+    want = max(float(np.dot(-L, [0, 1, 0])), 0.2) * np.array([0.5, 0.25, 1.0])  # closesthit.rchit:174-175, shadow ray misses
+    assert np.allclose(out["base32"][hit][:, :3], want.astype(np.float32), atol=1e-6)
+    assert np.allclose(out["normal32"][hit][:, :3], [0, 1, 0], atol=1e-6)
+
+
+def test_unlit_plane_gets_ambient_0_2_and_no_shadow_ray():
+    mat = default_material(diffuse=(0.5, 0.5, 0.5))
+    osc, ubo, W, H = _plane_scene(mat, light=(0.0, 1.0, 0.0))  # light from below
+    out = osc.render(ubo, W, H)
+    hit = out["primary"]["instance"] != NO_HIT
+    assert np.allclose(out["base32"][hit][:, :3], 0.1, atol=1e-7)
+    assert out["rays"] == W * H  # no shadow rays (dot < 0.07), no reflection
+
+
+def test_mirror_plane_reflects_the_sky():
+    mat = default_material(diffuse=(0.2, 0.2, 0.2), reflectivity=1.0, roughness=0.4, specular=(0.5, 1.0, 0.25))
+    L = np.array([0.4, -0.6, -0.8], np.float32)
+    L = L / np.linalg.norm(L)
+    osc, ubo, W, H = _plane_scene(mat, light=L)
+    out = osc.render(ubo, W, H)
+    hit = out["primary"]["instance"] != NO_HIT
+    ys, xs = np.nonzero(hit)
+    for y, x in list(zip(ys, xs))[::37]:
+        o, d = oracle_py.primary_ray(ubo, x, y, W, H)
+        r = d.astype(np.float64) * np.array([1, -1, 1])  # reflect about +y
+        want = oracle_py.sky(r.astype(np.float32), ubo["lightDir"]) * np.array([0.5, 1.0, 0.25])
+        assert np.allclose(out["rough32"][y, x, :3], want, rtol=1e-5, atol=1e-6)
+        # reflectDepth = 10000 (miss), so rough.a = min(10000/50*r, r/2.1) = r/2.1
+        assert np.isclose(out["rough32"][y, x, 3], np.float32(0.4) / np.float32(2.1), rtol=1e-6)
+    assert np.allclose(out["base32"][hit][:, 3], 1.0)  # reflectContribution = max(transp, refl)
+
+
+def test_max_recursions_zero_traces_primary_rays_only(c1):
+    setup, osc = c1
+    ubo = setup.ubo(64, 36)
+    ubo["maxRecursions"] = 0
+    out = osc.render(ubo, 64, 36)
+    assert out["rays"] == 64 * 36 and out["sky"] == 0
+    hit = out["primary"]["instance"] != NO_HIT
+    assert np.allclose(out["rough32"][hit][:, :3], 1.0)  # reflect/refract colours stay (1,1,1)
+
+
+def test_sky_only_scene():
+    from raygun_amd.records import INSTANCE
+    mesh = _mesh_from_tris(np.array([[[0, 0, 100], [1, 0, 100], [0, 1, 100]]]))  # behind the camera
+    osc = oracle_py.OracleScene(mesh.vertices, mesh.indices, np.array([default_material()]))
+    osc.build_blas(0, 0, 1)
+    inst = np.zeros(1, INSTANCE)
+    inst["transform"] = np.eye(4)[:3].reshape(12)
+    inst["mask"] = 0xFF
+    osc.build_tlas(inst)
+    cam = Camera(fov_deg=60.0)
+    cam.look_at((0, 0, 0), (0, 0, -1))
+    ubo = camera_ubo(cam, 16, 8)
+    out = osc.render(ubo, 16, 8)
+    assert np.all(out["primary"]["instance"] == NO_HIT)
+    for y, x in [(0, 0), (7, 15), (3, 9)]:
+        o, d = oracle_py.primary_ray(ubo, x, y, 16, 8)
+        assert np.allclose(out["base32"][y, x, :3], oracle_py.sky(d, ubo["lightDir"]), rtol=0, atol=0)
+        assert np.isclose(out["normal32"][y, x, 3], np.log(np.float32(10000.0)) * 0.25, rtol=1e-6)
+
+
+# ---------------------------------------------------------------------------
+# post passes
+# ---------------------------------------------------------------------------
+def _f16(a):
+    return np.asarray(a, np.float32).astype(np.float16).view(np.uint16)
+
+
+def _ubo():
+    cam = Camera()
+    return camera_ubo(cam, 8, 8)
+
+
+def test_blur_impulse_response_is_the_separable_3_tap_power():
+    W = H = 48
+    t = 0.2
+    rough = np.zeros((H, W, 4), np.float32)
+    rough[..., 3] = t  # uniform roughness alpha -> transition t (identical normals)
+    rough[24, 20, :3] = 1.0
+    normal = np.zeros((H, W, 4), np.float32)
+    normal[..., 1] = 1.0
+    base = np.zeros((H, W, 4), np.float32)
+    out = oracle_py.postprocess(_ubo(), _f16(base), _f16(normal), _f16(rough), use_fxaa=False)
+    tq = np.float32(np.rint(np.float32(t) * 127) / 127)  # R8_SNORM transition
+    assert np.all(out["transitions"][1:-1, 1:-1] == np.rint(t * 127))
+    assert np.all(out["transitions"][0] == 0) and np.all(out["transitions"][:, -1] == 0)  # out-of-bounds neighbours read 0
+    k = np.array([tq, 1 - 2 * tq, tq], np.float64)
+    ker = np.array([1.0])
+    for _ in range(10):
+        ker = np.convolve(ker, k)
+    want = np.zeros((H, W))
+    want[24 - 10:24 + 11, 20 - 10:20 + 11] = np.outer(ker, ker)
+    got = half(out["colorsA"])[..., 0]
+    assert np.abs(got - want).max() < 20 * 2 ** -11 * want.max()
+
+
+def test_constant_image_passes_through_post_and_fxaa():
+    W, H = 40, 24
+    c = np.array([0.3, 0.6, 0.2, 0.5], np.float32)
+    img = np.broadcast_to(c, (H, W, 4)).copy()
+    normal = np.zeros((H, W, 4), np.float32)
+    base = img.copy()
+    base[..., 3] = 0.0  # no rough contribution
+    out = oracle_py.postprocess(_ubo(), _f16(base), _f16(normal), _f16(img), use_fxaa=True)
+    fin = half(out["final"])
+    assert np.allclose(fin[..., :3], half(_f16(c))[:3])
+    luma = np.float32(0.299) * half(_f16(c))[0] + np.float32(0.587) * half(_f16(c))[1] + np.float32(0.114) * half(_f16(c))[2]
+    assert np.allclose(fin[..., 3], half(_f16(luma)))
+
+
+def test_fxaa_ignores_edges_without_green_contrast():
+    W, H = 32, 32
+    base = np.zeros((H, W, 4), np.float32)
+    base[..., 1] = 0.5
+    base[:, W // 2:, 0] = 1.0  # red-only edge: FXAA_GREEN_AS_LUMA sees no contrast
+    out = oracle_py.postprocess(_ubo(), _f16(base), _f16(np.zeros_like(base)), _f16(np.zeros_like(base)), use_fxaa=True)
+    assert np.array_equal(half(out["final"])[..., :3], half(out["base"])[..., :3])
+
+
+def test_fxaa_smooths_a_green_step_and_stays_in_range():
+    W, H = 32, 32
+    base = np.zeros((H, W, 4), np.float32)
+    yy, xx = np.mgrid[:H, :W]
+    base[..., 1] = (xx > 0.6 * yy + 5).astype(np.float32)  # slanted green step
+    out = oracle_py.postprocess(_ubo(), _f16(base), _f16(np.zeros_like(base)), _f16(np.zeros_like(base)), use_fxaa=True)
+    g = half(out["final"])[..., 1]
+    src = half(out["base"])[..., 1]
+    assert g.min() >= 0.0 and g.max() <= 1.0
+    changed = g != src
+    assert changed.any()
+    # only pixels next to the step change
+    edge = np.zeros_like(changed)
+    edge[:, 1:] |= src[:, 1:] != src[:, :-1]
+    edge[:, :-1] |= src[:, 1:] != src[:, :-1]
+    edge[1:, :] |= src[1:, :] != src[:-1, :]
+    edge[:-1, :] |= src[1:, :] != src[:-1, :]
+    assert not (changed & ~edge).any()
+
+
+def test_composite_clamps_and_writes_luma():
+    W, H = 8, 4
+    rng = np.random.default_rng(3)
+    base = rng.uniform(-0.5, 1.5, (H, W, 4)).astype(np.float32)
+    base[..., 3] = 0.0
+    out = oracle_py.postprocess(_ubo(), _f16(base), _f16(np.zeros_like(base)), _f16(np.zeros_like(base)), use_fxaa=False)
+    b = half(_f16(base))
+    col = np.clip(b[..., :3] * np.float32(1.0) + np.float32(0.0), 0, 1).astype(np.float32)
+    luma = (col[..., 0] * np.float32(0.299) + col[..., 1] * np.float32(0.587)) + col[..., 2] * np.float32(0.114)
+    fin = out["final"]
+    assert np.array_equal(fin[..., :3], _f16(col))
+    assert np.all(np.abs(half(fin)[..., 3] - luma) <= 2 ** -10)
