@@ -209,6 +209,11 @@ rgrt_status rgrt_blas_info(rgrt_ctx* ctx, rgrt_blas blas, uint32_t* numNodes, ui
  * original primitive index of every leaf in Morton order. */
 rgrt_status rgrt_export_blas(rgrt_ctx* ctx, rgrt_blas blas, rgrt_bvh_node* nodes, uint32_t nodeCapacity,
                              uint32_t* leafPrimitive, uint32_t leafCapacity);
+/* The 4-wide tree the traversal walks, collapsed from the binary tree
+ * breadth-first (128-byte nodes: per-axis lo/hi of 4 children, 4 child refs,
+ * 16 bytes padding). With nodes == NULL only *numNodes is written. */
+rgrt_status rgrt_export_blas_wide(rgrt_ctx* ctx, rgrt_blas blas, void* nodes, uint32_t capacity, uint32_t* numNodes);
+rgrt_status rgrt_export_tlas_wide(rgrt_ctx* ctx, void* nodes, uint32_t capacity, uint32_t* numNodes);
 
 /* Replaces TopLevelAS(cmd, scene) + instanceOffsetTable()
  * (raygun/render/acceleration_structure.cpp:55-138). The caller keeps the
@@ -282,9 +287,28 @@ typedef struct rgrt_stats {
     uint32_t traceLaunches; /* launches of the traversal kernel            */
     float traceKernelMs;  /* summed duration of those launches (hipEvents) */
     float shadeKernelMs;  /* summed duration of the shading launches       */
+    float drainKernelMs;  /* duration of the drain launch (0 if none)      */
+    uint32_t drained;     /* 1 if the frame's tail ran in the drain kernel */
 } rgrt_stats;
 
+/* Launch-shape parameters (see rgrt_set_tuning). */
+typedef struct rgrt_tuning {
+    uint32_t wavefrontPasses;   /* wavefront passes before the persistent drain takes the tail
+                                   (0 = drain everything, >= 64 = never drain)                */
+    uint32_t traceBlocksPerCU;  /* persistent traversal grid, blocks of 256 per CU            */
+    uint32_t shadeBlocksPerCU;  /* shading grid, blocks of 256 per CU                         */
+    uint32_t drainBlocksPerCU;  /* drain grid, blocks of 256 per CU                           */
+} rgrt_tuning;
+
+/* Sets / reads the launch shape (defaults tuned for MI355X). Values outside
+ * 1..32 blocks per CU are rejected. */
+rgrt_status rgrt_set_tuning(rgrt_ctx* ctx, const rgrt_tuning* t);
+rgrt_status rgrt_get_tuning(rgrt_ctx* ctx, rgrt_tuning* t);
+
 rgrt_status rgrt_get_stats(rgrt_ctx* ctx, rgrt_stats* out);
+/* Rays traced in each wavefront pass of the last rgrt_trace / rgrt_render
+ * (the recursion depth profile of the frame); *count = number of passes. */
+rgrt_status rgrt_get_pass_counts(rgrt_ctx* ctx, uint32_t* counts, uint32_t capacity, uint32_t* count);
 
 /* Library version string. */
 const char* rgrt_version(void);

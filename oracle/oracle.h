@@ -33,6 +33,11 @@ int32_t orc_update_vertices(orc_scene* s, uint64_t firstVertex, const void* vert
 uint32_t orc_blas_num_nodes(orc_scene* s, int32_t blas);
 /* Canonical nodes (64 B each, rgrt_bvh_node layout) + leaf primitive ids. */
 int32_t orc_export_blas(orc_scene* s, int32_t blas, void* nodes, uint32_t* leafPrim);
+/* The 4-wide tree the traversal walks (128-byte nodes, breadth-first). */
+uint32_t orc_blas_num_wide(orc_scene* s, int32_t blas);
+int32_t orc_export_blas_wide(orc_scene* s, int32_t blas, void* nodes);
+uint32_t orc_tlas_num_wide(orc_scene* s);
+int32_t orc_export_tlas_wide(orc_scene* s, void* nodes);
 /* Morton codes of the primitives in original order (for the sort check). */
 int32_t orc_blas_morton(orc_scene* s, int32_t blas, uint32_t* codes);
 

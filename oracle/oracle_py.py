@@ -39,6 +39,14 @@ def lib():
         L.orc_blas_num_nodes.argtypes = [P, i32]
         L.orc_export_blas.restype = i32
         L.orc_export_blas.argtypes = [P, i32, P, P]
+        L.orc_blas_num_wide.restype = u32
+        L.orc_blas_num_wide.argtypes = [P, i32]
+        L.orc_export_blas_wide.restype = i32
+        L.orc_export_blas_wide.argtypes = [P, i32, P]
+        L.orc_tlas_num_wide.restype = u32
+        L.orc_tlas_num_wide.argtypes = [P]
+        L.orc_export_tlas_wide.restype = i32
+        L.orc_export_tlas_wide.argtypes = [P, P]
         L.orc_blas_morton.restype = i32
         L.orc_blas_morton.argtypes = [P, i32, P]
         L.orc_build_tlas.restype = i32
@@ -116,6 +124,18 @@ class OracleScene:
         leaf = np.zeros(self.blas_faces[b], np.uint32)
         lib().orc_export_blas(self.h, b, _p(nodes), _p(leaf))
         return nodes, leaf
+
+    def export_blas_wide(self, b):
+        from raygun_amd.records import WIDE_NODE
+        nodes = np.zeros(lib().orc_blas_num_wide(self.h, b), WIDE_NODE)
+        lib().orc_export_blas_wide(self.h, b, _p(nodes))
+        return nodes
+
+    def export_tlas_wide(self):
+        from raygun_amd.records import WIDE_NODE
+        nodes = np.zeros(lib().orc_tlas_num_wide(self.h), WIDE_NODE)
+        lib().orc_export_tlas_wide(self.h, _p(nodes))
+        return nodes
 
     def blas_morton(self, b):
         codes = np.zeros(self.blas_faces[b], np.uint32)

@@ -277,8 +277,18 @@ uint32_t morton3(float x, float y, float z)
     return (expand_bits10(quantize10(x)) << 2) | (expand_bits10(quantize10(y)) << 1) | expand_bits10(quantize10(z));
 }
 
+// 4-wide node, 128 bytes: per-axis box bounds of the four children, then
+// the child references (>= 0 wide node, < 0 leaf ~k, EMPTY_CHILD unused).
+struct Node4 {
+    float lox[4], hix[4], loy[4], hiy[4], loz[4], hiz[4];
+    int32_t child[4];
+    int32_t pad[4];
+};
+static_assert(sizeof(Node4) == 128, "wide node");
+
 struct BVH {
     std::vector<CanonNode> nodes;
+    std::vector<Node4> wide;        // the binary tree collapsed to 4 children per node
     std::vector<uint32_t> leafPrim; // Morton order -> original primitive index
     std::vector<uint32_t> morton;   // original order
     std::vector<Box> primBox;       // original order
@@ -338,6 +348,91 @@ void compute_node_boxes(BVH& b)
         nodeBox[i] = box_union(cb[0], cb[1]);
     }
     b.root = nodeBox[0];
+}
+
+inline float half_area(const float lo[3], const float hi[3])
+{
+    const float dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+    return ((dx * dy) + (dy * dz)) + (dz * dx);
+}
+
+// Collapses the binary tree into 4-wide nodes. Each wide node starts from a
+// binary node's two children and repeatedly opens the internal child of
+// largest surface area (the first one on equal areas) until it has four
+// children or only leaves. Wide nodes are numbered in breadth-first order.
+void collapse4(BVH& b)
+{
+    const int n = (int)b.leafPrim.size();
+    const float inf = std::numeric_limits<float>::infinity();
+    auto empty_node = [&]() {
+        Node4 w;
+        for(int c = 0; c < 4; ++c) {
+            w.lox[c] = w.loy[c] = w.loz[c] = inf;
+            w.hix[c] = w.hiy[c] = w.hiz[c] = -inf;
+            w.child[c] = EMPTY_CHILD;
+            w.pad[c] = 0;
+        }
+        return w;
+    };
+    struct E {
+        int32_t ref;
+        float lo[3], hi[3];
+    };
+    auto slot = [&](int node, int c) {
+        const CanonNode& nd = b.nodes[node];
+        E e;
+        e.ref = nd.child[c];
+        for(int k = 0; k < 3; ++k) {
+            e.lo[k] = nd.lo[c][k];
+            e.hi[k] = nd.hi[c][k];
+        }
+        return e;
+    };
+    b.wide.clear();
+    std::vector<int> queue{0};
+    for(size_t qi = 0; qi < queue.size(); ++qi) {
+        const int r = queue[qi];
+        E S[4];
+        int ns = 0;
+        for(int c = 0; c < 2; ++c)
+            if(b.nodes[r].child[c] != EMPTY_CHILD) S[ns++] = slot(r, c);
+        while(ns < 4) {
+            int j = -1;
+            float best = -1.0f;
+            for(int k = 0; k < ns; ++k) {
+                if(S[k].ref < 0) continue;
+                const float a = half_area(S[k].lo, S[k].hi);
+                if(a > best) {
+                    best = a;
+                    j = k;
+                }
+            }
+            if(j < 0) break;
+            const int e = S[j].ref;
+            for(int k = ns; k > j + 1; --k) S[k] = S[k - 1];
+            S[j] = slot(e, 0);
+            S[j + 1] = slot(e, 1);
+            ++ns;
+        }
+        Node4 w = empty_node();
+        for(int k = 0; k < ns; ++k) {
+            w.lox[k] = S[k].lo[0];
+            w.loy[k] = S[k].lo[1];
+            w.loz[k] = S[k].lo[2];
+            w.hix[k] = S[k].hi[0];
+            w.hiy[k] = S[k].hi[1];
+            w.hiz[k] = S[k].hi[2];
+            if(S[k].ref >= 0) {
+                w.child[k] = (int32_t)queue.size();
+                queue.push_back(S[k].ref);
+            }
+            else {
+                w.child[k] = S[k].ref;
+            }
+        }
+        b.wide.push_back(w);
+    }
+    (void)n;
 }
 
 // Builds the canonical LBVH over primitive boxes.
@@ -402,6 +497,7 @@ void build_lbvh(BVH& b)
         }
     }
     compute_node_boxes(b);
+    collapse4(b);
 }
 
 // --------------------------------------------------------------------------
@@ -604,9 +700,11 @@ struct Hit {
 
 constexpr float TIE_REL = 1e-6f;
 
-// Closest hit over the TLAS. Depth-first over child references (>= 0 an
-// internal node, < 0 a leaf ~k); of two hit children the nearer is visited
-// first (child 0 on equal entry distance), the other waits on the stack.
+// Closest hit over the TLAS, on the 4-wide trees. Depth-first over child
+// references (>= 0 a wide node, < 0 a leaf ~k): the hit children of a node
+// are ordered by entry distance with the five-comparator sorting network
+// (0,1) (2,3) (0,2) (1,3) (1,2), swapping only on a strictly smaller
+// distance, and pushed farthest first so the nearest is visited next.
 // Candidates are accepted if t < best, so the first found wins an exact tie;
 // near-ties are recorded for the parity tests.
 bool trace_closest(const orc_scene& s, const float o[3], const float d[3], float tmin, float tmax, uint32_t cullMask, Hit& h)
@@ -627,32 +725,42 @@ bool trace_closest(const orc_scene& s, const float o[3], const float d[3], float
     // that near-tie candidates are seen; acceptance still needs t < best.
     auto widen = [](float b) { return b * (1.0f + 8.0f * TIE_REL); };
 
-    auto visit = [&](const std::vector<CanonNode>& nodes, const RayPre& r, int32_t ref, int32_t* stack, int& sp) {
-        const CanonNode& nd = nodes[ref];
+    auto visit = [&](const std::vector<Node4>& nodes, const RayPre& r, int32_t ref, int32_t* stack, int& sp) {
+        const Node4& w = nodes[ref];
         h.nodes++;
-        float tn[2];
-        bool hc[2];
-        for(int c = 0; c < 2; ++c) hc[c] = nd.child[c] != EMPTY_CHILD && ray_box(r, nd.lo[c], nd.hi[c], tmin, widen(best), tn[c]);
-        if(hc[0] && hc[1]) {
-            int nearC = tn[1] < tn[0] ? 1 : 0;
-            stack[sp++] = nd.child[1 - nearC];
-            stack[sp++] = nd.child[nearC];
+        float tn[4];
+        int32_t rf[4];
+        for(int c = 0; c < 4; ++c) {
+            const float lo[3] = {w.lox[c], w.loy[c], w.loz[c]}, hi[3] = {w.hix[c], w.hiy[c], w.hiz[c]};
+            float t;
+            const bool hit = w.child[c] != EMPTY_CHILD && ray_box(r, lo, hi, tmin, widen(best), t);
+            tn[c] = hit ? t : std::numeric_limits<float>::infinity();
+            rf[c] = hit ? w.child[c] : EMPTY_CHILD;
         }
-        else if(hc[0]) {
-            stack[sp++] = nd.child[0];
-        }
-        else if(hc[1]) {
-            stack[sp++] = nd.child[1];
-        }
+        auto cx = [&](int i, int j) {
+            if(tn[j] < tn[i]) {
+                std::swap(tn[i], tn[j]);
+                std::swap(rf[i], rf[j]);
+            }
+        };
+        cx(0, 1);
+        cx(2, 3);
+        cx(0, 2);
+        cx(1, 3);
+        cx(1, 2);
+        for(int c = 3; c >= 0; --c)
+            if(rf[c] != EMPTY_CHILD) stack[sp++] = rf[c];
     };
 
-    int32_t tstack[128];
+    // A wide node pushes at most 3 entries and a tree is at most 63 levels
+    // deep, so 3 * 63 + 1 entries per level always suffice.
+    int32_t tstack[192];
     int tsp = 0;
     tstack[tsp++] = 0;
     while(tsp) {
         int32_t ref = tstack[--tsp];
         if(ref >= 0) {
-            visit(s.tlas.nodes, wr, ref, tstack, tsp);
+            visit(s.tlas.wide, wr, ref, tstack, tsp);
             continue;
         }
         uint32_t ii = s.tlas.leafPrim[~ref];
@@ -669,13 +777,13 @@ bool trace_closest(const orc_scene& s, const float o[3], const float d[3], float
         }
         RayPre r;
         ray_pre(r, oo, od);
-        int32_t bstack[128];
+        int32_t bstack[192];
         int bsp = 0;
         bstack[bsp++] = 0;
         while(bsp) {
             int32_t bref = bstack[--bsp];
             if(bref >= 0) {
-                visit(b.bvh.nodes, r, bref, bstack, bsp);
+                visit(b.bvh.wide, r, bref, bstack, bsp);
                 continue;
             }
             uint32_t prim = b.bvh.leafPrim[~bref];
@@ -1480,10 +1588,28 @@ int32_t orc_refit_blas(orc_scene* s, int32_t id)
     Blas& b = s->blas[id];
     load_tris(*s, b);
     compute_node_boxes(b.bvh);
+    collapse4(b.bvh); // the wide topology follows the refitted areas
     return 0;
 }
 
 uint32_t orc_blas_num_nodes(orc_scene* s, int32_t id) { return (uint32_t)s->blas[id].bvh.nodes.size(); }
+
+uint32_t orc_blas_num_wide(orc_scene* s, int32_t id) { return (uint32_t)s->blas[id].bvh.wide.size(); }
+
+int32_t orc_export_blas_wide(orc_scene* s, int32_t id, void* nodes)
+{
+    const BVH& b = s->blas[id].bvh;
+    std::memcpy(nodes, b.wide.data(), b.wide.size() * sizeof(Node4));
+    return 0;
+}
+
+uint32_t orc_tlas_num_wide(orc_scene* s) { return (uint32_t)s->tlas.wide.size(); }
+
+int32_t orc_export_tlas_wide(orc_scene* s, void* nodes)
+{
+    std::memcpy(nodes, s->tlas.wide.data(), s->tlas.wide.size() * sizeof(Node4));
+    return 0;
+}
 
 int32_t orc_export_blas(orc_scene* s, int32_t id, void* nodes, uint32_t* leafPrim)
 {

@@ -20,7 +20,9 @@ namespace {
 
 struct BlasRec {
     rgrt_blas_desc desc{};
-    Node* nodes = nullptr;
+    Node* nodes = nullptr;  // canonical binary tree
+    Node4* wide = nullptr;  // the 4-wide tree the traversal walks
+    uint32_t* numWide = nullptr;
     Tri* tris = nullptr;
     uint32_t* leafPrim = nullptr;
     bool valid = false;
@@ -78,6 +80,8 @@ struct rgrt_ctx {
     hipEvent_t stageDone = nullptr;
     rgrt_instance* dInstIn = nullptr;
     Node* tlasNodes = nullptr;
+    Node4* tlasWide = nullptr;
+    uint32_t* tlasNumWide = nullptr;
     uint32_t* tlasLeaf = nullptr;
     DevInstance* dInst = nullptr;
     InstanceShade* dShade = nullptr;
@@ -93,7 +97,32 @@ struct rgrt_ctx {
     Images img;
     WavefrontState st;
     int32_t* spill = nullptr;
-    uint32_t traceGrid = 0, shadeGrid = 0;
+    size_t spillBytes = 0;
+    uint32_t traceGrid = 0, shadeGrid = 0, drainGrid = 0;
+    rgrt_tuning tuning{4, 6, 8, 4}; // measured on config 2 (1080p, 2M triangles)
+    hipEvent_t evDrainStart = nullptr, evDrainEnd = nullptr;
+    bool lastDrained = false;
+
+    // Grids from the tuning; the traversal spill region covers the larger of
+    // the persistent traversal and drain grids (they never run at once).
+    void apply_tuning()
+    {
+        traceGrid = numCUs * tuning.traceBlocksPerCU;
+        shadeGrid = numCUs * tuning.shadeBlocksPerCU;
+        drainGrid = numCUs * tuning.drainBlocksPerCU;
+        const size_t need = std::max((size_t)traceGrid * trace_block_size() * trace_spill_words_per_thread(),
+                                     (size_t)drainGrid * drain_block_size() * drain_spill_words_per_thread()) *
+                            4;
+        if(need > spillBytes) {
+            if(spill) {
+                check(hipStreamSynchronize(stream));
+                check(hipFree(spill));
+                spill = nullptr;
+            }
+            check(hipMalloc((void**)&spill, need));
+            spillBytes = need;
+        }
+    }
 
     rgrt_ray* dRays = nullptr;
     rgrt_hit* dHits = nullptr;
@@ -115,7 +144,7 @@ struct rgrt_ctx {
     TraceScene scene() const
     {
         TraceScene sc;
-        sc.tlasNodes = tlasNodes;
+        sc.tlasNodes = tlasWide;
         sc.instances = dInst;
         sc.blas = dBlasTable;
         sc.numInstances = haveTlas ? numInstances : 0;
@@ -243,6 +272,8 @@ void ensure_inst_cap(rgrt_ctx* c, uint32_t n)
     check(hipHostMalloc((void**)&c->hInstStage, cap * sizeof(rgrt_instance), hipHostMallocDefault));
     dev_alloc(c->dInstIn, cap);
     dev_alloc(c->tlasNodes, cap);
+    dev_alloc(c->tlasWide, cap);
+    if(!c->tlasNumWide) dev_alloc(c->tlasNumWide, 1);
     dev_alloc(c->tlasLeaf, cap);
     dev_alloc(c->dInst, cap);
     dev_alloc(c->dShade, cap);
@@ -321,12 +352,25 @@ rgrt_status do_trace(rgrt_ctx* c)
     int cur = 0;
     uint32_t it = 0;
     uint32_t traceLaunches = 0;
+    bool drained = false;
     for(; it < maxIters; ++it) {
         // The count of iteration it-2's input queue is known by now (the GPU
         // runs ahead by at most two iterations); zero means every pixel is done.
         if(it >= 2) {
             check(hipEventSynchronize(c->iterEvents[(it - 2) % ring]));
             if(__atomic_load_n(&c->hostCounts[it - 2], __ATOMIC_ACQUIRE) == 0u) break;
+        }
+        if(it == c->tuning.wavefrontPasses) {
+            // the persistent drain finishes every pixel still in flight
+            if(timing) {
+                make_event(c->evDrainStart);
+                make_event(c->evDrainEnd);
+                check(hipEventRecord(c->evDrainStart, s));
+            }
+            launch_drain(s, p, sc, cur, c->drainGrid, c->spill);
+            if(timing) check(hipEventRecord(c->evDrainEnd, s));
+            drained = true;
+            break;
         }
         if(timing) {
             while(c->traceEvents.size() < 2 * (traceLaunches + 1)) {
@@ -352,7 +396,7 @@ rgrt_status do_trace(rgrt_ctx* c)
         ++traceLaunches;
         cur ^= 1;
     }
-    if(it == maxIters) {
+    if(!drained && it == maxIters) {
         // all pixels must be done after the bound; check the last count
         check(hipStreamSynchronize(s));
         uint32_t remaining = 0;
@@ -361,6 +405,7 @@ rgrt_status do_trace(rgrt_ctx* c)
     }
     if(timing) check(hipEventRecord(c->evRtOnlyEnd, s));
     c->lastTraceLaunches = traceLaunches;
+    c->lastDrained = drained;
     c->lastWaves = it;
     c->timedRt = timing;
     return RGRT_OK;
@@ -410,9 +455,7 @@ rgrt_status rgrt_create(rgrt_ctx** out, const rgrt_config* cfg)
         hipDeviceProp_t prop;
         check(hipGetDeviceProperties(&prop, c->device));
         c->numCUs = (uint32_t)std::max(1, prop.multiProcessorCount);
-        c->traceGrid = c->numCUs * 8;
-        c->shadeGrid = c->numCUs * 8;
-        check(hipMalloc((void**)&c->spill, (size_t)c->traceGrid * trace_block_size() * trace_spill_words_per_thread() * 4));
+        c->apply_tuning();
         check(hipEventCreateWithFlags(&c->stageDone, hipEventDisableTiming));
         if(cfg->width && cfg->height) alloc_images(c, cfg->width, cfg->height);
         return RGRT_OK;
@@ -433,6 +476,8 @@ void rgrt_destroy(rgrt_ctx* c)
     free_images(c);
     for(auto& b: c->blas) {
         dev_free(b.nodes);
+        dev_free(b.wide);
+        dev_free(b.numWide);
         dev_free(b.tris);
         dev_free(b.leafPrim);
     }
@@ -443,6 +488,8 @@ void rgrt_destroy(rgrt_ctx* c)
     dev_free(c->dBlasRoot);
     dev_free(c->dInstIn);
     dev_free(c->tlasNodes);
+    dev_free(c->tlasWide);
+    dev_free(c->tlasNumWide);
     dev_free(c->tlasLeaf);
     dev_free(c->dInst);
     dev_free(c->dShade);
@@ -457,7 +504,8 @@ void rgrt_destroy(rgrt_ctx* c)
     for(hipEvent_t e: c->iterEvents) (void)hipEventDestroy(e);
     for(hipEvent_t e: c->traceEvents) (void)hipEventDestroy(e);
     for(hipEvent_t e: c->shadeEvents) (void)hipEventDestroy(e);
-    for(hipEvent_t e: {c->evAsStart, c->evAsEnd, c->evRtStart, c->evRtOnlyEnd, c->evRoughEnd, c->evPostEnd, c->stageDone})
+    for(hipEvent_t e: {c->evAsStart, c->evAsEnd, c->evRtStart, c->evRtOnlyEnd, c->evRoughEnd, c->evPostEnd, c->stageDone,
+                       c->evDrainStart, c->evDrainEnd})
         if(e) (void)hipEventDestroy(e);
     if(c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -539,13 +587,16 @@ rgrt_status rgrt_build_blas(rgrt_ctx* c, const rgrt_blas_desc* d, rgrt_blas* out
         b.desc = *d;
         const uint32_t n = d->numFaces;
         dev_alloc(b.nodes, std::max<uint32_t>(n, 2) - 1);
+        dev_alloc(b.wide, std::max<uint32_t>(n, 2) - 1);
+        dev_alloc(b.numWide, 1);
         dev_alloc(b.tris, n);
         dev_alloc(b.leafPrim, n);
         c->builder.reserve(std::max<uint32_t>(n, 1));
         launch_tri_boxes(s, c->vb, c->ib, d->firstVertex, (uint32_t)io, n, c->builder.boxLo, c->builder.boxHi);
         c->builder.build(s, n, b.nodes, b.leafPrim, c->dBlasRoot + 2 * id);
+        c->builder.collapse(s, n, b.nodes, b.wide, b.numWide);
         launch_tri_gather(s, c->vb, c->ib, d->firstVertex, (uint32_t)io, b.leafPrim, n, b.tris);
-        DevBlas db{b.nodes, b.tris, n, 0};
+        DevBlas db{b.wide, b.tris, n, 0};
         check(hipMemcpyAsync(c->dBlasTable + id, &db, sizeof(DevBlas), hipMemcpyHostToDevice, s));
         check(hipStreamSynchronize(s)); // the reference blocks on a fence (raytracer.cpp:71-73)
         b.valid = true;
@@ -566,6 +617,7 @@ rgrt_status rgrt_refit_blas(rgrt_ctx* c, rgrt_blas id)
         c->builder.reserve(std::max<uint32_t>(n, 1));
         launch_tri_boxes(s, c->vb, c->ib, b.desc.firstVertex, io, n, c->builder.boxLo, c->builder.boxHi);
         c->builder.refit(s, n, b.nodes, b.leafPrim, c->dBlasRoot + 2 * id);
+        c->builder.collapse(s, n, b.nodes, b.wide, b.numWide); // wide topology follows the new areas
         launch_tri_gather(s, c->vb, c->ib, b.desc.firstVertex, io, b.leafPrim, n, b.tris);
         check(hipStreamSynchronize(s));
         return RGRT_OK;
@@ -611,6 +663,38 @@ rgrt_status rgrt_export_blas(rgrt_ctx* c, rgrt_blas id, rgrt_bvh_node* nodes, ui
     });
 }
 
+rgrt_status rgrt_export_blas_wide(rgrt_ctx* c, rgrt_blas id, void* nodes, uint32_t capacity, uint32_t* numNodes)
+{
+    return guarded(c, [&]() {
+        if(id >= c->blas.size() || !c->blas[id].valid) return fail(c, RGRT_ERR_INVALID_ARG, "export_blas_wide: bad handle");
+        check(hipStreamSynchronize(c->stream));
+        uint32_t nw = 0;
+        check(hipMemcpy(&nw, c->blas[id].numWide, 4, hipMemcpyDeviceToHost));
+        if(numNodes) *numNodes = nw;
+        if(nodes) {
+            if(capacity < nw) return fail(c, RGRT_ERR_INVALID_ARG, "export_blas_wide: capacity");
+            check(hipMemcpy(nodes, c->blas[id].wide, nw * sizeof(Node4), hipMemcpyDeviceToHost));
+        }
+        return RGRT_OK;
+    });
+}
+
+rgrt_status rgrt_export_tlas_wide(rgrt_ctx* c, void* nodes, uint32_t capacity, uint32_t* numNodes)
+{
+    return guarded(c, [&]() {
+        if(!c->haveTlas) return fail(c, RGRT_ERR_NOT_READY, "export_tlas_wide: no TLAS");
+        check(hipStreamSynchronize(c->stream));
+        uint32_t nw = 0;
+        check(hipMemcpy(&nw, c->tlasNumWide, 4, hipMemcpyDeviceToHost));
+        if(numNodes) *numNodes = nw;
+        if(nodes) {
+            if(capacity < nw) return fail(c, RGRT_ERR_INVALID_ARG, "export_tlas_wide: capacity");
+            check(hipMemcpy(nodes, c->tlasWide, nw * sizeof(Node4), hipMemcpyDeviceToHost));
+        }
+        return RGRT_OK;
+    });
+}
+
 rgrt_status rgrt_build_tlas(rgrt_ctx* c, const rgrt_instance* in, uint32_t n)
 {
     return guarded(c, [&]() {
@@ -636,6 +720,7 @@ rgrt_status rgrt_build_tlas(rgrt_ctx* c, const rgrt_instance* in, uint32_t n)
         c->builder.reserve(std::max<uint32_t>(n, 1));
         launch_inst_prep(s, c->dInstIn, n, c->dBlasRoot, c->builder.boxLo, c->builder.boxHi, c->dW2O, c->dShade);
         c->builder.build(s, n, c->tlasNodes, c->tlasLeaf, c->tlasRoot);
+        c->builder.collapse(s, n, c->tlasNodes, c->tlasWide, c->tlasNumWide);
         launch_inst_gather(s, c->dW2O, c->dInstIn, c->tlasLeaf, n, c->dInst);
         if(timing) check(hipEventRecord(c->evAsEnd, s));
         c->timedAs = timing;
@@ -816,6 +901,7 @@ rgrt_status rgrt_get_stats(rgrt_ctx* c, rgrt_stats* out)
         }
         out->waves = c->lastWaves;
         out->traceLaunches = c->lastTraceLaunches;
+        out->drained = c->lastDrained ? 1u : 0u;
         if(c->timedAs) out->asBuildMs = elapsed(c->evAsStart, c->evAsEnd);
         if(c->timedRt) {
             out->rtOnlyMs = elapsed(c->evRtStart, c->evRtOnlyEnd);
@@ -826,12 +912,46 @@ rgrt_status rgrt_get_stats(rgrt_ctx* c, rgrt_stats* out)
             }
             out->traceKernelMs = tk;
             out->shadeKernelMs = sk;
+            if(c->lastDrained) out->drainKernelMs = elapsed(c->evDrainStart, c->evDrainEnd);
         }
         if(c->timedPost) {
             out->roughMs = elapsed(c->evRtOnlyEnd, c->evRoughEnd);
             out->postprocMs = elapsed(c->evRtOnlyEnd, c->evPostEnd);
             out->rtTotalMs = elapsed(c->evRtStart, c->evPostEnd);
         }
+        return RGRT_OK;
+    });
+}
+
+rgrt_status rgrt_set_tuning(rgrt_ctx* c, const rgrt_tuning* t)
+{
+    return guarded(c, [&]() {
+        if(!t) return fail(c, RGRT_ERR_INVALID_ARG, "set_tuning: null");
+        for(uint32_t v: {t->traceBlocksPerCU, t->shadeBlocksPerCU, t->drainBlocksPerCU})
+            if(v < 1 || v > 32) return fail(c, RGRT_ERR_INVALID_ARG, "set_tuning: blocks per CU outside 1..32");
+        c->tuning = *t;
+        c->apply_tuning();
+        return RGRT_OK;
+    });
+}
+
+rgrt_status rgrt_get_tuning(rgrt_ctx* c, rgrt_tuning* t)
+{
+    return guarded(c, [&]() {
+        if(!t) return fail(c, RGRT_ERR_INVALID_ARG, "get_tuning: null");
+        *t = c->tuning;
+        return RGRT_OK;
+    });
+}
+
+rgrt_status rgrt_get_pass_counts(rgrt_ctx* c, uint32_t* counts, uint32_t capacity, uint32_t* count)
+{
+    return guarded(c, [&]() {
+        check(hipStreamSynchronize(c->stream));
+        const uint32_t n = std::min(c->lastTraceLaunches, c->hostCountsCap);
+        if(count) *count = n;
+        if(counts)
+            for(uint32_t i = 0; i < std::min(n, capacity); ++i) counts[i] = c->hostCounts[i];
         return RGRT_OK;
     });
 }

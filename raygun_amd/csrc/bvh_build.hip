@@ -413,6 +413,135 @@ __global__ void k_small_tree(const float4* __restrict__ boxLo, const float4* __r
 }
 
 // ---------------------------------------------------------------------------
+// 7. collapse to 4-wide nodes
+// ---------------------------------------------------------------------------
+struct WideEntry {
+    int32_t ref;
+    float lo[3], hi[3];
+};
+
+__device__ __forceinline__ WideEntry slot_of(const Node* nodes, int node, int c)
+{
+    const float* f = reinterpret_cast<const float*>(nodes + node);
+    WideEntry e;
+    e.ref = reinterpret_cast<const int32_t*>(nodes + node)[12 + c];
+    for(int k = 0; k < 3; ++k) {
+        e.lo[k] = f[3 * c + k];
+        e.hi[k] = f[6 + 3 * c + k];
+    }
+    return e;
+}
+
+__device__ __forceinline__ float half_area(const float lo[3], const float hi[3])
+{
+    const float dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+    return ((dx * dy) + (dy * dz)) + (dz * dx);
+}
+
+constexpr int kCollapseThreads = 1024;
+
+// One workgroup walks the binary tree breadth-first. A wide node starts from
+// its binary node's two children and opens the internal child of largest
+// surface area (the first on equal areas) until it has four children or only
+// leaves; its internal children become the next level's wide nodes, numbered
+// in order (a block-wide prefix sum over the per-node counts).
+__global__ __launch_bounds__(kCollapseThreads) void k_collapse(const Node* __restrict__ nodes, Node4* __restrict__ out,
+                                                               int32_t* frontA, int32_t* frontB, uint32_t* numWide)
+{
+    __shared__ int32_t scan[kCollapseThreads];
+    const int tid = threadIdx.x;
+    int32_t* front = frontA;
+    int32_t* next = frontB;
+    if(tid == 0) front[0] = 0;
+    __syncthreads();
+    int fcount = 1, base = 0;
+    for(int level = 0; level < 128 && fcount > 0; ++level) {
+        int nextCount = 0;
+        for(int chunk = 0; chunk < fcount; chunk += kCollapseThreads) {
+            const int k = chunk + tid;
+            const bool valid = k < fcount;
+            WideEntry S[4];
+            int ns = 0;
+            if(valid) {
+                const int r = front[k];
+                for(int c = 0; c < 2; ++c) {
+                    const WideEntry e = slot_of(nodes, r, c);
+                    if(e.ref != kEmptyChild) S[ns++] = e;
+                }
+                while(ns < 4) {
+                    int j = -1;
+                    float best = -1.0f;
+                    for(int q = 0; q < ns; ++q) {
+                        if(S[q].ref < 0) continue;
+                        const float a = half_area(S[q].lo, S[q].hi);
+                        if(a > best) {
+                            best = a;
+                            j = q;
+                        }
+                    }
+                    if(j < 0) break;
+                    const int e = S[j].ref;
+                    for(int q = ns; q > j + 1; --q) S[q] = S[q - 1];
+                    S[j] = slot_of(nodes, e, 0);
+                    S[j + 1] = slot_of(nodes, e, 1);
+                    ++ns;
+                }
+            }
+            int cnt = 0;
+            for(int q = 0; q < ns; ++q) cnt += S[q].ref >= 0 ? 1 : 0;
+            scan[tid] = cnt;
+            __syncthreads();
+            for(int off = 1; off < kCollapseThreads; off <<= 1) {
+                const int v = tid >= off ? scan[tid - off] : 0;
+                __syncthreads();
+                scan[tid] += v;
+                __syncthreads();
+            }
+            const int excl = scan[tid] - cnt;
+            const int total = scan[kCollapseThreads - 1];
+            if(valid) {
+                Node4 w;
+                for(int c = 0; c < 4; ++c) {
+                    w.lox[c] = w.loy[c] = w.loz[c] = INFINITY;
+                    w.hix[c] = w.hiy[c] = w.hiz[c] = -INFINITY;
+                    w.child[c] = kEmptyChild;
+                    w.pad[c] = 0;
+                }
+                int rank = 0;
+                for(int q = 0; q < ns; ++q) {
+                    w.lox[q] = S[q].lo[0];
+                    w.loy[q] = S[q].lo[1];
+                    w.loz[q] = S[q].lo[2];
+                    w.hix[q] = S[q].hi[0];
+                    w.hiy[q] = S[q].hi[1];
+                    w.hiz[q] = S[q].hi[2];
+                    if(S[q].ref >= 0) {
+                        const int idx = nextCount + excl + rank;
+                        w.child[q] = base + fcount + idx;
+                        next[idx] = S[q].ref;
+                        ++rank;
+                    }
+                    else {
+                        w.child[q] = S[q].ref;
+                    }
+                }
+                out[base + k] = w;
+            }
+            nextCount += total;
+            __syncthreads(); // scan[] and next[] settle before the next chunk / level
+        }
+        base += fcount;
+        fcount = nextCount;
+        int32_t* t = front;
+        front = next;
+        next = t;
+        __threadfence_block();
+        __syncthreads();
+    }
+    if(tid == 0) *numWide = (uint32_t)base;
+}
+
+// ---------------------------------------------------------------------------
 // leaf payloads
 // ---------------------------------------------------------------------------
 __global__ void k_tri_gather(const rgrt_vertex* __restrict__ vb, const uint32_t* __restrict__ ib, uint32_t firstVertex,
@@ -544,6 +673,15 @@ __global__ void k_leaf_parents(const Node* __restrict__ nodes, uint32_t numInter
     if(nd.child1 < 0 && nd.child1 != kEmptyChild) leafParent[~nd.child1] = i;
 }
 } // namespace
+
+void LbvhBuilder::collapse(hipStream_t s, uint32_t n, const Node* nodes, Node4* wide, uint32_t* numWide)
+{
+    // the sort's second key/value buffers are free after the build: they
+    // hold the two breadth-first frontiers (at most n - 1 entries each)
+    (void)n;
+    k_collapse<<<1, kCollapseThreads, 0, s>>>(nodes, wide, reinterpret_cast<int32_t*>(keysB), reinterpret_cast<int32_t*>(valsB),
+                                              numWide);
+}
 
 void LbvhBuilder::recompute_leaf_parents(hipStream_t s, uint32_t n, const Node* nodes)
 {

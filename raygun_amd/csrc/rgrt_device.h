@@ -33,6 +33,16 @@ struct __align__(16) Node {
 static_assert(sizeof(Node) == 64, "node is 64 bytes");
 static_assert(sizeof(Node) == sizeof(rgrt_bvh_node), "canonical node layout");
 
+// 4-wide node the traversal walks, 128 bytes: the per-axis bounds of the
+// four child boxes (one 16-byte vector each), then the four child refs
+// (>= 0 wide node, < 0 leaf ~k, kEmptyChild for an unused slot).
+struct __align__(16) Node4 {
+    float lox[4], hix[4], loy[4], hiy[4], loz[4], hiz[4];
+    int32_t child[4];
+    int32_t pad[4];
+};
+static_assert(sizeof(Node4) == 128, "wide node is 128 bytes");
+
 // Leaf triangle in Morton order: the three vertices and the original
 // primitive index (gl_PrimitiveID), 48 bytes.
 struct __align__(16) Tri {
@@ -56,7 +66,7 @@ struct __align__(16) DevInstance {
 static_assert(sizeof(DevInstance) == 64, "instance leaf is 64 bytes");
 
 struct DevBlas {
-    const Node* nodes;
+    const Node4* nodes;
     const Tri* tris;
     uint32_t numFaces;
     uint32_t pad;
@@ -72,7 +82,7 @@ static_assert(sizeof(InstanceShade) == 64, "shade record is 64 bytes");
 
 // Scene seen by the traversal.
 struct TraceScene {
-    const Node* tlasNodes;
+    const Node4* tlasNodes;
     const DevInstance* instances; // Morton order
     const DevBlas* blas;
     uint32_t numInstances;

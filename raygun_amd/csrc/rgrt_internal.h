@@ -37,6 +37,9 @@ struct LbvhBuilder {
     void build(hipStream_t s, uint32_t n, Node* nodes, uint32_t* leafPrimOut, float4* rootBox);
     void refit(hipStream_t s, uint32_t n, Node* nodes, const uint32_t* leafPrim, float4* rootBox);
     void recompute_leaf_parents(hipStream_t s, uint32_t n, const Node* nodes);
+    // Collapses the binary tree of n leaves into 4-wide nodes (breadth-first);
+    // writes the wide node count to *numWide (device).
+    void collapse(hipStream_t s, uint32_t n, const Node* nodes, Node4* wide, uint32_t* numWide);
 };
 
 void launch_tri_boxes(hipStream_t s, const rgrt_vertex* vb, const uint32_t* ib, uint32_t firstVertex, uint32_t indexOffset,
@@ -109,6 +112,9 @@ struct ShadeParams {
 void launch_raygen(hipStream_t s, const ShadeParams& p);
 void launch_trace_queue(hipStream_t s, const TraceScene& sc, const WavefrontState& st, int cur, uint32_t gridBlocks);
 void launch_shade(hipStream_t s, const ShadeParams& p, int cur, uint32_t gridBlocks);
+void launch_drain(hipStream_t s, const ShadeParams& p, const TraceScene& sc, int cur, uint32_t gridBlocks, int32_t* spill);
+uint32_t drain_spill_words_per_thread();
+uint32_t drain_block_size();
 void launch_trace_rays(hipStream_t s, const TraceScene& sc, const rgrt_ray* rays, uint32_t n, uint32_t cullMask, rgrt_hit* hits);
 void launch_post(hipStream_t s, const Images& img, const rgrt_ubo& ubo, int useFxaa, hipEvent_t roughEnd);
 

@@ -11,7 +11,7 @@ namespace {
 
 constexpr int kTraceBlock = 256;
 constexpr int kLdsStack = 16;
-constexpr int kSpill = 112; // LBVH depth <= 62 per level, two levels + sentinel < 128
+constexpr int kSpill = 368; // with the 16 LDS entries: the 381-entry worst case of two wide LBVH levels
 
 // Wavefront trace pass: rays [0, count) of queue `cur` -> hit records.
 // Waves fetch 64 rays at a time from a global counter (persistent grid).
@@ -33,10 +33,13 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_queue(TraceScene sc, cons
     const uint32_t gtid = blockIdx.x * kTraceBlock + threadIdx.x;
     TraversalStack<kLdsStack, kTraceBlock, kSpill> stack{ldsStack, spillBase + (size_t)gtid * kSpill, (int)threadIdx.x};
     uint32_t* fetch = counters + 2 + cur;
+    // Wave w first takes rays [64w, 64w + 64); rays beyond the grid's static
+    // share are fetched 64 at a time from a counter, so a pass with fewer
+    // rays than the grid has lanes does no atomics at all.
+    const uint32_t wavesInGrid = gridDim.x * (kTraceBlock / 64);
+    const uint32_t dynStart = wavesInGrid * 64u;
+    uint32_t base = (blockIdx.x * (kTraceBlock / 64) + (threadIdx.x >> 6)) * 64u;
     for(;;) {
-        uint32_t base = 0;
-        if(lane == 0) base = atomicAdd(fetch, 64u);
-        base = __shfl(base, 0);
         if(base >= count) break;
         const uint32_t i = base + lane;
         if(i < count) {
@@ -57,6 +60,10 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_queue(TraceScene sc, cons
             hits[3 * (size_t)npix + i] = __uint_as_float(h.inst);
             hits[4 * (size_t)npix + i] = __uint_as_float(h.prim);
         }
+        if(count <= dynStart) break; // no rays beyond the static share
+        uint32_t b = 0;
+        if(lane == 0) b = dynStart + atomicAdd(fetch, 64u);
+        base = __shfl(b, 0);
     }
 }
 

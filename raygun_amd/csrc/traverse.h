@@ -6,16 +6,18 @@
 // raygun/render/acceleration_structure.cpp:41), instance mask test
 // (mask & cullMask), t measured along the unnormalised world ray.
 //
-// Depth-first over child references (>= 0 internal node, < 0 leaf ~k). At a
-// node whose two children are hit, the nearer is visited next (child 0 on an
-// equal entry distance) and the farther waits on the stack. An instance leaf
-// maps the ray to object space, pushes a sentinel and continues at the BLAS
-// root; popping the sentinel restores the world ray. A candidate is accepted
-// when tmin <= t < tbest, so the first triangle found wins an exact tie.
+// Depth-first over the 4-wide trees (child refs >= 0 wide node, < 0 leaf ~k):
+// the hit children of a node are ordered by entry distance, the nearest is
+// visited next and the others wait on the stack. An instance leaf maps the
+// ray to object space, pushes a sentinel and continues at the BLAS root;
+// popping the sentinel restores the world ray. A candidate is accepted when
+// tmin <= t < tbest, so the first triangle found wins an exact tie.
 //
 // The stack lives in LDS (kLdsStack entries per thread, lane-interleaved so
 // a wave's pushes hit distinct banks) and spills to a per-thread global
-// region beyond that depth.
+// region beyond that depth. A wide node pushes at most 3 entries and an
+// LBVH is at most 63 levels deep, so two levels need at most
+// 2 x (3 x 63 + 1) + 1 = 381 entries: kLdsStack + spill = 384.
 #pragma once
 
 #include "rgrt_device.h"
@@ -76,6 +78,58 @@ __device__ __forceinline__ void box_test2(const BoxRay& r, const float4 a, const
     h1 = n1 <= f1 * 1.0000004f;
     tn0 = n0;
     tn1 = n1;
+}
+
+// Slab test of one child box of a wide node (same arithmetic as box_test2).
+__device__ __forceinline__ float box_enter(const BoxRay& r, float lx, float hx, float ly, float hy, float lz, float hz, float tmin,
+                                           float tmax)
+{
+    const float x0 = __builtin_fmaf(lx, r.ix, r.nox), x1 = __builtin_fmaf(hx, r.ix, r.nox);
+    const float y0 = __builtin_fmaf(ly, r.iy, r.noy), y1 = __builtin_fmaf(hy, r.iy, r.noy);
+    const float z0 = __builtin_fmaf(lz, r.iz, r.noz), z1 = __builtin_fmaf(hz, r.iz, r.noz);
+    const float n = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), tmin));
+    const float f = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tmax));
+    return n <= f * 1.0000004f ? n : INFINITY; // entry distance, +inf on a miss
+}
+
+__device__ __forceinline__ void cswap(float& ta, int32_t& ra, float& tb, int32_t& rb)
+{
+    const bool sw = tb < ta;
+    const float t = sw ? tb : ta;
+    const int32_t r = sw ? rb : ra;
+    tb = sw ? ta : tb;
+    rb = sw ? ra : rb;
+    ta = t;
+    ra = r;
+}
+
+// Visits a wide node: tests the four child boxes, orders the hit children by
+// entry distance with the comparator network (0,1) (2,3) (0,2) (1,3) (1,2)
+// (a swap only on a strictly smaller distance), pushes all but the nearest
+// farthest first and returns the nearest (kEmptyChild if none was hit).
+template<typename Stack>
+__device__ __forceinline__ int32_t visit_wide(const Node4* nodes, int32_t ref, const BoxRay& br, float tmin, float tbest, Stack& stack)
+{
+    const float4* np = reinterpret_cast<const float4*>(nodes + ref);
+    const float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
+    const int4 ch = reinterpret_cast<const int4*>(np)[6];
+    float t0 = ch.x != kEmptyChild ? box_enter(br, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, tmin, tbest) : INFINITY;
+    float t1 = ch.y != kEmptyChild ? box_enter(br, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, tmin, tbest) : INFINITY;
+    float t2 = ch.z != kEmptyChild ? box_enter(br, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, tmin, tbest) : INFINITY;
+    float t3 = ch.w != kEmptyChild ? box_enter(br, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, tmin, tbest) : INFINITY;
+    int32_t r0 = t0 != INFINITY ? ch.x : kEmptyChild;
+    int32_t r1 = t1 != INFINITY ? ch.y : kEmptyChild;
+    int32_t r2 = t2 != INFINITY ? ch.z : kEmptyChild;
+    int32_t r3 = t3 != INFINITY ? ch.w : kEmptyChild;
+    cswap(t0, r0, t1, r1);
+    cswap(t2, r2, t3, r3);
+    cswap(t0, r0, t2, r2);
+    cswap(t1, r1, t3, r3);
+    cswap(t1, r1, t2, r2);
+    if(r3 != kEmptyChild) stack.push(r3);
+    if(r2 != kEmptyChild) stack.push(r2);
+    if(r1 != kEmptyChild) stack.push(r1);
+    return r0;
 }
 
 // Per-ray setup of the watertight test (Woop, Benthin, Wald 2013).
@@ -166,9 +220,9 @@ struct TraversalStack {
     int32_t* spill; // this thread's global overflow region (kSpillWords)
     int tid;
     int sp = 0;
-    // A two-level LBVH is at most 2 x 63 levels deep plus the sentinel, so
-    // kLdsStack + kSpillWords >= 128 never drops an entry; the bound check
-    // only keeps every access inside the allocation.
+    // kLdsStack + kSpillWords >= 384 covers the deepest two-level tree, so
+    // no entry is ever dropped; the bound check only keeps every access
+    // inside the allocation.
     __device__ __forceinline__ void push(int32_t v)
     {
         if(sp < kLdsStack) lds[sp * kBlock + tid] = v;
@@ -211,7 +265,7 @@ __device__ HitRec traverse(const TraceScene& sc, const RayIn& ray, uint32_t cull
     BoxRay br;
     box_ray_setup(br, ray.ox, ray.oy, ray.oz, ray.dx, ray.dy, ray.dz);
     WtRay wt;
-    const Node* nodes = sc.tlasNodes;
+    const Node4* nodes = sc.tlasNodes;
     const Tri* tris = nullptr;
     bool inBlas = false;
     uint32_t curInst = 0;
@@ -224,24 +278,9 @@ __device__ HitRec traverse(const TraceScene& sc, const RayIn& ray, uint32_t cull
     // two-level tree ends far below this cap; it only bounds the loop.
     for(uint32_t guard = 0; guard < (1u << 26); ++guard) {
         if(ref >= 0) {
-            const float4* np = reinterpret_cast<const float4*>(nodes + ref);
-            const float4 a = np[0], b = np[1], c = np[2];
-            const int4 d = reinterpret_cast<const int4*>(np)[3];
-            bool h0, h1;
-            float tn0, tn1;
-            box_test2(br, a, b, c, tmin, tbest, h0, h1, tn0, tn1);
-            h0 = h0 && d.x != kEmptyChild;
-            h1 = h1 && d.y != kEmptyChild;
-            if(h0 && h1) {
-                const bool near1 = tn1 < tn0;
-                stack.push(near1 ? d.x : d.y);
-                ref = near1 ? d.y : d.x;
-            }
-            else if(h0) {
-                ref = d.x;
-            }
-            else if(h1) {
-                ref = d.y;
+            const int32_t nearest = visit_wide(nodes, ref, br, tmin, tbest, stack);
+            if(nearest != kEmptyChild) {
+                ref = nearest;
             }
             else {
                 if(stack.sp == 0) break;

@@ -26,6 +26,7 @@
 #pragma clang fp contract(off)
 
 #include "rgrt_internal.h"
+#include "traverse.h"
 
 namespace rgrt {
 
@@ -590,6 +591,87 @@ __global__ __launch_bounds__(kShadeBlock) void k_raygen(ShadeParams p)
     write_ray(p.st.rays[0], p.st.npix, pix, r, pix, kFlagPrimary | (0xffu << kMaskShift));
 }
 
+// One closest-hit or miss stage for a traced ray of pixel `pix`, then the
+// continuations it returns into. Returns true with `out` set when the pixel
+// issues its next ray; false once its last sample is accumulated and the
+// outputs are written.
+__device__ bool advance_pixel(const Ctx& c, uint32_t pix, uint32_t flags, F3 O, F3 D, const HitRec& h, Payload& P, int& sp,
+                              int& sample, OutRay& out)
+{
+    const ShadeParams& p = c.p;
+    unsigned long long* skyCount = reinterpret_cast<unsigned long long*>(p.st.counters + 6);
+    unsigned long long* zeroCount = reinterpret_cast<unsigned long long*>(p.st.counters + 8);
+    if((flags & kFlagPrimary) && p.keepAov) {
+        rgrt_hit a;
+        const bool hit = h.inst != kNoHit;
+        a.t = hit ? h.t : 0.0f;
+        a.u = hit ? h.u : 0.0f;
+        a.v = hit ? h.v : 0.0f;
+        a.instance = hit ? p.inst[min(h.inst, p.numInstances - 1)].customIndex : kNoHit;
+        a.primitive = hit ? h.prim : 0u;
+        p.img.primary[pix] = a;
+    }
+    Frame F;
+    int step;
+    if(h.inst != kNoHit) {
+        step = closest_hit(c, P, h.inst, h.prim, h.t, h.u, h.v, O, D, F, out);
+    }
+    else {
+        run_miss(c, P, D, flags & kFlagMiss1);
+        step = RET;
+    }
+    for(;;) {
+        if(step == EMIT_FRAME) {
+            if(sp < kMaxFrames) store_frame(p, pix, sp, F);
+            sp++;
+            step = EMIT;
+        }
+        if(step == EMIT) {
+            if(out.mask == 0u) {
+                atomicAdd(skyCount, 1ull);
+                run_miss(c, P, out.d, out.miss1);
+                step = RET;
+            }
+            else if(out.d.x == 0.0f && out.d.y == 0.0f && out.d.z == 0.0f) {
+                atomicAdd(zeroCount, 1ull);
+                run_miss(c, P, out.d, out.miss1);
+                step = RET;
+            }
+            else {
+                return true;
+            }
+        }
+        // step == RET: the innermost call returned
+        if(sp == 0) {
+            accumulate(p, pix, P);
+            sample++;
+            if(sample < p.ubo.numSamples) {
+                init_payload(P);
+                primary_ray(p.ubo, pix % p.width, pix / p.width, p.width, p.height, sample, out);
+                step = EMIT;
+                continue;
+            }
+            finalize(p, pix);
+            return false;
+        }
+        sp--;
+        if(sp < kMaxFrames) load_frame(p, pix, sp, F);
+        step = resume(c, F, P, out);
+    }
+}
+
+__device__ __forceinline__ HitRec load_hit(const float* hits, size_t n, uint32_t i)
+{
+    HitRec h;
+    h.t = hits[0 * n + i];
+    h.u = hits[1 * n + i];
+    h.v = hits[2 * n + i];
+    h.inst = __float_as_uint(hits[3 * n + i]);
+    h.prim = __float_as_uint(hits[4 * n + i]);
+    return h;
+}
+
+// Wavefront shading pass over the rays traced by the preceding traversal pass.
 __global__ __launch_bounds__(kShadeBlock) void k_shade(ShadeParams p, int cur)
 {
     const uint32_t count = p.st.counters[cur];
@@ -597,8 +679,6 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(ShadeParams p, int cur)
     const float* __restrict__ rays = p.st.rays[cur];
     float* __restrict__ outRays = p.st.rays[1 - cur];
     uint32_t* outCount = p.st.counters + (1 - cur);
-    unsigned long long* skyCount = reinterpret_cast<unsigned long long*>(p.st.counters + 6);
-    unsigned long long* zeroCount = reinterpret_cast<unsigned long long*>(p.st.counters + 8);
     const int lane = threadIdx.x & 63;
     const uint64_t ltMask = lane == 0 ? 0ull : ((~0ull) >> (64 - lane));
     const Ctx c{p, f3(p.ubo.lightDir[0], p.ubo.lightDir[1], p.ubo.lightDir[2]), p.ubo.maxRecursions};
@@ -614,75 +694,17 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(ShadeParams p, int cur)
             const uint32_t flags = __float_as_uint(rays[9 * n + i]);
             const F3 O = f3(rays[0 * n + i], rays[1 * n + i], rays[2 * n + i]);
             const F3 D = f3(rays[4 * n + i], rays[5 * n + i], rays[6 * n + i]);
-            const float ht = p.st.hits[0 * n + i];
-            const float hu = p.st.hits[1 * n + i];
-            const float hv = p.st.hits[2 * n + i];
-            const uint32_t hinst = __float_as_uint(p.st.hits[3 * n + i]);
-            const uint32_t hprim = __float_as_uint(p.st.hits[4 * n + i]);
-            if((flags & kFlagPrimary) && p.keepAov) {
-                rgrt_hit a;
-                a.t = hinst != kNoHit ? ht : 0.0f;
-                a.u = hinst != kNoHit ? hu : 0.0f;
-                a.v = hinst != kNoHit ? hv : 0.0f;
-                a.instance = hinst != kNoHit ? p.inst[min(hinst, p.numInstances - 1)].customIndex : kNoHit;
-                a.primitive = hinst != kNoHit ? hprim : 0u;
-                p.img.primary[pix] = a;
-            }
+            const HitRec h = load_hit(p.st.hits, n, i);
             Payload P;
             load_payload(p, pix, P);
-            uint32_t meta = p.st.meta[pix];
+            const uint32_t meta = p.st.meta[pix];
             int sp = (int)(meta & 0xffu);
             int sample = (int)(meta >> 8);
-            Frame F;
-            int step;
-            if(hinst != kNoHit) {
-                step = closest_hit(c, P, hinst, hprim, ht, hu, hv, O, D, F, out);
+            emit = advance_pixel(c, pix, flags, O, D, h, P, sp, sample, out);
+            if(emit) {
+                store_payload(p, pix, P);
+                p.st.meta[pix] = (uint32_t)sp | ((uint32_t)sample << 8);
             }
-            else {
-                run_miss(c, P, D, flags & kFlagMiss1);
-                step = RET;
-            }
-            for(;;) {
-                if(step == EMIT_FRAME) {
-                    if(sp < kMaxFrames) store_frame(p, pix, sp, F);
-                    sp++;
-                    step = EMIT;
-                }
-                if(step == EMIT) {
-                    if(out.mask == 0u) {
-                        atomicAdd(skyCount, 1ull);
-                        run_miss(c, P, out.d, out.miss1);
-                        step = RET;
-                    }
-                    else if(out.d.x == 0.0f && out.d.y == 0.0f && out.d.z == 0.0f) {
-                        atomicAdd(zeroCount, 1ull);
-                        run_miss(c, P, out.d, out.miss1);
-                        step = RET;
-                    }
-                    else {
-                        emit = true;
-                        break;
-                    }
-                }
-                // step == RET: the innermost call returned
-                if(sp == 0) {
-                    accumulate(p, pix, P);
-                    sample++;
-                    if(sample < p.ubo.numSamples) {
-                        init_payload(P);
-                        primary_ray(p.ubo, pix % p.width, pix / p.width, p.width, p.height, sample, out);
-                        step = EMIT;
-                        continue;
-                    }
-                    finalize(p, pix);
-                    break;
-                }
-                sp--;
-                if(sp < kMaxFrames) load_frame(p, pix, sp, F);
-                step = resume(c, F, P, out);
-            }
-            store_payload(p, pix, P);
-            p.st.meta[pix] = (uint32_t)sp | ((uint32_t)sample << 8);
         }
         // append the emitted rays to the next queue, one atomic per wave
         const uint64_t m = __ballot(emit);
@@ -693,6 +715,92 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(ShadeParams p, int cur)
             if(emit) write_ray(outRays, n, wbase + (uint32_t)__popcll(m & ltMask), out, pix, (out.mask << kMaskShift) | out.miss1);
         }
     }
+}
+
+constexpr int kDrainBlock = 256;
+constexpr int kDrainLdsStack = 16;
+constexpr int kDrainSpill = 368; // see traverse.h: 16 + 368 >= 381
+
+// Drain: finishes every pixel still in flight after the wavefront passes.
+// Each lane takes a queued ray, traces and shades it, and keeps tracing that
+// pixel's next rays itself until the pixel is done; idle lanes refill from
+// the queue (one atomic per wave, a ballot to hand out the entries). The
+// recursion tail thus costs the longest remaining chain of one pixel rather
+// than one launch per recursion step.
+__global__ __launch_bounds__(kDrainBlock) void k_drain(ShadeParams p, TraceScene sc, int cur, int32_t* __restrict__ spillBase)
+{
+    __shared__ int32_t ldsStack[kDrainLdsStack * kDrainBlock];
+    const uint32_t count = p.st.counters[cur];
+    const size_t n = p.st.npix;
+    const float* __restrict__ rays = p.st.rays[cur];
+    uint32_t* fetch = p.st.counters + 2 + cur;
+    const int lane = threadIdx.x & 63;
+    const uint64_t ltMask = lane == 0 ? 0ull : ((~0ull) >> (64 - lane));
+    const Ctx c{p, f3(p.ubo.lightDir[0], p.ubo.lightDir[1], p.ubo.lightDir[2]), p.ubo.maxRecursions};
+    const uint32_t gtid = blockIdx.x * kDrainBlock + threadIdx.x;
+    TraversalStack<kDrainLdsStack, kDrainBlock, kDrainSpill> stack{ldsStack, spillBase + (size_t)gtid * kDrainSpill,
+                                                                   (int)threadIdx.x};
+    bool active = false;
+    bool exhausted = false;
+    uint32_t pix = 0, flags = 0;
+    RayIn r;
+    Payload P;
+    int sp = 0, sample = 0;
+    uint64_t traced = 0;
+    for(;;) {
+        if(!exhausted) {
+            const uint64_t idle = __ballot(!active);
+            if(idle) {
+                uint32_t b = 0;
+                if(lane == 0) b = atomicAdd(fetch, (uint32_t)__popcll(idle));
+                b = __shfl(b, 0);
+                if(b + (uint32_t)__popcll(idle) >= count) exhausted = true;
+                if(!active) {
+                    const uint32_t i = b + (uint32_t)__popcll(idle & ltMask);
+                    if(i < count) {
+                        pix = __float_as_uint(rays[8 * n + i]);
+                        flags = __float_as_uint(rays[9 * n + i]);
+                        r.ox = rays[0 * n + i];
+                        r.oy = rays[1 * n + i];
+                        r.oz = rays[2 * n + i];
+                        r.tmin = rays[3 * n + i];
+                        r.dx = rays[4 * n + i];
+                        r.dy = rays[5 * n + i];
+                        r.dz = rays[6 * n + i];
+                        r.tmax = rays[7 * n + i];
+                        load_payload(p, pix, P);
+                        const uint32_t meta = p.st.meta[pix];
+                        sp = (int)(meta & 0xffu);
+                        sample = (int)(meta >> 8);
+                        active = true;
+                    }
+                }
+            }
+        }
+        if(!__ballot(active)) break;
+        if(active) {
+            const HitRec h = traverse(sc, r, (flags >> kMaskShift) & 0xffu, stack);
+            ++traced;
+            OutRay out;
+            if(advance_pixel(c, pix, flags, f3(r.ox, r.oy, r.oz), f3(r.dx, r.dy, r.dz), h, P, sp, sample, out)) {
+                r.ox = out.o.x;
+                r.oy = out.o.y;
+                r.oz = out.o.z;
+                r.tmin = out.tmin;
+                r.dx = out.d.x;
+                r.dy = out.d.y;
+                r.dz = out.d.z;
+                r.tmax = out.tmax;
+                flags = (out.mask << kMaskShift) | out.miss1;
+            }
+            else {
+                active = false;
+            }
+        }
+    }
+    // rays traced here count like the wavefront passes' rays (counters[4..5])
+    for(int off = 32; off > 0; off >>= 1) traced += __shfl_xor(traced, off);
+    if(lane == 0 && traced) atomicAdd(reinterpret_cast<unsigned long long*>(p.st.counters + 4), (unsigned long long)traced);
 }
 
 } // namespace
@@ -706,6 +814,14 @@ void launch_raygen(hipStream_t s, const ShadeParams& p)
 void launch_shade(hipStream_t s, const ShadeParams& p, int cur, uint32_t gridBlocks)
 {
     k_shade<<<gridBlocks, kShadeBlock, 0, s>>>(p, cur);
+}
+
+uint32_t drain_spill_words_per_thread() { return kDrainSpill; }
+uint32_t drain_block_size() { return kDrainBlock; }
+
+void launch_drain(hipStream_t s, const ShadeParams& p, const TraceScene& sc, int cur, uint32_t gridBlocks, int32_t* spill)
+{
+    k_drain<<<gridBlocks, kDrainBlock, 0, s>>>(p, sc, cur, spill);
 }
 
 } // namespace rgrt

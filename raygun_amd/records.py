@@ -48,7 +48,14 @@ STATS_FIELDS = [
     ("pad_", "<u4"),
     ("rays", "<u8"), ("skyLookups", "<u8"), ("zeroDirRays", "<u8"),
     ("waves", "<u4"), ("traceLaunches", "<u4"), ("traceKernelMs", "<f4"), ("shadeKernelMs", "<f4"),
+    ("drainKernelMs", "<f4"), ("drained", "<u4"),
 ]
+
+TUNING = np.dtype([("wavefrontPasses", "<u4"), ("traceBlocksPerCU", "<u4"), ("shadeBlocksPerCU", "<u4"),
+                   ("drainBlocksPerCU", "<u4")])
+
+WIDE_NODE = np.dtype([("lox", "<f4", 4), ("hix", "<f4", 4), ("loy", "<f4", 4), ("hiy", "<f4", 4), ("loz", "<f4", 4),
+                      ("hiz", "<f4", 4), ("child", "<i4", 4), ("pad", "<i4", 4)])
 
 NO_HIT = 0xFFFFFFFF
 EMPTY_CHILD = -(2 ** 31)
@@ -60,6 +67,7 @@ assert INSTANCE.itemsize == 80
 assert RAY.itemsize == 32
 assert HIT.itemsize == 20
 assert BVH_NODE.itemsize == 64
+assert WIDE_NODE.itemsize == 128
 
 
 def default_material(**kw):

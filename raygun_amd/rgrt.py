@@ -8,7 +8,7 @@ import os
 
 import numpy as np
 
-from .records import BVH_NODE, HIT, INSTANCE, MATERIAL, RAY, STATS_FIELDS, UBO, VERTEX
+from .records import BVH_NODE, HIT, INSTANCE, MATERIAL, RAY, STATS_FIELDS, TUNING, UBO, VERTEX, WIDE_NODE
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "librgrt.so")
@@ -29,7 +29,8 @@ EXPORTS = [
     "rgrt_build_blas", "rgrt_refit_blas", "rgrt_blas_info", "rgrt_export_blas",
     "rgrt_build_tlas", "rgrt_export_tlas", "rgrt_set_frame", "rgrt_resize",
     "rgrt_trace_rays", "rgrt_trace_rays_device", "rgrt_render", "rgrt_trace", "rgrt_postprocess",
-    "rgrt_read_image", "rgrt_image_device_ptr", "rgrt_write_image", "rgrt_get_stats", "rgrt_version",
+    "rgrt_read_image", "rgrt_image_device_ptr", "rgrt_write_image", "rgrt_get_stats", "rgrt_get_pass_counts",
+    "rgrt_set_tuning", "rgrt_get_tuning", "rgrt_export_blas_wide", "rgrt_export_tlas_wide", "rgrt_version",
 ]
 
 
@@ -49,7 +50,7 @@ class _BlasDesc(ctypes.Structure):
 
 
 STATS = np.dtype(STATS_FIELDS)
-assert STATS.itemsize == 64
+assert STATS.itemsize == 72
 
 _lib = None
 
@@ -88,6 +89,11 @@ def lib():
             "rgrt_image_device_ptr": (i32, [P, i32, ctypes.POINTER(P), ctypes.POINTER(u64)]),
             "rgrt_write_image": (i32, [P, i32, P, u64]),
             "rgrt_get_stats": (i32, [P, P]),
+            "rgrt_get_pass_counts": (i32, [P, P, u32, ctypes.POINTER(u32)]),
+            "rgrt_set_tuning": (i32, [P, P]),
+            "rgrt_export_blas_wide": (i32, [P, u32, P, u32, ctypes.POINTER(u32)]),
+            "rgrt_export_tlas_wide": (i32, [P, P, u32, ctypes.POINTER(u32)]),
+            "rgrt_get_tuning": (i32, [P, P]),
             "rgrt_version": (ctypes.c_char_p, []),
         }
         for name, (res, args) in sig.items():
@@ -175,6 +181,20 @@ class Context:
         self._ok(self._L.rgrt_export_blas(self.h, b, _p(nodes), nn, _p(leaf), nl))
         return nodes, leaf
 
+    def export_blas_wide(self, b):
+        nn = ctypes.c_uint32()
+        self._ok(self._L.rgrt_export_blas_wide(self.h, b, None, 0, ctypes.byref(nn)))
+        nodes = np.zeros(nn.value, WIDE_NODE)
+        self._ok(self._L.rgrt_export_blas_wide(self.h, b, _p(nodes), nn.value, ctypes.byref(nn)))
+        return nodes
+
+    def export_tlas_wide(self):
+        nn = ctypes.c_uint32()
+        self._ok(self._L.rgrt_export_tlas_wide(self.h, None, 0, ctypes.byref(nn)))
+        nodes = np.zeros(nn.value, WIDE_NODE)
+        self._ok(self._L.rgrt_export_tlas_wide(self.h, _p(nodes), nn.value, ctypes.byref(nn)))
+        return nodes
+
     def build_tlas(self, instances):
         inst = np.ascontiguousarray(instances, INSTANCE)
         self._ok(self._L.rgrt_build_tlas(self.h, _p(inst), len(inst)))
@@ -245,6 +265,25 @@ class Context:
         s = np.zeros((), STATS)
         self._ok(self._L.rgrt_get_stats(self.h, _p(s)))
         return {k: s[k].item() for k in STATS.names if not k.startswith("pad")}
+
+    def tuning(self):
+        t = np.zeros((), TUNING)
+        self._ok(self._L.rgrt_get_tuning(self.h, _p(t)))
+        return {k: int(t[k]) for k in TUNING.names}
+
+    def set_tuning(self, **kw):
+        t = np.zeros((), TUNING)
+        self._ok(self._L.rgrt_get_tuning(self.h, _p(t)))
+        for k, v in kw.items():
+            t[k] = v
+        self._ok(self._L.rgrt_set_tuning(self.h, _p(t)))
+
+    def pass_counts(self):
+        n = ctypes.c_uint32()
+        self._ok(self._L.rgrt_get_pass_counts(self.h, None, 0, ctypes.byref(n)))
+        out = np.zeros(n.value, np.uint32)
+        self._ok(self._L.rgrt_get_pass_counts(self.h, _p(out), n.value, ctypes.byref(n)))
+        return out
 
 
 def version():

@@ -70,6 +70,24 @@ def test_blas_and_tlas_match_oracle_config2_small(c2s):
     _nodes_equal(g_nodes, o_nodes)
 
 
+def _wide_equal(a, b):
+    assert len(a) == len(b)
+    assert np.array_equal(a["child"], b["child"])
+    for k in ("lox", "hix", "loy", "hiy", "loz", "hiz"):
+        assert np.array_equal(a[k], b[k]), k
+
+
+def test_wide_trees_match_oracle(c1, c2s):
+    _print_case("wide-trees")
+    setup, osc, ctx = c1
+    _wide_equal(ctx.export_blas_wide(0), osc.export_blas_wide(0))
+    _wide_equal(ctx.export_tlas_wide(), osc.export_tlas_wide())
+    setup, osc, ctx = c2s
+    for b in range(5):
+        _wide_equal(ctx.export_blas_wide(b), osc.export_blas_wide(b))
+    _wide_equal(ctx.export_tlas_wide(), osc.export_tlas_wide())
+
+
 @pytest.mark.parametrize("which", ["config1", "config2s"])
 def test_ray_queries_match_oracle(which, c1, c2s):
     setup, osc, ctx = c1 if which == "config1" else c2s
