@@ -155,14 +155,17 @@ def main():
         k = id(m.mesh)
         if k not in tb:
             tb[k] = tctx.build_blas(m.mesh.first_vertex, m.mesh.index_offset_bytes, m.mesh.num_faces(), len(m.mesh.vertices))
+    tctx.set_tuning(**ctx.tuning())
     tctx.build_tlas(instances)
     tctx.set_frame(ubo)
     tctx.render(use_fxaa)
     tctx.build_tlas(instances)
     tctx.render(use_fxaa)
     ts = tctx.stats()
+    passes = tctx.pass_counts()
     tctx.close()
     log(f"[bench] stages: {json.dumps({k: round(v, 4) if isinstance(v, float) else v for k, v in ts.items()})}")
+    log(f"[bench] wavefront pass ray counts: {passes.tolist()}")
 
     # roofline of the traversal kernel: algorithmic bytes per ray from the
     # oracle's visit counts on the same trees (a seeded band of rows)
@@ -183,25 +186,41 @@ def main():
         rows = list(range(13, H, max(1, H // 40)))
         vis = osc.render_counts(ubo, W, H, rows, threads=cores)
         e_nodes, e_tris, p_hit = vis["nodes"] / vis["rays"], vis["tris"] / vis["rays"], vis["hits"] / vis["rays"]
-        bytes_per_ray = RAY_BYTES + HIT_BYTES + NODE_BYTES * e_nodes + TRI_BYTES * e_tris + GATHER_BYTES * p_hit
+        # traversal bytes of one ray: queue entry + hit record + nodes + triangles
+        trav_bytes = RAY_BYTES + HIT_BYTES + NODE_BYTES * e_nodes + TRI_BYTES * e_tris
         trace_ms = ts["traceKernelMs"]
         launches = max(1, ts["traceLaunches"])
-        achieved = rays * bytes_per_ray / (trace_ms * 1e-3) / 1e9
+        wf_rays = int(passes.sum())  # rays the wavefront traversal passes traced
+        achieved = wf_rays * trav_bytes / (trace_ms * 1e-3) / 1e9
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                 "kernel": "k_trace_queue", "avg_launch_ms": round(trace_ms / launches, 5), "launches_per_frame": launches,
-                "bytes_per_ray": round(bytes_per_ray, 1), "E_nodes": round(e_nodes, 2), "E_tris": round(e_tris, 2),
-                "P_hit": round(p_hit, 4)}
+                "rays_in_launches": wf_rays, "bytes_per_ray": round(trav_bytes, 1),
+                "node_bytes": NODE_BYTES, "tri_bytes": TRI_BYTES,
+                "E_nodes": round(e_nodes, 2), "E_tris": round(e_tris, 2), "P_hit": round(p_hit, 4)}
+        if ts.get("drained"):
+            drain_rays = rays - wf_rays
+            drain_bytes = trav_bytes + GATHER_BYTES * p_hit
+            dach = drain_rays * drain_bytes / (ts["drainKernelMs"] * 1e-3) / 1e9
+            roof["drain"] = {"kernel": "k_drain", "ms": round(ts["drainKernelMs"], 4), "rays": drain_rays,
+                             "achieved": round(dach, 1), "frac": round(dach / HBM_PEAK_GBS, 4),
+                             "bytes_per_ray": round(drain_bytes, 1)}
         log(f"[bench] roofline {roof}")
         if not args.no_cpu_baseline:
-            band = args.cpu_rows or 24
-            y0 = H // 2 - band // 2
+            # a band of rows sized for ~15 s of wall time on these cores,
+            # from a quick 16-row probe of the oracle's rate
+            y0 = H // 2 - 8
+            tc = time.time()
+            probe = osc.render(ubo, W, H, threads=cores, rows=(y0, y0 + 16))
+            rate = probe["rays"] / max(time.time() - tc, 1e-3)
+            band = args.cpu_rows or int(min(H, max(16, 16 * (15.0 * rate / max(probe["rays"], 1)))))
+            y0 = max(0, H // 2 - band // 2)
             tc = time.time()
             out = osc.render(ubo, W, H, threads=cores, rows=(y0, y0 + band))
             dt = time.time() - tc
             cpu = {"value": round(out["rays"] / dt / 1e6, 3), "unit": "Mrays/s", "cores": cores, "kind": "port",
-                   "sample": f"rows {y0}..{y0 + band - 1} of the 1920x1080 frame ({out['rays']} rays, {dt:.1f}s), "
-                             "CPU oracle (C++ restatement, not lavapipe)"}
+                   "sample": f"rows {y0}..{y0 + band - 1} of the 1920x1080 frame ({out['rays']} rays, {dt:.1f}s wall), "
+                             "CPU oracle (C++ restatement of the same trees and shading, not lavapipe)"}
             log(f"[bench] cpu baseline {cpu}")
     except Exception as e:  # the GPU number stands without the CPU side
         log(f"[bench] oracle side failed: {e!r}")

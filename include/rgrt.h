@@ -209,11 +209,17 @@ rgrt_status rgrt_blas_info(rgrt_ctx* ctx, rgrt_blas blas, uint32_t* numNodes, ui
  * original primitive index of every leaf in Morton order. */
 rgrt_status rgrt_export_blas(rgrt_ctx* ctx, rgrt_blas blas, rgrt_bvh_node* nodes, uint32_t nodeCapacity,
                              uint32_t* leafPrimitive, uint32_t leafCapacity);
-/* The 4-wide tree the traversal walks, collapsed from the binary tree
- * breadth-first (128-byte nodes: per-axis lo/hi of 4 children, 4 child refs,
- * 16 bytes padding). With nodes == NULL only *numNodes is written. */
-rgrt_status rgrt_export_blas_wide(rgrt_ctx* ctx, rgrt_blas blas, void* nodes, uint32_t capacity, uint32_t* numNodes);
-rgrt_status rgrt_export_tlas_wide(rgrt_ctx* ctx, void* nodes, uint32_t capacity, uint32_t* numNodes);
+/* The 4-wide tree collapsed from the binary tree breadth-first. With
+ * compressed == 0: 128-byte nodes (per-axis lo/hi of 4 children, 4 child
+ * refs, 16 bytes padding). With compressed != 0: the 64-byte form the
+ * traversal walks (node box min corner as 3 floats, a power-of-two exponent
+ * byte per axis and a pad byte, the children's lo/hi as 8-bit grid offsets
+ * per axis rounded outwards, 8 bytes padding, 4 child refs). Child refs are
+ * >= 0 for a wide node, ~k for leaf k, INT32_MIN for an empty slot. With
+ * nodes == NULL only *numNodes is written. */
+rgrt_status rgrt_export_blas_wide(rgrt_ctx* ctx, rgrt_blas blas, int32_t compressed, void* nodes, uint32_t capacity,
+                                  uint32_t* numNodes);
+rgrt_status rgrt_export_tlas_wide(rgrt_ctx* ctx, int32_t compressed, void* nodes, uint32_t capacity, uint32_t* numNodes);
 
 /* Replaces TopLevelAS(cmd, scene) + instanceOffsetTable()
  * (raygun/render/acceleration_structure.cpp:55-138). The caller keeps the

@@ -33,11 +33,14 @@ int32_t orc_update_vertices(orc_scene* s, uint64_t firstVertex, const void* vert
 uint32_t orc_blas_num_nodes(orc_scene* s, int32_t blas);
 /* Canonical nodes (64 B each, rgrt_bvh_node layout) + leaf primitive ids. */
 int32_t orc_export_blas(orc_scene* s, int32_t blas, void* nodes, uint32_t* leafPrim);
-/* The 4-wide tree the traversal walks (128-byte nodes, breadth-first). */
+/* The 4-wide tree (breadth-first): 128-byte nodes, or with compressed != 0
+ * the 64-byte quantised nodes the traversal reads. */
 uint32_t orc_blas_num_wide(orc_scene* s, int32_t blas);
-int32_t orc_export_blas_wide(orc_scene* s, int32_t blas, void* nodes);
+int32_t orc_export_blas_wide(orc_scene* s, int32_t blas, int32_t compressed, void* nodes);
 uint32_t orc_tlas_num_wide(orc_scene* s);
-int32_t orc_export_tlas_wide(orc_scene* s, void* nodes);
+int32_t orc_export_tlas_wide(orc_scene* s, int32_t compressed, void* nodes);
+/* Hierarchy of later BLAS builds: 0 Karras (default), 1 PLOC. */
+void orc_set_hierarchy(int32_t h);
 /* Morton codes of the primitives in original order (for the sort check). */
 int32_t orc_blas_morton(orc_scene* s, int32_t blas, uint32_t* codes);
 

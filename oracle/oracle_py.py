@@ -42,11 +42,13 @@ def lib():
         L.orc_blas_num_wide.restype = u32
         L.orc_blas_num_wide.argtypes = [P, i32]
         L.orc_export_blas_wide.restype = i32
-        L.orc_export_blas_wide.argtypes = [P, i32, P]
+        L.orc_export_blas_wide.argtypes = [P, i32, i32, P]
         L.orc_tlas_num_wide.restype = u32
         L.orc_tlas_num_wide.argtypes = [P]
         L.orc_export_tlas_wide.restype = i32
-        L.orc_export_tlas_wide.argtypes = [P, P]
+        L.orc_export_tlas_wide.argtypes = [P, i32, P]
+        L.orc_set_hierarchy.restype = None
+        L.orc_set_hierarchy.argtypes = [i32]
         L.orc_blas_morton.restype = i32
         L.orc_blas_morton.argtypes = [P, i32, P]
         L.orc_build_tlas.restype = i32
@@ -125,16 +127,16 @@ class OracleScene:
         lib().orc_export_blas(self.h, b, _p(nodes), _p(leaf))
         return nodes, leaf
 
-    def export_blas_wide(self, b):
-        from raygun_amd.records import WIDE_NODE
-        nodes = np.zeros(lib().orc_blas_num_wide(self.h, b), WIDE_NODE)
-        lib().orc_export_blas_wide(self.h, b, _p(nodes))
+    def export_blas_wide(self, b, compressed=False):
+        from raygun_amd.records import QWIDE_NODE, WIDE_NODE
+        nodes = np.zeros(lib().orc_blas_num_wide(self.h, b), QWIDE_NODE if compressed else WIDE_NODE)
+        lib().orc_export_blas_wide(self.h, b, int(compressed), _p(nodes))
         return nodes
 
-    def export_tlas_wide(self):
-        from raygun_amd.records import WIDE_NODE
-        nodes = np.zeros(lib().orc_tlas_num_wide(self.h), WIDE_NODE)
-        lib().orc_export_tlas_wide(self.h, _p(nodes))
+    def export_tlas_wide(self, compressed=False):
+        from raygun_amd.records import QWIDE_NODE, WIDE_NODE
+        nodes = np.zeros(lib().orc_tlas_num_wide(self.h), QWIDE_NODE if compressed else WIDE_NODE)
+        lib().orc_export_tlas_wide(self.h, int(compressed), _p(nodes))
         return nodes
 
     def blas_morton(self, b):
@@ -202,12 +204,12 @@ class OracleScene:
     def render_counts(self, ubo, width, height, rows, threads=None):
         """Traversal work of rendering `rows`: rays, nodes, tris, hits, sky."""
         rows = np.ascontiguousarray(rows, np.int32)
-        out = np.zeros(5, np.uint64)
+        out = np.zeros(7, np.uint64)
         ubo = np.ascontiguousarray(ubo)
         rc = lib().orc_render_counts(self.h, _p(ubo), width, height, _p(rows), len(rows), threads or default_threads(), _p(out))
         if rc != 0:
             raise RuntimeError("oracle render_counts failed")
-        return dict(zip(("rays", "nodes", "tris", "hits", "sky"), (int(x) for x in out)))
+        return dict(zip(("rays", "nodes", "tris", "hits", "sky", "inst_leaves", "blas_entries"), (int(x) for x in out)))
 
 
 def postprocess(ubo, base16, normal16, rough16, use_fxaa=True):

@@ -286,9 +286,23 @@ struct Node4 {
 };
 static_assert(sizeof(Node4) == 128, "wide node");
 
+// Compressed 4-wide node, 64 bytes: the node box's min corner, a
+// power-of-two scale per axis (exponent bytes) and each child box as 8-bit
+// multiples of the scale, rounded outwards (lo down, hi up).
+struct Node4q {
+    float ox, oy, oz;
+    int8_t ex, ey, ez, pad0;
+    uint8_t qlox[4], qhix[4], qloy[4], qhiy[4];
+    uint8_t qloz[4], qhiz[4];
+    uint32_t pad1[2];
+    int32_t child[4];
+};
+static_assert(sizeof(Node4q) == 64, "compressed wide node");
+
 struct BVH {
     std::vector<CanonNode> nodes;
     std::vector<Node4> wide;        // the binary tree collapsed to 4 children per node
+    std::vector<Node4q> wideq;      // the same nodes compressed (what the traversal reads)
     std::vector<uint32_t> leafPrim; // Morton order -> original primitive index
     std::vector<uint32_t> morton;   // original order
     std::vector<Box> primBox;       // original order
@@ -435,8 +449,180 @@ void collapse4(BVH& b)
     (void)n;
 }
 
+// 2^e as a float, e in [-126, 127].
+inline float pow2f(int e)
+{
+    uint32_t b = (uint32_t)(e + 127) << 23;
+    float f;
+    std::memcpy(&f, &b, 4);
+    return f;
+}
+
+// Smallest e in [-126, 127] with 254 * 2^e >= ext: the 8-bit grid spans the
+// node box with one step to spare for outward rounding.
+inline int quant_exponent(float ext)
+{
+    if(!(ext > 0.0f)) return -126;
+    uint32_t bits;
+    std::memcpy(&bits, &ext, 4);
+    int e = (int)((bits >> 23) & 0xffu) - 127 - 7;
+    e = std::min(127, std::max(-126, e));
+    while(e < 127 && 254.0f * pow2f(e) < ext) ++e;
+    while(e > -126 && 254.0f * pow2f(e - 1) >= ext) --e;
+    return e;
+}
+
+// Outward-rounded 8-bit offsets of [lo, hi] from origin o on the grid 2^e.
+inline void quantize_range(float o, int e, float lo, float hi, uint8_t& qlo, uint8_t& qhi)
+{
+    const float s = pow2f(e), inv = pow2f(-std::min(e, 126));
+    int a = (int)std::floor((lo - o) * inv);
+    a = std::min(255, std::max(0, a));
+    while(a > 0 && o + (float)a * s > lo) --a;
+    int b = (int)std::ceil((hi - o) * inv);
+    b = std::min(255, std::max(0, b));
+    while(b < 255 && o + (float)b * s < hi) ++b;
+    qlo = (uint8_t)a;
+    qhi = (uint8_t)b;
+}
+
+void quantize4(BVH& b)
+{
+    b.wideq.resize(b.wide.size());
+    for(size_t i = 0; i < b.wide.size(); ++i) {
+        const Node4& w = b.wide[i];
+        Node4q q;
+        std::memset(&q, 0, sizeof(q));
+        float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        const float* clo[3] = {w.lox, w.loy, w.loz};
+        const float* chi[3] = {w.hix, w.hiy, w.hiz};
+        for(int c = 0; c < 4; ++c) {
+            if(w.child[c] == EMPTY_CHILD) continue;
+            for(int k = 0; k < 3; ++k) {
+                lo[k] = std::fmin(lo[k], clo[k][c]);
+                hi[k] = std::fmax(hi[k], chi[k][c]);
+            }
+        }
+        int e[3];
+        float org[3];
+        for(int k = 0; k < 3; ++k) {
+            const bool any = lo[k] <= hi[k];
+            org[k] = any ? lo[k] : 0.0f;
+            e[k] = any ? quant_exponent(hi[k] - lo[k]) : -126;
+        }
+        q.ox = org[0];
+        q.oy = org[1];
+        q.oz = org[2];
+        q.ex = (int8_t)e[0];
+        q.ey = (int8_t)e[1];
+        q.ez = (int8_t)e[2];
+        uint8_t* qlo[3] = {q.qlox, q.qloy, q.qloz};
+        uint8_t* qhi[3] = {q.qhix, q.qhiy, q.qhiz};
+        for(int c = 0; c < 4; ++c) {
+            q.child[c] = w.child[c];
+            if(w.child[c] == EMPTY_CHILD) continue;
+            for(int k = 0; k < 3; ++k) quantize_range(org[k], e[k], clo[k][c], chi[k][c], qlo[k][c], qhi[k][c]);
+        }
+        b.wideq[i] = q;
+    }
+}
+
+// Parallel locally-ordered clustering (Meister & Bittner 2018) over the
+// Morton-ordered primitives: each cluster's nearest neighbour is the
+// cluster within +-kPlocRadius positions whose union box has the smallest
+// surface area (the lowest position on equal areas); mutual nearest
+// neighbours merge (the left one keeps the merged cluster), and the
+// clusters are compacted in order. Internal nodes are numbered from n-2
+// down in merge order, so the last merge is the root, node 0.
+constexpr int kPlocRadius = 16;
+
+void build_ploc(BVH& b)
+{
+    const int n = (int)b.leafPrim.size();
+    struct C {
+        int32_t ref;
+        float lo[3], hi[3];
+    };
+    std::vector<C> cl(n), nxt;
+    for(int i = 0; i < n; ++i) {
+        const Box& pb = b.primBox[b.leafPrim[i]];
+        cl[i].ref = ~i;
+        for(int k = 0; k < 3; ++k) {
+            cl[i].lo[k] = pb.lo[k];
+            cl[i].hi[k] = pb.hi[k];
+        }
+    }
+    auto merged_area = [&](const C& a, const C& c) {
+        float lo[3], hi[3];
+        for(int k = 0; k < 3; ++k) {
+            lo[k] = std::fmin(a.lo[k], c.lo[k]);
+            hi[k] = std::fmax(a.hi[k], c.hi[k]);
+        }
+        return half_area(lo, hi);
+    };
+    int nextNode = n - 2;
+    std::vector<int> nn;
+    for(int iter = 0; (int)cl.size() > 1 && iter < 100000; ++iter) {
+        const int m = (int)cl.size();
+        nn.assign(m, -1);
+        for(int i = 0; i < m; ++i) {
+            float best = std::numeric_limits<float>::infinity();
+            int bj = -1;
+            for(int j = std::max(0, i - kPlocRadius); j <= std::min(m - 1, i + kPlocRadius); ++j) {
+                if(j == i) continue;
+                const float a = merged_area(cl[i], cl[j]);
+                if(a < best) {
+                    best = a;
+                    bj = j;
+                }
+            }
+            nn[i] = bj;
+        }
+        nxt.clear();
+        int merges = 0;
+        for(int i = 0; i < m; ++i) {
+            const int j = nn[i];
+            const bool mutual = j >= 0 && nn[j] == i;
+            if(mutual && j < i) continue; // merged into the left cluster
+            if(mutual) {
+                const int node = nextNode--;
+                CanonNode& nd = b.nodes[node];
+                nd.child[0] = cl[i].ref;
+                nd.child[1] = cl[j].ref;
+                for(int k = 0; k < 3; ++k) {
+                    nd.lo[0][k] = cl[i].lo[k];
+                    nd.hi[0][k] = cl[i].hi[k];
+                    nd.lo[1][k] = cl[j].lo[k];
+                    nd.hi[1][k] = cl[j].hi[k];
+                }
+                for(int c = 0; c < 2; ++c)
+                    if(nd.child[c] >= 0) b.nodes[nd.child[c]].parent = (uint32_t)node;
+                C u;
+                u.ref = node;
+                for(int k = 0; k < 3; ++k) {
+                    u.lo[k] = std::fmin(cl[i].lo[k], cl[j].lo[k]);
+                    u.hi[k] = std::fmax(cl[i].hi[k], cl[j].hi[k]);
+                }
+                nxt.push_back(u);
+                ++merges;
+            }
+            else {
+                nxt.push_back(cl[i]);
+            }
+        }
+        cl.swap(nxt);
+        (void)merges;
+    }
+    b.nodes[0].parent = 0xffffffffu;
+    b.root = Box{{cl[0].lo[0], cl[0].lo[1], cl[0].lo[2]}, {cl[0].hi[0], cl[0].hi[1], cl[0].hi[2]}};
+}
+
+// Hierarchy of the next builds: 0 = Karras 2012 (fast build), 1 = PLOC
+// (fast trace).
+int g_hierarchy = 0;
+
 // Builds the canonical LBVH over primitive boxes.
-void build_lbvh(BVH& b)
+void build_lbvh(BVH& b, int hierarchy)
 {
     const int n = (int)b.primBox.size();
     // Scene box: union of all primitive boxes.
@@ -469,7 +655,10 @@ void build_lbvh(BVH& b)
         nd.child[0] = nd.child[1] = EMPTY_CHILD;
     }
     if(n == 1) b.nodes[0].child[0] = ~0;
-    if(n >= 2) {
+    if(n >= 2 && hierarchy == 1) {
+        build_ploc(b);
+    }
+    else if(n >= 2) {
         for(int i = 0; i < n - 1; ++i) {
             int d = (delta(keys, n, i, i + 1) - delta(keys, n, i, i - 1)) >= 0 ? 1 : -1;
             int dmin = delta(keys, n, i, i - d);
@@ -498,6 +687,7 @@ void build_lbvh(BVH& b)
     }
     compute_node_boxes(b);
     collapse4(b);
+    quantize4(b);
 }
 
 // --------------------------------------------------------------------------
@@ -696,6 +886,7 @@ struct Hit {
     uint32_t prim = 0;
     bool tie = false;
     uint32_t nodes = 0, tris = 0;
+    uint32_t instLeaves = 0, blasEntries = 0; // instance leaves reached, BLASes entered
 };
 
 constexpr float TIE_REL = 1e-6f;
@@ -720,22 +911,31 @@ bool trace_closest(const orc_scene& s, const float o[3], const float d[3], float
         int32_t inst;
         uint32_t prim;
     };
-    std::vector<Cand> cands;
+    static thread_local std::vector<Cand> cands;
+    cands.clear();
     // The box and triangle tests run against a slightly widened limit so
     // that near-tie candidates are seen; acceptance still needs t < best.
     auto widen = [](float b) { return b * (1.0f + 8.0f * TIE_REL); };
 
-    auto visit = [&](const std::vector<Node4>& nodes, const RayPre& r, int32_t ref, int32_t* stack, int& sp) {
-        const Node4& w = nodes[ref];
+    // Child boxes of a compressed node: plane distance t = q * (2^e / d) +
+    // (origin - o) / d, one fused multiply-add per plane.
+    auto visit = [&](const std::vector<Node4q>& nodes, const RayPre& r, int32_t ref, int32_t* stack, int& sp) {
+        const Node4q& q = nodes[ref];
         h.nodes++;
+        const float ax = pow2f(q.ex) * r.inv[0], ay = pow2f(q.ey) * r.inv[1], az = pow2f(q.ez) * r.inv[2];
+        const float bx = (q.ox - r.o[0]) * r.inv[0], by = (q.oy - r.o[1]) * r.inv[1], bz = (q.oz - r.o[2]) * r.inv[2];
+        const float lim = widen(best);
         float tn[4];
         int32_t rf[4];
         for(int c = 0; c < 4; ++c) {
-            const float lo[3] = {w.lox[c], w.loy[c], w.loz[c]}, hi[3] = {w.hix[c], w.hiy[c], w.hiz[c]};
-            float t;
-            const bool hit = w.child[c] != EMPTY_CHILD && ray_box(r, lo, hi, tmin, widen(best), t);
-            tn[c] = hit ? t : std::numeric_limits<float>::infinity();
-            rf[c] = hit ? w.child[c] : EMPTY_CHILD;
+            const float x0 = std::fma((float)q.qlox[c], ax, bx), x1 = std::fma((float)q.qhix[c], ax, bx);
+            const float y0 = std::fma((float)q.qloy[c], ay, by), y1 = std::fma((float)q.qhiy[c], ay, by);
+            const float z0 = std::fma((float)q.qloz[c], az, bz), z1 = std::fma((float)q.qhiz[c], az, bz);
+            const float n = std::fmax(std::fmax(std::fmin(x0, x1), std::fmin(y0, y1)), std::fmax(std::fmin(z0, z1), tmin));
+            const float f = std::fmin(std::fmin(std::fmax(x0, x1), std::fmax(y0, y1)), std::fmin(std::fmax(z0, z1), lim));
+            const bool hit = q.child[c] != EMPTY_CHILD && n <= f * 1.0000004f;
+            tn[c] = hit ? n : std::numeric_limits<float>::infinity();
+            rf[c] = hit ? q.child[c] : EMPTY_CHILD;
         }
         auto cx = [&](int i, int j) {
             if(tn[j] < tn[i]) {
@@ -760,15 +960,17 @@ bool trace_closest(const orc_scene& s, const float o[3], const float d[3], float
     while(tsp) {
         int32_t ref = tstack[--tsp];
         if(ref >= 0) {
-            visit(s.tlas.wide, wr, ref, tstack, tsp);
+            visit(s.tlas.wideq, wr, ref, tstack, tsp);
             continue;
         }
         uint32_t ii = s.tlas.leafPrim[~ref];
         const InstanceX& ix = s.inst[ii];
         h.nodes++; // the 64-byte instance leaf record is fetched like a node
+        h.instLeaves++;
         if((ix.rec.mask & cullMask & 0xffu) == 0) continue;
         const Blas& b = s.blas[ix.rec.blas];
         if(b.numFaces == 0) continue;
+        h.blasEntries++;
         const float* w = ix.w2o;
         float oo[3], od[3];
         for(int row = 0; row < 3; ++row) {
@@ -783,7 +985,7 @@ bool trace_closest(const orc_scene& s, const float o[3], const float d[3], float
         while(bsp) {
             int32_t bref = bstack[--bsp];
             if(bref >= 0) {
-                visit(b.bvh.wide, r, bref, bstack, bsp);
+                visit(b.bvh.wideq, r, bref, bstack, bsp);
                 continue;
             }
             uint32_t prim = b.bvh.leafPrim[~bref];
@@ -892,6 +1094,7 @@ struct Ctx {
     V3 lightDir;
     uint64_t rays = 0, sky = 0, zeroDir = 0;
     uint64_t nodes = 0, tris = 0, hits = 0; // traversal work of the traced rays
+    uint64_t instLeaves = 0, blasEntries = 0;
     bool tie = false;
     bool recordPrimary = false;
     HitOut primary{0, 0, 0, NO_HIT, 0};
@@ -978,6 +1181,8 @@ void trace_ray(Ctx& c, Payload& P, uint32_t cullMask, int missIndex, V3 O, float
         if(hit && h.tie) c.tie = true;
         c.nodes += h.nodes;
         c.tris += h.tris;
+        c.instLeaves += h.instLeaves;
+        c.blasEntries += h.blasEntries;
         c.hits += hit ? 1 : 0;
     }
     if(primary && c.recordPrimary) {
@@ -1570,7 +1775,7 @@ int32_t orc_build_blas(orc_scene* s, uint32_t firstVertex, uint32_t indexOffsetB
     b.numFaces = numFaces;
     if((uint64_t)b.indexOffset + 3ull * numFaces > s->ib.size()) return -1;
     load_tris(*s, b);
-    build_lbvh(b.bvh);
+    build_lbvh(b.bvh, g_hierarchy);
     s->blas.push_back(std::move(b));
     return (int32_t)s->blas.size() - 1;
 }
@@ -1589,6 +1794,7 @@ int32_t orc_refit_blas(orc_scene* s, int32_t id)
     load_tris(*s, b);
     compute_node_boxes(b.bvh);
     collapse4(b.bvh); // the wide topology follows the refitted areas
+    quantize4(b.bvh);
     return 0;
 }
 
@@ -1596,18 +1802,20 @@ uint32_t orc_blas_num_nodes(orc_scene* s, int32_t id) { return (uint32_t)s->blas
 
 uint32_t orc_blas_num_wide(orc_scene* s, int32_t id) { return (uint32_t)s->blas[id].bvh.wide.size(); }
 
-int32_t orc_export_blas_wide(orc_scene* s, int32_t id, void* nodes)
+int32_t orc_export_blas_wide(orc_scene* s, int32_t id, int32_t compressed, void* nodes)
 {
     const BVH& b = s->blas[id].bvh;
-    std::memcpy(nodes, b.wide.data(), b.wide.size() * sizeof(Node4));
+    if(compressed) std::memcpy(nodes, b.wideq.data(), b.wideq.size() * sizeof(Node4q));
+    else std::memcpy(nodes, b.wide.data(), b.wide.size() * sizeof(Node4));
     return 0;
 }
 
 uint32_t orc_tlas_num_wide(orc_scene* s) { return (uint32_t)s->tlas.wide.size(); }
 
-int32_t orc_export_tlas_wide(orc_scene* s, void* nodes)
+int32_t orc_export_tlas_wide(orc_scene* s, int32_t compressed, void* nodes)
 {
-    std::memcpy(nodes, s->tlas.wide.data(), s->tlas.wide.size() * sizeof(Node4));
+    if(compressed) std::memcpy(nodes, s->tlas.wideq.data(), s->tlas.wideq.size() * sizeof(Node4q));
+    else std::memcpy(nodes, s->tlas.wide.data(), s->tlas.wide.size() * sizeof(Node4));
     return 0;
 }
 
@@ -1638,7 +1846,7 @@ int32_t orc_build_tlas(orc_scene* s, const void* instances, uint32_t n)
         invert_affine(in[i].transform, s->inst[i].w2o);
         s->tlas.primBox[i] = transform_box(in[i].transform, s->blas[in[i].blas].bvh.root);
     }
-    build_lbvh(s->tlas);
+    build_lbvh(s->tlas, 0);
     s->hasTlas = true;
     return 0;
 }
@@ -1746,7 +1954,8 @@ int32_t orc_render_counts(orc_scene* s, const void* uboPtr, uint32_t W, uint32_t
     UBO ubo;
     std::memcpy(&ubo, uboPtr, sizeof(UBO));
     const int nt = std::max(1, (int)threads);
-    std::vector<uint64_t> acc(5 * nt, 0);
+    constexpr int K = 7;
+    std::vector<uint64_t> acc(K * nt, 0);
     parallel_rows(0, nrows, nt, [&](int r, int t) {
         Ctx c;
         c.s = s;
@@ -1757,15 +1966,17 @@ int32_t orc_render_counts(orc_scene* s, const void* uboPtr, uint32_t W, uint32_t
             float o[3][4];
             raygen(c, x, y, W, H, o);
         }
-        acc[5 * t + 0] += c.rays;
-        acc[5 * t + 1] += c.nodes;
-        acc[5 * t + 2] += c.tris;
-        acc[5 * t + 3] += c.hits;
-        acc[5 * t + 4] += c.sky;
+        acc[K * t + 0] += c.rays;
+        acc[K * t + 1] += c.nodes;
+        acc[K * t + 2] += c.tris;
+        acc[K * t + 3] += c.hits;
+        acc[K * t + 4] += c.sky;
+        acc[K * t + 5] += c.instLeaves;
+        acc[K * t + 6] += c.blasEntries;
     });
-    for(int k = 0; k < 5; ++k) {
+    for(int k = 0; k < K; ++k) {
         out[k] = 0;
-        for(int t = 0; t < nt; ++t) out[k] += acc[5 * t + k];
+        for(int t = 0; t < nt; ++t) out[k] += acc[K * t + k];
     }
     return 0;
 }
@@ -1820,6 +2031,8 @@ void orc_sky(const float* dir, const float* lightDir, float* rgb)
 }
 
 uint32_t orc_morton3(float x, float y, float z) { return morton3(x, y, z); }
+
+void orc_set_hierarchy(int32_t h) { g_hierarchy = h; }
 
 void orc_primary_ray(const void* uboPtr, uint32_t x, uint32_t y, uint32_t W, uint32_t H, int32_t sample, float* origin, float* dir)
 {

@@ -21,7 +21,8 @@ namespace {
 struct BlasRec {
     rgrt_blas_desc desc{};
     Node* nodes = nullptr;  // canonical binary tree
-    Node4* wide = nullptr;  // the 4-wide tree the traversal walks
+    Node4* wide = nullptr;  // the 4-wide tree (full-precision boxes)
+    Node4q* wideq = nullptr; // its compressed form, which the traversal walks
     uint32_t* numWide = nullptr;
     Tri* tris = nullptr;
     uint32_t* leafPrim = nullptr;
@@ -81,6 +82,7 @@ struct rgrt_ctx {
     rgrt_instance* dInstIn = nullptr;
     Node* tlasNodes = nullptr;
     Node4* tlasWide = nullptr;
+    Node4q* tlasWideq = nullptr;
     uint32_t* tlasNumWide = nullptr;
     uint32_t* tlasLeaf = nullptr;
     DevInstance* dInst = nullptr;
@@ -99,7 +101,7 @@ struct rgrt_ctx {
     int32_t* spill = nullptr;
     size_t spillBytes = 0;
     uint32_t traceGrid = 0, shadeGrid = 0, drainGrid = 0;
-    rgrt_tuning tuning{4, 6, 8, 4}; // measured on config 2 (1080p, 2M triangles)
+    rgrt_tuning tuning{6, 6, 8, 4}; // measured on config 2 (1080p, 2M triangles)
     hipEvent_t evDrainStart = nullptr, evDrainEnd = nullptr;
     bool lastDrained = false;
 
@@ -144,7 +146,7 @@ struct rgrt_ctx {
     TraceScene scene() const
     {
         TraceScene sc;
-        sc.tlasNodes = tlasWide;
+        sc.tlasNodes = tlasWideq;
         sc.instances = dInst;
         sc.blas = dBlasTable;
         sc.numInstances = haveTlas ? numInstances : 0;
@@ -273,6 +275,7 @@ void ensure_inst_cap(rgrt_ctx* c, uint32_t n)
     dev_alloc(c->dInstIn, cap);
     dev_alloc(c->tlasNodes, cap);
     dev_alloc(c->tlasWide, cap);
+    dev_alloc(c->tlasWideq, cap);
     if(!c->tlasNumWide) dev_alloc(c->tlasNumWide, 1);
     dev_alloc(c->tlasLeaf, cap);
     dev_alloc(c->dInst, cap);
@@ -477,6 +480,7 @@ void rgrt_destroy(rgrt_ctx* c)
     for(auto& b: c->blas) {
         dev_free(b.nodes);
         dev_free(b.wide);
+        dev_free(b.wideq);
         dev_free(b.numWide);
         dev_free(b.tris);
         dev_free(b.leafPrim);
@@ -489,6 +493,7 @@ void rgrt_destroy(rgrt_ctx* c)
     dev_free(c->dInstIn);
     dev_free(c->tlasNodes);
     dev_free(c->tlasWide);
+    dev_free(c->tlasWideq);
     dev_free(c->tlasNumWide);
     dev_free(c->tlasLeaf);
     dev_free(c->dInst);
@@ -588,6 +593,7 @@ rgrt_status rgrt_build_blas(rgrt_ctx* c, const rgrt_blas_desc* d, rgrt_blas* out
         const uint32_t n = d->numFaces;
         dev_alloc(b.nodes, std::max<uint32_t>(n, 2) - 1);
         dev_alloc(b.wide, std::max<uint32_t>(n, 2) - 1);
+        dev_alloc(b.wideq, std::max<uint32_t>(n, 2) - 1);
         dev_alloc(b.numWide, 1);
         dev_alloc(b.tris, n);
         dev_alloc(b.leafPrim, n);
@@ -595,8 +601,9 @@ rgrt_status rgrt_build_blas(rgrt_ctx* c, const rgrt_blas_desc* d, rgrt_blas* out
         launch_tri_boxes(s, c->vb, c->ib, d->firstVertex, (uint32_t)io, n, c->builder.boxLo, c->builder.boxHi);
         c->builder.build(s, n, b.nodes, b.leafPrim, c->dBlasRoot + 2 * id);
         c->builder.collapse(s, n, b.nodes, b.wide, b.numWide);
+        c->builder.quantize(s, std::max<uint32_t>(n, 2) - 1, b.wide, b.numWide, b.wideq);
         launch_tri_gather(s, c->vb, c->ib, d->firstVertex, (uint32_t)io, b.leafPrim, n, b.tris);
-        DevBlas db{b.wide, b.tris, n, 0};
+        DevBlas db{b.wideq, b.tris, n, 0};
         check(hipMemcpyAsync(c->dBlasTable + id, &db, sizeof(DevBlas), hipMemcpyHostToDevice, s));
         check(hipStreamSynchronize(s)); // the reference blocks on a fence (raytracer.cpp:71-73)
         b.valid = true;
@@ -618,6 +625,7 @@ rgrt_status rgrt_refit_blas(rgrt_ctx* c, rgrt_blas id)
         launch_tri_boxes(s, c->vb, c->ib, b.desc.firstVertex, io, n, c->builder.boxLo, c->builder.boxHi);
         c->builder.refit(s, n, b.nodes, b.leafPrim, c->dBlasRoot + 2 * id);
         c->builder.collapse(s, n, b.nodes, b.wide, b.numWide); // wide topology follows the new areas
+        c->builder.quantize(s, std::max<uint32_t>(n, 2) - 1, b.wide, b.numWide, b.wideq);
         launch_tri_gather(s, c->vb, c->ib, b.desc.firstVertex, io, b.leafPrim, n, b.tris);
         check(hipStreamSynchronize(s));
         return RGRT_OK;
@@ -663,7 +671,8 @@ rgrt_status rgrt_export_blas(rgrt_ctx* c, rgrt_blas id, rgrt_bvh_node* nodes, ui
     });
 }
 
-rgrt_status rgrt_export_blas_wide(rgrt_ctx* c, rgrt_blas id, void* nodes, uint32_t capacity, uint32_t* numNodes)
+rgrt_status rgrt_export_blas_wide(rgrt_ctx* c, rgrt_blas id, int32_t compressed, void* nodes, uint32_t capacity,
+                                  uint32_t* numNodes)
 {
     return guarded(c, [&]() {
         if(id >= c->blas.size() || !c->blas[id].valid) return fail(c, RGRT_ERR_INVALID_ARG, "export_blas_wide: bad handle");
@@ -673,13 +682,14 @@ rgrt_status rgrt_export_blas_wide(rgrt_ctx* c, rgrt_blas id, void* nodes, uint32
         if(numNodes) *numNodes = nw;
         if(nodes) {
             if(capacity < nw) return fail(c, RGRT_ERR_INVALID_ARG, "export_blas_wide: capacity");
-            check(hipMemcpy(nodes, c->blas[id].wide, nw * sizeof(Node4), hipMemcpyDeviceToHost));
+            if(compressed) check(hipMemcpy(nodes, c->blas[id].wideq, nw * sizeof(Node4q), hipMemcpyDeviceToHost));
+            else check(hipMemcpy(nodes, c->blas[id].wide, nw * sizeof(Node4), hipMemcpyDeviceToHost));
         }
         return RGRT_OK;
     });
 }
 
-rgrt_status rgrt_export_tlas_wide(rgrt_ctx* c, void* nodes, uint32_t capacity, uint32_t* numNodes)
+rgrt_status rgrt_export_tlas_wide(rgrt_ctx* c, int32_t compressed, void* nodes, uint32_t capacity, uint32_t* numNodes)
 {
     return guarded(c, [&]() {
         if(!c->haveTlas) return fail(c, RGRT_ERR_NOT_READY, "export_tlas_wide: no TLAS");
@@ -689,7 +699,8 @@ rgrt_status rgrt_export_tlas_wide(rgrt_ctx* c, void* nodes, uint32_t capacity, u
         if(numNodes) *numNodes = nw;
         if(nodes) {
             if(capacity < nw) return fail(c, RGRT_ERR_INVALID_ARG, "export_tlas_wide: capacity");
-            check(hipMemcpy(nodes, c->tlasWide, nw * sizeof(Node4), hipMemcpyDeviceToHost));
+            if(compressed) check(hipMemcpy(nodes, c->tlasWideq, nw * sizeof(Node4q), hipMemcpyDeviceToHost));
+            else check(hipMemcpy(nodes, c->tlasWide, nw * sizeof(Node4), hipMemcpyDeviceToHost));
         }
         return RGRT_OK;
     });
@@ -721,6 +732,7 @@ rgrt_status rgrt_build_tlas(rgrt_ctx* c, const rgrt_instance* in, uint32_t n)
         launch_inst_prep(s, c->dInstIn, n, c->dBlasRoot, c->builder.boxLo, c->builder.boxHi, c->dW2O, c->dShade);
         c->builder.build(s, n, c->tlasNodes, c->tlasLeaf, c->tlasRoot);
         c->builder.collapse(s, n, c->tlasNodes, c->tlasWide, c->tlasNumWide);
+        c->builder.quantize(s, std::max<uint32_t>(n, 2) - 1, c->tlasWide, c->tlasNumWide, c->tlasWideq);
         launch_inst_gather(s, c->dW2O, c->dInstIn, c->tlasLeaf, n, c->dInst);
         if(timing) check(hipEventRecord(c->evAsEnd, s));
         c->timedAs = timing;

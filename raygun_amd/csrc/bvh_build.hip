@@ -542,6 +542,81 @@ __global__ __launch_bounds__(kCollapseThreads) void k_collapse(const Node* __res
 }
 
 // ---------------------------------------------------------------------------
+// 8. compress the wide nodes
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float pow2f(int e) { return __uint_as_float((uint32_t)(e + 127) << 23); }
+
+// Smallest e in [-126, 127] with 254 * 2^e >= ext (one grid step to spare
+// for the outward rounding).
+__device__ __forceinline__ int quant_exponent(float ext)
+{
+    if(!(ext > 0.0f)) return -126;
+    int e = (int)((__float_as_uint(ext) >> 23) & 0xffu) - 127 - 7;
+    e = min(127, max(-126, e));
+    while(e < 127 && 254.0f * pow2f(e) < ext) ++e;
+    while(e > -126 && 254.0f * pow2f(e - 1) >= ext) --e;
+    return e;
+}
+
+__device__ __forceinline__ void quantize_range(float o, int e, float lo, float hi, uint8_t& qlo, uint8_t& qhi)
+{
+    const float s = pow2f(e), inv = pow2f(-min(e, 126));
+    int a = (int)floorf((lo - o) * inv);
+    a = min(255, max(0, a));
+    while(a > 0 && o + (float)a * s > lo) --a;
+    int b = (int)ceilf((hi - o) * inv);
+    b = min(255, max(0, b));
+    while(b < 255 && o + (float)b * s < hi) ++b;
+    qlo = (uint8_t)a;
+    qhi = (uint8_t)b;
+}
+
+__global__ void k_quantize(const Node4* __restrict__ wide, const uint32_t* __restrict__ numWide, uint32_t capacity,
+                           Node4q* __restrict__ out)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if(i >= min(*numWide, capacity)) return;
+    const Node4 w = wide[i];
+    Node4q q;
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    const float* clo[3] = {w.lox, w.loy, w.loz};
+    const float* chi[3] = {w.hix, w.hiy, w.hiz};
+    for(int c = 0; c < 4; ++c) {
+        if(w.child[c] == kEmptyChild) continue;
+        for(int k = 0; k < 3; ++k) {
+            lo[k] = fminf(lo[k], clo[k][c]);
+            hi[k] = fmaxf(hi[k], chi[k][c]);
+        }
+    }
+    int e[3];
+    float org[3];
+    for(int k = 0; k < 3; ++k) {
+        const bool any = lo[k] <= hi[k];
+        org[k] = any ? lo[k] : 0.0f;
+        e[k] = any ? quant_exponent(hi[k] - lo[k]) : -126;
+    }
+    q.ox = org[0];
+    q.oy = org[1];
+    q.oz = org[2];
+    q.ex = (int8_t)e[0];
+    q.ey = (int8_t)e[1];
+    q.ez = (int8_t)e[2];
+    q.pad0 = 0;
+    q.pad1[0] = q.pad1[1] = 0;
+    uint8_t* qlo[3] = {q.qlox, q.qloy, q.qloz};
+    uint8_t* qhi[3] = {q.qhix, q.qhiy, q.qhiz};
+    for(int c = 0; c < 4; ++c) {
+        q.child[c] = w.child[c];
+        for(int k = 0; k < 3; ++k) {
+            qlo[k][c] = 0;
+            qhi[k][c] = 0;
+            if(w.child[c] != kEmptyChild) quantize_range(org[k], e[k], clo[k][c], chi[k][c], qlo[k][c], qhi[k][c]);
+        }
+    }
+    out[i] = q;
+}
+
+// ---------------------------------------------------------------------------
 // leaf payloads
 // ---------------------------------------------------------------------------
 __global__ void k_tri_gather(const rgrt_vertex* __restrict__ vb, const uint32_t* __restrict__ ib, uint32_t firstVertex,
@@ -681,6 +756,11 @@ void LbvhBuilder::collapse(hipStream_t s, uint32_t n, const Node* nodes, Node4* 
     (void)n;
     k_collapse<<<1, kCollapseThreads, 0, s>>>(nodes, wide, reinterpret_cast<int32_t*>(keysB), reinterpret_cast<int32_t*>(valsB),
                                               numWide);
+}
+
+void LbvhBuilder::quantize(hipStream_t s, uint32_t capacity, const Node4* wide, const uint32_t* numWide, Node4q* out)
+{
+    k_quantize<<<blocks_for(capacity, 256), 256, 0, s>>>(wide, numWide, capacity, out);
 }
 
 void LbvhBuilder::recompute_leaf_parents(hipStream_t s, uint32_t n, const Node* nodes)

@@ -43,6 +43,21 @@ struct __align__(16) Node4 {
 };
 static_assert(sizeof(Node4) == 128, "wide node is 128 bytes");
 
+// Compressed 4-wide node, 64 bytes — what the traversal reads. The node's
+// box min corner, a power-of-two scale 2^e per axis, and every child box as
+// 8-bit multiples of the scale from that corner, rounded outwards. Read as
+// four 16-byte vectors: (origin, exponent bytes), (qlo/qhi x, qlo/qhi y),
+// (qlo/qhi z, pad), child refs.
+struct __align__(16) Node4q {
+    float ox, oy, oz;
+    int8_t ex, ey, ez, pad0;
+    uint8_t qlox[4], qhix[4], qloy[4], qhiy[4];
+    uint8_t qloz[4], qhiz[4];
+    uint32_t pad1[2];
+    int32_t child[4];
+};
+static_assert(sizeof(Node4q) == 64, "compressed wide node is 64 bytes");
+
 // Leaf triangle in Morton order: the three vertices and the original
 // primitive index (gl_PrimitiveID), 48 bytes.
 struct __align__(16) Tri {
@@ -66,7 +81,7 @@ struct __align__(16) DevInstance {
 static_assert(sizeof(DevInstance) == 64, "instance leaf is 64 bytes");
 
 struct DevBlas {
-    const Node4* nodes;
+    const Node4q* nodes;
     const Tri* tris;
     uint32_t numFaces;
     uint32_t pad;
@@ -82,7 +97,7 @@ static_assert(sizeof(InstanceShade) == 64, "shade record is 64 bytes");
 
 // Scene seen by the traversal.
 struct TraceScene {
-    const Node4* tlasNodes;
+    const Node4q* tlasNodes;
     const DevInstance* instances; // Morton order
     const DevBlas* blas;
     uint32_t numInstances;

@@ -40,6 +40,8 @@ struct LbvhBuilder {
     // Collapses the binary tree of n leaves into 4-wide nodes (breadth-first);
     // writes the wide node count to *numWide (device).
     void collapse(hipStream_t s, uint32_t n, const Node* nodes, Node4* wide, uint32_t* numWide);
+    // Compresses the wide nodes (count read from *numWide, at most capacity).
+    void quantize(hipStream_t s, uint32_t capacity, const Node4* wide, const uint32_t* numWide, Node4q* out);
 };
 
 void launch_tri_boxes(hipStream_t s, const rgrt_vertex* vb, const uint32_t* ib, uint32_t firstVertex, uint32_t indexOffset,

@@ -43,54 +43,21 @@ __device__ __forceinline__ float safe_rcp(float d)
 }
 
 struct BoxRay {
-    float ix, iy, iz;    // 1/d
-    float nox, noy, noz; // -o/d
+    float ox, oy, oz; // origin
+    float ix, iy, iz; // 1/d (zero components made finite)
 };
 
 __device__ __forceinline__ void box_ray_setup(BoxRay& b, float ox, float oy, float oz, float dx, float dy, float dz)
 {
+    b.ox = ox;
+    b.oy = oy;
+    b.oz = oz;
     b.ix = safe_rcp(dx);
     b.iy = safe_rcp(dy);
     b.iz = safe_rcp(dz);
-    b.nox = -ox * b.ix;
-    b.noy = -oy * b.iy;
-    b.noz = -oz * b.iz;
 }
 
-// Slab test of both children of a node. Only culls, so it uses fused
-// multiply-adds; the far distance is widened by 1 + 2*gamma(3) (Ize 2013)
-// to stay conservative.
-__device__ __forceinline__ void box_test2(const BoxRay& r, const float4 a, const float4 b, const float4 c, float tmin,
-                                          float tmax, bool& h0, bool& h1, float& tn0, float& tn1)
-{
-    // a = (lo0x lo0y lo0z lo1x), b = (lo1y lo1z hi0x hi0y), c = (hi0z hi1x hi1y hi1z)
-    const float x00 = __builtin_fmaf(a.x, r.ix, r.nox), x01 = __builtin_fmaf(b.z, r.ix, r.nox);
-    const float y00 = __builtin_fmaf(a.y, r.iy, r.noy), y01 = __builtin_fmaf(b.w, r.iy, r.noy);
-    const float z00 = __builtin_fmaf(a.z, r.iz, r.noz), z01 = __builtin_fmaf(c.x, r.iz, r.noz);
-    const float x10 = __builtin_fmaf(a.w, r.ix, r.nox), x11 = __builtin_fmaf(c.y, r.ix, r.nox);
-    const float y10 = __builtin_fmaf(b.x, r.iy, r.noy), y11 = __builtin_fmaf(c.z, r.iy, r.noy);
-    const float z10 = __builtin_fmaf(b.y, r.iz, r.noz), z11 = __builtin_fmaf(c.w, r.iz, r.noz);
-    const float n0 = fmaxf(fmaxf(fminf(x00, x01), fminf(y00, y01)), fmaxf(fminf(z00, z01), tmin));
-    const float f0 = fminf(fminf(fmaxf(x00, x01), fmaxf(y00, y01)), fminf(fmaxf(z00, z01), tmax));
-    const float n1 = fmaxf(fmaxf(fminf(x10, x11), fminf(y10, y11)), fmaxf(fminf(z10, z11), tmin));
-    const float f1 = fminf(fminf(fmaxf(x10, x11), fmaxf(y10, y11)), fminf(fmaxf(z10, z11), tmax));
-    h0 = n0 <= f0 * 1.0000004f;
-    h1 = n1 <= f1 * 1.0000004f;
-    tn0 = n0;
-    tn1 = n1;
-}
-
-// Slab test of one child box of a wide node (same arithmetic as box_test2).
-__device__ __forceinline__ float box_enter(const BoxRay& r, float lx, float hx, float ly, float hy, float lz, float hz, float tmin,
-                                           float tmax)
-{
-    const float x0 = __builtin_fmaf(lx, r.ix, r.nox), x1 = __builtin_fmaf(hx, r.ix, r.nox);
-    const float y0 = __builtin_fmaf(ly, r.iy, r.noy), y1 = __builtin_fmaf(hy, r.iy, r.noy);
-    const float z0 = __builtin_fmaf(lz, r.iz, r.noz), z1 = __builtin_fmaf(hz, r.iz, r.noz);
-    const float n = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), tmin));
-    const float f = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tmax));
-    return n <= f * 1.0000004f ? n : INFINITY; // entry distance, +inf on a miss
-}
+__device__ __forceinline__ float pow2i(int e) { return __uint_as_float((uint32_t)(e + 127) << 23); }
 
 __device__ __forceinline__ void cswap(float& ta, int32_t& ra, float& tb, int32_t& rb)
 {
@@ -103,20 +70,42 @@ __device__ __forceinline__ void cswap(float& ta, int32_t& ra, float& tb, int32_t
     ra = r;
 }
 
-// Visits a wide node: tests the four child boxes, orders the hit children by
-// entry distance with the comparator network (0,1) (2,3) (0,2) (1,3) (1,2)
-// (a swap only on a strictly smaller distance), pushes all but the nearest
-// farthest first and returns the nearest (kEmptyChild if none was hit).
-template<typename Stack>
-__device__ __forceinline__ int32_t visit_wide(const Node4* nodes, int32_t ref, const BoxRay& br, float tmin, float tbest, Stack& stack)
+// Entry distance of child c of a compressed node, +inf on a miss. A plane at
+// origin + q * 2^e lies at t = q * (2^e / d) + (origin - o) / d: one fused
+// multiply-add per plane (the box test only culls). The far distance is
+// widened by 1 + 2*gamma(3) (Ize 2013) to stay conservative.
+__device__ __forceinline__ float qbox_enter(uint32_t wx, uint32_t hxw, uint32_t wy, uint32_t hyw, uint32_t wz, uint32_t hzw, int c,
+                                            float ax, float bx, float ay, float by, float az, float bz, float tmin, float tmax)
 {
-    const float4* np = reinterpret_cast<const float4*>(nodes + ref);
-    const float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
-    const int4 ch = reinterpret_cast<const int4*>(np)[6];
-    float t0 = ch.x != kEmptyChild ? box_enter(br, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, tmin, tbest) : INFINITY;
-    float t1 = ch.y != kEmptyChild ? box_enter(br, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, tmin, tbest) : INFINITY;
-    float t2 = ch.z != kEmptyChild ? box_enter(br, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, tmin, tbest) : INFINITY;
-    float t3 = ch.w != kEmptyChild ? box_enter(br, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, tmin, tbest) : INFINITY;
+    const int sh = 8 * c;
+    const float x0 = __builtin_fmaf((float)((wx >> sh) & 0xffu), ax, bx), x1 = __builtin_fmaf((float)((hxw >> sh) & 0xffu), ax, bx);
+    const float y0 = __builtin_fmaf((float)((wy >> sh) & 0xffu), ay, by), y1 = __builtin_fmaf((float)((hyw >> sh) & 0xffu), ay, by);
+    const float z0 = __builtin_fmaf((float)((wz >> sh) & 0xffu), az, bz), z1 = __builtin_fmaf((float)((hzw >> sh) & 0xffu), az, bz);
+    const float n = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), tmin));
+    const float f = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tmax));
+    return n <= f * 1.0000004f ? n : INFINITY;
+}
+
+// Visits a compressed wide node (64 bytes, four 16-byte loads): tests the
+// four child boxes, orders the hit children by entry distance with the
+// comparator network (0,1) (2,3) (0,2) (1,3) (1,2) (a swap only on a strictly
+// smaller distance), pushes all but the nearest farthest first and returns
+// the nearest (kEmptyChild if none was hit).
+template<typename Stack>
+__device__ __forceinline__ int32_t visit_wide(const Node4q* nodes, int32_t ref, const BoxRay& br, float tmin, float tbest, Stack& stack)
+{
+    const uint4* np = reinterpret_cast<const uint4*>(nodes + ref);
+    const uint4 v0 = np[0], v1 = np[1], v2 = np[2];
+    const int4 ch = reinterpret_cast<const int4*>(np)[3];
+    const int ex = (int)(int8_t)(v0.w & 0xffu), ey = (int)(int8_t)((v0.w >> 8) & 0xffu), ez = (int)(int8_t)((v0.w >> 16) & 0xffu);
+    const float ax = pow2i(ex) * br.ix, ay = pow2i(ey) * br.iy, az = pow2i(ez) * br.iz;
+    const float bx = (__uint_as_float(v0.x) - br.ox) * br.ix;
+    const float by = (__uint_as_float(v0.y) - br.oy) * br.iy;
+    const float bz = (__uint_as_float(v0.z) - br.oz) * br.iz;
+    float t0 = ch.x != kEmptyChild ? qbox_enter(v1.x, v1.y, v1.z, v1.w, v2.x, v2.y, 0, ax, bx, ay, by, az, bz, tmin, tbest) : INFINITY;
+    float t1 = ch.y != kEmptyChild ? qbox_enter(v1.x, v1.y, v1.z, v1.w, v2.x, v2.y, 1, ax, bx, ay, by, az, bz, tmin, tbest) : INFINITY;
+    float t2 = ch.z != kEmptyChild ? qbox_enter(v1.x, v1.y, v1.z, v1.w, v2.x, v2.y, 2, ax, bx, ay, by, az, bz, tmin, tbest) : INFINITY;
+    float t3 = ch.w != kEmptyChild ? qbox_enter(v1.x, v1.y, v1.z, v1.w, v2.x, v2.y, 3, ax, bx, ay, by, az, bz, tmin, tbest) : INFINITY;
     int32_t r0 = t0 != INFINITY ? ch.x : kEmptyChild;
     int32_t r1 = t1 != INFINITY ? ch.y : kEmptyChild;
     int32_t r2 = t2 != INFINITY ? ch.z : kEmptyChild;
@@ -265,7 +254,7 @@ __device__ HitRec traverse(const TraceScene& sc, const RayIn& ray, uint32_t cull
     BoxRay br;
     box_ray_setup(br, ray.ox, ray.oy, ray.oz, ray.dx, ray.dy, ray.dz);
     WtRay wt;
-    const Node4* nodes = sc.tlasNodes;
+    const Node4q* nodes = sc.tlasNodes;
     const Tri* tris = nullptr;
     bool inBlas = false;
     uint32_t curInst = 0;

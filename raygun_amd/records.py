@@ -57,6 +57,10 @@ TUNING = np.dtype([("wavefrontPasses", "<u4"), ("traceBlocksPerCU", "<u4"), ("sh
 WIDE_NODE = np.dtype([("lox", "<f4", 4), ("hix", "<f4", 4), ("loy", "<f4", 4), ("hiy", "<f4", 4), ("loz", "<f4", 4),
                       ("hiz", "<f4", 4), ("child", "<i4", 4), ("pad", "<i4", 4)])
 
+QWIDE_NODE = np.dtype([("origin", "<f4", 3), ("exp", "i1", 3), ("pad0", "i1"),
+                       ("qlox", "u1", 4), ("qhix", "u1", 4), ("qloy", "u1", 4), ("qhiy", "u1", 4),
+                       ("qloz", "u1", 4), ("qhiz", "u1", 4), ("pad1", "<u4", 2), ("child", "<i4", 4)])
+
 NO_HIT = 0xFFFFFFFF
 EMPTY_CHILD = -(2 ** 31)
 
@@ -68,6 +72,7 @@ assert RAY.itemsize == 32
 assert HIT.itemsize == 20
 assert BVH_NODE.itemsize == 64
 assert WIDE_NODE.itemsize == 128
+assert QWIDE_NODE.itemsize == 64
 
 
 def default_material(**kw):

@@ -8,7 +8,7 @@ import os
 
 import numpy as np
 
-from .records import BVH_NODE, HIT, INSTANCE, MATERIAL, RAY, STATS_FIELDS, TUNING, UBO, VERTEX, WIDE_NODE
+from .records import BVH_NODE, HIT, INSTANCE, MATERIAL, RAY, STATS_FIELDS, TUNING, QWIDE_NODE, UBO, VERTEX, WIDE_NODE
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "librgrt.so")
@@ -91,8 +91,8 @@ def lib():
             "rgrt_get_stats": (i32, [P, P]),
             "rgrt_get_pass_counts": (i32, [P, P, u32, ctypes.POINTER(u32)]),
             "rgrt_set_tuning": (i32, [P, P]),
-            "rgrt_export_blas_wide": (i32, [P, u32, P, u32, ctypes.POINTER(u32)]),
-            "rgrt_export_tlas_wide": (i32, [P, P, u32, ctypes.POINTER(u32)]),
+            "rgrt_export_blas_wide": (i32, [P, u32, i32, P, u32, ctypes.POINTER(u32)]),
+            "rgrt_export_tlas_wide": (i32, [P, i32, P, u32, ctypes.POINTER(u32)]),
             "rgrt_get_tuning": (i32, [P, P]),
             "rgrt_version": (ctypes.c_char_p, []),
         }
@@ -181,18 +181,22 @@ class Context:
         self._ok(self._L.rgrt_export_blas(self.h, b, _p(nodes), nn, _p(leaf), nl))
         return nodes, leaf
 
-    def export_blas_wide(self, b):
+    def export_blas_wide(self, b, compressed=False):
+        """4-wide tree of BLAS b: WIDE_NODE records, or QWIDE_NODE (the form
+        the traversal walks) with compressed=True."""
+        c = int(bool(compressed))
         nn = ctypes.c_uint32()
-        self._ok(self._L.rgrt_export_blas_wide(self.h, b, None, 0, ctypes.byref(nn)))
-        nodes = np.zeros(nn.value, WIDE_NODE)
-        self._ok(self._L.rgrt_export_blas_wide(self.h, b, _p(nodes), nn.value, ctypes.byref(nn)))
+        self._ok(self._L.rgrt_export_blas_wide(self.h, b, c, None, 0, ctypes.byref(nn)))
+        nodes = np.zeros(nn.value, QWIDE_NODE if c else WIDE_NODE)
+        self._ok(self._L.rgrt_export_blas_wide(self.h, b, c, _p(nodes), nn.value, ctypes.byref(nn)))
         return nodes
 
-    def export_tlas_wide(self):
+    def export_tlas_wide(self, compressed=False):
+        c = int(bool(compressed))
         nn = ctypes.c_uint32()
-        self._ok(self._L.rgrt_export_tlas_wide(self.h, None, 0, ctypes.byref(nn)))
-        nodes = np.zeros(nn.value, WIDE_NODE)
-        self._ok(self._L.rgrt_export_tlas_wide(self.h, _p(nodes), nn.value, ctypes.byref(nn)))
+        self._ok(self._L.rgrt_export_tlas_wide(self.h, c, None, 0, ctypes.byref(nn)))
+        nodes = np.zeros(nn.value, QWIDE_NODE if c else WIDE_NODE)
+        self._ok(self._L.rgrt_export_tlas_wide(self.h, c, _p(nodes), nn.value, ctypes.byref(nn)))
         return nodes
 
     def build_tlas(self, instances):

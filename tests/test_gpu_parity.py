@@ -77,15 +77,19 @@ def _wide_equal(a, b):
         assert np.array_equal(a[k], b[k]), k
 
 
+def _qwide_equal(a, b):
+    assert len(a) == len(b)
+    assert a.tobytes() == b.tobytes()
+
+
 def test_wide_trees_match_oracle(c1, c2s):
     _print_case("wide-trees")
-    setup, osc, ctx = c1
-    _wide_equal(ctx.export_blas_wide(0), osc.export_blas_wide(0))
-    _wide_equal(ctx.export_tlas_wide(), osc.export_tlas_wide())
-    setup, osc, ctx = c2s
-    for b in range(5):
-        _wide_equal(ctx.export_blas_wide(b), osc.export_blas_wide(b))
-    _wide_equal(ctx.export_tlas_wide(), osc.export_tlas_wide())
+    for (setup, osc, ctx), nblas in ((c1, 1), (c2s, 5)):
+        for b in range(nblas):
+            _wide_equal(ctx.export_blas_wide(b), osc.export_blas_wide(b))
+            _qwide_equal(ctx.export_blas_wide(b, compressed=True), osc.export_blas_wide(b, compressed=True))
+        _wide_equal(ctx.export_tlas_wide(), osc.export_tlas_wide())
+        _qwide_equal(ctx.export_tlas_wide(compressed=True), osc.export_tlas_wide(compressed=True))
 
 
 @pytest.mark.parametrize("which", ["config1", "config2s"])
