@@ -27,7 +27,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-NODE_BYTES = 64   # rgrt::Node (also the 64-byte TLAS instance leaf)
+NODE_BYTES = 64   # compressed 4-wide node (rgrt::Node4q)
+INST_BYTES = 64   # TLAS instance leaf (rgrt::DevInstance)
 TRI_BYTES = 48    # rgrt::Tri
 RAY_BYTES = 40    # queue entry: origin, tmin, dir, tmax, pixel, flags
 HIT_BYTES = 20    # t, u, v, instance, primitive
@@ -64,6 +65,32 @@ class HipEvents:
         return ms.value
 
 
+def measured_traffic(kernel, rays_per_launch):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    (FETCH_SIZE + WRITE_SIZE per traced ray, scaled by the copy-kernel
+    calibration), at this run's rays per launch; (None, None) without one."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+        return round(t["hbm_bytes_per_ray"] * rays_per_launch), f"profiles/{kernel}_traffic.json"
+    except (OSError, KeyError, ValueError):
+        return None, None
+
+
+def copy_probe(mib):
+    """A plain elementwise copy of `mib` MiB (reads and writes that many
+    bytes): the calibration kernel for the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        torch.mul(a, 1.0, out=b)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_mib": mib, "bytes_read": n * 4, "bytes_written": n * 4, "launches": 3}), flush=True)
+
+
 def host_cores():
     try:
         return len(os.sched_getaffinity(0))
@@ -83,7 +110,12 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-rows", type=int, default=0, help="rows of the frame the CPU baseline traces (0 = auto)")
     ap.add_argument("--no-fxaa", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="run only the counter-calibration copy kernel over MIB MiB")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU only (the frame is one GPU's share); "
@@ -185,26 +217,45 @@ def main():
         # visit counts: every 27th row (40 rows of 1080p), all ray types in their frame proportions
         rows = list(range(13, H, max(1, H // 40)))
         vis = osc.render_counts(ubo, W, H, rows, threads=cores)
-        e_nodes, e_tris, p_hit = vis["nodes"] / vis["rays"], vis["tris"] / vis["rays"], vis["hits"] / vis["rays"]
-        # traversal bytes of one ray: queue entry + hit record + nodes + triangles
-        trav_bytes = RAY_BYTES + HIT_BYTES + NODE_BYTES * e_nodes + TRI_BYTES * e_tris
-        trace_ms = ts["traceKernelMs"]
-        launches = max(1, ts["traceLaunches"])
+        r = vis["rays"]
+        e_nodes, e_tris, p_hit = vis["nodes"] / r, vis["tris"] / r, vis["hits"] / r
+        e_inst = vis["inst_leaves"] / r
+        # traversal bytes of one ray: queue entry + hit record + wide nodes
+        # (TLAS and BLAS) + instance leaves + triangles
+        trav_bytes = RAY_BYTES + HIT_BYTES + NODE_BYTES * e_nodes + INST_BYTES * e_inst + TRI_BYTES * e_tris
         wf_rays = int(passes.sum())  # rays the wavefront traversal passes traced
-        achieved = wf_rays * trav_bytes / (trace_ms * 1e-3) / 1e9
-        roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "k_trace_queue", "avg_launch_ms": round(trace_ms / launches, 5), "launches_per_frame": launches,
-                "rays_in_launches": wf_rays, "bytes_per_ray": round(trav_bytes, 1),
-                "node_bytes": NODE_BYTES, "tri_bytes": TRI_BYTES,
-                "E_nodes": round(e_nodes, 2), "E_tris": round(e_tris, 2), "P_hit": round(p_hit, 4)}
-        if ts.get("drained"):
-            drain_rays = rays - wf_rays
-            drain_bytes = trav_bytes + GATHER_BYTES * p_hit
-            dach = drain_rays * drain_bytes / (ts["drainKernelMs"] * 1e-3) / 1e9
-            roof["drain"] = {"kernel": "k_drain", "ms": round(ts["drainKernelMs"], 4), "rays": drain_rays,
-                             "achieved": round(dach, 1), "frac": round(dach / HBM_PEAK_GBS, 4),
-                             "bytes_per_ray": round(drain_bytes, 1)}
+        drain_rays = rays - wf_rays
+        kernels = {
+            "k_trace_queue": {"ms": ts["traceKernelMs"], "launches": max(1, ts["traceLaunches"]), "rays": wf_rays,
+                              "bytes_per_ray": trav_bytes},
+            # the drain kernel also does closest-hit shading: its gathers count
+            "k_drain": {"ms": ts["drainKernelMs"], "launches": 1 if ts.get("drained") else 0, "rays": drain_rays,
+                        "bytes_per_ray": trav_bytes + GATHER_BYTES * p_hit},
+        }
+        dom = max(kernels, key=lambda k: kernels[k]["ms"] if kernels[k]["launches"] else -1.0)
+        roof = None
+        for name in (dom,) + tuple(k for k in kernels if k != dom):
+            kk = kernels[name]
+            if not kk["launches"] or kk["ms"] <= 0:
+                continue
+            avg_ms = kk["ms"] / kk["launches"]
+            alg_per_launch = kk["rays"] * kk["bytes_per_ray"] / kk["launches"]
+            achieved = alg_per_launch / (avg_ms * 1e-3) / 1e9
+            traffic, tsrc = measured_traffic(name, kk["rays"] / kk["launches"])
+            obj = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                   "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                   "kernel": name, "avg_launch_ms": round(avg_ms, 5), "launches_per_frame": kk["launches"],
+                   "rays_per_launch": round(kk["rays"] / kk["launches"]), "bytes_per_ray": round(kk["bytes_per_ray"], 1),
+                   "algorithmic_bytes_per_launch": round(alg_per_launch)}
+            if tsrc:
+                obj["traffic_source"] = tsrc
+            if roof is None:
+                roof = obj
+                roof.update({"node_bytes": NODE_BYTES, "inst_bytes": INST_BYTES, "tri_bytes": TRI_BYTES,
+                             "E_nodes": round(e_nodes, 2), "E_inst_leaves": round(e_inst, 3), "E_tris": round(e_tris, 2),
+                             "P_hit": round(p_hit, 4), "others": []})
+            else:
+                roof["others"].append(obj)
         log(f"[bench] roofline {roof}")
         if not args.no_cpu_baseline:
             # a band of rows sized for ~15 s of wall time on these cores,
@@ -244,7 +295,8 @@ def main():
                    "rays_per_frame": rays, "sky_lookups_per_frame": st["skyLookups"],
                    "width": W, "height": H, "seed": args.seed,
                    "stage_ms": {k: round(ts[k], 4) for k in ("asBuildMs", "rtOnlyMs", "roughMs", "postprocMs", "rtTotalMs",
-                                                             "traceKernelMs", "shadeKernelMs")}},
+                                                             "traceKernelMs", "shadeKernelMs", "drainKernelMs")},
+                   "tuning": ctx.tuning(), "wavefront_pass_rays": passes.tolist()},
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -18,6 +18,20 @@ constexpr int32_t kEmptyChild = (int32_t)0x80000000;
 constexpr int32_t kSentinel = (int32_t)0x80000000; // stack marker: return to the TLAS
 constexpr uint32_t kNoHit = 0xffffffffu;
 
+// Loads through pointers that the compiler cannot prove global (they are
+// read from other records) would otherwise be flat instructions, which also
+// wait on LDS traffic; these go through the global address space.
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ u32x4 ld_global16(const void* p)
+{
+    return *(const __attribute__((address_space(1))) u32x4*)p;
+}
+__device__ __forceinline__ u32x2 ld_global8(const void* p)
+{
+    return *(const __attribute__((address_space(1))) u32x2*)p;
+}
+
 // ---------------------------------------------------------------------------
 // BVH records in HBM
 // ---------------------------------------------------------------------------
@@ -80,12 +94,13 @@ struct __align__(16) DevInstance {
 };
 static_assert(sizeof(DevInstance) == 64, "instance leaf is 64 bytes");
 
-struct DevBlas {
+struct __align__(16) DevBlas {
     const Node4q* nodes;
     const Tri* tris;
     uint32_t numFaces;
-    uint32_t pad;
+    uint32_t pad[3];
 };
+static_assert(sizeof(DevBlas) == 32, "BLAS table entry is 32 bytes");
 
 // Per-instance data the shading reads, indexed by instance position.
 struct __align__(16) InstanceShade {

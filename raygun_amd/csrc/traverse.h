@@ -94,18 +94,24 @@ __device__ __forceinline__ float qbox_enter(uint32_t wx, uint32_t hxw, uint32_t 
 template<typename Stack>
 __device__ __forceinline__ int32_t visit_wide(const Node4q* nodes, int32_t ref, const BoxRay& br, float tmin, float tbest, Stack& stack)
 {
-    const uint4* np = reinterpret_cast<const uint4*>(nodes + ref);
-    const uint4 v0 = np[0], v1 = np[1], v2 = np[2];
-    const int4 ch = reinterpret_cast<const int4*>(np)[3];
+    const u32x4* np = reinterpret_cast<const u32x4*>(nodes + ref);
+    const u32x4 v0 = ld_global16(np), v1 = ld_global16(np + 1), v2 = ld_global16(np + 2), v3 = ld_global16(np + 3);
+    const int4 ch = make_int4((int)v3.x, (int)v3.y, (int)v3.z, (int)v3.w);
     const int ex = (int)(int8_t)(v0.w & 0xffu), ey = (int)(int8_t)((v0.w >> 8) & 0xffu), ez = (int)(int8_t)((v0.w >> 16) & 0xffu);
     const float ax = pow2i(ex) * br.ix, ay = pow2i(ey) * br.iy, az = pow2i(ez) * br.iz;
     const float bx = (__uint_as_float(v0.x) - br.ox) * br.ix;
     const float by = (__uint_as_float(v0.y) - br.oy) * br.iy;
     const float bz = (__uint_as_float(v0.z) - br.oz) * br.iz;
-    float t0 = ch.x != kEmptyChild ? qbox_enter(v1.x, v1.y, v1.z, v1.w, v2.x, v2.y, 0, ax, bx, ay, by, az, bz, tmin, tbest) : INFINITY;
-    float t1 = ch.y != kEmptyChild ? qbox_enter(v1.x, v1.y, v1.z, v1.w, v2.x, v2.y, 1, ax, bx, ay, by, az, bz, tmin, tbest) : INFINITY;
-    float t2 = ch.z != kEmptyChild ? qbox_enter(v1.x, v1.y, v1.z, v1.w, v2.x, v2.y, 2, ax, bx, ay, by, az, bz, tmin, tbest) : INFINITY;
-    float t3 = ch.w != kEmptyChild ? qbox_enter(v1.x, v1.y, v1.z, v1.w, v2.x, v2.y, 3, ax, bx, ay, by, az, bz, tmin, tbest) : INFINITY;
+    // empty slots hold zero offsets, so their (discarded) tests are harmless
+    // and all four run without branches
+    const float e0 = qbox_enter(v1.x, v1.y, v1.z, v1.w, v2.x, v2.y, 0, ax, bx, ay, by, az, bz, tmin, tbest);
+    const float e1 = qbox_enter(v1.x, v1.y, v1.z, v1.w, v2.x, v2.y, 1, ax, bx, ay, by, az, bz, tmin, tbest);
+    const float e2 = qbox_enter(v1.x, v1.y, v1.z, v1.w, v2.x, v2.y, 2, ax, bx, ay, by, az, bz, tmin, tbest);
+    const float e3 = qbox_enter(v1.x, v1.y, v1.z, v1.w, v2.x, v2.y, 3, ax, bx, ay, by, az, bz, tmin, tbest);
+    float t0 = ch.x != kEmptyChild ? e0 : INFINITY;
+    float t1 = ch.y != kEmptyChild ? e1 : INFINITY;
+    float t2 = ch.z != kEmptyChild ? e2 : INFINITY;
+    float t3 = ch.w != kEmptyChild ? e3 : INFINITY;
     int32_t r0 = t0 != INFINITY ? ch.x : kEmptyChild;
     int32_t r1 = t1 != INFINITY ? ch.y : kEmptyChild;
     int32_t r2 = t2 != INFINITY ? ch.z : kEmptyChild;
@@ -240,87 +246,131 @@ __device__ __forceinline__ void xform_ray(const float* w, float ox, float oy, fl
     rdz = (w[8] * dx + w[9] * dy) + w[10] * dz;
 }
 
+// Traversal state of one ray between steps.
+struct Traversal {
+    RayIn ray;
+    BoxRay br;
+    WtRay wt;
+    const Node4q* nodes;
+    const Tri* tris;
+    float tbest;
+    int32_t ref;
+    uint32_t cullMask;
+    uint32_t curInst;
+    bool inBlas;
+    HitRec h;
+};
+
+// Starts a ray at the TLAS root; false if there is nothing to traverse.
+template<typename Stack>
+__device__ __forceinline__ bool trav_begin(Traversal& T, const TraceScene& sc, const RayIn& ray, uint32_t cullMask, Stack& stack)
+{
+    T.ray = ray;
+    T.h.t = 0.0f;
+    T.h.u = 0.0f;
+    T.h.v = 0.0f;
+    T.h.inst = kNoHit;
+    T.h.prim = 0;
+    T.nodes = sc.tlasNodes;
+    T.tris = nullptr;
+    T.inBlas = false;
+    T.curInst = 0;
+    T.tbest = ray.tmax;
+    T.ref = 0;
+    T.cullMask = cullMask;
+    stack.sp = 0;
+    box_ray_setup(T.br, ray.ox, ray.oy, ray.oz, ray.dx, ray.dy, ray.dz);
+    return sc.numInstances != 0;
+}
+
+// One step: a wide node, a sentinel, an instance leaf or a triangle.
+// Returns false once the ray is finished (T.h holds the closest hit).
+template<typename Stack>
+__device__ __forceinline__ bool trav_step(Traversal& T, const TraceScene& sc, Stack& stack)
+{
+    if(T.ref >= 0) {
+        const int32_t nearest = visit_wide(T.nodes, T.ref, T.br, T.ray.tmin, T.tbest, stack);
+        if(nearest != kEmptyChild) {
+            T.ref = nearest;
+            return true;
+        }
+    }
+    else if(T.ref == kSentinel) {
+        T.inBlas = false;
+        T.nodes = sc.tlasNodes;
+        box_ray_setup(T.br, T.ray.ox, T.ray.oy, T.ray.oz, T.ray.dx, T.ray.dy, T.ray.dz);
+    }
+    else if(!T.inBlas) {
+        const int32_t leaf = ~T.ref;
+        const u32x4* ip = reinterpret_cast<const u32x4*>(sc.instances + leaf);
+        const u32x4 i3 = ld_global16(ip + 3); // blas, mask, position, customIndex
+        const u32x4* bp = reinterpret_cast<const u32x4*>(sc.blas + i3.x);
+        const u32x4 b0 = ld_global16(bp);     // nodes, tris
+        const uint32_t numFaces = ld_global8(bp + 1).x;
+        if((i3.y & T.cullMask) != 0u && numFaces > 0u) {
+            float w[12];
+            for(int q = 0; q < 3; ++q) {
+                const u32x4 r = ld_global16(ip + q);
+                w[4 * q + 0] = __uint_as_float(r.x);
+                w[4 * q + 1] = __uint_as_float(r.y);
+                w[4 * q + 2] = __uint_as_float(r.z);
+                w[4 * q + 3] = __uint_as_float(r.w);
+            }
+            float rox, roy, roz, rdx, rdy, rdz;
+            xform_ray(w, T.ray.ox, T.ray.oy, T.ray.oz, T.ray.dx, T.ray.dy, T.ray.dz, rox, roy, roz, rdx, rdy, rdz);
+            box_ray_setup(T.br, rox, roy, roz, rdx, rdy, rdz);
+            wt_setup(T.wt, rox, roy, roz, rdx, rdy, rdz);
+            stack.push(kSentinel);
+            T.nodes = reinterpret_cast<const Node4q*>((uint64_t)b0.x | ((uint64_t)b0.y << 32));
+            T.tris = reinterpret_cast<const Tri*>((uint64_t)b0.z | ((uint64_t)b0.w << 32));
+            T.inBlas = true;
+            T.curInst = i3.z;
+            T.ref = 0;
+            return true;
+        }
+    }
+    else {
+        const int32_t leaf = ~T.ref;
+        const u32x4* tp = reinterpret_cast<const u32x4*>(T.tris + leaf);
+        const u32x4 a = ld_global16(tp), b = ld_global16(tp + 1), cc = ld_global16(tp + 2);
+        Tri tri;
+        tri.v0x = __uint_as_float(a.x);
+        tri.v0y = __uint_as_float(a.y);
+        tri.v0z = __uint_as_float(a.z);
+        tri.prim = a.w;
+        tri.v1x = __uint_as_float(b.x);
+        tri.v1y = __uint_as_float(b.y);
+        tri.v1z = __uint_as_float(b.z);
+        tri.v2x = __uint_as_float(cc.x);
+        tri.v2y = __uint_as_float(cc.y);
+        tri.v2z = __uint_as_float(cc.z);
+        float t, u, v;
+        if(wt_intersect(T.wt, tri, T.ray.tmin, T.tbest, t, u, v)) {
+            T.tbest = t;
+            T.h.t = t;
+            T.h.u = u;
+            T.h.v = v;
+            T.h.inst = T.curInst;
+            T.h.prim = tri.prim;
+        }
+    }
+    if(stack.sp == 0) return false;
+    T.ref = stack.pop();
+    return true;
+}
+
+// Whole traversal of one ray. Each node and leaf is visited at most once
+// per level, so a well-formed two-level tree ends far below the step cap;
+// the cap only bounds the loop.
 template<typename Stack>
 __device__ HitRec traverse(const TraceScene& sc, const RayIn& ray, uint32_t cullMask, Stack& stack)
 {
-    HitRec h;
-    h.t = 0.0f;
-    h.u = 0.0f;
-    h.v = 0.0f;
-    h.inst = kNoHit;
-    h.prim = 0;
-    if(sc.numInstances == 0) return h;
-
-    BoxRay br;
-    box_ray_setup(br, ray.ox, ray.oy, ray.oz, ray.dx, ray.dy, ray.dz);
-    WtRay wt;
-    const Node4q* nodes = sc.tlasNodes;
-    const Tri* tris = nullptr;
-    bool inBlas = false;
-    uint32_t curInst = 0;
-    float tbest = ray.tmax;
-    const float tmin = ray.tmin;
-    int32_t ref = 0;
-    stack.sp = 0;
-
-    // Each node and leaf is visited at most once per level, so a well-formed
-    // two-level tree ends far below this cap; it only bounds the loop.
-    for(uint32_t guard = 0; guard < (1u << 26); ++guard) {
-        if(ref >= 0) {
-            const int32_t nearest = visit_wide(nodes, ref, br, tmin, tbest, stack);
-            if(nearest != kEmptyChild) {
-                ref = nearest;
-            }
-            else {
-                if(stack.sp == 0) break;
-                ref = stack.pop();
-            }
-            continue;
-        }
-        if(ref == kSentinel) {
-            inBlas = false;
-            nodes = sc.tlasNodes;
-            box_ray_setup(br, ray.ox, ray.oy, ray.oz, ray.dx, ray.dy, ray.dz);
-            if(stack.sp == 0) break;
-            ref = stack.pop();
-            continue;
-        }
-        const int32_t leaf = ~ref;
-        if(!inBlas) {
-            const DevInstance& di = sc.instances[leaf];
-            const DevBlas bl = sc.blas[di.blas];
-            if((di.mask & cullMask) != 0u && bl.numFaces > 0u) {
-                float w[12];
-                for(int q = 0; q < 12; ++q) w[q] = di.w2o[q];
-                float rox, roy, roz, rdx, rdy, rdz;
-                xform_ray(w, ray.ox, ray.oy, ray.oz, ray.dx, ray.dy, ray.dz, rox, roy, roz, rdx, rdy, rdz);
-                box_ray_setup(br, rox, roy, roz, rdx, rdy, rdz);
-                wt_setup(wt, rox, roy, roz, rdx, rdy, rdz);
-                stack.push(kSentinel);
-                nodes = bl.nodes;
-                tris = bl.tris;
-                inBlas = true;
-                curInst = di.position;
-                ref = 0;
-                continue;
-            }
-        }
-        else {
-            const Tri tri = tris[leaf];
-            float t, u, v;
-            if(wt_intersect(wt, tri, tmin, tbest, t, u, v)) {
-                tbest = t;
-                h.t = t;
-                h.u = u;
-                h.v = v;
-                h.inst = curInst;
-                h.prim = tri.prim;
-            }
-        }
-        if(stack.sp == 0) break;
-        ref = stack.pop();
+    Traversal T;
+    if(trav_begin(T, sc, ray, cullMask, stack)) {
+        for(uint32_t guard = 0; guard < (1u << 26); ++guard)
+            if(!trav_step(T, sc, stack)) break;
     }
-    return h;
+    return T.h;
 }
 
 } // namespace rgrt

@@ -35,6 +35,15 @@ namespace {
 constexpr int RT_GENERIC = 0, RT_SHADOW_TRACE = 1, RT_SHADOW_INTERNAL = 2;
 constexpr int kShadeBlock = 256;
 
+// Minimum waves per SIMD the shading kernels are compiled for (the register
+// budget is then 512 / waves VGPRs; what does not fit spills to scratch).
+#ifndef RGRT_SHADE_WAVES
+#define RGRT_SHADE_WAVES 1
+#endif
+#ifndef RGRT_DRAIN_WAVES
+#define RGRT_DRAIN_WAVES 1
+#endif
+
 enum Stage : int { G_SHADOW = 1, G_REFL = 2, G_REFR = 3, SI_TRACE = 4, SI_SKY = 5, ST_INTERNAL = 6 };
 enum Step : int { RET = 0, EMIT_FRAME = 1, EMIT = 2, DONE = 3 };
 
@@ -532,14 +541,35 @@ __device__ void accumulate(const ShadeParams& p, uint32_t pix, const Payload& P)
     a[11 * n] = a[11 * n] + P.depth;
 }
 
-__device__ void finalize(const ShadeParams& p, uint32_t pix)
+// Pixel of wavefront slot `slot`. Slots run over 8x8 tiles (row-major
+// within a tile, tiles row-major within bands of 8 rows; the last band and
+// the last tile of a band may be smaller), so the 64 rays of a wave come from
+// one compact tile and take similar paths through the trees.
+__device__ __forceinline__ void slot_pixel(uint32_t slot, uint32_t W, uint32_t H, uint32_t& px, uint32_t& py)
 {
-    const float* a = p.st.accum + pix;
+    const uint32_t band = slot / (8u * W);
+    const uint32_t y0 = band * 8u;
+    const uint32_t bh = min(8u, H - y0);
+    const uint32_t o = slot - y0 * W;
+    const uint32_t tc = o / (8u * bh);
+    const uint32_t i = o - tc * 8u * bh;
+    const uint32_t x0 = tc * 8u;
+    const uint32_t tw = min(8u, W - x0);
+    px = x0 + i % tw;
+    py = y0 + i / tw;
+}
+
+__device__ void finalize(const ShadeParams& p, uint32_t slot)
+{
+    const float* a = p.st.accum + slot;
     const size_t n = p.st.npix;
     const float ns = (float)p.ubo.numSamples;
     const float r0[4] = {a[0 * n] / ns, a[1 * n] / ns, a[2 * n] / ns, a[10 * n] / ns};
     const float r1[4] = {a[3 * n] / ns, a[4 * n] / ns, a[5 * n] / ns, (logf(a[11 * n]) * 0.25f) / ns};
     const float r2[4] = {a[6 * n] / ns, a[7 * n] / ns, a[8 * n] / ns, a[9 * n] / ns};
+    uint32_t px, py;
+    slot_pixel(slot, p.width, p.height, px, py);
+    const uint32_t pix = py * p.width + px;
     write_rgba16(p.img.base, pix, r0[0], r0[1], r0[2], r0[3]);
     write_rgba16(p.img.normal, pix, r1[0], r1[1], r1[2], r1[3]);
     write_rgba16(p.img.rough, pix, r2[0], r2[1], r2[2], r2[3]);
@@ -579,7 +609,8 @@ __global__ __launch_bounds__(kShadeBlock) void k_raygen(ShadeParams p)
         for(int k = 4; k < 12; ++k) c[k] = 0; // ray / sky / zero-dir counters (u64)
     }
     if(pix >= p.st.npix) return;
-    const uint32_t px = pix % p.width, py = pix / p.width;
+    uint32_t px, py;
+    slot_pixel(pix, p.width, p.height, px, py);
     OutRay r;
     primary_ray(p.ubo, px, py, p.width, p.height, 0, r);
     Payload P;
@@ -609,7 +640,9 @@ __device__ bool advance_pixel(const Ctx& c, uint32_t pix, uint32_t flags, F3 O, 
         a.v = hit ? h.v : 0.0f;
         a.instance = hit ? p.inst[min(h.inst, p.numInstances - 1)].customIndex : kNoHit;
         a.primitive = hit ? h.prim : 0u;
-        p.img.primary[pix] = a;
+        uint32_t px, py;
+        slot_pixel(pix, p.width, p.height, px, py);
+        p.img.primary[py * p.width + px] = a;
     }
     Frame F;
     int step;
@@ -647,7 +680,9 @@ __device__ bool advance_pixel(const Ctx& c, uint32_t pix, uint32_t flags, F3 O, 
             sample++;
             if(sample < p.ubo.numSamples) {
                 init_payload(P);
-                primary_ray(p.ubo, pix % p.width, pix / p.width, p.width, p.height, sample, out);
+                uint32_t px, py;
+                slot_pixel(pix, p.width, p.height, px, py);
+                primary_ray(p.ubo, px, py, p.width, p.height, sample, out);
                 step = EMIT;
                 continue;
             }
@@ -672,7 +707,7 @@ __device__ __forceinline__ HitRec load_hit(const float* hits, size_t n, uint32_t
 }
 
 // Wavefront shading pass over the rays traced by the preceding traversal pass.
-__global__ __launch_bounds__(kShadeBlock) void k_shade(ShadeParams p, int cur)
+__global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(RGRT_SHADE_WAVES))) void k_shade(ShadeParams p, int cur)
 {
     const uint32_t count = p.st.counters[cur];
     const size_t n = p.st.npix;
@@ -727,7 +762,7 @@ constexpr int kDrainSpill = 368; // see traverse.h: 16 + 368 >= 381
 // the queue (one atomic per wave, a ballot to hand out the entries). The
 // recursion tail thus costs the longest remaining chain of one pixel rather
 // than one launch per recursion step.
-__global__ __launch_bounds__(kDrainBlock) void k_drain(ShadeParams p, TraceScene sc, int cur, int32_t* __restrict__ spillBase)
+__global__ __launch_bounds__(kDrainBlock) __attribute__((amdgpu_waves_per_eu(RGRT_DRAIN_WAVES))) void k_drain(ShadeParams p, TraceScene sc, int cur, int32_t* __restrict__ spillBase)
 {
     __shared__ int32_t ldsStack[kDrainLdsStack * kDrainBlock];
     const uint32_t count = p.st.counters[cur];

@@ -11,7 +11,8 @@ import numpy as np
 from .records import BVH_NODE, HIT, INSTANCE, MATERIAL, RAY, STATS_FIELDS, TUNING, QWIDE_NODE, UBO, VERTEX, WIDE_NODE
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "librgrt.so")
+# RGRT_LIBRARY selects another build of the library (e.g. a tuning variant).
+LIB_PATH = os.environ.get("RGRT_LIBRARY") or os.path.join(HERE, "librgrt.so")
 
 CFG_FP32_RADIANCE = 0x1
 CFG_PRIMARY_AOV = 0x2
