@@ -472,16 +472,30 @@ inline int quant_exponent(float ext)
     return e;
 }
 
-// Outward-rounded 8-bit offsets of [lo, hi] from origin o on the grid 2^e.
+// Sign of the exact (o + p) - x for floats o, p, x: the rounded sum and its
+// error from TwoSum (Knuth) decide it without rounding.
+inline int sum_cmp(float o, float p, float x)
+{
+    const float sum = o + p;
+    if(sum < x) return -1;
+    if(sum > x) return 1;
+    const float bv = sum - o;
+    const float err = (o - (sum - bv)) + (p - bv);
+    return err < 0.0f ? -1 : (err > 0.0f ? 1 : 0);
+}
+
+// Outward-rounded 8-bit offsets of [lo, hi] from origin o on the grid 2^e:
+// the largest a with o + a*2^e <= lo and the smallest b with o + b*2^e >= hi,
+// exactly (a*2^e is exact for a < 256).
 inline void quantize_range(float o, int e, float lo, float hi, uint8_t& qlo, uint8_t& qhi)
 {
     const float s = pow2f(e), inv = pow2f(-std::min(e, 126));
     int a = (int)std::floor((lo - o) * inv);
     a = std::min(255, std::max(0, a));
-    while(a > 0 && o + (float)a * s > lo) --a;
+    while(a > 0 && sum_cmp(o, (float)a * s, lo) > 0) --a;
     int b = (int)std::ceil((hi - o) * inv);
     b = std::min(255, std::max(0, b));
-    while(b < 255 && o + (float)b * s < hi) ++b;
+    while(b < 255 && sum_cmp(o, (float)b * s, hi) < 0) ++b;
     qlo = (uint8_t)a;
     qhi = (uint8_t)b;
 }

@@ -558,15 +558,30 @@ __device__ __forceinline__ int quant_exponent(float ext)
     return e;
 }
 
+// Sign of the exact (o + p) - x: the rounded sum and its TwoSum error
+// (Knuth) decide it without rounding.
+__device__ __forceinline__ int sum_cmp(float o, float p, float x)
+{
+    const float sum = o + p;
+    if(sum < x) return -1;
+    if(sum > x) return 1;
+    const float bv = sum - o;
+    const float err = (o - (sum - bv)) + (p - bv);
+    return err < 0.0f ? -1 : (err > 0.0f ? 1 : 0);
+}
+
+// Outward-rounded 8-bit offsets of [lo, hi] from origin o on the grid 2^e:
+// the largest a with o + a*2^e <= lo and the smallest b with o + b*2^e >= hi,
+// exactly (a*2^e is exact for a < 256).
 __device__ __forceinline__ void quantize_range(float o, int e, float lo, float hi, uint8_t& qlo, uint8_t& qhi)
 {
     const float s = pow2f(e), inv = pow2f(-min(e, 126));
     int a = (int)floorf((lo - o) * inv);
     a = min(255, max(0, a));
-    while(a > 0 && o + (float)a * s > lo) --a;
+    while(a > 0 && sum_cmp(o, (float)a * s, lo) > 0) --a;
     int b = (int)ceilf((hi - o) * inv);
     b = min(255, max(0, b));
-    while(b < 255 && o + (float)b * s < hi) ++b;
+    while(b < 255 && sum_cmp(o, (float)b * s, hi) < 0) ++b;
     qlo = (uint8_t)a;
     qhi = (uint8_t)b;
 }

@@ -8,6 +8,7 @@ oracle is pinned here by a closed form, an invariant, an independent
 double-precision computation or brute force on small inputs.
 """
 import math
+from fractions import Fraction
 
 import numpy as np
 import pytest
@@ -131,6 +132,116 @@ def test_lbvh_empty_mesh_has_one_empty_node():
     nodes, leaf = osc.export_blas(b)
     assert len(nodes) == 1 and len(leaf) == 0
     assert nodes["child"][0].tolist() == [-(2 ** 31), -(2 ** 31)]
+
+
+def _wide_checks(wide, qwide, n, boxes):
+    """Invariants of the 4-wide tree and of its compressed form."""
+    EMPTY = -(2 ** 31)
+    child = wide["child"]
+    assert np.array_equal(qwide["child"], child)
+    seen_leaf = np.zeros(n, int)
+    seen_node = np.zeros(len(wide), int)
+    for w in range(len(wide)):
+        refs = [int(c) for c in child[w]]
+        used = [c for c in refs if c != EMPTY]
+        assert 1 <= len(used) <= 4
+        assert refs[:len(used)] == used  # children fill the low slots
+        # fewer than four children only when none of them could be opened
+        if len(used) < 4:
+            assert all(c < 0 for c in used)
+        for c in used:
+            if c < 0:
+                seen_leaf[~c] += 1
+            else:
+                seen_node[c] += 1
+    assert np.all(seen_leaf == 1)
+    assert seen_node[0] == 0 and np.all(seen_node[1:] == 1)
+
+    # every child box is the exact union of the leaf boxes below it, and the
+    # leaves below a child form a contiguous Morton range
+    import sys
+    sys.setrecursionlimit(10000)
+    def span(ref):
+        if ref < 0:
+            return boxes[~ref][0], boxes[~ref][1], ~ref, ~ref
+        lo, hi, a, b = None, None, None, None
+        for c in child[ref]:
+            if c == EMPTY:
+                continue
+            l, h, x, y = span(int(c))
+            lo = l if lo is None else np.minimum(lo, l)
+            hi = h if hi is None else np.maximum(hi, h)
+            assert b is None or b + 1 == x
+            a = x if a is None else a
+            b = y
+        return lo, hi, a, b
+    assert span(0)[2:] == (0, n - 1)
+    for w in range(len(wide)):
+        for k in range(4):
+            c = int(child[w][k])
+            if c == EMPTY:
+                continue
+            lo, hi, _, _ = span(c)
+            got_lo = np.array([wide["lox"][w][k], wide["loy"][w][k], wide["loz"][w][k]])
+            got_hi = np.array([wide["hix"][w][k], wide["hiy"][w][k], wide["hiz"][w][k]])
+            assert np.array_equal(got_lo, lo) and np.array_equal(got_hi, hi)
+
+    # compressed boxes: origin = node box min corner; 2^e the smallest power
+    # of two with 254 * 2^e >= the node extent; every child box contains the
+    # exact one and exceeds it by less than one grid step per side
+    for w in range(len(wide)):
+        used = [k for k in range(4) if child[w][k] != EMPTY]
+        for ax, (lk, hk, qlk, qhk, ok, ek) in enumerate(
+                [("lox", "hix", "qlox", "qhix", 0, 0), ("loy", "hiy", "qloy", "qhiy", 1, 1),
+                 ("loz", "hiz", "qloz", "qhiz", 2, 2)]):
+            lo = np.array([wide[lk][w][k] for k in used], np.float64)
+            hi = np.array([wide[hk][w][k] for k in used], np.float64)
+            o = float(qwide["origin"][w][ok])
+            e = int(qwide["exp"][w][ek])
+            assert o == lo.min()
+            ext = np.float32(np.float32(hi.max()) - np.float32(lo.min()))
+            if ext > 0:
+                assert 254.0 * 2.0 ** e >= ext
+                assert e == -126 or 254.0 * 2.0 ** (e - 1) < ext
+            step = Fraction(2) ** e
+            fo = Fraction(o)
+            for j, k in enumerate(used):
+                qlo = int(qwide[qlk][w][k])
+                qhi = int(qwide[qhk][w][k])
+                dlo = fo + qlo * step  # exact plane positions
+                dhi = fo + qhi * step
+                flo, fhi = Fraction(float(lo[j])), Fraction(float(hi[j]))
+                assert dlo <= flo and dhi >= fhi
+                assert dlo + step > flo  # the largest such grid plane
+                assert dhi - step < fhi
+
+
+@pytest.mark.parametrize("n,dup", [(1, False), (2, False), (5, False), (7, False), (500, False), (300, True)])
+def test_wide_and_compressed_trees(n, dup):
+    rng = np.random.default_rng(100 + n)
+    centers = rng.uniform(-5, 5, (n, 3))
+    if dup:
+        centers[: n // 2] = centers[0]
+    tris = centers[:, None, :] + rng.uniform(-0.3, 0.3, (n, 3, 3))
+    mesh = _mesh_from_tris(tris)
+    osc = oracle_py.OracleScene(mesh.vertices, mesh.indices, np.array([default_material()]))
+    b = osc.build_blas(0, 0, n)
+    _, leaf = osc.export_blas(b)
+    t32 = tris.astype(np.float32)
+    boxes = np.stack([t32.min(axis=1), t32.max(axis=1)], axis=1)[leaf]  # in Morton order
+    _wide_checks(osc.export_blas_wide(b), osc.export_blas_wide(b, compressed=True), n, boxes)
+
+
+def test_wide_tree_of_the_config1_mesh():
+    setup = procgen.config1()
+    osc, inst, (v, i, m) = oracle_scene(setup)
+    mesh = setup.scene.models()[0].mesh
+    _, leaf = osc.export_blas(0)
+    pos = v["position"].astype(np.float32)
+    io = mesh.index_offset_bytes // 4
+    tri = pos[mesh.first_vertex + i[io: io + 3 * mesh.num_faces()].reshape(-1, 3)]
+    boxes = np.stack([tri.min(axis=1), tri.max(axis=1)], axis=1)[leaf]
+    _wide_checks(osc.export_blas_wide(0), osc.export_blas_wide(0, compressed=True), mesh.num_faces(), boxes)
 
 
 # ---------------------------------------------------------------------------
