@@ -232,7 +232,7 @@ void alloc_images(rgrt_ctx* c, uint32_t w, uint32_t h)
     dev_alloc(s.payload, npix * kPayloadWords);
     dev_alloc(s.accum, npix * kAccumWords);
     dev_alloc(s.meta, npix);
-    dev_alloc(s.frames, npix * kFrameWords * kMaxFrames);
+    dev_alloc(s.frames, npix * kFrameStride * kMaxFrames);
     dev_alloc(s.rays[0], npix * kRayWords);
     dev_alloc(s.rays[1], npix * kRayWords);
     dev_alloc(s.hits, npix * kHitWords);

@@ -59,6 +59,7 @@ void launch_inst_gather(hipStream_t s, const float* w2o, const rgrt_instance* in
 constexpr int kPayloadWords = 16;
 constexpr int kAccumWords = 12;
 constexpr int kFrameWords = 34;
+constexpr int kFrameStride = 36; // words per stored frame: 34 + padding to 16-byte vectors
 constexpr int kMaxFrames = 8; // maxRecursions <= 7 keeps at most 7 frames waiting
 constexpr int kRayWords = 10; // origin, tmin, dir, tmax, pixel, flags
 constexpr int kHitWords = 5;
@@ -70,10 +71,12 @@ constexpr uint32_t kMaskShift = 8u;     // cull mask in bits 8..15
 
 struct WavefrontState {
     uint32_t npix = 0;
-    float* payload = nullptr; // [kPayloadWords][npix]
-    float* accum = nullptr;   // [kAccumWords][npix]
+    // Per-slot records are contiguous (a lane reads its slot's record with
+    // 16-byte loads, and consecutive lanes read consecutive records).
+    float* payload = nullptr; // [npix][kPayloadWords]
+    float* accum = nullptr;   // [npix][kAccumWords]
     uint32_t* meta = nullptr; // [npix]: sp | sample << 8
-    float* frames = nullptr;  // [kMaxFrames][kFrameWords][npix]
+    float* frames = nullptr;  // [kMaxFrames][npix][kFrameStride]
     float* rays[2] = {nullptr, nullptr}; // [kRayWords][npix]
     float* hits = nullptr;    // [kHitWords][npix]
     uint32_t* counters = nullptr; // [0..1] queue counts, [2..3] fetch, [4..] stats (u64 pairs)

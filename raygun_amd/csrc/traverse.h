@@ -249,6 +249,7 @@ __device__ __forceinline__ void xform_ray(const float* w, float ox, float oy, fl
 // Traversal state of one ray between steps.
 struct Traversal {
     RayIn ray;
+    float wix, wiy, wiz; // the world ray's slab reciprocals, kept for the return to the TLAS
     BoxRay br;
     WtRay wt;
     const Node4q* nodes;
@@ -280,6 +281,9 @@ __device__ __forceinline__ bool trav_begin(Traversal& T, const TraceScene& sc, c
     T.cullMask = cullMask;
     stack.sp = 0;
     box_ray_setup(T.br, ray.ox, ray.oy, ray.oz, ray.dx, ray.dy, ray.dz);
+    T.wix = T.br.ix;
+    T.wiy = T.br.iy;
+    T.wiz = T.br.iz;
     return sc.numInstances != 0;
 }
 
@@ -298,7 +302,12 @@ __device__ __forceinline__ bool trav_step(Traversal& T, const TraceScene& sc, St
     else if(T.ref == kSentinel) {
         T.inBlas = false;
         T.nodes = sc.tlasNodes;
-        box_ray_setup(T.br, T.ray.ox, T.ray.oy, T.ray.oz, T.ray.dx, T.ray.dy, T.ray.dz);
+        T.br.ox = T.ray.ox;
+        T.br.oy = T.ray.oy;
+        T.br.oz = T.ray.oz;
+        T.br.ix = T.wix;
+        T.br.iy = T.wiy;
+        T.br.iz = T.wiz;
     }
     else if(!T.inBlas) {
         const int32_t leaf = ~T.ref;
@@ -357,6 +366,13 @@ __device__ __forceinline__ bool trav_step(Traversal& T, const TraceScene& sc, St
     if(stack.sp == 0) return false;
     T.ref = stack.pop();
     return true;
+}
+
+// Kind of the step a ray takes next: 0 a wide node, 1 a triangle, 2 an
+// instance leaf or the sentinel (the return to the TLAS).
+__device__ __forceinline__ int trav_kind(const Traversal& T)
+{
+    return T.ref >= 0 ? 0 : ((T.ref != kSentinel && T.inBlas) ? 1 : 2);
 }
 
 // Whole traversal of one ray. Each node and leaf is visited at most once

@@ -105,62 +105,44 @@ struct Ctx {
 // ---------------------------------------------------------------------------
 // State in HBM
 // ---------------------------------------------------------------------------
+// The payload record is the Payload struct's 16 words in field order.
+static_assert(sizeof(Payload) == kPayloadWords * 4, "payload words");
+
 __device__ __forceinline__ void load_payload(const ShadeParams& p, uint32_t pix, Payload& P)
 {
-    const float* b = p.st.payload + pix;
-    const size_t n = p.st.npix;
-    P.hitValue = f3(b[0 * n], b[1 * n], b[2 * n]);
-    P.reflectContribution = b[3 * n];
-    P.normal = f3(b[4 * n], b[5 * n], b[6 * n]);
-    P.rv[0] = b[7 * n];
-    P.rv[1] = b[8 * n];
-    P.rv[2] = b[9 * n];
-    P.rv[3] = b[10 * n];
-    P.depth = b[11 * n];
-    P.curIOR = b[12 * n];
-    P.refDepth = b[13 * n];
-    P.rayType = __float_as_int(b[14 * n]);
-    P.recDepth = __float_as_int(b[15 * n]);
+    const float4* b = reinterpret_cast<const float4*>(p.st.payload) + (size_t)pix * 4;
+    float4 v[4];
+#pragma unroll
+    for(int k = 0; k < 4; ++k) v[k] = b[k];
+    __builtin_memcpy(&P, v, sizeof(Payload));
 }
 
 __device__ __forceinline__ void store_payload(const ShadeParams& p, uint32_t pix, const Payload& P)
 {
-    float* b = p.st.payload + pix;
-    const size_t n = p.st.npix;
-    b[0 * n] = P.hitValue.x;
-    b[1 * n] = P.hitValue.y;
-    b[2 * n] = P.hitValue.z;
-    b[3 * n] = P.reflectContribution;
-    b[4 * n] = P.normal.x;
-    b[5 * n] = P.normal.y;
-    b[6 * n] = P.normal.z;
-    b[7 * n] = P.rv[0];
-    b[8 * n] = P.rv[1];
-    b[9 * n] = P.rv[2];
-    b[10 * n] = P.rv[3];
-    b[11 * n] = P.depth;
-    b[12 * n] = P.curIOR;
-    b[13 * n] = P.refDepth;
-    b[14 * n] = __int_as_float(P.rayType);
-    b[15 * n] = __int_as_float(P.recDepth);
+    float4* b = reinterpret_cast<float4*>(p.st.payload) + (size_t)pix * 4;
+    float4 v[4];
+    __builtin_memcpy(v, &P, sizeof(Payload));
+#pragma unroll
+    for(int k = 0; k < 4; ++k) b[k] = v[k];
 }
 
 __device__ __forceinline__ void store_frame(const ShadeParams& p, uint32_t pix, int level, const Frame& F)
 {
-    const size_t n = p.st.npix;
-    float* b = p.st.frames + (size_t)level * kFrameWords * n + pix;
-    const float* w = reinterpret_cast<const float*>(&F);
+    float4* b = reinterpret_cast<float4*>(p.st.frames + ((size_t)level * p.st.npix + pix) * kFrameStride);
+    float4 v[kFrameStride / 4];
+    v[kFrameStride / 4 - 1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    __builtin_memcpy(v, &F, sizeof(Frame));
 #pragma unroll
-    for(int k = 0; k < kFrameWords; ++k) b[k * n] = w[k];
+    for(int k = 0; k < kFrameStride / 4; ++k) b[k] = v[k];
 }
 
 __device__ __forceinline__ void load_frame(const ShadeParams& p, uint32_t pix, int level, Frame& F)
 {
-    const size_t n = p.st.npix;
-    const float* b = p.st.frames + (size_t)level * kFrameWords * n + pix;
-    float* w = reinterpret_cast<float*>(&F);
+    const float4* b = reinterpret_cast<const float4*>(p.st.frames + ((size_t)level * p.st.npix + pix) * kFrameStride);
+    float4 v[kFrameStride / 4];
 #pragma unroll
-    for(int k = 0; k < kFrameWords; ++k) w[k] = b[k * n];
+    for(int k = 0; k < kFrameStride / 4; ++k) v[k] = b[k];
+    __builtin_memcpy(&F, v, sizeof(Frame));
 }
 
 __device__ __forceinline__ void init_payload(Payload& P) // raygen.h:92-100
@@ -523,22 +505,26 @@ __device__ __forceinline__ void write_rgba16(uint16_t* img, uint32_t pix, float 
 }
 
 // raygen.h:105-114 accumulation and raygen.rgen:35-38 stores.
+// Accumulator record: hitValue, normal, rv[4], reflectContribution, depth.
 __device__ void accumulate(const ShadeParams& p, uint32_t pix, const Payload& P)
 {
-    float* a = p.st.accum + pix;
-    const size_t n = p.st.npix;
-    a[0 * n] = a[0 * n] + P.hitValue.x;
-    a[1 * n] = a[1 * n] + P.hitValue.y;
-    a[2 * n] = a[2 * n] + P.hitValue.z;
-    a[3 * n] = a[3 * n] + P.normal.x;
-    a[4 * n] = a[4 * n] + P.normal.y;
-    a[5 * n] = a[5 * n] + P.normal.z;
-    a[6 * n] = a[6 * n] + P.rv[0];
-    a[7 * n] = a[7 * n] + P.rv[1];
-    a[8 * n] = a[8 * n] + P.rv[2];
-    a[9 * n] = a[9 * n] + P.rv[3];
-    a[10 * n] = a[10 * n] + P.reflectContribution;
-    a[11 * n] = a[11 * n] + P.depth;
+    float4* a = reinterpret_cast<float4*>(p.st.accum) + (size_t)pix * 3;
+    float4 a0 = a[0], a1 = a[1], a2 = a[2];
+    a0.x = a0.x + P.hitValue.x;
+    a0.y = a0.y + P.hitValue.y;
+    a0.z = a0.z + P.hitValue.z;
+    a0.w = a0.w + P.normal.x;
+    a1.x = a1.x + P.normal.y;
+    a1.y = a1.y + P.normal.z;
+    a1.z = a1.z + P.rv[0];
+    a1.w = a1.w + P.rv[1];
+    a2.x = a2.x + P.rv[2];
+    a2.y = a2.y + P.rv[3];
+    a2.z = a2.z + P.reflectContribution;
+    a2.w = a2.w + P.depth;
+    a[0] = a0;
+    a[1] = a1;
+    a[2] = a2;
 }
 
 // Pixel of wavefront slot `slot`. Slots run over 8x8 tiles (row-major
@@ -561,12 +547,13 @@ __device__ __forceinline__ void slot_pixel(uint32_t slot, uint32_t W, uint32_t H
 
 __device__ void finalize(const ShadeParams& p, uint32_t slot)
 {
-    const float* a = p.st.accum + slot;
-    const size_t n = p.st.npix;
+    const float4* av = reinterpret_cast<const float4*>(p.st.accum) + (size_t)slot * 3;
+    const float4 a0 = av[0], a1 = av[1], a2 = av[2];
+    const float a[12] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w, a2.x, a2.y, a2.z, a2.w};
     const float ns = (float)p.ubo.numSamples;
-    const float r0[4] = {a[0 * n] / ns, a[1 * n] / ns, a[2 * n] / ns, a[10 * n] / ns};
-    const float r1[4] = {a[3 * n] / ns, a[4 * n] / ns, a[5 * n] / ns, (logf(a[11 * n]) * 0.25f) / ns};
-    const float r2[4] = {a[6 * n] / ns, a[7 * n] / ns, a[8 * n] / ns, a[9 * n] / ns};
+    const float r0[4] = {a[0] / ns, a[1] / ns, a[2] / ns, a[10] / ns};
+    const float r1[4] = {a[3] / ns, a[4] / ns, a[5] / ns, (logf(a[11]) * 0.25f) / ns};
+    const float r2[4] = {a[6] / ns, a[7] / ns, a[8] / ns, a[9] / ns};
     uint32_t px, py;
     slot_pixel(slot, p.width, p.height, px, py);
     const uint32_t pix = py * p.width + px;
@@ -616,8 +603,8 @@ __global__ __launch_bounds__(kShadeBlock) void k_raygen(ShadeParams p)
     Payload P;
     init_payload(P);
     store_payload(p, pix, P);
-    float* a = p.st.accum + pix;
-    for(int k = 0; k < kAccumWords; ++k) a[(size_t)k * p.st.npix] = 0.0f;
+    float4* a = reinterpret_cast<float4*>(p.st.accum) + (size_t)pix * 3;
+    for(int k = 0; k < 3; ++k) a[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     p.st.meta[pix] = 0u;
     write_ray(p.st.rays[0], p.st.npix, pix, r, pix, kFlagPrimary | (0xffu << kMaskShift));
 }

@@ -1,0 +1,396 @@
+// raygun_rt.cpp — C++ host side of Raygun's ray-tracing path over the rgrt C ABI.
+// Class-by-class sources are listed in raygun_rt.hpp.
+#include "raygun_rt.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace raygun {
+
+namespace {
+
+void ok(rgrt_ctx* ctx, rgrt_status s, const char* what)
+{
+    if(s == RGRT_OK) return;
+    const char* msg = ctx ? rgrt_last_error(ctx) : nullptr;
+    throw Error(s, std::string(what) + ": " + (msg && *msg ? msg : "rgrt error " + std::to_string(s)));
+}
+
+vec3 cross(const vec3& a, const vec3& b)
+{
+    return {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+}
+
+vec3 normalized(const vec3& a)
+{
+    const double l = std::sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+    return {a[0] / l, a[1] / l, a[2] / l};
+}
+
+// rotation matrix of a unit quaternion (glm::toMat3), m[row][col]
+std::array<std::array<double, 3>, 3> quatToMat3(const quat& q)
+{
+    const double w = q[0], x = q[1], y = q[2], z = q[3];
+    return {{{1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)},
+             {2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)},
+             {2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)}}};
+}
+
+quat quatMul(const quat& a, const quat& b)
+{
+    return {a[0] * b[0] - a[1] * b[1] - a[2] * b[2] - a[3] * b[3], a[0] * b[1] + a[1] * b[0] + a[2] * b[3] - a[3] * b[2],
+            a[0] * b[2] - a[1] * b[3] + a[2] * b[0] + a[3] * b[1], a[0] * b[3] + a[1] * b[2] - a[2] * b[1] + a[3] * b[0]};
+}
+
+quat mat3ToQuat(const std::array<std::array<double, 3>, 3>& m)
+{
+    const double tr = m[0][0] + m[1][1] + m[2][2];
+    if(tr > 0) {
+        const double s = std::sqrt(tr + 1.0) * 2;
+        return {0.25 * s, (m[2][1] - m[1][2]) / s, (m[0][2] - m[2][0]) / s, (m[1][0] - m[0][1]) / s};
+    }
+    if(m[0][0] > m[1][1] && m[0][0] > m[2][2]) {
+        const double s = std::sqrt(1.0 + m[0][0] - m[1][1] - m[2][2]) * 2;
+        return {(m[2][1] - m[1][2]) / s, 0.25 * s, (m[0][1] + m[1][0]) / s, (m[0][2] + m[2][0]) / s};
+    }
+    if(m[1][1] > m[2][2]) {
+        const double s = std::sqrt(1.0 + m[1][1] - m[0][0] - m[2][2]) * 2;
+        return {(m[0][2] - m[2][0]) / s, (m[0][1] + m[1][0]) / s, 0.25 * s, (m[1][2] + m[2][1]) / s};
+    }
+    const double s = std::sqrt(1.0 + m[2][2] - m[0][0] - m[1][1]) * 2;
+    return {(m[1][0] - m[0][1]) / s, (m[0][2] + m[2][0]) / s, (m[1][2] + m[2][1]) / s, 0.25 * s};
+}
+
+} // namespace
+
+mat4 identity4()
+{
+    mat4 m{};
+    for(int i = 0; i < 4; ++i) m[i][i] = 1.0;
+    return m;
+}
+
+mat4 mul(const mat4& a, const mat4& b)
+{
+    mat4 r{};
+    for(int i = 0; i < 4; ++i)
+        for(int j = 0; j < 4; ++j) {
+            double s = 0.0;
+            for(int k = 0; k < 4; ++k) s += a[i][k] * b[k][j];
+            r[i][j] = s;
+        }
+    return r;
+}
+
+// Gauss-Jordan elimination with partial pivoting, in double.
+mat4 inverse(const mat4& m)
+{
+    double a[4][8];
+    for(int i = 0; i < 4; ++i)
+        for(int j = 0; j < 4; ++j) {
+            a[i][j] = m[i][j];
+            a[i][4 + j] = i == j ? 1.0 : 0.0;
+        }
+    for(int c = 0; c < 4; ++c) {
+        int p = c;
+        for(int r = c + 1; r < 4; ++r)
+            if(std::fabs(a[r][c]) > std::fabs(a[p][c])) p = r;
+        if(a[p][c] == 0.0) throw Error(RGRT_ERR_INVALID_ARG, "inverse: singular matrix");
+        if(p != c)
+            for(int j = 0; j < 8; ++j) std::swap(a[p][j], a[c][j]);
+        const double d = a[c][c];
+        for(int j = 0; j < 8; ++j) a[c][j] /= d;
+        for(int r = 0; r < 4; ++r) {
+            if(r == c) continue;
+            const double f = a[r][c];
+            if(f == 0.0) continue;
+            for(int j = 0; j < 8; ++j) a[r][j] -= f * a[c][j];
+        }
+    }
+    mat4 r{};
+    for(int i = 0; i < 4; ++i)
+        for(int j = 0; j < 4; ++j) r[i][j] = a[i][4 + j];
+    return r;
+}
+
+// glm::quatLookAt (right-handed): the rotation whose -Z axis is `direction`
+quat quatLookAt(const vec3& direction, const vec3& up)
+{
+    const vec3 d = normalized(direction);
+    const vec3 nd = {-d[0], -d[1], -d[2]};
+    const vec3 right = normalized(cross(up, nd));
+    const vec3 upv = cross(nd, right);
+    std::array<std::array<double, 3>, 3> m{};
+    for(int r = 0; r < 3; ++r) {
+        m[r][0] = right[r];
+        m[r][1] = upv[r];
+        m[r][2] = nd[r];
+    }
+    return mat3ToQuat(m);
+}
+
+// transform.hpp:38-46: translate * toMat4(rotation) * scale
+mat4 Transform::toMat4() const
+{
+    const auto R = quatToMat3(rotation);
+    mat4 m = identity4();
+    for(int r = 0; r < 3; ++r) {
+        for(int c = 0; c < 3; ++c) m[r][c] = R[r][c] * scaling[c];
+        m[r][3] = position[r];
+    }
+    return m;
+}
+
+bool Transform::isZeroVolume() const { return scaling[0] * scaling[1] * scaling[2] == 0.0; }
+
+// transform.hpp:113-120
+Transform Transform::operator*(const Transform& y) const
+{
+    const auto R = quatToMat3(rotation);
+    Transform t;
+    for(int r = 0; r < 3; ++r) {
+        double s = 0.0;
+        for(int c = 0; c < 3; ++c) s += R[r][c] * (scaling[c] * y.position[c]);
+        t.position[r] = s + position[r];
+    }
+    t.rotation = quatMul(rotation, y.rotation);
+    for(int k = 0; k < 3; ++k) t.scaling[k] = scaling[k] * y.scaling[k];
+    return t;
+}
+
+// entity.cpp:187-199
+Transform Entity::globalTransform() const
+{
+    const Transform parentTransform = parent ? parent->globalTransform() : Transform();
+    return parentTransform * localTransform;
+}
+
+Entity& Entity::addChild(std::shared_ptr<Entity> e)
+{
+    e->parent = this;
+    children.push_back(std::move(e));
+    return *children.back();
+}
+
+// entity.hpp:67-84
+void Entity::forEachEntity(const std::function<bool(const Entity&)>& f) const
+{
+    if(!f(*this)) return;
+    for(const auto& c: children) c->forEachEntity(f);
+}
+
+void Camera::lookAt(const vec3& eye, const vec3& target)
+{
+    localTransform.position = eye;
+    localTransform.rotation = quatLookAt({target[0] - eye[0], target[1] - eye[1], target[2] - eye[2]});
+}
+
+// camera.cpp:34-47: glm::perspectiveRH_ZO (depth zero to one) with the
+// Y axis flipped for Vulkan's clip space.
+mat4 Camera::projection(uint32_t width, uint32_t height) const
+{
+    double aspect = (double)((float)width / (float)height);
+    if(!std::isfinite(aspect)) aspect = 16.0 / 9.0;
+    const double f = (double)(float)(fovDeg * (3.14159265358979323846 / 180.0)); // glm::radians
+    const double th = std::tan(f / 2.0);
+    mat4 p{};
+    p[0][0] = 1.0 / (aspect * th);
+    p[1][1] = -(1.0 / th);
+    p[2][2] = zFar / (zNear - zFar);
+    p[3][2] = -1.0;
+    p[2][3] = -(zFar * zNear) / (zFar - zNear);
+    return p;
+}
+
+std::vector<std::shared_ptr<render::Model>> Scene::models() const
+{
+    std::vector<std::shared_ptr<render::Model>> out;
+    root->forEachEntity([&](const Entity& e) {
+        if(e.model && std::find(out.begin(), out.end(), e.model) == out.end()) out.push_back(e.model);
+        return true;
+    });
+    return out;
+}
+
+namespace render {
+
+rgrt_material defaultMaterial()
+{
+    rgrt_material m{};
+    m.diffuse[0] = 1.0f;
+    m.diffuse[1] = 0.0f;
+    m.diffuse[2] = 1.0f;
+    m.transparency = 0.0f;
+    m.specular[0] = m.specular[1] = m.specular[2] = 1.0f;
+    m.reflectivity = 0.0f;
+    m.roughness = 0.0f;
+    m.ior = 1.0f;
+    m.effectId = 0;
+    m.rayConsumption = 1;
+    m.emission = 0.0f;
+    return m;
+}
+
+// acceleration_structure.cpp:140-194 (the range info of :167-170)
+BottomLevelAS::BottomLevelAS(rgrt_ctx* ctx, const Mesh& mesh)
+{
+    rgrt_blas_desc d{};
+    d.firstVertex = mesh.vertexBufferRef.offsetInElements();
+    d.indexOffsetBytes = mesh.indexBufferRef.offsetInElements() * 4u;
+    d.numFaces = mesh.numFaces();
+    d.maxVertex = (uint32_t)mesh.vertices.size();
+    ok(ctx, rgrt_build_blas(ctx, &d, &handle), "BottomLevelAS");
+}
+
+std::vector<rgrt_instance> TopLevelAS::collectInstances(const Scene& scene)
+{
+    std::vector<rgrt_instance> instances;
+    scene.root->forEachEntity([&](const Entity& entity) {
+        if(!entity.isVisible()) return false;          // do not descend
+        if(entity.transform().isZeroVolume()) return false;
+        if(!entity.model) return true;                  // children may still render
+        if(!entity.model->mesh || !entity.model->mesh->bottomLevelAS)
+            throw Error(RGRT_ERR_NOT_READY, "TopLevelAS: entity '" + entity.name + "' has no bottom-level structure");
+        rgrt_instance in{};
+        const mat4 m = entity.globalTransform().toMat4();
+        for(int r = 0; r < 3; ++r)
+            for(int c = 0; c < 4; ++c) in.transform[r * 4 + c] = (float)m[r][c];
+        in.customIndex = (uint32_t)instances.size();
+        in.mask = 0xffu;
+        in.flags = RGRT_INSTANCE_TRIANGLE_CULL_DISABLE;
+        in.blas = entity.model->mesh->bottomLevelAS->handle;
+        in.offsets.vertexBufferOffset = entity.model->mesh->vertexBufferRef.offsetInElements();
+        in.offsets.indexBufferOffset = entity.model->mesh->indexBufferRef.offsetInElements();
+        in.offsets.materialBufferOffset = entity.model->materialBufferRef.offsetInElements();
+        instances.push_back(in);
+        return true;
+    });
+    return instances;
+}
+
+TopLevelAS::TopLevelAS(rgrt_ctx* ctx, const Scene& scene) : m_instances(collectInstances(scene))
+{
+    ok(ctx, rgrt_build_tlas(ctx, m_instances.data(), (uint32_t)m_instances.size()), "TopLevelAS");
+}
+
+// raytracer.cpp:54-74: one structure per distinct mesh (the reference
+// builds one per model; models sharing a mesh get identical trees)
+void Raytracer::setupBottomLevelAS(const Scene& scene)
+{
+    for(const auto& model: scene.models())
+        if(model->mesh && !model->mesh->bottomLevelAS) model->mesh->bottomLevelAS = std::make_shared<BottomLevelAS>(m_ctx, *model->mesh);
+}
+
+// raytracer.cpp:76-85: rebuilt every frame
+void Raytracer::setupTopLevelAS(const Scene& scene) { m_topLevelAS = std::make_unique<TopLevelAS>(m_ctx, scene); }
+
+void Raytracer::updateRenderTarget(const rgrt_ubo& uniformBuffer) { ok(m_ctx, rgrt_set_frame(m_ctx, &uniformBuffer), "updateRenderTarget"); }
+
+// raytracer.cpp:87-147 + selectResultImage (:379-392)
+rgrt_image Raytracer::doRaytracing()
+{
+    ok(m_ctx, rgrt_render(m_ctx, useFXAA ? 1 : 0), "doRaytracing");
+    return RGRT_IMAGE_FINAL;
+}
+
+void packModels(const std::vector<std::shared_ptr<Model>>& models, std::vector<rgrt_vertex>& vertices,
+                std::vector<uint32_t>& indices, std::vector<rgrt_material>& materials)
+{
+    std::vector<Mesh*> meshes;
+    for(const auto& m: models)
+        if(m->mesh && std::find(meshes.begin(), meshes.end(), m->mesh.get()) == meshes.end()) meshes.push_back(m->mesh.get());
+    vertices.clear();
+    indices.clear();
+    materials.clear();
+    for(Mesh* mesh: meshes) {
+        mesh->vertexBufferRef.offset = (uint32_t)vertices.size();
+        mesh->indexBufferRef.offset = (uint32_t)indices.size();
+        vertices.insert(vertices.end(), mesh->vertices.begin(), mesh->vertices.end());
+        indices.insert(indices.end(), mesh->indices.begin(), mesh->indices.end());
+    }
+    for(const auto& m: models) {
+        m->materialBufferRef.offset = (uint32_t)materials.size();
+        materials.insert(materials.end(), m->materials.begin(), m->materials.end());
+    }
+}
+
+rgrt_ubo cameraUbo(const Camera& camera, uint32_t width, uint32_t height, const vec3& lightDir, int32_t numSamples,
+                   int32_t maxRecursions)
+{
+    rgrt_ubo u;
+    std::memset(&u, 0, sizeof(u));
+    const mat4 viewInverse = camera.globalTransform().toMat4();
+    const mat4 projInverse = inverse(camera.projection(width, height));
+    for(int c = 0; c < 4; ++c)
+        for(int r = 0; r < 4; ++r) {
+            u.viewInverse[c * 4 + r] = (float)viewInverse[r][c]; // column-major
+            u.projInverse[c * 4 + r] = (float)projInverse[r][c];
+        }
+    u.clearColor[0] = u.clearColor[1] = u.clearColor[2] = 0.2f;
+    u.numSamples = numSamples;
+    const vec3 l = normalized(lightDir);
+    for(int k = 0; k < 3; ++k) u.lightDir[k] = (float)l[k];
+    u.maxRecursions = maxRecursions;
+    u.time = 0.0f;
+    u.showAlpha = 0;
+    return u;
+}
+
+RenderSystem::RenderSystem(uint32_t width, uint32_t height, uint32_t flags, int32_t device) : m_width(width), m_height(height)
+{
+    rgrt_config cfg{device, width, height, flags};
+    const rgrt_status s = rgrt_create(&m_ctx, &cfg);
+    if(s != RGRT_OK) {
+        std::string msg = "RenderSystem: rgrt_create failed (status " + std::to_string(s) + ")";
+        if(m_ctx) {
+            msg += std::string(": ") + rgrt_last_error(m_ctx);
+            rgrt_destroy(m_ctx);
+        }
+        throw Error(s, msg);
+    }
+    m_raytracer = std::make_unique<Raytracer>(m_ctx);
+}
+
+RenderSystem::~RenderSystem()
+{
+    m_raytracer.reset();
+    if(m_ctx) rgrt_destroy(m_ctx);
+}
+
+void RenderSystem::setupModelBuffers(const Scene& scene)
+{
+    std::vector<rgrt_vertex> vertices;
+    std::vector<uint32_t> indices;
+    std::vector<rgrt_material> materials;
+    const auto models = scene.models();
+    for(const auto& m: models)
+        if(m->mesh) m->mesh->bottomLevelAS.reset(); // buffers change: trees are rebuilt
+    packModels(models, vertices, indices, materials);
+    ok(m_ctx, rgrt_upload_geometry(m_ctx, vertices.data(), vertices.size(), indices.data(), indices.size()), "setupModelBuffers");
+    ok(m_ctx, rgrt_upload_materials(m_ctx, materials.data(), (uint32_t)materials.size()), "setupModelBuffers");
+    m_raytracer->setupBottomLevelAS(scene);
+}
+
+rgrt_ubo RenderSystem::updateUniformBuffer(const Camera& camera, int32_t maxRecursions, int32_t numSamples, const vec3& lightDir)
+{
+    const rgrt_ubo u = cameraUbo(camera, m_width, m_height, lightDir, numSamples, maxRecursions);
+    m_raytracer->updateRenderTarget(u);
+    return u;
+}
+
+rgrt_image RenderSystem::render(const Scene& scene)
+{
+    m_raytracer->setupTopLevelAS(scene);
+    return m_raytracer->doRaytracing();
+}
+
+std::vector<uint16_t> RenderSystem::readImage(rgrt_image which) const
+{
+    std::vector<uint16_t> out((size_t)m_width * m_height * 4);
+    ok(m_ctx, rgrt_read_image(m_ctx, which, out.data(), out.size() * 2), "readImage");
+    return out;
+}
+
+} // namespace render
+} // namespace raygun

@@ -1,0 +1,209 @@
+// raygun_rt.hpp — C++ host side of Raygun's ray-tracing path over the rgrt C ABI.
+//
+// Mirrors the classes a Raygun renderer calls, with the reference's names and
+// call order, as far as the ray-tracing path consumes them:
+//
+//   Transform          raygun/transform.hpp:29-120 (T * R * S, operator*)
+//   Entity             raygun/entity.hpp:32-131, entity.cpp:180-199
+//   Camera             raygun/camera.hpp:29-48, camera.cpp:34-47
+//   Scene              raygun/scene.hpp:36-52
+//   render::Mesh       raygun/render/mesh.hpp:30-52 (+ BufferRef offsets)
+//   render::Model      raygun/render/model.hpp:33-42
+//   render::BottomLevelAS / TopLevelAS
+//                      raygun/render/acceleration_structure.hpp:34-73
+//   render::Raytracer  raygun/render/raytracer.hpp:36-46
+//   render::RenderSystem (setupModelBuffers, updateUniformBuffer, render)
+//                      raygun/render/render_system.cpp:88-330
+//
+// Vulkan objects become rgrt handles; the swapchain blit, ImGui and present
+// that follow doRaytracing in the reference are out of scope. Errors: the
+// reference aborts through RAYGUN_FATAL; here a failing rgrt call throws
+// raygun::Error carrying the library's status and message.
+#pragma once
+
+#include <array>
+#include <cstdint>
+#include <functional>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/rgrt.h"
+
+namespace raygun {
+
+struct Error : std::runtime_error {
+    rgrt_status status;
+    Error(rgrt_status s, const std::string& what) : std::runtime_error(what), status(s) {}
+};
+
+using vec3 = std::array<double, 3>;
+using quat = std::array<double, 4>; // w, x, y, z (glm order)
+using mat4 = std::array<std::array<double, 4>, 4>; // m[row][col], column vectors
+
+mat4 identity4();
+mat4 mul(const mat4& a, const mat4& b);
+mat4 inverse(const mat4& m);
+quat quatLookAt(const vec3& direction, const vec3& up = {0.0, 1.0, 0.0});
+
+// transform.hpp:29-120
+struct Transform {
+    vec3 position = {0.0, 0.0, 0.0};
+    quat rotation = {1.0, 0.0, 0.0, 0.0};
+    vec3 scaling = {1.0, 1.0, 1.0};
+
+    mat4 toMat4() const; // translate * toMat4(rotation) * scale
+    bool isZeroVolume() const;
+    Transform operator*(const Transform& y) const;
+};
+
+namespace render {
+
+struct BottomLevelAS;
+struct RenderSystem;
+
+// A range of elements in one of the packed scene buffers (gpu::BufferRef).
+struct BufferRef {
+    uint32_t offset = 0; // in elements
+    uint32_t offsetInElements() const { return offset; }
+};
+
+// mesh.hpp:30-52
+struct Mesh {
+    std::string name;
+    std::vector<rgrt_vertex> vertices;
+    std::vector<uint32_t> indices; // mesh-local
+    BufferRef vertexBufferRef, indexBufferRef;
+    std::shared_ptr<BottomLevelAS> bottomLevelAS; // one tree per distinct mesh
+
+    uint32_t numFaces() const { return (uint32_t)(indices.size() / 3); }
+};
+
+// model.hpp:33-42
+struct Model {
+    std::shared_ptr<Mesh> mesh;
+    std::vector<rgrt_material> materials;
+    BufferRef materialBufferRef;
+};
+
+rgrt_material defaultMaterial(); // gpu_material.def:11-23
+
+} // namespace render
+
+// entity.hpp:32-131
+struct Entity {
+    std::string name;
+    Transform localTransform;
+    std::shared_ptr<render::Model> model;
+    bool visible = true;
+    Entity* parent = nullptr;
+    std::vector<std::shared_ptr<Entity>> children;
+
+    explicit Entity(std::string n = "") : name(std::move(n)) {}
+    virtual ~Entity() = default;
+
+    const Transform& transform() const { return localTransform; }
+    bool isVisible() const { return visible; }
+    Transform globalTransform() const; // parentTransform() * m_transform
+    Entity& addChild(std::shared_ptr<Entity> e);
+    // pre-order; the callback returns false to skip the subtree
+    void forEachEntity(const std::function<bool(const Entity&)>& f) const;
+};
+
+// camera.hpp:29-48; the field of view is a parameter (reference: 45 degrees)
+struct Camera : Entity {
+    double fovDeg = 45.0, zNear = 0.1, zFar = 100.0;
+    Camera() : Entity("camera") {}
+    void lookAt(const vec3& eye, const vec3& target);
+    mat4 projection(uint32_t width, uint32_t height) const; // perspectiveRH_ZO, [1][1] *= -1
+};
+
+// scene.hpp:36-52
+struct Scene {
+    std::shared_ptr<Entity> root = std::make_shared<Entity>("root");
+    std::shared_ptr<Camera> camera = std::make_shared<Camera>();
+    // distinct models in entity pre-order (first use)
+    std::vector<std::shared_ptr<render::Model>> models() const;
+};
+
+namespace render {
+
+// acceleration_structure.hpp:55-73: one tree over a mesh's triangles.
+struct BottomLevelAS {
+    BottomLevelAS(rgrt_ctx* ctx, const Mesh& mesh);
+    explicit BottomLevelAS(rgrt_blas h) : handle(h) {} // a tree built elsewhere
+    rgrt_blas handle = 0;
+};
+
+// acceleration_structure.hpp:34-53 + instanceOffsetTable(): the DFS over
+// the entity tree (acceleration_structure.cpp:55-85). Invisible and
+// zero-volume subtrees are skipped, entities without a model are descended
+// into; instance i gets customIndex i, mask 0xff, the transposed global
+// transform's first three rows and its offset-table entry.
+struct TopLevelAS {
+    TopLevelAS(rgrt_ctx* ctx, const Scene& scene);
+    static std::vector<rgrt_instance> collectInstances(const Scene& scene);
+    const std::vector<rgrt_instance>& instances() const { return m_instances; }
+
+  private:
+    std::vector<rgrt_instance> m_instances;
+};
+
+// raytracer.hpp:36-46
+struct Raytracer {
+    explicit Raytracer(rgrt_ctx* ctx) : m_ctx(ctx) {}
+
+    void setupBottomLevelAS(const Scene& scene);
+    void setupTopLevelAS(const Scene& scene);
+    void updateRenderTarget(const rgrt_ubo& uniformBuffer);
+    // trace + rough prepare + 10 x (H, V) blur + postprocess + FXAA + swap;
+    // returns the image the reference's selectResultImage() would return
+    rgrt_image doRaytracing();
+
+    bool useFXAA = true; // m_useFXAA (raytracer.hpp:80)
+
+  private:
+    rgrt_ctx* m_ctx;
+    std::unique_ptr<TopLevelAS> m_topLevelAS;
+};
+
+// render_system.cpp: the parts around the ray tracer.
+struct RenderSystem {
+    RenderSystem(uint32_t width, uint32_t height, uint32_t flags = 0, int32_t device = 0);
+    ~RenderSystem();
+    RenderSystem(const RenderSystem&) = delete;
+    RenderSystem& operator=(const RenderSystem&) = delete;
+
+    // packs the distinct meshes and the models' materials into the shared
+    // buffers (render_system.cpp:192-223, 270-330) and builds the BLASes
+    // (Raygun::finalizeLoadScene, raygun.cpp:276-295)
+    void setupModelBuffers(const Scene& scene);
+    // resetUniformBuffer + updateUniformBuffer (render_system.cpp:235-268)
+    rgrt_ubo updateUniformBuffer(const Camera& camera, int32_t maxRecursions = 5, int32_t numSamples = 1,
+                                 const vec3& lightDir = {0.4, -0.6, -0.8});
+    // per frame: TLAS + doRaytracing (render_system.cpp:88-100)
+    rgrt_image render(const Scene& scene);
+    std::vector<uint16_t> readImage(rgrt_image which) const; // rgba16f bits
+
+    Raytracer& raytracer() { return *m_raytracer; }
+    rgrt_ctx* context() const { return m_ctx; }
+    uint32_t width() const { return m_width; }
+    uint32_t height() const { return m_height; }
+
+  private:
+    rgrt_ctx* m_ctx = nullptr;
+    uint32_t m_width, m_height;
+    std::unique_ptr<Raytracer> m_raytracer;
+};
+
+// packing without a context: fills the buffer refs, returns the buffers
+void packModels(const std::vector<std::shared_ptr<Model>>& models, std::vector<rgrt_vertex>& vertices,
+                std::vector<uint32_t>& indices, std::vector<rgrt_material>& materials);
+
+// UBO of a frame (render_system.cpp:235-268; matrices column-major)
+rgrt_ubo cameraUbo(const Camera& camera, uint32_t width, uint32_t height, const vec3& lightDir, int32_t numSamples,
+                   int32_t maxRecursions);
+
+} // namespace render
+} // namespace raygun
