@@ -92,6 +92,47 @@ def test_wide_trees_match_oracle(c1, c2s):
         _qwide_equal(ctx.export_tlas_wide(compressed=True), osc.export_tlas_wide(compressed=True))
 
 
+def test_refit_matches_oracle():
+    """Deformed vertices + refit on both sides: the same binary, wide and
+    compressed trees, and the same closest hits."""
+    from test_oracle import _refit_case
+    from raygun_amd.records import INSTANCE, default_material
+    _print_case("refit", n=400)
+    mesh, moved = _refit_case()
+    n = mesh.num_faces()
+    mats = np.array([default_material()])
+    osc = oracle_py.OracleScene(mesh.vertices, mesh.indices, mats)
+    ob = osc.build_blas(0, 0, n)
+    ctx = rgrt.Context(64, 64, 0)
+    try:
+        ctx.upload_geometry(mesh.vertices, mesh.indices)
+        ctx.upload_materials(mats)
+        gb = ctx.build_blas(0, 0, n, len(mesh.vertices))
+        osc.update_vertices(0, moved.vertices)
+        osc.refit_blas(ob)
+        ctx.update_vertices(0, moved.vertices)
+        ctx.refit_blas(gb)
+        g_nodes, g_leaf = ctx.export_blas(gb)
+        o_nodes, o_leaf = osc.export_blas(ob)
+        assert np.array_equal(g_leaf, o_leaf)
+        _nodes_equal(g_nodes, o_nodes)
+        _wide_equal(ctx.export_blas_wide(gb), osc.export_blas_wide(ob))
+        _qwide_equal(ctx.export_blas_wide(gb, compressed=True), osc.export_blas_wide(ob, compressed=True))
+        inst = np.zeros(1, INSTANCE)
+        inst["transform"] = [1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0]
+        inst["mask"] = 0xFF
+        inst["blas"] = gb
+        ctx.build_tlas(inst)
+        inst["blas"] = ob
+        osc.build_tlas(inst)
+        rays = random_rays(50_000, seed=13, radius=12.0)
+        hg = ctx.trace_rays(rays)
+        ho, tie = osc.trace_rays(rays)
+        assert len(compare_hits(hg, ho, tie)) == 0
+    finally:
+        ctx.close()
+
+
 @pytest.mark.parametrize("which", ["config1", "config2s"])
 def test_ray_queries_match_oracle(which, c1, c2s):
     setup, osc, ctx = c1 if which == "config1" else c2s

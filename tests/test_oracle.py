@@ -16,7 +16,7 @@ import pytest
 import oracle_py
 from helpers import NO_HIT, half, oracle_scene, random_rays
 from raygun_amd import procgen
-from raygun_amd.records import RAY, VERTEX, default_material
+from raygun_amd.records import INSTANCE, RAY, VERTEX, default_material
 from raygun_amd.scene import Camera, Entity, Mesh, Model, Scene, Transform, camera_ubo
 
 
@@ -242,6 +242,48 @@ def test_wide_tree_of_the_config1_mesh():
     tri = pos[mesh.first_vertex + i[io: io + 3 * mesh.num_faces()].reshape(-1, 3)]
     boxes = np.stack([tri.min(axis=1), tri.max(axis=1)], axis=1)[leaf]
     _wide_checks(osc.export_blas_wide(0), osc.export_blas_wide(0, compressed=True), mesh.num_faces(), boxes)
+
+
+def _refit_case(seed=7, n=400):
+    """A random triangle soup, its BLAS, and the soup deformed (same topology)."""
+    rng = np.random.default_rng(seed)
+    centers = rng.uniform(-5, 5, (n, 3))
+    tris = centers[:, None, :] + rng.uniform(-0.4, 0.4, (n, 3, 3))
+    moved = tris + rng.normal(0, 0.3, (n, 3, 3))
+    moved[: n // 10] += np.array([7.0, -3.0, 1.0])  # some triangles travel far
+    return _mesh_from_tris(tris), _mesh_from_tris(moved)
+
+
+def test_refit_equals_exact_boxes_of_the_moved_triangles():
+    """refit_blas keeps the topology and rewrites every box as the exact union
+    of the moved leaf boxes; the wide and compressed trees follow."""
+    mesh, moved = _refit_case()
+    n = mesh.num_faces()
+    osc = oracle_py.OracleScene(mesh.vertices, mesh.indices, np.array([default_material()]))
+    b = osc.build_blas(0, 0, n)
+    _, leaf0 = osc.export_blas(b)
+    osc.update_vertices(0, moved.vertices)
+    osc.refit_blas(b)
+    nodes, leaf = osc.export_blas(b)
+    assert np.array_equal(leaf, leaf0)  # same leaves in the same order
+    t32 = moved.vertices["position"].reshape(-1, 3, 3).astype(np.float32)
+    boxes = np.stack([t32.min(axis=1), t32.max(axis=1)], axis=1)
+    _check_lbvh(nodes, leaf, boxes, np.argsort(leaf).astype(np.uint32))
+    _wide_checks(osc.export_blas_wide(b), osc.export_blas_wide(b, compressed=True), n, boxes[leaf])
+    # and the refitted tree traces like brute force over the moved triangles
+    inst = np.zeros(1, INSTANCE)
+    inst["transform"] = [1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0]
+    inst["mask"] = 0xFF
+    inst["blas"] = b
+    osc.build_tlas(inst)
+    rays = random_rays(2000, seed=9, radius=12.0)
+    h, tie = osc.trace_rays(rays)
+    t64, inst64, prim64, uv64 = osc.trace_rays_fp64(rays)
+    hit32, hit64 = h["instance"] != NO_HIT, inst64 != NO_HIT
+    assert np.mean(hit32 == hit64) > 0.998
+    both = hit32 & hit64 & ~tie
+    assert both.sum() > 200
+    assert np.mean(h["primitive"][both] == prim64[both]) > 0.99
 
 
 # ---------------------------------------------------------------------------
