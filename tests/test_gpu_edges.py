@@ -1,0 +1,130 @@
+"""Edge cases of the scene description on the GPU, each against the oracle:
+no instances, a one-triangle BLAS, an empty (zero-face) BLAS next to a real
+one, instance masks against the cull mask, and an instance whose transform
+mirrors its mesh."""
+import numpy as np
+import pytest
+
+import oracle_py
+from helpers import NO_HIT, compare_hits, random_rays
+from raygun_amd import rgrt
+from raygun_amd.records import INSTANCE, VERTEX, default_material
+from raygun_amd.scene import Camera, camera_ubo
+
+pytestmark = pytest.mark.gpu
+
+
+def _tris_mesh(tris):
+    tris = np.asarray(tris, np.float64).reshape(-1, 3, 3)
+    v = np.zeros(len(tris) * 3, VERTEX)
+    v["position"] = tris.reshape(-1, 3)
+    v["normal"] = (0, 0, 1)
+    return v, np.arange(len(v), dtype=np.uint32)
+
+
+def _instance(blas, transform=None, mask=0xFF, custom=0, offsets=(0, 0, 0)):
+    r = np.zeros((), INSTANCE)
+    r["transform"] = np.eye(4)[:3].reshape(12) if transform is None else np.asarray(transform, np.float32).reshape(12)
+    r["mask"] = mask
+    r["flags"] = 1
+    r["blas"] = blas
+    r["customIndex"] = custom
+    r["offsets"] = offsets
+    return r
+
+
+def _both(v, i, mats, blas_ranges, instances, W=32, H=16):
+    """Same scene in a GPU context and in the oracle."""
+    ctx = rgrt.Context(W, H, rgrt.CFG_FP32_RADIANCE | rgrt.CFG_PRIMARY_AOV)
+    ctx.upload_geometry(v, i)
+    ctx.upload_materials(mats)
+    osc = oracle_py.OracleScene(v, i, mats)
+    for (first, io, nf) in blas_ranges:
+        assert ctx.build_blas(first, io, nf, len(v)) == osc.build_blas(first, io, nf)
+    inst = np.array(instances, INSTANCE)
+    ctx.build_tlas(inst)
+    osc.build_tlas(inst)
+    return ctx, osc
+
+
+def _render_both(ctx, osc, W=32, H=16):
+    cam = Camera(fov_deg=60.0)
+    cam.look_at((0.2, 0.3, 4.0), (0.0, 0.0, 0.0))
+    ubo = camera_ubo(cam, W, H, max_recursions=3)
+    ctx.set_frame(ubo)
+    ctx.trace()
+    g = ctx.read_image(rgrt.IMAGE_BASE_F32)
+    gp = ctx.read_image(rgrt.IMAGE_PRIMARY_HIT)
+    o = osc.render(ubo, W, H)
+    ok = ~o["tie"].astype(bool)
+    assert np.array_equal(gp["instance"][ok], o["primary"]["instance"][ok])
+    assert np.abs(g - o["base32"])[ok].max() <= 1e-3
+    return gp
+
+
+def test_no_instances_renders_sky():
+    print("[case] edge no-instances", flush=True)
+    v, i = _tris_mesh([[[0, 0, 0], [1, 0, 0], [0, 1, 0]]])
+    mats = np.array([default_material()])
+    ctx, osc = _both(v, i, mats, [(0, 0, 1)], [])
+    try:
+        gp = _render_both(ctx, osc)
+        assert np.all(gp["instance"] == NO_HIT)
+        h = ctx.trace_rays(random_rays(1000, seed=1))
+        assert np.all(h["instance"] == NO_HIT)
+    finally:
+        ctx.close()
+
+
+def test_one_triangle_and_empty_blas():
+    print("[case] edge one-triangle + empty BLAS", flush=True)
+    v, i = _tris_mesh([[[-1, -1, 0], [1, -1, 0], [0, 1, 0]], [[0, 0, 0], [1, 0, 0], [0, 1, 0]]])
+    mats = np.array([default_material(diffuse=(0.2, 0.7, 0.3))])
+    # BLAS 0: the first triangle; BLAS 1: zero faces
+    ctx, osc = _both(v, i, mats, [(0, 0, 1), (0, 12, 0)], [_instance(1, custom=0), _instance(0, custom=1)])
+    try:
+        gp = _render_both(ctx, osc)
+        assert np.any(gp["instance"] == 1) and not np.any(gp["instance"] == 0)
+        rays = random_rays(20_000, seed=2, radius=4.0)
+        hg = ctx.trace_rays(rays)
+        ho, tie = osc.trace_rays(rays)
+        assert len(compare_hits(hg, ho, tie)) == 0
+        assert (hg["instance"] != NO_HIT).sum() > 100
+    finally:
+        ctx.close()
+
+
+def test_instance_masks_and_cull_masks():
+    print("[case] edge instance masks", flush=True)
+    v, i = _tris_mesh([[[-1, -1, 0], [1, -1, 0], [0, 1, 0]]])
+    mats = np.array([default_material()])
+    far = [1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, -2.0]  # the same triangle 2 units behind
+    ctx, osc = _both(v, i, mats, [(0, 0, 1)], [_instance(0, mask=0x01, custom=0), _instance(0, far, mask=0x02, custom=1)])
+    try:
+        rays = np.zeros(64, rgrt.RAY)
+        rays["origin"] = np.stack([np.linspace(-0.3, 0.3, 64), np.full(64, -0.2), np.full(64, 3.0)], axis=1)
+        rays["dir"] = (0, 0, -1)
+        rays["tmin"], rays["tmax"] = 0.001, 100.0
+        for cull, expect in [(0xFF, 0), (0x01, 0), (0x02, 1), (0x04, NO_HIT), (0x00, NO_HIT)]:
+            hg = ctx.trace_rays(rays, cull_mask=cull)
+            ho, _ = osc.trace_rays(rays, cull_mask=cull)
+            assert np.all(hg["instance"] == expect), (cull, hg["instance"][:4])
+            assert np.array_equal(hg["instance"], ho["instance"]) and np.array_equal(hg["t"], ho["t"])
+    finally:
+        ctx.close()
+
+
+def test_mirrored_instance_transform():
+    print("[case] edge mirrored instance", flush=True)
+    v, i = _tris_mesh([[[-1, -1, 0], [1, -1, 0], [0, 1, 0]], [[0.5, 0.5, 0.3], [1.5, 0.5, 0.3], [0.5, 1.5, 0.3]]])
+    mats = np.array([default_material(diffuse=(0.9, 0.4, 0.1), reflectivity=0.3)])
+    mirror = [-1, 0, 0, 0.2, 0, 1, 0, 0, 0, 0, 1, 0]  # x -> -x: negative determinant
+    ctx, osc = _both(v, i, mats, [(0, 0, 2)], [_instance(0, mirror)])
+    try:
+        _render_both(ctx, osc)
+        rays = random_rays(20_000, seed=3, radius=4.0)
+        hg = ctx.trace_rays(rays)
+        ho, tie = osc.trace_rays(rays)
+        assert len(compare_hits(hg, ho, tie)) == 0
+    finally:
+        ctx.close()
