@@ -1,4 +1,5 @@
 // api.hip — the C ABI (include/rgrt.h) over the HIP kernels.
+#include <cstdio>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -150,6 +151,9 @@ struct rgrt_ctx {
         sc.instances = dInst;
         sc.blas = dBlasTable;
         sc.numInstances = haveTlas ? numInstances : 0;
+        sc.tlasNodeCap = std::max<uint32_t>(sc.numInstances, 2u) - 1u;
+        sc.numBlas = (uint32_t)blas.size();
+        sc.errors = st.counters + kTraversalErrorWord;
         return sc;
     }
 };
@@ -160,6 +164,20 @@ rgrt_status fail(rgrt_ctx* c, rgrt_status code, const std::string& msg)
 {
     if(c) c->err = msg;
     return code;
+}
+
+// The traversal kernels flag tree or table indices out of range (a corrupt
+// structure) instead of reading past an allocation; report it once the
+// stream is idle. The flag stays set: results after it are not trusted.
+rgrt_status traversal_errors(rgrt_ctx* c)
+{
+    uint32_t flags = 0;
+    check(hipMemcpy(&flags, c->st.counters + kTraversalErrorWord, 4, hipMemcpyDeviceToHost));
+    if(!flags) return RGRT_OK;
+    char buf[160];
+    std::snprintf(buf, sizeof(buf), "traversal found indices out of range (flags 0x%x: 1 node, 2 instance, 4 BLAS, 8 triangle)",
+                  flags);
+    return fail(c, RGRT_ERR_HIP, buf);
 }
 
 template<typename F>
@@ -524,7 +542,7 @@ rgrt_status rgrt_synchronize(rgrt_ctx* c)
 {
     return guarded(c, [&]() {
         check(hipStreamSynchronize(c->stream));
-        return RGRT_OK;
+        return traversal_errors(c);
     });
 }
 
@@ -807,7 +825,7 @@ rgrt_status rgrt_trace_rays(rgrt_ctx* c, const rgrt_ray* rays, uint32_t n, uint3
         launch_trace_rays_impl(c->stream, c->scene(), c->dRays, n, cullMask, c->dHits, c->dShade, c->spill, c->traceGrid);
         check(hipMemcpyAsync(hits, c->dHits, n * sizeof(rgrt_hit), hipMemcpyDeviceToHost, c->stream));
         check(hipStreamSynchronize(c->stream));
-        return RGRT_OK;
+        return traversal_errors(c);
     });
 }
 
@@ -869,7 +887,7 @@ rgrt_status rgrt_read_image(rgrt_ctx* c, rgrt_image which, void* dst, uint64_t b
         if(bytes != sz || !dst) return fail(c, RGRT_ERR_INVALID_ARG, "read_image: size mismatch");
         check(hipStreamSynchronize(c->stream));
         check(hipMemcpy(dst, p, sz, hipMemcpyDeviceToHost));
-        return RGRT_OK;
+        return traversal_errors(c);
     });
 }
 

@@ -116,7 +116,15 @@ struct TraceScene {
     const DevInstance* instances; // Morton order
     const DevBlas* blas;
     uint32_t numInstances;
+    uint32_t tlasNodeCap; // TLAS wide nodes allocated
+    uint32_t numBlas;     // entries of the BLAS table
+    uint32_t* errors;     // sticky flags of indices found out of range (kTravErr*)
 };
+
+// Flags a traversal sets when a tree or table index is out of range (the
+// ray then ends as a miss): a node, an instance leaf, a BLAS handle, a
+// triangle leaf.
+constexpr uint32_t kTravErrNode = 1u, kTravErrInstance = 2u, kTravErrBlas = 4u, kTravErrTriangle = 8u;
 
 // ---------------------------------------------------------------------------
 // GLSL built-ins (fp32, evaluated as written)

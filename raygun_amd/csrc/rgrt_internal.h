@@ -63,6 +63,7 @@ constexpr int kFrameStride = 36; // words per stored frame: 34 + padding to 16-b
 constexpr int kMaxFrames = 8; // maxRecursions <= 7 keeps at most 7 frames waiting
 constexpr int kRayWords = 10; // origin, tmin, dir, tmax, pixel, flags
 constexpr int kHitWords = 5;
+constexpr int kTraversalErrorWord = 12; // in WavefrontState::counters, sticky (cleared at allocation)
 
 // Ray flags in a queue entry
 constexpr uint32_t kFlagMiss1 = 1u;     // miss shader index 1 (shadow miss)
@@ -79,7 +80,7 @@ struct WavefrontState {
     float* frames = nullptr;  // [kMaxFrames][npix][kFrameStride]
     float* rays[2] = {nullptr, nullptr}; // [kRayWords][npix]
     float* hits = nullptr;    // [kHitWords][npix]
-    uint32_t* counters = nullptr; // [0..1] queue counts, [2..3] fetch, [4..] stats (u64 pairs)
+    uint32_t* counters = nullptr; // [0..1] queue counts, [2..3] fetch, [4..11] stats (u64 pairs), [12] traversal errors
 };
 
 struct Images {

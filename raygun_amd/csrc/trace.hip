@@ -142,12 +142,12 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_rays(TraceScene sc, const
     }
 }
 
-__global__ void k_hit_custom_index(rgrt_hit* hits, uint32_t n, const InstanceShade* __restrict__ inst)
+__global__ void k_hit_custom_index(rgrt_hit* hits, uint32_t n, const InstanceShade* __restrict__ inst, uint32_t numInstances)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if(i >= n) return;
     const uint32_t p = hits[i].instance;
-    if(p != kNoHit) hits[i].instance = inst[p].customIndex;
+    if(p != kNoHit) hits[i].instance = p < numInstances ? inst[p].customIndex : kNoHit;
 }
 
 } // namespace
@@ -167,7 +167,7 @@ void launch_trace_rays_impl(hipStream_t s, const TraceScene& sc, const rgrt_ray*
     if(!n) return;
     const uint32_t blocks = std::min<uint32_t>(gridBlocks, (n + kTraceBlock - 1) / kTraceBlock);
     k_trace_rays<<<blocks, kTraceBlock, 0, s>>>(sc, rays, n, cullMask, hits, spill);
-    k_hit_custom_index<<<(n + 255) / 256, 256, 0, s>>>(hits, n, inst);
+    k_hit_custom_index<<<(n + 255) / 256, 256, 0, s>>>(hits, n, inst, sc.numInstances);
 }
 
 } // namespace rgrt

@@ -249,7 +249,6 @@ __device__ __forceinline__ void xform_ray(const float* w, float ox, float oy, fl
 // Traversal state of one ray between steps.
 struct Traversal {
     RayIn ray;
-    float wix, wiy, wiz; // the world ray's slab reciprocals, kept for the return to the TLAS
     BoxRay br;
     WtRay wt;
     const Node4q* nodes;
@@ -258,9 +257,16 @@ struct Traversal {
     int32_t ref;
     uint32_t cullMask;
     uint32_t curInst;
+    uint32_t nodeCap;  // nodes of the tree being walked
+    uint32_t numFaces; // triangles of the BLAS being walked
     bool inBlas;
     HitRec h;
 };
+
+// Every index a traversal step uses is checked against the table it
+// indexes; a bad one (a corrupt tree or table) ends the ray as a miss and
+// raises a sticky flag that the host turns into an error.
+__device__ __noinline__ void trav_error(const TraceScene& sc, uint32_t flag) { atomicOr(sc.errors, flag); }
 
 // Starts a ray at the TLAS root; false if there is nothing to traverse.
 template<typename Stack>
@@ -279,11 +285,10 @@ __device__ __forceinline__ bool trav_begin(Traversal& T, const TraceScene& sc, c
     T.tbest = ray.tmax;
     T.ref = 0;
     T.cullMask = cullMask;
+    T.nodeCap = sc.tlasNodeCap;
+    T.numFaces = 0;
     stack.sp = 0;
     box_ray_setup(T.br, ray.ox, ray.oy, ray.oz, ray.dx, ray.dy, ray.dz);
-    T.wix = T.br.ix;
-    T.wiy = T.br.iy;
-    T.wiz = T.br.iz;
     return sc.numInstances != 0;
 }
 
@@ -293,6 +298,11 @@ template<typename Stack>
 __device__ __forceinline__ bool trav_step(Traversal& T, const TraceScene& sc, Stack& stack)
 {
     if(T.ref >= 0) {
+        if((uint32_t)T.ref >= T.nodeCap) {
+            trav_error(sc, kTravErrNode);
+            T.h.inst = kNoHit;
+            return false;
+        }
         const int32_t nearest = visit_wide(T.nodes, T.ref, T.br, T.ray.tmin, T.tbest, stack);
         if(nearest != kEmptyChild) {
             T.ref = nearest;
@@ -302,17 +312,23 @@ __device__ __forceinline__ bool trav_step(Traversal& T, const TraceScene& sc, St
     else if(T.ref == kSentinel) {
         T.inBlas = false;
         T.nodes = sc.tlasNodes;
-        T.br.ox = T.ray.ox;
-        T.br.oy = T.ray.oy;
-        T.br.oz = T.ray.oz;
-        T.br.ix = T.wix;
-        T.br.iy = T.wiy;
-        T.br.iz = T.wiz;
+        T.nodeCap = sc.tlasNodeCap;
+        box_ray_setup(T.br, T.ray.ox, T.ray.oy, T.ray.oz, T.ray.dx, T.ray.dy, T.ray.dz);
     }
     else if(!T.inBlas) {
         const int32_t leaf = ~T.ref;
+        if((uint32_t)leaf >= sc.numInstances) {
+            trav_error(sc, kTravErrInstance);
+            T.h.inst = kNoHit;
+            return false;
+        }
         const u32x4* ip = reinterpret_cast<const u32x4*>(sc.instances + leaf);
         const u32x4 i3 = ld_global16(ip + 3); // blas, mask, position, customIndex
+        if(i3.x >= sc.numBlas) {
+            trav_error(sc, kTravErrBlas);
+            T.h.inst = kNoHit;
+            return false;
+        }
         const u32x4* bp = reinterpret_cast<const u32x4*>(sc.blas + i3.x);
         const u32x4 b0 = ld_global16(bp);     // nodes, tris
         const uint32_t numFaces = ld_global8(bp + 1).x;
@@ -334,12 +350,19 @@ __device__ __forceinline__ bool trav_step(Traversal& T, const TraceScene& sc, St
             T.tris = reinterpret_cast<const Tri*>((uint64_t)b0.z | ((uint64_t)b0.w << 32));
             T.inBlas = true;
             T.curInst = i3.z;
+            T.numFaces = numFaces;
+            T.nodeCap = numFaces > 2u ? numFaces - 1u : 1u;
             T.ref = 0;
             return true;
         }
     }
     else {
         const int32_t leaf = ~T.ref;
+        if((uint32_t)leaf >= T.numFaces) {
+            trav_error(sc, kTravErrTriangle);
+            T.h.inst = kNoHit;
+            return false;
+        }
         const u32x4* tp = reinterpret_cast<const u32x4*>(T.tris + leaf);
         const u32x4 a = ld_global16(tp), b = ld_global16(tp + 1), cc = ld_global16(tp + 2);
         Tri tri;
