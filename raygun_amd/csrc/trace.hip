@@ -13,34 +13,6 @@ constexpr int kTraceBlock = 256;
 constexpr int kLdsStack = 16;
 constexpr int kSpill = 368; // with the 16 LDS entries: the 381-entry worst case of two wide LBVH levels
 
-#ifndef RGRT_PHASED
-#define RGRT_PHASED 1
-#endif
-
-// Traverses one ray per lane, the wave choosing one kind of step per
-// iteration: the kind that most of its unfinished lanes need next (node
-// visits on ties). Only lanes needing that kind step; the branches of the
-// step for the other kinds then have no active lanes and are skipped, so an
-// iteration costs one path instead of the node, triangle and instance paths
-// one after the other. Each ray still takes exactly the steps it takes
-// alone, in the same order.
-template<typename Stack>
-__device__ __forceinline__ HitRec traverse_wave(const TraceScene& sc, bool valid, const RayIn& ray, uint32_t cullMask, Stack& stack)
-{
-    Traversal T;
-    bool active = valid && trav_begin(T, sc, ray, cullMask, stack);
-    if(!valid) T.h.inst = kNoHit;
-    for(uint32_t guard = 0; guard < (1u << 26); ++guard) {
-        const int kind = active ? trav_kind(T) : 3;
-        const uint64_t mNode = __ballot(kind == 0), mTri = __ballot(kind == 1), mInst = __ballot(kind == 2);
-        if((mNode | mTri | mInst) == 0ull) break;
-        const int cNode = __popcll(mNode), cTri = __popcll(mTri), cInst = __popcll(mInst);
-        const int phase = (cNode >= cTri && cNode >= cInst) ? 0 : (cTri >= cInst ? 1 : 2);
-        if(kind == phase) active = trav_step(T, sc, stack);
-    }
-    return T.h;
-}
-
 // Wavefront trace pass: rays [0, count) of queue `cur` -> hit records.
 // Waves fetch 64 rays at a time from a global counter (persistent grid).
 __global__ __launch_bounds__(kTraceBlock) void k_trace_queue(TraceScene sc, const float* __restrict__ rays,
@@ -70,29 +42,6 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_queue(TraceScene sc, cons
     for(;;) {
         if(base >= count) break;
         const uint32_t i = base + lane;
-#if RGRT_PHASED
-        // every lane of the wave takes part in the wave-wide step choice
-        const bool valid = i < count;
-        const uint32_t ii = valid ? i : 0u;
-        RayIn r;
-        r.ox = rays[0 * (size_t)npix + ii];
-        r.oy = rays[1 * (size_t)npix + ii];
-        r.oz = rays[2 * (size_t)npix + ii];
-        r.tmin = rays[3 * (size_t)npix + ii];
-        r.dx = rays[4 * (size_t)npix + ii];
-        r.dy = rays[5 * (size_t)npix + ii];
-        r.dz = rays[6 * (size_t)npix + ii];
-        r.tmax = rays[7 * (size_t)npix + ii];
-        const uint32_t flags = __float_as_uint(rays[9 * (size_t)npix + ii]);
-        const HitRec h = traverse_wave(sc, valid, r, (flags >> kMaskShift) & 0xffu, stack);
-        if(valid) {
-            hits[0 * (size_t)npix + i] = h.t;
-            hits[1 * (size_t)npix + i] = h.u;
-            hits[2 * (size_t)npix + i] = h.v;
-            hits[3 * (size_t)npix + i] = __uint_as_float(h.inst);
-            hits[4 * (size_t)npix + i] = __uint_as_float(h.prim);
-        }
-#else
         if(i < count) {
             RayIn r;
             r.ox = rays[0 * (size_t)npix + i];
@@ -111,7 +60,6 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_queue(TraceScene sc, cons
             hits[3 * (size_t)npix + i] = __uint_as_float(h.inst);
             hits[4 * (size_t)npix + i] = __uint_as_float(h.prim);
         }
-#endif
         if(count <= dynStart) break; // no rays beyond the static share
         uint32_t b = 0;
         if(lane == 0) b = dynStart + atomicAdd(fetch, 64u);

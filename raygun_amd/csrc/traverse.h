@@ -391,13 +391,6 @@ __device__ __forceinline__ bool trav_step(Traversal& T, const TraceScene& sc, St
     return true;
 }
 
-// Kind of the step a ray takes next: 0 a wide node, 1 a triangle, 2 an
-// instance leaf or the sentinel (the return to the TLAS).
-__device__ __forceinline__ int trav_kind(const Traversal& T)
-{
-    return T.ref >= 0 ? 0 : ((T.ref != kSentinel && T.inBlas) ? 1 : 2);
-}
-
 // Whole traversal of one ray. Each node and leaf is visited at most once
 // per level, so a well-formed two-level tree ends far below the step cap;
 // the cap only bounds the loop.

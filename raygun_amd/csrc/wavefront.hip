@@ -43,9 +43,6 @@ constexpr int kShadeBlock = 256;
 #ifndef RGRT_DRAIN_WAVES
 #define RGRT_DRAIN_WAVES 1
 #endif
-#ifndef RGRT_PHASED_DRAIN
-#define RGRT_PHASED_DRAIN 0
-#endif
 
 enum Stage : int { G_SHADOW = 1, G_REFL = 2, G_REFR = 3, SI_TRACE = 4, SI_SKY = 5, ST_INTERNAL = 6 };
 enum Step : int { RET = 0, EMIT_FRAME = 1, EMIT = 2, DONE = 3 };
@@ -765,98 +762,6 @@ __global__ __launch_bounds__(kDrainBlock) __attribute__((amdgpu_waves_per_eu(RGR
     const uint32_t gtid = blockIdx.x * kDrainBlock + threadIdx.x;
     TraversalStack<kDrainLdsStack, kDrainBlock, kDrainSpill> stack{ldsStack, spillBase + (size_t)gtid * kDrainSpill,
                                                                    (int)threadIdx.x};
-#if RGRT_PHASED_DRAIN
-    // One step per iteration, chosen for the whole wave: a traversal step
-    // kind (node, triangle, instance/sentinel) or shading (lanes whose ray is
-    // finished run the closest-hit / miss stage and start the pixel's next
-    // ray), whichever most lanes need. Idle lanes refill from the queue after
-    // a shading step (when lanes can have become idle) or when all are idle.
-    bool active = false, tracing = false;
-    bool exhausted = false, refill = true;
-    uint32_t pix = 0, flags = 0;
-    RayIn r;
-    Traversal T;
-    Payload P;
-    int sp = 0, sample = 0;
-    uint64_t traced = 0;
-    // every iteration takes at least one traversal step or shading stage of
-    // the finite work, or fetches; the cap only bounds the loop
-    for(uint32_t guard = 0; guard < (1u << 24); ++guard) {
-        if(!exhausted && refill) {
-            const uint64_t idle = __ballot(!active);
-            if(idle) {
-                uint32_t b = 0;
-                if(lane == 0) b = atomicAdd(fetch, (uint32_t)__popcll(idle));
-                b = __shfl(b, 0);
-                if(b + (uint32_t)__popcll(idle) >= count) exhausted = true;
-                if(!active) {
-                    const uint32_t i = b + (uint32_t)__popcll(idle & ltMask);
-                    if(i < count) {
-                        pix = __float_as_uint(rays[8 * n + i]);
-                        flags = __float_as_uint(rays[9 * n + i]);
-                        r.ox = rays[0 * n + i];
-                        r.oy = rays[1 * n + i];
-                        r.oz = rays[2 * n + i];
-                        r.tmin = rays[3 * n + i];
-                        r.dx = rays[4 * n + i];
-                        r.dy = rays[5 * n + i];
-                        r.dz = rays[6 * n + i];
-                        r.tmax = rays[7 * n + i];
-                        load_payload(p, pix, P);
-                        const uint32_t meta = p.st.meta[pix];
-                        sp = (int)(meta & 0xffu);
-                        sample = (int)(meta >> 8);
-                        active = true;
-                        tracing = trav_begin(T, sc, r, (flags >> kMaskShift) & 0xffu, stack);
-                    }
-                }
-            }
-        }
-        const int kind = !active ? 4 : (tracing ? trav_kind(T) : 3);
-        const uint64_t m0 = __ballot(kind == 0), m1 = __ballot(kind == 1), m2 = __ballot(kind == 2), m3 = __ballot(kind == 3);
-        if((m0 | m1 | m2 | m3) == 0ull) {
-            if(exhausted) break;
-            refill = true; // every lane idle: fetch
-            continue;
-        }
-        const int c0 = __popcll(m0), c1 = __popcll(m1), c2 = __popcll(m2), c3 = __popcll(m3);
-        int phase = 0, best = c0;
-        if(c1 > best) {
-            phase = 1;
-            best = c1;
-        }
-        if(c2 > best) {
-            phase = 2;
-            best = c2;
-        }
-        if(c3 > best) phase = 3;
-        refill = phase == 3;
-        if(kind == phase) {
-            if(phase < 3) {
-                tracing = trav_step(T, sc, stack);
-            }
-            else {
-                ++traced;
-                OutRay out;
-                if(advance_pixel(c, pix, flags, f3(r.ox, r.oy, r.oz), f3(r.dx, r.dy, r.dz), T.h, P, sp, sample, out)) {
-                    r.ox = out.o.x;
-                    r.oy = out.o.y;
-                    r.oz = out.o.z;
-                    r.tmin = out.tmin;
-                    r.dx = out.d.x;
-                    r.dy = out.d.y;
-                    r.dz = out.d.z;
-                    r.tmax = out.tmax;
-                    flags = (out.mask << kMaskShift) | out.miss1;
-                    tracing = trav_begin(T, sc, r, (flags >> kMaskShift) & 0xffu, stack);
-                }
-                else {
-                    active = false;
-                }
-            }
-        }
-    }
-#else
     bool active = false;
     bool exhausted = false;
     uint32_t pix = 0, flags = 0;
@@ -915,7 +820,6 @@ __global__ __launch_bounds__(kDrainBlock) __attribute__((amdgpu_waves_per_eu(RGR
             }
         }
     }
-#endif
     // rays traced here count like the wavefront passes' rays (counters[4..5])
     for(int off = 32; off > 0; off >>= 1) traced += __shfl_xor(traced, off);
     if(lane == 0 && traced) atomicAdd(reinterpret_cast<unsigned long long*>(p.st.counters + 4), (unsigned long long)traced);
