@@ -304,10 +304,13 @@ typedef struct rgrt_tuning {
     uint32_t traceBlocksPerCU;  /* persistent traversal grid, blocks of 256 per CU            */
     uint32_t shadeBlocksPerCU;  /* shading grid, blocks of 256 per CU                         */
     uint32_t drainBlocksPerCU;  /* drain grid, blocks of 256 per CU                           */
+    uint32_t hostLag;           /* wavefront passes the GPU may run ahead of the host's check
+                                   for an empty queue (1..16); the passes queued after the
+                                   last ray are empty launches                               */
 } rgrt_tuning;
 
 /* Sets / reads the launch shape (defaults tuned for MI355X). Values outside
- * 1..32 blocks per CU are rejected. */
+ * 1..32 blocks per CU or a hostLag outside 1..16 are rejected. */
 rgrt_status rgrt_set_tuning(rgrt_ctx* ctx, const rgrt_tuning* t);
 rgrt_status rgrt_get_tuning(rgrt_ctx* ctx, rgrt_tuning* t);
 

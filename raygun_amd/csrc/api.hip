@@ -102,7 +102,7 @@ struct rgrt_ctx {
     int32_t* spill = nullptr;
     size_t spillBytes = 0;
     uint32_t traceGrid = 0, shadeGrid = 0, drainGrid = 0;
-    rgrt_tuning tuning{6, 6, 8, 4}; // measured on config 2 (1080p, 2M triangles)
+    rgrt_tuning tuning{6, 6, 8, 4, 2}; // measured on config 2 (1080p, 2M triangles)
     hipEvent_t evDrainStart = nullptr, evDrainEnd = nullptr;
     bool lastDrained = false;
 
@@ -356,7 +356,8 @@ rgrt_status do_trace(rgrt_ctx* c)
     const uint32_t perSample = max_sequential_rays(c->ubo.maxRecursions);
     const uint32_t maxIters = perSample * (uint32_t)c->ubo.numSamples + 2;
     ensure_host_counts(c, maxIters + 2);
-    const int ring = 8;
+    const uint32_t lag = c->tuning.hostLag;
+    const int ring = (int)std::max<uint32_t>(8u, lag + 2u);
     while((int)c->iterEvents.size() < ring) {
         hipEvent_t e;
         check(hipEventCreateWithFlags(&e, hipEventDisableTiming));
@@ -375,11 +376,11 @@ rgrt_status do_trace(rgrt_ctx* c)
     uint32_t traceLaunches = 0;
     bool drained = false;
     for(; it < maxIters; ++it) {
-        // The count of iteration it-2's input queue is known by now (the GPU
-        // runs ahead by at most two iterations); zero means every pixel is done.
-        if(it >= 2) {
-            check(hipEventSynchronize(c->iterEvents[(it - 2) % ring]));
-            if(__atomic_load_n(&c->hostCounts[it - 2], __ATOMIC_ACQUIRE) == 0u) break;
+        // The count of iteration it-lag's input queue is known by now (the GPU
+        // runs ahead by at most `lag` iterations); zero means every pixel is done.
+        if(it >= lag) {
+            check(hipEventSynchronize(c->iterEvents[(it - lag) % ring]));
+            if(__atomic_load_n(&c->hostCounts[it - lag], __ATOMIC_ACQUIRE) == 0u) break;
         }
         if(it == c->tuning.wavefrontPasses) {
             // the persistent drain finishes every pixel still in flight
@@ -959,6 +960,7 @@ rgrt_status rgrt_set_tuning(rgrt_ctx* c, const rgrt_tuning* t)
         if(!t) return fail(c, RGRT_ERR_INVALID_ARG, "set_tuning: null");
         for(uint32_t v: {t->traceBlocksPerCU, t->shadeBlocksPerCU, t->drainBlocksPerCU})
             if(v < 1 || v > 32) return fail(c, RGRT_ERR_INVALID_ARG, "set_tuning: blocks per CU outside 1..32");
+        if(t->hostLag < 1 || t->hostLag > 16) return fail(c, RGRT_ERR_INVALID_ARG, "set_tuning: hostLag outside 1..16");
         c->tuning = *t;
         c->apply_tuning();
         return RGRT_OK;

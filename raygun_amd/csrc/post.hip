@@ -95,15 +95,25 @@ __global__ __launch_bounds__(kPostThreads) void k_rough_post(Images img, float4 
 
     // 10 x (H: A -> B, V: B -> A). Region-border pixels read a missing
     // neighbour as 0; that error stays inside the halo (one pixel per pass).
+    // A pass only computes the pixels the remaining passes still read: the
+    // tile grown by one pixel in x per H pass after it and one pixel in y per
+    // V pass after it (these sets shrink from pass to pass, so every pixel a
+    // pass computes was computed by all earlier passes too).
+    const int tcol = threadIdx.x % 32, trow = threadIdx.x / 32;
     for(int pass = 0; pass < 20; ++pass) {
         const bool horiz = (pass & 1) == 0;
         const ushort4* in = horiz ? A : B;
         ushort4* out = horiz ? B : A;
         const int ox = horiz ? 1 : 0, oy = horiz ? 0 : 1;
-        for(int q = threadIdx.x; q < kRegionPix; q += kPostThreads) {
+        const int hAfter = 9 - pass / 2;                     // H passes after this one
+        const int vAfter = horiz ? 10 - pass / 2 : 9 - pass / 2; // V passes after this one
+        const int x0 = kHalo - hAfter, x1 = kHalo + kTile + hAfter;
+        const int y0 = kHalo - vAfter, y1 = kHalo + kTile + vAfter;
+        for(int ly = y0 + trow; ly < y1; ly += kPostThreads / 32)
+            for(int lx = x0 + tcol; lx < x1; lx += 32) {
+            const int q = ly * kRegion + lx;
             const float t = T[q];
             if(t < 0.001f) continue;
-            const int lx = q % kRegion, ly = q / kRegion;
             const ushort4 c = in[q];
             const int q1x = lx + ox, q1y = ly + oy, q2x = lx - ox, q2y = ly - oy;
             const ushort4 z = make_ushort4(0, 0, 0, 0);
