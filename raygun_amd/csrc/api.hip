@@ -747,12 +747,15 @@ rgrt_status rgrt_build_tlas(rgrt_ctx* c, const rgrt_instance* in, uint32_t n)
         if(n) std::memcpy(c->hInstStage, in, n * sizeof(rgrt_instance));
         if(n) check(hipMemcpyAsync(c->dInstIn, c->hInstStage, n * sizeof(rgrt_instance), hipMemcpyHostToDevice, s));
         check(hipEventRecord(c->stageDone, s));
-        c->builder.reserve(std::max<uint32_t>(n, 1));
-        launch_inst_prep(s, c->dInstIn, n, c->dBlasRoot, c->builder.boxLo, c->builder.boxHi, c->dW2O, c->dShade);
-        c->builder.build(s, n, c->tlasNodes, c->tlasLeaf, c->tlasRoot);
-        c->builder.collapse(s, n, c->tlasNodes, c->tlasWide, c->tlasNumWide);
-        c->builder.quantize(s, std::max<uint32_t>(n, 2) - 1, c->tlasWide, c->tlasNumWide, c->tlasWideq);
-        launch_inst_gather(s, c->dW2O, c->dInstIn, c->tlasLeaf, n, c->dInst);
+        if(!launch_tlas_fused(s, c->dInstIn, n, c->dBlasRoot, c->dW2O, c->dShade, c->tlasNodes, c->tlasLeaf, c->tlasRoot,
+                              c->tlasWide, c->tlasWideq, c->tlasNumWide, c->dInst)) {
+            c->builder.reserve(std::max<uint32_t>(n, 1));
+            launch_inst_prep(s, c->dInstIn, n, c->dBlasRoot, c->builder.boxLo, c->builder.boxHi, c->dW2O, c->dShade);
+            c->builder.build(s, n, c->tlasNodes, c->tlasLeaf, c->tlasRoot);
+            c->builder.collapse(s, n, c->tlasNodes, c->tlasWide, c->tlasNumWide);
+            c->builder.quantize(s, std::max<uint32_t>(n, 2) - 1, c->tlasWide, c->tlasNumWide, c->tlasWideq);
+            launch_inst_gather(s, c->dW2O, c->dInstIn, c->tlasLeaf, n, c->dInst);
+        }
         if(timing) check(hipEventRecord(c->evAsEnd, s));
         c->timedAs = timing;
         c->numInstances = n;

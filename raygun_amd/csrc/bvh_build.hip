@@ -67,17 +67,12 @@ __global__ void k_tri_boxes(const rgrt_vertex* __restrict__ vb, const uint32_t* 
 
 // World box of each instance (Arvo's method on the BLAS root box), the
 // world-to-object inverse in double, and the per-position shading record.
-__global__ void k_inst_prep(const rgrt_instance* __restrict__ in, uint32_t n, const float4* __restrict__ blasRoot,
-                            float4* __restrict__ boxLo, float4* __restrict__ boxHi, float* __restrict__ w2oOut,
-                            InstanceShade* __restrict__ shade)
+__device__ __forceinline__ void inst_prep_one(const rgrt_instance& r, const float4* __restrict__ blasRoot, float wlo[3],
+                                              float whi[3], float w[12], InstanceShade& s)
 {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if(i >= n) return;
-    const rgrt_instance r = in[i];
     const float* m = r.transform;
     const float4 rlo = blasRoot[2 * r.blas], rhi = blasRoot[2 * r.blas + 1];
     const float blo[3] = {rlo.x, rlo.y, rlo.z}, bhi[3] = {rhi.x, rhi.y, rhi.z};
-    float wlo[3], whi[3];
     if(!(blo[0] <= bhi[0])) {
         for(int c = 0; c < 3; ++c) {
             wlo[c] = blo[c];
@@ -97,8 +92,6 @@ __global__ void k_inst_prep(const rgrt_instance* __restrict__ in, uint32_t n, co
             whi[row] = hi;
         }
     }
-    boxLo[i] = make_float4(wlo[0], wlo[1], wlo[2], 0.0f);
-    boxHi[i] = make_float4(whi[0], whi[1], whi[2], 0.0f);
 
     // inverse of the affine map: adjugate of the 3x3 part in double
     const double a = m[0], b = m[1], c = m[2], d = m[4], e = m[5], f = m[6], g = m[8], h = m[9], k = m[10];
@@ -108,17 +101,30 @@ __global__ void k_inst_prep(const rgrt_instance* __restrict__ in, uint32_t n, co
     const double inv[9] = {A * rr, -(b * k - c * h) * rr, (b * f - c * e) * rr, B * rr, (a * k - c * g) * rr,
                            -(a * f - c * d) * rr, C * rr, -(a * h - b * g) * rr, (a * e - b * d) * rr};
     const double tx = m[3], ty = m[7], tz = m[11];
-    float* w = w2oOut + 12 * (size_t)i;
     for(int row = 0; row < 3; ++row) {
         w[row * 4 + 0] = (float)inv[row * 3 + 0];
         w[row * 4 + 1] = (float)inv[row * 3 + 1];
         w[row * 4 + 2] = (float)inv[row * 3 + 2];
         w[row * 4 + 3] = (float)(-(inv[row * 3 + 0] * tx + inv[row * 3 + 1] * ty + inv[row * 3 + 2] * tz));
     }
-    InstanceShade s;
     for(int q = 0; q < 12; ++q) s.o2w[q] = m[q];
     s.offsets = r.offsets;
     s.customIndex = r.customIndex;
+}
+
+__global__ void k_inst_prep(const rgrt_instance* __restrict__ in, uint32_t n, const float4* __restrict__ blasRoot,
+                            float4* __restrict__ boxLo, float4* __restrict__ boxHi, float* __restrict__ w2oOut,
+                            InstanceShade* __restrict__ shade)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if(i >= n) return;
+    const rgrt_instance r = in[i];
+    float wlo[3], whi[3], w[12];
+    InstanceShade s;
+    inst_prep_one(r, blasRoot, wlo, whi, w, s);
+    boxLo[i] = make_float4(wlo[0], wlo[1], wlo[2], 0.0f);
+    boxHi[i] = make_float4(whi[0], whi[1], whi[2], 0.0f);
+    for(int q = 0; q < 12; ++q) w2oOut[12 * (size_t)i + q] = w[q];
     shade[i] = s;
 }
 
@@ -156,15 +162,11 @@ __global__ void k_scene_box(const float4* __restrict__ boxLo, const float4* __re
     else if(threadIdx.x < 6) atomicMax(&sb[threadIdx.x], red[threadIdx.x]);
 }
 
-__global__ void k_morton(const float4* __restrict__ boxLo, const float4* __restrict__ boxHi, uint32_t n,
-                         const uint32_t* __restrict__ sb, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals)
+// Morton code of a box centroid in the scene box sb (order-preserving integer encodings).
+__device__ __forceinline__ uint32_t morton_key(const float lo[3], const float hi[3], const uint32_t* sb)
 {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if(i >= n) return;
     const float slo[3] = {ordered_to_float(sb[0]), ordered_to_float(sb[1]), ordered_to_float(sb[2])};
     const float shi[3] = {ordered_to_float(sb[3]), ordered_to_float(sb[4]), ordered_to_float(sb[5])};
-    const float4 a = boxLo[i], b = boxHi[i];
-    const float lo[3] = {a.x, a.y, a.z}, hi[3] = {b.x, b.y, b.z};
     uint32_t q[3];
     for(int c = 0; c < 3; ++c) {
         const float ext = shi[c] - slo[c];
@@ -172,7 +174,17 @@ __global__ void k_morton(const float4* __restrict__ boxLo, const float4* __restr
         const float x = ext > 0.0f ? (centroid - slo[c]) / ext : 0.0f;
         q[c] = quantize10(x);
     }
-    keys[i] = (expand_bits10(q[0]) << 2) | (expand_bits10(q[1]) << 1) | expand_bits10(q[2]);
+    return (expand_bits10(q[0]) << 2) | (expand_bits10(q[1]) << 1) | expand_bits10(q[2]);
+}
+
+__global__ void k_morton(const float4* __restrict__ boxLo, const float4* __restrict__ boxHi, uint32_t n,
+                         const uint32_t* __restrict__ sb, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if(i >= n) return;
+    const float4 a = boxLo[i], b = boxHi[i];
+    const float lo[3] = {a.x, a.y, a.z}, hi[3] = {b.x, b.y, b.z};
+    keys[i] = morton_key(lo, hi, sb);
     vals[i] = i;
 }
 
@@ -280,7 +292,7 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(const uint32_t* _
 // ---------------------------------------------------------------------------
 // 5. Karras hierarchy
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ int karras_delta(const uint32_t* __restrict__ k, int n, int i, int j)
+__device__ __forceinline__ int karras_delta(const uint32_t* k, int n, int i, int j)
 {
     if(j < 0 || j >= n) return -1;
     const uint32_t a = k[i], b = k[j];
@@ -288,10 +300,10 @@ __device__ __forceinline__ int karras_delta(const uint32_t* __restrict__ k, int 
     return __clz((int)(a ^ b));
 }
 
-__global__ void k_karras(const uint32_t* __restrict__ keys, int n, Node* __restrict__ nodes, uint32_t* __restrict__ leafParent)
+// Children of internal node i (Karras 2012): refs >= 0 are internal nodes,
+// ~k the leaf at sorted position k; gamma is the split position.
+__device__ __forceinline__ void karras_node(const uint32_t* keys, int n, int i, int& left, int& right, int& gamma)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if(i >= n - 1) return;
     const int d = (karras_delta(keys, n, i, i + 1) - karras_delta(keys, n, i, i - 1)) >= 0 ? 1 : -1;
     const int dmin = karras_delta(keys, n, i, i - d);
     int lmax = 2;
@@ -307,10 +319,18 @@ __global__ void k_karras(const uint32_t* __restrict__ keys, int n, Node* __restr
         if(karras_delta(keys, n, i, i + (s + t) * d) > dnode) s += t;
         if(t <= 1) break;
     }
-    const int gamma = i + s * d + min(d, 0);
+    gamma = i + s * d + min(d, 0);
     const int lo = min(i, j), hi = max(i, j);
-    const int left = (lo == gamma) ? ~gamma : gamma;
-    const int right = (hi == gamma + 1) ? ~(gamma + 1) : gamma + 1;
+    left = (lo == gamma) ? ~gamma : gamma;
+    right = (hi == gamma + 1) ? ~(gamma + 1) : gamma + 1;
+}
+
+__global__ void k_karras(const uint32_t* __restrict__ keys, int n, Node* __restrict__ nodes, uint32_t* __restrict__ leafParent)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if(i >= n - 1) return;
+    int left, right, gamma;
+    karras_node(keys, n, i, left, right, gamma);
     Node& nd = nodes[i];
     nd.child0 = left;
     nd.child1 = right;
@@ -420,17 +440,21 @@ struct WideEntry {
     float lo[3], hi[3];
 };
 
-__device__ __forceinline__ WideEntry slot_of(const Node* nodes, int node, int c)
-{
-    const float* f = reinterpret_cast<const float*>(nodes + node);
-    WideEntry e;
-    e.ref = reinterpret_cast<const int32_t*>(nodes + node)[12 + c];
-    for(int k = 0; k < 3; ++k) {
-        e.lo[k] = f[3 * c + k];
-        e.hi[k] = f[6 + 3 * c + k];
+// Child slot c of binary node `node`, read from the node array in HBM.
+struct GlobalNodes {
+    const Node* nodes;
+    __device__ __forceinline__ WideEntry slot(int node, int c) const
+    {
+        const float* f = reinterpret_cast<const float*>(nodes + node);
+        WideEntry e;
+        e.ref = reinterpret_cast<const int32_t*>(nodes + node)[12 + c];
+        for(int k = 0; k < 3; ++k) {
+            e.lo[k] = f[3 * c + k];
+            e.hi[k] = f[6 + 3 * c + k];
+        }
+        return e;
     }
-    return e;
-}
+};
 
 __device__ __forceinline__ float half_area(const float lo[3], const float hi[3])
 {
@@ -440,16 +464,37 @@ __device__ __forceinline__ float half_area(const float lo[3], const float hi[3])
 
 constexpr int kCollapseThreads = 1024;
 
-// One workgroup walks the binary tree breadth-first. A wide node starts from
-// its binary node's two children and opens the internal child of largest
-// surface area (the first on equal areas) until it has four children or only
-// leaves; its internal children become the next level's wide nodes, numbered
-// in order (a block-wide prefix sum over the per-node counts).
-__global__ __launch_bounds__(kCollapseThreads) void k_collapse(const Node* __restrict__ nodes, Node4* __restrict__ out,
-                                                               int32_t* frontA, int32_t* frontB, uint32_t* numWide)
+__device__ Node4q quantize_node(const Node4& w);
+
+// Field-wise c ? a : b (keeps the entries in registers).
+__device__ __forceinline__ WideEntry pick(bool c, const WideEntry& a, const WideEntry& b)
 {
-    __shared__ int32_t scan[kCollapseThreads];
+    WideEntry r;
+    r.ref = c ? a.ref : b.ref;
+#pragma unroll
+    for(int k = 0; k < 3; ++k) {
+        r.lo[k] = c ? a.lo[k] : b.lo[k];
+        r.hi[k] = c ? a.hi[k] : b.hi[k];
+    }
+    return r;
+}
+
+// One workgroup of kCollapseThreads walks the binary tree breadth-first. A
+// wide node starts from its binary node's two children and opens the
+// internal child of largest surface area (the first on equal areas) until it
+// has four children or only leaves; its internal children become the next
+// level's wide nodes, numbered in order (a block-wide prefix sum over the
+// per-node counts: a shuffle scan in each wave, then the wave totals). The
+// frontiers hold at most n - 1 entries each; `waveTot` is
+// kCollapseThreads / 64 ints of LDS. With outq set, every wide node is also
+// written compressed. Returns the wide node count (in every thread).
+template <class Src>
+__device__ int collapse_bfs(const Src src, Node4* __restrict__ out, Node4q* __restrict__ outq, int32_t* frontA,
+                            int32_t* frontB, int32_t* waveTot)
+{
+    constexpr int kWavesPerBlock = kCollapseThreads / 64;
     const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
     int32_t* front = frontA;
     int32_t* next = frontB;
     if(tid == 0) front[0] = 0;
@@ -464,43 +509,55 @@ __global__ __launch_bounds__(kCollapseThreads) void k_collapse(const Node* __res
             int ns = 0;
             if(valid) {
                 const int r = front[k];
-                for(int c = 0; c < 2; ++c) {
-                    const WideEntry e = slot_of(nodes, r, c);
-                    if(e.ref != kEmptyChild) S[ns++] = e;
-                }
+                const WideEntry e0 = src.slot(r, 0), e1 = src.slot(r, 1);
+                const bool h0 = e0.ref != kEmptyChild, h1 = e1.ref != kEmptyChild;
+                S[0] = pick(h0, e0, e1);
+                S[1] = e1;
+                ns = (int)h0 + (int)h1;
                 while(ns < 4) {
-                    int j = -1;
+                    int j = -1, e = 0;
                     float best = -1.0f;
-                    for(int q = 0; q < ns; ++q) {
-                        if(S[q].ref < 0) continue;
-                        const float a = half_area(S[q].lo, S[q].hi);
-                        if(a > best) {
-                            best = a;
-                            j = q;
+#pragma unroll
+                    for(int q = 0; q < 4; ++q) {
+                        if(q < ns && S[q].ref >= 0) {
+                            const float a = half_area(S[q].lo, S[q].hi);
+                            if(a > best) {
+                                best = a;
+                                j = q;
+                                e = S[q].ref;
+                            }
                         }
                     }
                     if(j < 0) break;
-                    const int e = S[j].ref;
-                    for(int q = ns; q > j + 1; --q) S[q] = S[q - 1];
-                    S[j] = slot_of(nodes, e, 0);
-                    S[j + 1] = slot_of(nodes, e, 1);
+                    const WideEntry a0 = src.slot(e, 0), a1 = src.slot(e, 1);
+#pragma unroll
+                    for(int q = 3; q >= 1; --q) S[q] = pick(q > j + 1 && q <= ns, S[q - 1], S[q]);
+#pragma unroll
+                    for(int q = 0; q < 4; ++q) S[q] = pick(q == j, a0, pick(q == j + 1, a1, S[q]));
                     ++ns;
                 }
             }
             int cnt = 0;
-            for(int q = 0; q < ns; ++q) cnt += S[q].ref >= 0 ? 1 : 0;
-            scan[tid] = cnt;
-            __syncthreads();
-            for(int off = 1; off < kCollapseThreads; off <<= 1) {
-                const int v = tid >= off ? scan[tid - off] : 0;
-                __syncthreads();
-                scan[tid] += v;
-                __syncthreads();
+#pragma unroll
+            for(int q = 0; q < 4; ++q) cnt += (q < ns && S[q].ref >= 0) ? 1 : 0;
+            int incl = cnt;
+#pragma unroll
+            for(int off = 1; off < 64; off <<= 1) {
+                const int t = __shfl_up(incl, off);
+                if(lane >= off) incl += t;
             }
-            const int excl = scan[tid] - cnt;
-            const int total = scan[kCollapseThreads - 1];
+            if(lane == 63) waveTot[wave] = incl;
+            __syncthreads();
+            int excl = incl - cnt, total = 0;
+#pragma unroll
+            for(int w = 0; w < kWavesPerBlock; ++w) {
+                const int t = waveTot[w];
+                excl += w < wave ? t : 0;
+                total += t;
+            }
             if(valid) {
                 Node4 w;
+#pragma unroll
                 for(int c = 0; c < 4; ++c) {
                     w.lox[c] = w.loy[c] = w.loz[c] = INFINITY;
                     w.hix[c] = w.hiy[c] = w.hiz[c] = -INFINITY;
@@ -508,27 +565,31 @@ __global__ __launch_bounds__(kCollapseThreads) void k_collapse(const Node* __res
                     w.pad[c] = 0;
                 }
                 int rank = 0;
-                for(int q = 0; q < ns; ++q) {
-                    w.lox[q] = S[q].lo[0];
-                    w.loy[q] = S[q].lo[1];
-                    w.loz[q] = S[q].lo[2];
-                    w.hix[q] = S[q].hi[0];
-                    w.hiy[q] = S[q].hi[1];
-                    w.hiz[q] = S[q].hi[2];
-                    if(S[q].ref >= 0) {
-                        const int idx = nextCount + excl + rank;
-                        w.child[q] = base + fcount + idx;
-                        next[idx] = S[q].ref;
-                        ++rank;
-                    }
-                    else {
-                        w.child[q] = S[q].ref;
+#pragma unroll
+                for(int q = 0; q < 4; ++q) {
+                    if(q < ns) {
+                        w.lox[q] = S[q].lo[0];
+                        w.loy[q] = S[q].lo[1];
+                        w.loz[q] = S[q].lo[2];
+                        w.hix[q] = S[q].hi[0];
+                        w.hiy[q] = S[q].hi[1];
+                        w.hiz[q] = S[q].hi[2];
+                        if(S[q].ref >= 0) {
+                            const int idx = nextCount + excl + rank;
+                            w.child[q] = base + fcount + idx;
+                            next[idx] = S[q].ref;
+                            ++rank;
+                        }
+                        else {
+                            w.child[q] = S[q].ref;
+                        }
                     }
                 }
                 out[base + k] = w;
+                if(outq) outq[base + k] = quantize_node(w);
             }
             nextCount += total;
-            __syncthreads(); // scan[] and next[] settle before the next chunk / level
+            __syncthreads(); // waveTot[] and next[] settle before the next chunk / level
         }
         base += fcount;
         fcount = nextCount;
@@ -538,7 +599,15 @@ __global__ __launch_bounds__(kCollapseThreads) void k_collapse(const Node* __res
         __threadfence_block();
         __syncthreads();
     }
-    if(tid == 0) *numWide = (uint32_t)base;
+    return base;
+}
+
+__global__ __launch_bounds__(kCollapseThreads) void k_collapse(const Node* __restrict__ nodes, Node4* __restrict__ out,
+                                                               int32_t* frontA, int32_t* frontB, uint32_t* numWide)
+{
+    __shared__ int32_t waveTot[kCollapseThreads / 64];
+    const int base = collapse_bfs(GlobalNodes{nodes}, out, nullptr, frontA, frontB, waveTot);
+    if(threadIdx.x == 0) *numWide = (uint32_t)base;
 }
 
 // ---------------------------------------------------------------------------
@@ -586,30 +655,43 @@ __device__ __forceinline__ void quantize_range(float o, int e, float lo, float h
     qhi = (uint8_t)b;
 }
 
-__global__ void k_quantize(const Node4* __restrict__ wide, const uint32_t* __restrict__ numWide, uint32_t capacity,
-                           Node4q* __restrict__ out)
+__device__ Node4q quantize_node(const Node4& w)
 {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if(i >= min(*numWide, capacity)) return;
-    const Node4 w = wide[i];
-    Node4q q;
     float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-    const float* clo[3] = {w.lox, w.loy, w.loz};
-    const float* chi[3] = {w.hix, w.hiy, w.hiz};
+#pragma unroll
     for(int c = 0; c < 4; ++c) {
         if(w.child[c] == kEmptyChild) continue;
-        for(int k = 0; k < 3; ++k) {
-            lo[k] = fminf(lo[k], clo[k][c]);
-            hi[k] = fmaxf(hi[k], chi[k][c]);
-        }
+        lo[0] = fminf(lo[0], w.lox[c]);
+        lo[1] = fminf(lo[1], w.loy[c]);
+        lo[2] = fminf(lo[2], w.loz[c]);
+        hi[0] = fmaxf(hi[0], w.hix[c]);
+        hi[1] = fmaxf(hi[1], w.hiy[c]);
+        hi[2] = fmaxf(hi[2], w.hiz[c]);
     }
     int e[3];
     float org[3];
+#pragma unroll
     for(int k = 0; k < 3; ++k) {
         const bool any = lo[k] <= hi[k];
         org[k] = any ? lo[k] : 0.0f;
         e[k] = any ? quant_exponent(hi[k] - lo[k]) : -126;
     }
+    // child offsets packed four bytes to a word (byte c = child c)
+    uint32_t plo[3] = {0u, 0u, 0u}, phi[3] = {0u, 0u, 0u};
+#pragma unroll
+    for(int c = 0; c < 4; ++c) {
+        if(w.child[c] == kEmptyChild) continue;
+        const float clo[3] = {w.lox[c], w.loy[c], w.loz[c]};
+        const float chi[3] = {w.hix[c], w.hiy[c], w.hiz[c]};
+#pragma unroll
+        for(int k = 0; k < 3; ++k) {
+            uint8_t qa, qb;
+            quantize_range(org[k], e[k], clo[k], chi[k], qa, qb);
+            plo[k] |= (uint32_t)qa << (8 * c);
+            phi[k] |= (uint32_t)qb << (8 * c);
+        }
+    }
+    Node4q q;
     q.ox = org[0];
     q.oy = org[1];
     q.oz = org[2];
@@ -617,18 +699,24 @@ __global__ void k_quantize(const Node4* __restrict__ wide, const uint32_t* __res
     q.ey = (int8_t)e[1];
     q.ez = (int8_t)e[2];
     q.pad0 = 0;
+    __builtin_memcpy(q.qlox, &plo[0], 4);
+    __builtin_memcpy(q.qhix, &phi[0], 4);
+    __builtin_memcpy(q.qloy, &plo[1], 4);
+    __builtin_memcpy(q.qhiy, &phi[1], 4);
+    __builtin_memcpy(q.qloz, &plo[2], 4);
+    __builtin_memcpy(q.qhiz, &phi[2], 4);
     q.pad1[0] = q.pad1[1] = 0;
-    uint8_t* qlo[3] = {q.qlox, q.qloy, q.qloz};
-    uint8_t* qhi[3] = {q.qhix, q.qhiy, q.qhiz};
-    for(int c = 0; c < 4; ++c) {
-        q.child[c] = w.child[c];
-        for(int k = 0; k < 3; ++k) {
-            qlo[k][c] = 0;
-            qhi[k][c] = 0;
-            if(w.child[c] != kEmptyChild) quantize_range(org[k], e[k], clo[k][c], chi[k][c], qlo[k][c], qhi[k][c]);
-        }
-    }
-    out[i] = q;
+#pragma unroll
+    for(int c = 0; c < 4; ++c) q.child[c] = w.child[c];
+    return q;
+}
+
+__global__ void k_quantize(const Node4* __restrict__ wide, const uint32_t* __restrict__ numWide, uint32_t capacity,
+                           Node4q* __restrict__ out)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if(i >= min(*numWide, capacity)) return;
+    out[i] = quantize_node(wide[i]);
 }
 
 // ---------------------------------------------------------------------------
@@ -673,6 +761,174 @@ __global__ void k_inst_gather(const float* __restrict__ w2o, const rgrt_instance
     d.position = i;
     d.customIndex = in[i].customIndex;
     out[k] = d;
+}
+
+// ---------------------------------------------------------------------------
+// TLAS of 2..kFusedTlasMax instances in one workgroup
+// ---------------------------------------------------------------------------
+// The same steps as the multi-kernel path (instance boxes, scene box,
+// Morton codes, stable sort, Karras, refit, collapse, compression, leaf
+// records), with the intermediate arrays in LDS and one launch instead of
+// about twenty. The sort ranks each key by counting the keys that precede
+// it in (key, index) order, which is the order the stable radix sort
+// produces; the refit combines child boxes in the same order as k_refit.
+// The trees are therefore identical to the multi-kernel path's.
+constexpr int kFusedTlasMax = kCollapseThreads;
+
+// Binary nodes held in LDS: box[node][12] in Node's float order, child refs.
+struct LdsNodes {
+    const float* box;
+    const int32_t* child0;
+    const int32_t* child1;
+    __device__ __forceinline__ WideEntry slot(int node, int c) const
+    {
+        WideEntry e;
+        e.ref = c ? child1[node] : child0[node];
+        const float* f = box + 12 * node;
+        for(int k = 0; k < 3; ++k) {
+            e.lo[k] = f[3 * c + k];
+            e.hi[k] = f[6 + 3 * c + k];
+        }
+        return e;
+    }
+};
+
+__global__ __launch_bounds__(kCollapseThreads) void k_tlas_fused(
+    const rgrt_instance* __restrict__ in, int n, const float4* __restrict__ blasRoot, float* __restrict__ w2oOut,
+    InstanceShade* __restrict__ shade, Node* __restrict__ nodes, uint32_t* __restrict__ leafOut, float4* __restrict__ rootBox,
+    Node4* __restrict__ wide, Node4q* __restrict__ wideq, uint32_t* __restrict__ numWide, DevInstance* __restrict__ instOut)
+{
+    __shared__ float ilo[3][kFusedTlasMax], ihi[3][kFusedTlasMax]; // instance world boxes
+    __shared__ uint32_t key0[kFusedTlasMax];                         // by instance
+    __shared__ uint32_t keys[kFusedTlasMax];                         // sorted
+    __shared__ uint32_t order[kFusedTlasMax];                        // sorted position -> instance
+    __shared__ int32_t child0[kFusedTlasMax], child1[kFusedTlasMax];
+    __shared__ uint32_t parent[kFusedTlasMax], leafParent[kFusedTlasMax], arrive[kFusedTlasMax];
+    __shared__ float box[12 * (kFusedTlasMax - 1)];
+    __shared__ int32_t frontA[kFusedTlasMax], frontB[kFusedTlasMax], waveTot[kCollapseThreads / 64];
+    __shared__ uint32_t sb[6];
+
+    const int tid = threadIdx.x;
+    const bool mine = tid < n;
+    if(tid < 3) sb[tid] = float_to_ordered(INFINITY);
+    else if(tid < 6) sb[tid] = float_to_ordered(-INFINITY);
+    __syncthreads();
+
+    // instance boxes, inverses, shading records; scene box
+    float w[12];
+    uint32_t blas = 0, mask = 0, custom = 0;
+    float lo[3], hi[3];
+    if(mine) {
+        const rgrt_instance r = in[tid];
+        InstanceShade sh;
+        inst_prep_one(r, blasRoot, lo, hi, w, sh);
+        for(int q = 0; q < 12; ++q) w2oOut[12 * (size_t)tid + q] = w[q];
+        shade[tid] = sh;
+        blas = r.blas;
+        mask = r.mask & 0xffu;
+        custom = r.customIndex;
+        for(int c = 0; c < 3; ++c) {
+            ilo[c][tid] = lo[c];
+            ihi[c][tid] = hi[c];
+            atomicMin(&sb[c], float_to_ordered(lo[c]));
+            atomicMax(&sb[3 + c], float_to_ordered(hi[c]));
+        }
+    }
+    __syncthreads();
+
+    // Morton codes, then the rank of each in (key, index) order
+    uint32_t key = 0;
+    if(mine) {
+        key = morton_key(lo, hi, sb);
+        key0[tid] = key;
+    }
+    __syncthreads();
+    uint32_t rank = 0;
+    if(mine) {
+        for(int j = 0; j < n; ++j) {
+            const uint32_t kj = key0[j];
+            rank += (kj < key || (kj == key && j < tid)) ? 1u : 0u;
+        }
+        keys[rank] = key;
+        order[rank] = (uint32_t)tid;
+        // instance leaf at its Morton position
+        DevInstance d;
+        for(int q = 0; q < 12; ++q) d.w2o[q] = w[q];
+        d.blas = blas;
+        d.mask = mask;
+        d.position = (uint32_t)tid;
+        d.customIndex = custom;
+        instOut[rank] = d;
+    }
+    __syncthreads();
+
+    // hierarchy
+    if(tid < n - 1) {
+        int left, right, gamma;
+        karras_node(keys, n, tid, left, right, gamma);
+        child0[tid] = left;
+        child1[tid] = right;
+        arrive[tid] = 0;
+        if(tid == 0) parent[0] = 0xffffffffu;
+        if(left >= 0) parent[left] = (uint32_t)tid;
+        else leafParent[gamma] = (uint32_t)tid;
+        if(right >= 0) parent[right] = (uint32_t)tid;
+        else leafParent[gamma + 1] = (uint32_t)tid;
+    }
+    __syncthreads();
+
+    // bottom-up refit: the second arrival at a node combines its two slots
+    if(mine) {
+        const uint32_t prim = order[tid];
+        float blo[3], bhi[3];
+        for(int c = 0; c < 3; ++c) {
+            blo[c] = ilo[c][prim];
+            bhi[c] = ihi[c][prim];
+        }
+        int child = ~tid;
+        uint32_t p = leafParent[tid];
+        for(int level = 0; level < 128; ++level) {
+            if(p >= (uint32_t)(n - 1)) break;
+            float* f = box + 12 * p;
+            const int slot = child0[p] == child ? 0 : 1;
+            for(int c = 0; c < 3; ++c) {
+                f[3 * slot + c] = blo[c];
+                f[6 + 3 * slot + c] = bhi[c];
+            }
+            __threadfence_block();
+            if(atomicAdd(&arrive[p], 1u) == 0u) break; // the sibling's thread finishes this node
+            __threadfence_block();
+            for(int c = 0; c < 3; ++c) {
+                blo[c] = fminf(f[c], f[3 + c]);
+                bhi[c] = fmaxf(f[6 + c], f[9 + c]);
+            }
+            if(p == 0) {
+                rootBox[0] = make_float4(blo[0], blo[1], blo[2], 0.0f);
+                rootBox[1] = make_float4(bhi[0], bhi[1], bhi[2], 0.0f);
+                break;
+            }
+            child = (int)p;
+            p = parent[p];
+        }
+    }
+    __syncthreads();
+
+    // binary nodes and leaf order for export
+    if(tid < n - 1) {
+        Node nd;
+        float* f = reinterpret_cast<float*>(&nd);
+        for(int q = 0; q < 12; ++q) f[q] = box[12 * tid + q];
+        nd.child0 = child0[tid];
+        nd.child1 = child1[tid];
+        nd.parent = parent[tid];
+        nd.pad = 0;
+        nodes[tid] = nd;
+    }
+    if(mine) leafOut[tid] = order[tid];
+
+    // wide + compressed nodes
+    const int nw = collapse_bfs(LdsNodes{box, child0, child1}, wide, wideq, frontA, frontB, waveTot);
+    if(tid == 0) *numWide = (uint32_t)nw;
 }
 
 inline uint32_t blocks_for(uint64_t n, uint32_t t) { return (uint32_t)((n + t - 1) / t); }
@@ -805,6 +1061,16 @@ void launch_inst_gather(hipStream_t s, const float* w2o, const rgrt_instance* in
                         DevInstance* out)
 {
     if(n) k_inst_gather<<<blocks_for(n, 128), 128, 0, s>>>(w2o, in, leafInst, n, out);
+}
+
+bool launch_tlas_fused(hipStream_t s, const rgrt_instance* in, uint32_t n, const float4* blasRoot, float* w2o,
+                       InstanceShade* shade, Node* nodes, uint32_t* leaf, float4* rootBox, Node4* wide, Node4q* wideq,
+                       uint32_t* numWide, DevInstance* inst)
+{
+    if(n < 2 || n > (uint32_t)kFusedTlasMax) return false;
+    k_tlas_fused<<<1, kCollapseThreads, 0, s>>>(in, (int)n, blasRoot, w2o, shade, nodes, leaf, rootBox, wide, wideq, numWide,
+                                                inst);
+    return true;
 }
 
 } // namespace rgrt
