@@ -226,6 +226,16 @@ rgrt_status rgrt_export_tlas_wide(rgrt_ctx* ctx, int32_t compressed, void* nodes
  * scene DFS (:63-85) so customIndex is the DFS order. Asynchronous on the
  * context stream. */
 rgrt_status rgrt_build_tlas(rgrt_ctx* ctx, const rgrt_instance* instances, uint32_t numInstances);
+/* Per-frame update of the TLAS when only instance transforms changed (the
+ * scene DFS of raygun/render/acceleration_structure.cpp:63-85 then yields
+ * the same instances in the same order, entity.cpp:180-199 moving them):
+ * same count and BLAS per position as the last rgrt_build_tlas, new
+ * transforms / masks / offsets. Keeps the hierarchy and the leaf order,
+ * recomputes every box bottom-up, then the wide and compressed nodes. No
+ * counterpart in the reference, which rebuilds the TLAS every frame
+ * (raygun/render/raytracer.cpp:76-85). RGRT_ERR_INVALID_ARG if the count or
+ * a BLAS differs. Asynchronous on the context stream. */
+rgrt_status rgrt_refit_tlas(rgrt_ctx* ctx, const rgrt_instance* instances, uint32_t numInstances);
 rgrt_status rgrt_export_tlas(rgrt_ctx* ctx, rgrt_bvh_node* nodes, uint32_t nodeCapacity, uint32_t* leafInstance,
                              uint32_t leafCapacity, uint32_t* numNodes);
 

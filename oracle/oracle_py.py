@@ -53,6 +53,8 @@ def lib():
         L.orc_blas_morton.argtypes = [P, i32, P]
         L.orc_build_tlas.restype = i32
         L.orc_build_tlas.argtypes = [P, P, u32]
+        L.orc_refit_tlas.restype = i32
+        L.orc_refit_tlas.argtypes = [P, P, u32]
         L.orc_tlas_num_nodes.restype = u32
         L.orc_tlas_num_nodes.argtypes = [P]
         L.orc_export_tlas.restype = i32
@@ -148,6 +150,12 @@ class OracleScene:
         from raygun_amd.records import INSTANCE
         self._inst = np.ascontiguousarray(instances, INSTANCE)
         assert lib().orc_build_tlas(self.h, _p(self._inst), len(self._inst)) == 0
+
+    def refit_tlas(self, instances):
+        from raygun_amd.records import INSTANCE
+        inst = np.ascontiguousarray(instances, INSTANCE)
+        assert lib().orc_refit_tlas(self.h, _p(inst), len(inst)) == 0
+        self._inst = inst
 
     def export_tlas(self):
         from raygun_amd.records import BVH_NODE

@@ -1865,6 +1865,27 @@ int32_t orc_build_tlas(orc_scene* s, const void* instances, uint32_t n)
     return 0;
 }
 
+// Refit of the TLAS for moved instances: same count and BLAS per position
+// (the entity DFS keeps positions when only transforms change), new boxes,
+// the existing hierarchy and leaf order, then collapse and compression as in
+// a build (the wide topology follows the refitted areas).
+int32_t orc_refit_tlas(orc_scene* s, const void* instances, uint32_t n)
+{
+    if(!s->hasTlas || n != s->inst.size()) return -1;
+    const Instance* in = (const Instance*)instances;
+    for(uint32_t i = 0; i < n; ++i)
+        if(in[i].blas != s->inst[i].rec.blas) return -1;
+    for(uint32_t i = 0; i < n; ++i) {
+        s->inst[i].rec = in[i];
+        invert_affine(in[i].transform, s->inst[i].w2o);
+        s->tlas.primBox[i] = transform_box(in[i].transform, s->blas[in[i].blas].bvh.root);
+    }
+    compute_node_boxes(s->tlas);
+    collapse4(s->tlas);
+    quantize4(s->tlas);
+    return 0;
+}
+
 uint32_t orc_tlas_num_nodes(orc_scene* s) { return (uint32_t)s->tlas.nodes.size(); }
 
 int32_t orc_export_tlas(orc_scene* s, void* nodes, uint32_t* leafInstance)

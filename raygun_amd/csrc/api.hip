@@ -78,6 +78,7 @@ struct rgrt_ctx {
     uint32_t instCap = 0;
     uint32_t numInstances = 0;
     bool haveTlas = false;
+    std::vector<uint32_t> tlasBlas; // BLAS per instance position of the last build
     rgrt_instance* hInstStage = nullptr; // pinned
     hipEvent_t stageDone = nullptr;
     rgrt_instance* dInstIn = nullptr;
@@ -455,6 +456,41 @@ rgrt_status do_post(rgrt_ctx* c, int32_t useFxaa)
     return RGRT_OK;
 }
 
+// Validates the instances, starts the timing event and copies them to the
+// device through the pinned staging buffer.
+rgrt_status stage_instances(rgrt_ctx* c, const rgrt_instance* in, uint32_t n, const char* who)
+{
+    if(n && !in) return fail(c, RGRT_ERR_INVALID_ARG, std::string(who) + ": null instances");
+    for(uint32_t i = 0; i < n; ++i) {
+        if(in[i].blas >= c->blas.size() || !c->blas[in[i].blas].valid)
+            return fail(c, RGRT_ERR_INVALID_ARG, std::string(who) + ": instance references an invalid BLAS");
+        if(in[i].customIndex >= n)
+            return fail(c, RGRT_ERR_INVALID_ARG, std::string(who) + ": customIndex must index the offset table");
+    }
+    hipStream_t s = c->stream;
+    if(c->flags & RGRT_CFG_TIMING) {
+        make_event(c->evAsStart);
+        make_event(c->evAsEnd);
+        check(hipEventRecord(c->evAsStart, s));
+    }
+    ensure_inst_cap(c, std::max<uint32_t>(n, 1));
+    check(hipEventSynchronize(c->stageDone)); // previous frame's upload has left the staging buffer
+    if(n) std::memcpy(c->hInstStage, in, n * sizeof(rgrt_instance));
+    if(n) check(hipMemcpyAsync(c->dInstIn, c->hInstStage, n * sizeof(rgrt_instance), hipMemcpyHostToDevice, s));
+    check(hipEventRecord(c->stageDone, s));
+    return RGRT_OK;
+}
+
+void finish_tlas(rgrt_ctx* c, const rgrt_instance* in, uint32_t n)
+{
+    const bool timing = (c->flags & RGRT_CFG_TIMING) != 0;
+    if(timing) check(hipEventRecord(c->evAsEnd, c->stream));
+    c->timedAs = timing;
+    c->numInstances = n;
+    c->tlasBlas.resize(n);
+    for(uint32_t i = 0; i < n; ++i) c->tlasBlas[i] = in[i].blas;
+    c->haveTlas = true;
+}
 } // namespace
 
 extern "C" {
@@ -728,27 +764,11 @@ rgrt_status rgrt_export_tlas_wide(rgrt_ctx* c, int32_t compressed, void* nodes, 
 rgrt_status rgrt_build_tlas(rgrt_ctx* c, const rgrt_instance* in, uint32_t n)
 {
     return guarded(c, [&]() {
-        if(n && !in) return fail(c, RGRT_ERR_INVALID_ARG, "build_tlas: null instances");
-        for(uint32_t i = 0; i < n; ++i) {
-            if(in[i].blas >= c->blas.size() || !c->blas[in[i].blas].valid)
-                return fail(c, RGRT_ERR_INVALID_ARG, "build_tlas: instance references an invalid BLAS");
-            if(in[i].customIndex >= n)
-                return fail(c, RGRT_ERR_INVALID_ARG, "build_tlas: customIndex must index the offset table");
-        }
+        const rgrt_status st = stage_instances(c, in, n, "build_tlas");
+        if(st != RGRT_OK) return st;
         hipStream_t s = c->stream;
-        const bool timing = (c->flags & RGRT_CFG_TIMING) != 0;
-        if(timing) {
-            make_event(c->evAsStart);
-            make_event(c->evAsEnd);
-            check(hipEventRecord(c->evAsStart, s));
-        }
-        ensure_inst_cap(c, std::max<uint32_t>(n, 1));
-        check(hipEventSynchronize(c->stageDone)); // previous frame's upload has left the staging buffer
-        if(n) std::memcpy(c->hInstStage, in, n * sizeof(rgrt_instance));
-        if(n) check(hipMemcpyAsync(c->dInstIn, c->hInstStage, n * sizeof(rgrt_instance), hipMemcpyHostToDevice, s));
-        check(hipEventRecord(c->stageDone, s));
         if(!launch_tlas_fused(s, c->dInstIn, n, c->dBlasRoot, c->dW2O, c->dShade, c->tlasNodes, c->tlasLeaf, c->tlasRoot,
-                              c->tlasWide, c->tlasWideq, c->tlasNumWide, c->dInst)) {
+                              c->tlasWide, c->tlasWideq, c->tlasNumWide, c->dInst, false)) {
             c->builder.reserve(std::max<uint32_t>(n, 1));
             launch_inst_prep(s, c->dInstIn, n, c->dBlasRoot, c->builder.boxLo, c->builder.boxHi, c->dW2O, c->dShade);
             c->builder.build(s, n, c->tlasNodes, c->tlasLeaf, c->tlasRoot);
@@ -756,10 +776,32 @@ rgrt_status rgrt_build_tlas(rgrt_ctx* c, const rgrt_instance* in, uint32_t n)
             c->builder.quantize(s, std::max<uint32_t>(n, 2) - 1, c->tlasWide, c->tlasNumWide, c->tlasWideq);
             launch_inst_gather(s, c->dW2O, c->dInstIn, c->tlasLeaf, n, c->dInst);
         }
-        if(timing) check(hipEventRecord(c->evAsEnd, s));
-        c->timedAs = timing;
-        c->numInstances = n;
-        c->haveTlas = true;
+        finish_tlas(c, in, n);
+        return RGRT_OK;
+    });
+}
+
+rgrt_status rgrt_refit_tlas(rgrt_ctx* c, const rgrt_instance* in, uint32_t n)
+{
+    return guarded(c, [&]() {
+        if(!c->haveTlas) return fail(c, RGRT_ERR_NOT_READY, "refit_tlas: no TLAS built");
+        if(n != c->numInstances) return fail(c, RGRT_ERR_INVALID_ARG, "refit_tlas: instance count differs from the build");
+        for(uint32_t i = 0; i < n; ++i)
+            if(in && in[i].blas != c->tlasBlas[i])
+                return fail(c, RGRT_ERR_INVALID_ARG, "refit_tlas: an instance's BLAS differs from the build");
+        const rgrt_status st = stage_instances(c, in, n, "refit_tlas");
+        if(st != RGRT_OK) return st;
+        hipStream_t s = c->stream;
+        if(!launch_tlas_fused(s, c->dInstIn, n, c->dBlasRoot, c->dW2O, c->dShade, c->tlasNodes, c->tlasLeaf, c->tlasRoot,
+                              c->tlasWide, c->tlasWideq, c->tlasNumWide, c->dInst, true)) {
+            c->builder.reserve(std::max<uint32_t>(n, 1));
+            launch_inst_prep(s, c->dInstIn, n, c->dBlasRoot, c->builder.boxLo, c->builder.boxHi, c->dW2O, c->dShade);
+            c->builder.refit(s, n, c->tlasNodes, c->tlasLeaf, c->tlasRoot);
+            c->builder.collapse(s, n, c->tlasNodes, c->tlasWide, c->tlasNumWide);
+            c->builder.quantize(s, std::max<uint32_t>(n, 2) - 1, c->tlasWide, c->tlasNumWide, c->tlasWideq);
+            launch_inst_gather(s, c->dW2O, c->dInstIn, c->tlasLeaf, n, c->dInst);
+        }
+        finish_tlas(c, in, n);
         return RGRT_OK;
     });
 }

@@ -796,7 +796,8 @@ struct LdsNodes {
 __global__ __launch_bounds__(kCollapseThreads) void k_tlas_fused(
     const rgrt_instance* __restrict__ in, int n, const float4* __restrict__ blasRoot, float* __restrict__ w2oOut,
     InstanceShade* __restrict__ shade, Node* __restrict__ nodes, uint32_t* __restrict__ leafOut, float4* __restrict__ rootBox,
-    Node4* __restrict__ wide, Node4q* __restrict__ wideq, uint32_t* __restrict__ numWide, DevInstance* __restrict__ instOut)
+    Node4* __restrict__ wide, Node4q* __restrict__ wideq, uint32_t* __restrict__ numWide, DevInstance* __restrict__ instOut,
+    int refit)
 {
     __shared__ float ilo[3][kFusedTlasMax], ihi[3][kFusedTlasMax]; // instance world boxes
     __shared__ uint32_t key0[kFusedTlasMax];                         // by instance
@@ -836,21 +837,35 @@ __global__ __launch_bounds__(kCollapseThreads) void k_tlas_fused(
     }
     __syncthreads();
 
-    // Morton codes, then the rank of each in (key, index) order
-    uint32_t key = 0;
-    if(mine) {
-        key = morton_key(lo, hi, sb);
-        key0[tid] = key;
-    }
-    __syncthreads();
     uint32_t rank = 0;
-    if(mine) {
-        for(int j = 0; j < n; ++j) {
-            const uint32_t kj = key0[j];
-            rank += (kj < key || (kj == key && j < tid)) ? 1u : 0u;
+    if(refit) {
+        // the existing leaf order: key0 holds each instance's position in it
+        if(mine) {
+            const uint32_t i = leafOut[tid];
+            order[tid] = i < (uint32_t)n ? i : 0u;
+            if(i < (uint32_t)n) key0[i] = (uint32_t)tid;
         }
-        keys[rank] = key;
-        order[rank] = (uint32_t)tid;
+        __syncthreads();
+        if(mine) rank = min(key0[tid], (uint32_t)n - 1u);
+    }
+    else {
+        // Morton codes, then the rank of each in (key, index) order
+        uint32_t key = 0;
+        if(mine) {
+            key = morton_key(lo, hi, sb);
+            key0[tid] = key;
+        }
+        __syncthreads();
+        if(mine) {
+            for(int j = 0; j < n; ++j) {
+                const uint32_t kj = key0[j];
+                rank += (kj < key || (kj == key && j < tid)) ? 1u : 0u;
+            }
+            keys[rank] = key;
+            order[rank] = (uint32_t)tid;
+        }
+    }
+    if(mine) {
         // instance leaf at its Morton position
         DevInstance d;
         for(int q = 0; q < 12; ++q) d.w2o[q] = w[q];
@@ -862,18 +877,32 @@ __global__ __launch_bounds__(kCollapseThreads) void k_tlas_fused(
     }
     __syncthreads();
 
-    // hierarchy
+    // hierarchy: Karras, or the existing one for a refit
     if(tid < n - 1) {
-        int left, right, gamma;
-        karras_node(keys, n, tid, left, right, gamma);
+        int left, right;
+        if(refit) {
+            const Node nd = nodes[tid];
+            left = nd.child0;
+            right = nd.child1;
+        }
+        else {
+            int gamma;
+            karras_node(keys, n, tid, left, right, gamma);
+        }
         child0[tid] = left;
         child1[tid] = right;
         arrive[tid] = 0;
         if(tid == 0) parent[0] = 0xffffffffu;
-        if(left >= 0) parent[left] = (uint32_t)tid;
-        else leafParent[gamma] = (uint32_t)tid;
-        if(right >= 0) parent[right] = (uint32_t)tid;
-        else leafParent[gamma + 1] = (uint32_t)tid;
+        // every ref stays inside the tables (a refit reads nodes this
+        // library wrote; the check only keeps the accesses in bounds)
+        if(left >= 0) {
+            if(left < n - 1) parent[left] = (uint32_t)tid;
+        }
+        else if(~left < n) leafParent[~left] = (uint32_t)tid;
+        if(right >= 0) {
+            if(right < n - 1) parent[right] = (uint32_t)tid;
+        }
+        else if(~right < n) leafParent[~right] = (uint32_t)tid;
     }
     __syncthreads();
 
@@ -1065,11 +1094,11 @@ void launch_inst_gather(hipStream_t s, const float* w2o, const rgrt_instance* in
 
 bool launch_tlas_fused(hipStream_t s, const rgrt_instance* in, uint32_t n, const float4* blasRoot, float* w2o,
                        InstanceShade* shade, Node* nodes, uint32_t* leaf, float4* rootBox, Node4* wide, Node4q* wideq,
-                       uint32_t* numWide, DevInstance* inst)
+                       uint32_t* numWide, DevInstance* inst, bool refit)
 {
     if(n < 2 || n > (uint32_t)kFusedTlasMax) return false;
     k_tlas_fused<<<1, kCollapseThreads, 0, s>>>(in, (int)n, blasRoot, w2o, shade, nodes, leaf, rootBox, wide, wideq, numWide,
-                                                inst);
+                                                inst, refit ? 1 : 0);
     return true;
 }
 

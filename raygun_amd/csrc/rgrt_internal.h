@@ -54,9 +54,11 @@ void launch_inst_gather(hipStream_t s, const float* w2o, const rgrt_instance* in
                         DevInstance* out);
 // Whole TLAS build (instance prep through leaf records) in one workgroup for
 // 2 <= n <= 1024 instances; returns false, launching nothing, otherwise.
+// With refit set, the hierarchy and leaf order already in nodes / leaf are
+// kept and only the boxes, wide nodes and instance records are redone.
 bool launch_tlas_fused(hipStream_t s, const rgrt_instance* in, uint32_t n, const float4* blasRoot, float* w2o,
                        InstanceShade* shade, Node* nodes, uint32_t* leaf, float4* rootBox, Node4* wide, Node4q* wideq,
-                       uint32_t* numWide, DevInstance* inst);
+                       uint32_t* numWide, DevInstance* inst, bool refit);
 
 // ---------------------------------------------------------------------------
 // Wavefront state (device memory, structure of arrays, one slot per pixel)

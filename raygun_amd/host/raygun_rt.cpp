@@ -144,6 +144,81 @@ mat4 Transform::toMat4() const
 
 bool Transform::isZeroVolume() const { return scaling[0] * scaling[1] * scaling[2] == 0.0; }
 
+bool Transform::isIdentity() const
+{
+    return position == vec3{0.0, 0.0, 0.0} && rotation == quat{1.0, 0.0, 0.0, 0.0} && scaling == vec3{1.0, 1.0, 1.0};
+}
+
+Transform Transform::inverse() const
+{
+    Transform r;
+    const double n = rotation[0] * rotation[0] + rotation[1] * rotation[1] + rotation[2] * rotation[2] + rotation[3] * rotation[3];
+    for(int k = 0; k < 3; ++k) {
+        r.position[k] = -position[k];
+        r.scaling[k] = 1.0 / scaling[k];
+    }
+    r.rotation = {rotation[0] / n, -rotation[1] / n, -rotation[2] / n, -rotation[3] / n};
+    return r;
+}
+
+quat quatFromEuler(const vec3& a)
+{
+    const double cx = std::cos(a[0] * 0.5), cy = std::cos(a[1] * 0.5), cz = std::cos(a[2] * 0.5);
+    const double sx = std::sin(a[0] * 0.5), sy = std::sin(a[1] * 0.5), sz = std::sin(a[2] * 0.5);
+    return {cx * cy * cz + sx * sy * sz, sx * cy * cz - cx * sy * sz, cx * sy * cz + sx * cy * sz, cx * cy * sz - sx * sy * cz};
+}
+
+void Transform::move(const vec3& t)
+{
+    for(int k = 0; k < 3; ++k) position[k] += t[k];
+}
+
+void Transform::rotate(double angle, const vec3& axis)
+{
+    vec3 a = axis;
+    const double len = std::sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+    if(std::fabs(len - 1.0) > 0.001)
+        for(double& c: a) c /= len;
+    const double sn = std::sin(angle * 0.5);
+    rotation = quatMul(rotation, {std::cos(angle * 0.5), a[0] * sn, a[1] * sn, a[2] * sn});
+}
+
+void Transform::rotate(const vec3& eulerAngles) { rotation = quatMul(quatFromEuler(eulerAngles), rotation); }
+
+void Transform::rotateAround(const vec3& pivot, const vec3& eulerAngles)
+{
+    move({-pivot[0], -pivot[1], -pivot[2]});
+    rotate(eulerAngles);
+    move(pivot);
+}
+
+void Transform::lookAt(const vec3& target)
+{
+    rotation = quatLookAt(normalized({target[0] - position[0], target[1] - position[1], target[2] - position[2]}));
+}
+
+void Transform::scale(double f)
+{
+    for(double& c: scaling) c *= f;
+}
+
+void Transform::scale(const vec3& f)
+{
+    for(int k = 0; k < 3; ++k) scaling[k] *= f[k];
+}
+
+Transform interpolate(const Transform& x, const Transform& y, double factor)
+{
+    const double f = std::min(std::max(factor, 0.0), 1.0);
+    Transform r;
+    for(int k = 0; k < 3; ++k) {
+        r.position[k] = x.position[k] * (1.0 - f) + y.position[k] * f;
+        r.scaling[k] = x.scaling[k] * (1.0 - f) + y.scaling[k] * f;
+    }
+    for(int k = 0; k < 4; ++k) r.rotation[k] = x.rotation[k] * (1.0 - f) + y.rotation[k] * f;
+    return r;
+}
+
 // transform.hpp:113-120
 Transform Transform::operator*(const Transform& y) const
 {
@@ -159,16 +234,90 @@ Transform Transform::operator*(const Transform& y) const
     return t;
 }
 
-// entity.cpp:187-199
-Transform Entity::globalTransform() const
+// entity.cpp:180-270
+void Entity::setTransform(const Transform& t)
 {
-    const Transform parentTransform = parent ? parent->globalTransform() : Transform();
-    return parentTransform * localTransform;
+    invalidateChildrenCachedParentTransform();
+    m_transform = t;
 }
+
+void Entity::move(const vec3& translation)
+{
+    invalidateChildrenCachedParentTransform();
+    m_transform.move(translation);
+}
+
+void Entity::moveTo(const vec3& position)
+{
+    invalidateChildrenCachedParentTransform();
+    m_transform.position = position;
+}
+
+void Entity::rotate(double angle, const vec3& axis)
+{
+    invalidateChildrenCachedParentTransform();
+    m_transform.rotate(angle, axis);
+}
+
+void Entity::rotate(const vec3& eulerAngles)
+{
+    invalidateChildrenCachedParentTransform();
+    m_transform.rotate(eulerAngles);
+}
+
+void Entity::rotateAround(const vec3& pivot, const vec3& eulerAngles)
+{
+    invalidateChildrenCachedParentTransform();
+    m_transform.rotateAround(pivot, eulerAngles);
+}
+
+void Entity::lookAt(const vec3& target)
+{
+    invalidateChildrenCachedParentTransform();
+    m_transform.lookAt(target);
+}
+
+void Entity::scale(double factor)
+{
+    invalidateChildrenCachedParentTransform();
+    m_transform.scale(factor);
+}
+
+void Entity::scale(const vec3& factors)
+{
+    invalidateChildrenCachedParentTransform();
+    m_transform.scale(factors);
+}
+
+void Entity::setParent(const Entity* parent)
+{
+    invalidateCachedParentTransform();
+    m_parent = parent;
+}
+
+void Entity::invalidateCachedParentTransform()
+{
+    m_cachedParentTransform.reset();
+    invalidateChildrenCachedParentTransform();
+}
+
+void Entity::invalidateChildrenCachedParentTransform()
+{
+    for(const auto& c: children) c->invalidateCachedParentTransform();
+}
+
+// entity.cpp:187-199
+Transform Entity::parentTransform() const
+{
+    if(!m_cachedParentTransform) m_cachedParentTransform = m_parent ? m_parent->globalTransform() : Transform();
+    return *m_cachedParentTransform;
+}
+
+Transform Entity::globalTransform() const { return parentTransform() * m_transform; }
 
 Entity& Entity::addChild(std::shared_ptr<Entity> e)
 {
-    e->parent = this;
+    e->setParent(this);
     children.push_back(std::move(e));
     return *children.back();
 }
@@ -177,13 +326,48 @@ Entity& Entity::addChild(std::shared_ptr<Entity> e)
 void Entity::forEachEntity(const std::function<bool(const Entity&)>& f) const
 {
     if(!f(*this)) return;
+    for(const auto& c: children) static_cast<const Entity&>(*c).forEachEntity(f);
+}
+
+void Entity::forEachEntity(const std::function<bool(Entity&)>& f)
+{
+    if(!f(*this)) return;
     for(const auto& c: children) c->forEachEntity(f);
+}
+
+// entity.cpp:300-332
+bool EntityAnimation::update(double deltaTime, Entity& target)
+{
+    m_animationTime += deltaTime;
+    return runAnimation(target);
+}
+
+bool ScaleAnimation::runAnimation(Entity& target)
+{
+    const double t = m_animationTime / m_duration;
+    const double f = std::min(std::max(t, 0.0), 1.0);
+    Transform tr = target.transform();
+    for(int k = 0; k < 3; ++k) tr.scaling[k] = m_startScale[k] * (1.0 - f) + m_endScale[k] * f;
+    target.setTransform(tr);
+    return t <= 1.0;
+}
+
+void AnimatableEntity::update(double deltaTime)
+{
+    if(!m_animation) return;
+    if(!m_animation->update(deltaTime, *this)) {
+        m_animation.reset();
+        if(m_animationFinisher) (*m_animationFinisher)();
+        m_animationFinisher.reset();
+    }
 }
 
 void Camera::lookAt(const vec3& eye, const vec3& target)
 {
-    localTransform.position = eye;
-    localTransform.rotation = quatLookAt({target[0] - eye[0], target[1] - eye[1], target[2] - eye[2]});
+    Transform t = transform();
+    t.position = eye;
+    t.rotation = quatLookAt({target[0] - eye[0], target[1] - eye[1], target[2] - eye[2]});
+    setTransform(t);
 }
 
 // camera.cpp:34-47: glm::perspectiveRH_ZO (depth zero to one) with the
@@ -201,6 +385,14 @@ mat4 Camera::projection(uint32_t width, uint32_t height) const
     p[3][2] = -1.0;
     p[2][3] = -(zFar * zNear) / (zFar - zNear);
     return p;
+}
+
+void Scene::update(double deltaTime)
+{
+    root->forEachEntity([deltaTime](Entity& e) {
+        if(auto* a = dynamic_cast<AnimatableEntity*>(&e)) a->update(deltaTime);
+        return true;
+    });
 }
 
 std::vector<std::shared_ptr<render::Model>> Scene::models() const
@@ -274,6 +466,18 @@ TopLevelAS::TopLevelAS(rgrt_ctx* ctx, const Scene& scene) : m_instances(collectI
     ok(ctx, rgrt_build_tlas(ctx, m_instances.data(), (uint32_t)m_instances.size()), "TopLevelAS");
 }
 
+bool TopLevelAS::update(rgrt_ctx* ctx, const Scene& scene)
+{
+    std::vector<rgrt_instance> next = collectInstances(scene);
+    bool same = next.size() == m_instances.size();
+    for(size_t i = 0; same && i < next.size(); ++i) same = next[i].blas == m_instances[i].blas;
+    m_instances = std::move(next);
+    const uint32_t n = (uint32_t)m_instances.size();
+    if(same) ok(ctx, rgrt_refit_tlas(ctx, m_instances.data(), n), "TopLevelAS::update");
+    else ok(ctx, rgrt_build_tlas(ctx, m_instances.data(), n), "TopLevelAS::update");
+    return same;
+}
+
 // raytracer.cpp:54-74: one structure per distinct mesh (the reference
 // builds one per model; models sharing a mesh get identical trees)
 void Raytracer::setupBottomLevelAS(const Scene& scene)
@@ -282,8 +486,13 @@ void Raytracer::setupBottomLevelAS(const Scene& scene)
         if(model->mesh && !model->mesh->bottomLevelAS) model->mesh->bottomLevelAS = std::make_shared<BottomLevelAS>(m_ctx, *model->mesh);
 }
 
-// raytracer.cpp:76-85: rebuilt every frame
-void Raytracer::setupTopLevelAS(const Scene& scene) { m_topLevelAS = std::make_unique<TopLevelAS>(m_ctx, scene); }
+// raytracer.cpp:76-85: rebuilt every frame (refitted when refitTopLevelAS is
+// set and only transforms changed)
+void Raytracer::setupTopLevelAS(const Scene& scene)
+{
+    if(refitTopLevelAS && m_topLevelAS) m_topLevelAS->update(m_ctx, scene);
+    else m_topLevelAS = std::make_unique<TopLevelAS>(m_ctx, scene);
+}
 
 void Raytracer::updateRenderTarget(const rgrt_ubo& uniformBuffer) { ok(m_ctx, rgrt_set_frame(m_ctx, &uniformBuffer), "updateRenderTarget"); }
 

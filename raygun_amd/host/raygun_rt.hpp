@@ -3,10 +3,15 @@
 // Mirrors the classes a Raygun renderer calls, with the reference's names and
 // call order, as far as the ray-tracing path consumes them:
 //
-//   Transform          raygun/transform.hpp:29-120 (T * R * S, operator*)
-//   Entity             raygun/entity.hpp:32-131, entity.cpp:180-199
+//   Transform          raygun/transform.hpp:29-131 (T * R * S, operator*,
+//                      the editing helpers, interpolate)
+//   Entity             raygun/entity.hpp:32-131, entity.cpp:180-270 (cached
+//                      parent transform, dropped when an ancestor moves)
+//   EntityAnimation / ScaleAnimation / AnimatableEntity
+//                      raygun/entity.hpp:133-175, entity.cpp:300-332
 //   Camera             raygun/camera.hpp:29-48, camera.cpp:34-47
-//   Scene              raygun/scene.hpp:36-52
+//   Scene              raygun/scene.hpp:36-52; update() = the animation
+//                      step of the main loop (raygun.cpp:122-126)
 //   render::Mesh       raygun/render/mesh.hpp:30-52 (+ BufferRef offsets)
 //   render::Model      raygun/render/model.hpp:33-42
 //   render::BottomLevelAS / TopLevelAS
@@ -25,6 +30,7 @@
 #include <cstdint>
 #include <functional>
 #include <memory>
+#include <optional>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -47,6 +53,8 @@ mat4 mul(const mat4& a, const mat4& b);
 mat4 inverse(const mat4& m);
 quat quatLookAt(const vec3& direction, const vec3& up = {0.0, 1.0, 0.0});
 
+quat quatFromEuler(const vec3& angles); // glm quat(vec3 eulerAngles)
+
 // transform.hpp:29-120
 struct Transform {
     vec3 position = {0.0, 0.0, 0.0};
@@ -55,8 +63,23 @@ struct Transform {
 
     mat4 toMat4() const; // translate * toMat4(rotation) * scale
     bool isZeroVolume() const;
+    bool isIdentity() const;
+    Transform inverse() const; // component-wise, as transform.hpp:55-62 defines it
     Transform operator*(const Transform& y) const;
+
+    // transform.hpp:71-95
+    void move(const vec3& translation);
+    void rotate(double angle, const vec3& axis); // rotation * angleAxis (glm::rotate)
+    void rotate(const vec3& eulerAngles);        // quat(euler) * rotation
+    // move(-pivot), rotate, move(pivot): only the orientation changes, as in the reference
+    void rotateAround(const vec3& pivot, const vec3& eulerAngles);
+    void lookAt(const vec3& target);
+    void scale(double factor);
+    void scale(const vec3& factors);
 };
+
+// transform.hpp:122-131: component-wise lerp with the factor clamped to [0, 1]
+Transform interpolate(const Transform& x, const Transform& y, double factor);
 
 namespace render {
 
@@ -94,21 +117,86 @@ rgrt_material defaultMaterial(); // gpu_material.def:11-23
 // entity.hpp:32-131
 struct Entity {
     std::string name;
-    Transform localTransform;
     std::shared_ptr<render::Model> model;
     bool visible = true;
-    Entity* parent = nullptr;
     std::vector<std::shared_ptr<Entity>> children;
 
     explicit Entity(std::string n = "") : name(std::move(n)) {}
     virtual ~Entity() = default;
 
-    const Transform& transform() const { return localTransform; }
+    const Transform& transform() const { return m_transform; }
     bool isVisible() const { return visible; }
+    const Entity* parent() const { return m_parent; }
+
+    // entity.cpp:180-270: every change drops the cached parent transform of
+    // the descendants
+    void setTransform(const Transform& t);
+    void move(const vec3& translation);
+    void moveTo(const vec3& position);
+    void rotate(double angle, const vec3& axis);
+    void rotate(const vec3& eulerAngles);
+    void rotateAround(const vec3& pivot, const vec3& eulerAngles);
+    void lookAt(const vec3& target);
+    void scale(double factor);
+    void scale(const vec3& factors);
+
+    Transform parentTransform() const; // cached
     Transform globalTransform() const; // parentTransform() * m_transform
     Entity& addChild(std::shared_ptr<Entity> e);
     // pre-order; the callback returns false to skip the subtree
     void forEachEntity(const std::function<bool(const Entity&)>& f) const;
+    void forEachEntity(const std::function<bool(Entity&)>& f);
+
+  private:
+    void setParent(const Entity* parent);
+    void invalidateCachedParentTransform();
+    void invalidateChildrenCachedParentTransform();
+
+    Transform m_transform;
+    const Entity* m_parent = nullptr;
+    mutable std::optional<Transform> m_cachedParentTransform;
+};
+
+// entity.hpp:133-153, entity.cpp:300-320
+struct EntityAnimation {
+    virtual ~EntityAnimation() = default;
+    bool update(double deltaTime, Entity& target); // false once finished
+
+  protected:
+    virtual bool runAnimation(Entity& target) = 0;
+    double m_animationTime = 0.0;
+};
+
+struct ScaleAnimation : EntityAnimation {
+    ScaleAnimation(double duration, const vec3& startScale, const vec3& endScale)
+        : m_duration(duration), m_startScale(startScale), m_endScale(endScale)
+    {
+    }
+
+  protected:
+    bool runAnimation(Entity& target) override;
+
+  private:
+    double m_duration;
+    vec3 m_startScale, m_endScale;
+};
+
+// entity.hpp:155-175, entity.cpp:322-332
+struct AnimatableEntity : Entity {
+    explicit AnimatableEntity(std::string n = "") : Entity(std::move(n)) {}
+    virtual void update(double deltaTime);
+
+    template<typename T>
+    void setAnimation(const T& animation)
+    {
+        m_animation = std::make_unique<T>(animation);
+    }
+    void setAnimationFinisher(const std::function<void()>& f) { m_animationFinisher = f; }
+    bool animating() const { return m_animation != nullptr; }
+
+  protected:
+    std::unique_ptr<EntityAnimation> m_animation;
+    std::optional<std::function<void()>> m_animationFinisher;
 };
 
 // camera.hpp:29-48; the field of view is a parameter (reference: 45 degrees)
@@ -125,6 +213,8 @@ struct Scene {
     std::shared_ptr<Camera> camera = std::make_shared<Camera>();
     // distinct models in entity pre-order (first use)
     std::vector<std::shared_ptr<render::Model>> models() const;
+    // the animation step of the main loop (raygun.cpp:122-126)
+    void update(double deltaTime);
 };
 
 namespace render {
@@ -145,6 +235,11 @@ struct TopLevelAS {
     TopLevelAS(rgrt_ctx* ctx, const Scene& scene);
     static std::vector<rgrt_instance> collectInstances(const Scene& scene);
     const std::vector<rgrt_instance>& instances() const { return m_instances; }
+    // Per-frame update without a rebuild (no counterpart in the reference):
+    // when the scene yields the same instance count and BLASes (only
+    // transforms changed) the tree is refitted (rgrt_refit_tlas); otherwise
+    // it is rebuilt. Returns true for a refit.
+    bool update(rgrt_ctx* ctx, const Scene& scene);
 
   private:
     std::vector<rgrt_instance> m_instances;
@@ -162,6 +257,9 @@ struct Raytracer {
     rgrt_image doRaytracing();
 
     bool useFXAA = true; // m_useFXAA (raytracer.hpp:80)
+    // The reference rebuilds the TLAS every frame; with refitTopLevelAS set,
+    // frames that only move instances refit it (TopLevelAS::update).
+    bool refitTopLevelAS = false;
 
   private:
     rgrt_ctx* m_ctx;

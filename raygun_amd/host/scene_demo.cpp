@@ -8,6 +8,10 @@
 //   scene_demo render W H OUT          render one frame; write the final image
 //                                      (rgba16f) to OUT and the frame constants
 //                                      to OUT.ubo
+//   scene_demo animate FRAMES          run FRAMES frames of the per-frame update
+//                                      (a scale animation on cube2, the group
+//                                      turning, cube4 moving) and print the
+//                                      TLAS instances of every frame as JSON
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -77,18 +81,21 @@ quat axisAngle(double ax, double ay, double az, double angleDeg)
     return {std::cos(h), ax / l * s, ay / l * s, az / l * s};
 }
 
-std::shared_ptr<Entity> entity(const std::string& name, std::shared_ptr<Model> model, vec3 pos, quat rot = {1, 0, 0, 0},
-                               vec3 scale = {1, 1, 1})
+template<typename E = Entity>
+std::shared_ptr<E> entity(const std::string& name, std::shared_ptr<Model> model, vec3 pos, quat rot = {1, 0, 0, 0},
+                          vec3 scale = {1, 1, 1})
 {
-    auto e = std::make_shared<Entity>(name);
+    auto e = std::make_shared<E>(name);
     e->model = std::move(model);
-    e->localTransform.position = pos;
-    e->localTransform.rotation = rot;
-    e->localTransform.scaling = scale;
+    Transform t;
+    t.position = pos;
+    t.rotation = rot;
+    t.scaling = scale;
+    e->setTransform(t);
     return e;
 }
 
-Scene demoScene()
+Scene demoScene(bool animated = false)
 {
     auto cubeMesh = cube();
     auto groundMesh = ground();
@@ -123,7 +130,14 @@ Scene demoScene()
     s.root->addChild(entity("ground", floor, {0, 0, 0}, {1, 0, 0, 0}, {20, 1, 20}));
     auto& group = s.root->addChild(entity("group", nullptr, {0, 0.5, 0}, axisAngle(0, 1, 0, 30)));
     group.addChild(entity("cube1", red, {-1.2, 0, 0}));
-    group.addChild(entity("cube2", glass, {1.2, 0.2, 0}, {1, 0, 0, 0}, {0.8, 0.8, 0.8}));
+    if(animated) {
+        auto c2 = entity<AnimatableEntity>("cube2", glass, {1.2, 0.2, 0}, {1, 0, 0, 0}, {0.8, 0.8, 0.8});
+        c2->setAnimation(ScaleAnimation(1.0, {0.8, 0.8, 0.8}, {1.3, 1.3, 1.3}));
+        group.addChild(c2);
+    }
+    else {
+        group.addChild(entity("cube2", glass, {1.2, 0.2, 0}, {1, 0, 0, 0}, {0.8, 0.8, 0.8}));
+    }
     auto& hidden = group.addChild(entity("hidden", red, {0, 3, 0}));
     hidden.visible = false;
     hidden.addChild(entity("under-hidden", glass, {0, 1, 0}));
@@ -139,6 +153,16 @@ void printFloats(const float* f, int n)
     std::printf("[");
     for(int i = 0; i < n; ++i) std::printf("%s%.9g", i ? ", " : "", (double)f[i]);
     std::printf("]");
+}
+
+Entity* findEntity(Entity& root, const std::string& name)
+{
+    Entity* found = nullptr;
+    root.forEachEntity([&](Entity& e) {
+        if(e.name == name) found = &e;
+        return found == nullptr;
+    });
+    return found;
 }
 
 } // namespace
@@ -179,6 +203,33 @@ int main(int argc, char** argv)
             std::printf("}\n");
             return 0;
         }
+        if(mode == "animate" && argc == 3) {
+            Scene a = demoScene(true);
+            const int frames = std::atoi(argv[2]);
+            rgrt_blas next = 0;
+            for(const auto& mo: a.models())
+                if(!mo->mesh->bottomLevelAS) mo->mesh->bottomLevelAS = std::make_shared<render::BottomLevelAS>(next++);
+            Entity* group = findEntity(*a.root, "group");
+            Entity* cube4 = findEntity(*a.root, "cube4");
+            int finished = 0;
+            static_cast<AnimatableEntity*>(findEntity(*a.root, "cube2"))->setAnimationFinisher([&]() { ++finished; });
+            std::printf("{\"frames\": [");
+            for(int f = 0; f < frames; ++f) {
+                a.update(0.25);
+                group->rotate(vec3{0.0, 0.1, 0.0});
+                cube4->move({0.05, 0.0, 0.0});
+                const auto inst = render::TopLevelAS::collectInstances(a);
+                std::printf("%s\n  [", f ? "," : "");
+                for(size_t k = 0; k < inst.size(); ++k) {
+                    std::printf("%s{\"blas\": %u, \"transform\": ", k ? ", " : "", inst[k].blas);
+                    printFloats(inst[k].transform, 12);
+                    std::printf("}");
+                }
+                std::printf("]");
+            }
+            std::printf("],\n \"finished\": %d}\n", finished);
+            return 0;
+        }
         if(mode == "render" && argc == 5) {
             const uint32_t W = (uint32_t)std::atoi(argv[2]), H = (uint32_t)std::atoi(argv[3]);
             render::RenderSystem rs(W, H);
@@ -191,7 +242,7 @@ int main(int argc, char** argv)
             std::printf("rendered %ux%u, %zu instances\n", W, H, render::TopLevelAS::collectInstances(s).size());
             return 0;
         }
-        std::fprintf(stderr, "usage: scene_demo collect | render W H OUT\n");
+        std::fprintf(stderr, "usage: scene_demo collect | render W H OUT | animate FRAMES\n");
         return 2;
     }
     catch(const Error& e) {

@@ -5,6 +5,7 @@ Same method names and call order as the reference's host code:
 - ``RenderSystem.setupModelBuffers``  raygun/render/render_system.cpp:192-223
 - ``Raytracer.setupBottomLevelAS``     raygun/render/raytracer.cpp:54-74
 - ``Raytracer.setupTopLevelAS``        raygun/render/raytracer.cpp:76-85
+  (optionally a refit when only transforms changed, ``refit_tlas``)
 - ``RenderSystem.updateUniformBuffer`` raygun/render/render_system.cpp:246-268
 - ``Raytracer.doRaytracing``           raygun/render/raytracer.cpp:87-147
 - ``RenderSystem.render``              raygun/render/render_system.cpp:88-162
@@ -23,6 +24,11 @@ class Raytracer:
     def __init__(self, ctx):
         self.ctx = ctx
         self.use_fxaa = True  # m_useFXAA (raytracer.hpp:80)
+        # The reference rebuilds the TLAS every frame. With refit_tlas set, a
+        # frame whose instances keep their count and BLASes (only transforms
+        # changed) refits the previous tree instead.
+        self.refit_tlas = False
+        self._last_blas = None
 
     def setupBottomLevelAS(self, models):
         built = {}
@@ -37,7 +43,13 @@ class Raytracer:
             model.blas = built[key]
 
     def setupTopLevelAS(self, scene):
-        self.ctx.build_tlas(scene.collect_instances())
+        inst = scene.collect_instances()
+        blas = inst["blas"].copy()
+        if self.refit_tlas and self._last_blas is not None and np.array_equal(blas, self._last_blas):
+            self.ctx.refit_tlas(inst)
+        else:
+            self.ctx.build_tlas(inst)
+        self._last_blas = blas
 
     def updateRenderTarget(self, ubo):
         self.ctx.set_frame(ubo)

@@ -28,7 +28,7 @@ EXPORTS = [
     "rgrt_create", "rgrt_destroy", "rgrt_last_error", "rgrt_stream", "rgrt_synchronize",
     "rgrt_upload_geometry", "rgrt_upload_materials", "rgrt_update_vertices",
     "rgrt_build_blas", "rgrt_refit_blas", "rgrt_blas_info", "rgrt_export_blas",
-    "rgrt_build_tlas", "rgrt_export_tlas", "rgrt_set_frame", "rgrt_resize",
+    "rgrt_build_tlas", "rgrt_refit_tlas", "rgrt_export_tlas", "rgrt_set_frame", "rgrt_resize",
     "rgrt_trace_rays", "rgrt_trace_rays_device", "rgrt_render", "rgrt_trace", "rgrt_postprocess",
     "rgrt_read_image", "rgrt_image_device_ptr", "rgrt_write_image", "rgrt_get_stats", "rgrt_get_pass_counts",
     "rgrt_set_tuning", "rgrt_get_tuning", "rgrt_export_blas_wide", "rgrt_export_tlas_wide", "rgrt_version",
@@ -78,6 +78,7 @@ def lib():
             "rgrt_blas_info": (i32, [P, u32, ctypes.POINTER(u32), ctypes.POINTER(u32), P, P]),
             "rgrt_export_blas": (i32, [P, u32, P, u32, P, u32]),
             "rgrt_build_tlas": (i32, [P, P, u32]),
+            "rgrt_refit_tlas": (i32, [P, P, u32]),
             "rgrt_export_tlas": (i32, [P, P, u32, P, u32, ctypes.POINTER(u32)]),
             "rgrt_set_frame": (i32, [P, P]),
             "rgrt_resize": (i32, [P, u32, u32]),
@@ -203,6 +204,13 @@ class Context:
     def build_tlas(self, instances):
         inst = np.ascontiguousarray(instances, INSTANCE)
         self._ok(self._L.rgrt_build_tlas(self.h, _p(inst), len(inst)))
+        self._ninst = len(inst)
+
+    def refit_tlas(self, instances):
+        """Same instance count and BLAS per position as the last build: new
+        transforms on the existing hierarchy."""
+        inst = np.ascontiguousarray(instances, INSTANCE)
+        self._ok(self._L.rgrt_refit_tlas(self.h, _p(inst), len(inst)))
         self._ninst = len(inst)
 
     def export_tlas(self):

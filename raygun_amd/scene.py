@@ -6,8 +6,16 @@ raygun/camera.cpp:34-47, raygun/render/model.hpp:33-42,
 raygun/render/mesh.hpp:30-52) are restated here only as far as the
 ray-tracing path consumes them:
 
-- ``Transform.to_mat4``          transform.hpp:38-46  (T * R * S)
-- ``Entity.global_transform``    entity.cpp:187-199  (parent * local)
+- ``Transform.to_mat4``          transform.hpp:38-46  (T * R * S); the
+  editing helpers ``move`` / ``rotate`` / ``rotate_euler`` /
+  ``rotate_around`` / ``look_at`` / ``scale`` of transform.hpp:71-95 and
+  ``interpolate`` (:122-131)
+- ``Entity.global_transform``    entity.cpp:187-199  (parent * local, the
+  parent's global transform cached until the entity or an ancestor moves:
+  entity.cpp:180-270)
+- ``EntityAnimation``, ``ScaleAnimation``, ``AnimatableEntity``
+  entity.hpp:133-175, entity.cpp:300-332; ``Scene.update`` runs them once
+  per frame as the main loop does (raygun.cpp:122-126)
 - ``Scene.collect_instances``    acceleration_structure.cpp:55-85 (the DFS:
   invisible or zero-volume subtrees are skipped, entities without a model
   are descended into, customIndex = DFS order)
@@ -90,6 +98,60 @@ class Transform:
     def is_zero_volume(self):
         return float(self.scaling[0] * self.scaling[1] * self.scaling[2]) == 0.0
 
+    def copy(self):
+        return Transform(self.position.copy(), self.rotation.copy(), self.scaling.copy())
+
+    def is_identity(self):
+        return bool(np.all(self.position == 0) and np.all(self.rotation == (1, 0, 0, 0)) and np.all(self.scaling == 1))
+
+    def inverse(self):
+        """transform.hpp:55-62 (component-wise, as the reference defines it)."""
+        w, x, y, z = self.rotation
+        n = w * w + x * x + y * y + z * z
+        return Transform(-self.position, np.array([w, -x, -y, -z]) / n, 1.0 / self.scaling)
+
+    def up(self):
+        return quat_to_mat3(self.rotation) @ np.array([0.0, 1.0, 0.0])
+
+    def right(self):
+        return quat_to_mat3(self.rotation) @ np.array([1.0, 0.0, 0.0])
+
+    def forward(self):
+        return quat_to_mat3(self.rotation) @ np.array([0.0, 0.0, -1.0])
+
+    def move(self, translation):
+        self.position = self.position + np.asarray(translation, np.float64)
+
+    def rotate(self, angle, axis):
+        """glm::rotate(rotation, angle, axis): rotation * angleAxis (the axis
+        normalised when its length is off by more than 0.001)."""
+        a = np.asarray(axis, np.float64)
+        ln = float(np.linalg.norm(a))
+        if abs(ln - 1.0) > 0.001:
+            a = a / ln
+        sn = math.sin(angle * 0.5)
+        self.rotation = quat_mul(self.rotation, (math.cos(angle * 0.5), a[0] * sn, a[1] * sn, a[2] * sn))
+
+    def rotate_euler(self, angles):
+        """rotation = quat(eulerAngles) * rotation (transform.hpp:73)."""
+        self.rotation = quat_mul(quat_from_euler(angles), self.rotation)
+
+    def rotate_around(self, pivot, angles):
+        """transform.hpp:75-80: move(-pivot), rotate(angles), move(pivot). The
+        rotation does not touch the position, so only the orientation
+        changes, as in the reference."""
+        pv = np.asarray(pivot, np.float64)
+        self.move(-pv)
+        self.rotate_euler(angles)
+        self.move(pv)
+
+    def look_at(self, target):
+        d = np.asarray(target, np.float64) - self.position
+        self.rotation = quat_look_at(d / np.linalg.norm(d))
+
+    def scale(self, factors):
+        self.scaling = self.scaling * np.asarray(factors, np.float64)
+
     def __mul__(self, y):
         """operator* (transform.hpp:113-120)."""
         r = Transform()
@@ -97,6 +159,26 @@ class Transform:
         r.rotation = quat_mul(self.rotation, y.rotation)
         r.scaling = self.scaling * y.scaling
         return r
+
+
+def quat_from_euler(angles):
+    """glm quat(vec3 eulerAngles) (pitch x, yaw y, roll z)."""
+    c = np.cos(np.asarray(angles, np.float64) * 0.5)
+    sn = np.sin(np.asarray(angles, np.float64) * 0.5)
+    return np.array([
+        c[0] * c[1] * c[2] + sn[0] * sn[1] * sn[2],
+        sn[0] * c[1] * c[2] - c[0] * sn[1] * sn[2],
+        c[0] * sn[1] * c[2] + sn[0] * c[1] * sn[2],
+        c[0] * c[1] * sn[2] - sn[0] * sn[1] * c[2],
+    ])
+
+
+def interpolate(x, y, factor):
+    """transform.hpp:122-131: component-wise lerp, factor clamped to [0, 1]
+    (the quaternion too, without renormalisation, as glm::lerp does)."""
+    f = min(max(float(factor), 0.0), 1.0)
+    return Transform(x.position * (1 - f) + y.position * f, x.rotation * (1 - f) + y.rotation * f,
+                     x.scaling * (1 - f) + y.scaling * f)
 
 
 class Mesh:
@@ -124,25 +206,85 @@ class Model:
 
 
 class Entity:
-    """raygun/entity.hpp:32-131 (transform, optional model, children, visibility)."""
+    """raygun/entity.hpp:32-131 (transform, optional model, children, visibility).
+
+    ``transform`` is the local transform; assigning it (or calling one of the
+    editing methods) drops the cached parent transform of every descendant,
+    as Entity::setTransform does (entity.cpp:180-185). Changing the fields of
+    ``transform`` in place bypasses that: use ``set_transform``."""
 
     def __init__(self, name="", transform=None, model=None, visible=True):
         self.name = name
-        self.transform = transform or Transform()
+        self._transform = transform or Transform()
+        self._cached_parent = None
         self.model = model
         self.visible = visible
         self.children = []
         self.parent = None
 
+    @property
+    def transform(self):
+        return self._transform
+
+    @transform.setter
+    def transform(self, t):
+        self.set_transform(t)
+
+    def set_transform(self, t):
+        self._invalidate_children()
+        self._transform = t
+
     def add_child(self, e):
-        e.parent = self
+        e.set_parent(self)
         self.children.append(e)
         return e
 
+    def set_parent(self, parent):
+        self._invalidate()
+        self.parent = parent
+
+    def _invalidate(self):
+        self._cached_parent = None
+        self._invalidate_children()
+
+    def _invalidate_children(self):
+        for c in self.children:
+            c._invalidate()
+
+    def _edit(self, f):
+        self._invalidate_children()
+        f(self._transform)
+
+    def move(self, translation):
+        self._edit(lambda t: t.move(translation))
+
+    def move_to(self, position):
+        self._edit(lambda t: setattr(t, "position", np.asarray(position, np.float64)))
+
+    def rotate(self, angle, axis):
+        self._edit(lambda t: t.rotate(angle, axis))
+
+    def rotate_euler(self, angles):
+        self._edit(lambda t: t.rotate_euler(angles))
+
+    def rotate_around(self, pivot, angles):
+        self._edit(lambda t: t.rotate_around(pivot, angles))
+
+    def look_at_target(self, target):
+        self._edit(lambda t: t.look_at(target))
+
+    def scale(self, factors):
+        self._edit(lambda t: t.scale(factors))
+
+    def parent_transform(self):
+        """entity.cpp:187-194: the parent's global transform, cached."""
+        if self._cached_parent is None:
+            self._cached_parent = self.parent.global_transform() if self.parent is not None else Transform()
+        return self._cached_parent
+
     def global_transform(self):
-        """entity.cpp:187-199: parentTransform() * m_transform."""
-        parent = self.parent.global_transform() if self.parent is not None else Transform()
-        return parent * self.transform
+        """entity.cpp:196-199: parentTransform() * m_transform."""
+        return self.parent_transform() * self._transform
 
     def for_each_entity(self, f):
         """entity.hpp:67-84: pre-order; f returns False to skip the subtree."""
@@ -152,12 +294,72 @@ class Entity:
             c.for_each_entity(f)
 
 
+class EntityAnimation:
+    """entity.hpp:133-141, entity.cpp:300-304: accumulates time and runs."""
+
+    def __init__(self):
+        self.animation_time = 0.0
+
+    def update(self, delta_time, target):
+        self.animation_time += delta_time
+        return self.run_animation(target)
+
+    def run_animation(self, target):
+        raise NotImplementedError
+
+
+class ScaleAnimation(EntityAnimation):
+    """entity.cpp:306-320: scaling = mix(start, end, clamp(t / duration, 0, 1));
+    runs while t <= duration."""
+
+    def __init__(self, duration, start_scale, end_scale):
+        super().__init__()
+        self.duration = float(duration)
+        self.start_scale = np.asarray(start_scale, np.float64)
+        self.end_scale = np.asarray(end_scale, np.float64)
+
+    def run_animation(self, target):
+        t = self.animation_time / self.duration
+        f = min(max(t, 0.0), 1.0)
+        tr = target.transform.copy()
+        tr.scaling = self.start_scale * (1 - f) + self.end_scale * f
+        target.set_transform(tr)
+        return t <= 1.0
+
+
+class AnimatableEntity(Entity):
+    """entity.hpp:155-175, entity.cpp:322-332."""
+
+    def __init__(self, name="", transform=None, model=None, visible=True):
+        super().__init__(name, transform, model, visible)
+        self.animation = None
+        self.animation_finisher = None
+
+    def set_animation(self, animation):
+        self.animation = animation
+
+    def set_animation_finisher(self, f):
+        self.animation_finisher = f
+
+    def update(self, delta_time):
+        if self.animation is not None:
+            if not self.animation.update(delta_time, self):
+                self.animation = None
+                if self.animation_finisher is not None:
+                    self.animation_finisher()
+                self.animation_finisher = None
+
+
 class Scene:
     """raygun/scene.hpp:36-52: a camera and the entity tree."""
 
     def __init__(self):
         self.root = Entity("root")
         self.camera = Camera()
+
+    def update(self, delta_time):
+        """The per-frame animation step of the main loop (raygun.cpp:122-126)."""
+        self.root.for_each_entity(lambda e: e.update(delta_time) if isinstance(e, AnimatableEntity) else None)
 
     def models(self):
         out = []
@@ -231,8 +433,10 @@ class Camera(Entity):
         self.far = far
 
     def look_at(self, eye, target):
-        self.transform.position = np.asarray(eye, np.float64)
-        self.transform.rotation = quat_look_at(np.asarray(target, np.float64) - self.transform.position)
+        t = self.transform.copy()
+        t.position = np.asarray(eye, np.float64)
+        t.rotation = quat_look_at(np.asarray(target, np.float64) - t.position)
+        self.set_transform(t)
 
     def projection(self, width, height):
         """glm::perspectiveRH_ZO (GLM_FORCE_DEPTH_ZERO_TO_ONE) with [1][1] *= -1 (camera.cpp:34-47).

@@ -5,7 +5,7 @@ import math
 import numpy as np
 
 from raygun_amd.records import VERTEX, default_material
-from raygun_amd.scene import Entity, Mesh, Model, Scene, Transform
+from raygun_amd.scene import AnimatableEntity, Entity, Mesh, Model, ScaleAnimation, Scene, Transform
 
 
 def _vtx(p, n):
@@ -51,7 +51,9 @@ def _entity(name, model, pos, rot=(1, 0, 0, 0), scale=(1, 1, 1)):
     return Entity(name, Transform(pos, rot, scale), model)
 
 
-def demo_scene():
+def demo_scene(animated=False):
+    """animated: cube2 is an AnimatableEntity scaling from 0.8 to 1.3 over one
+    second (scene_demo animate)."""
     cube_mesh, ground_mesh = cube(), ground()
     red = Model(cube_mesh, [default_material(diffuse=(0.8, 0.1, 0.1), reflectivity=0.5, roughness=0.3)])
     glass = Model(cube_mesh, [default_material(diffuse=(0.9, 0.9, 1.0), transparency=0.9, ior=1.5)])
@@ -60,7 +62,12 @@ def demo_scene():
     sc.root.add_child(_entity("ground", floor, (0, 0, 0), scale=(20, 1, 20)))
     group = sc.root.add_child(_entity("group", None, (0, 0.5, 0), axis_angle(0, 1, 0, 30)))
     group.add_child(_entity("cube1", red, (-1.2, 0, 0)))
-    group.add_child(_entity("cube2", glass, (1.2, 0.2, 0), scale=(0.8, 0.8, 0.8)))
+    if animated:
+        c2 = AnimatableEntity("cube2", Transform((1.2, 0.2, 0), (1, 0, 0, 0), (0.8, 0.8, 0.8)), glass)
+        c2.set_animation(ScaleAnimation(1.0, (0.8, 0.8, 0.8), (1.3, 1.3, 1.3)))
+        group.add_child(c2)
+    else:
+        group.add_child(_entity("cube2", glass, (1.2, 0.2, 0), scale=(0.8, 0.8, 0.8)))
     hidden = group.add_child(_entity("hidden", red, (0, 3, 0)))
     hidden.visible = False
     hidden.add_child(_entity("under-hidden", glass, (0, 1, 0)))

@@ -96,3 +96,59 @@ def test_tlas_matches_oracle(scene, n):
     hit = ho["instance"] != 0xFFFFFFFF
     assert hit.mean() > 0.2
     assert np.array_equal(hg["t"][hit & ~tie], ho["t"][hit & ~tie])
+
+
+def _moved(inst, seed):
+    rng = np.random.default_rng(seed)
+    out = inst.copy()
+    t = out["transform"].reshape(-1, 3, 4)
+    t[:, :, 3] += rng.uniform(-8.0, 8.0, size=(len(inst), 3))
+    out["transform"] = t.reshape(-1, 12)
+    return out
+
+
+@pytest.mark.parametrize("n", [1, 2, 64, 1024, 1500])
+def test_tlas_refit_matches_oracle(scene, n):
+    """Build, move every instance, refit on both sides: the same binary, wide
+    and compressed trees (hierarchy and leaf order of the build kept) and the
+    same closest hits."""
+    osc, ctx = scene
+    print(f"[case] tlas-refit n={n}", flush=True)
+    inst = _instances(n, seed=500 + n)
+    ctx.build_tlas(inst)
+    osc.build_tlas(inst)
+    g0, l0 = ctx.export_tlas()
+    moved = _moved(inst, seed=n)
+    ctx.refit_tlas(moved)
+    osc.refit_tlas(moved)
+    g_nodes, g_leaf = ctx.export_tlas()
+    o_nodes, o_leaf = osc.export_tlas()
+    assert np.array_equal(g_leaf, l0) and np.array_equal(g_nodes["child"], g0["child"])
+    assert np.array_equal(g_leaf, o_leaf)
+    for k in ("child", "parent", "lo", "hi"):
+        assert np.array_equal(g_nodes[k], o_nodes[k]), k
+    gw, ow = ctx.export_tlas_wide(), osc.export_tlas_wide()
+    assert len(gw) == len(ow)
+    for k in ("child", "lox", "hix", "loy", "hiy", "loz", "hiz"):
+        assert np.array_equal(gw[k], ow[k]), k
+    assert ctx.export_tlas_wide(compressed=True).tobytes() == osc.export_tlas_wide(compressed=True).tobytes()
+    rays = _rays(20_000, seed=7 * n, inst=moved)
+    hg = ctx.trace_rays(rays)
+    ho, tie = osc.trace_rays(rays)
+    assert len(compare_hits(hg, ho, tie)) == 0
+    hit = ho["instance"] != 0xFFFFFFFF
+    assert hit.mean() > 0.2
+    assert np.array_equal(hg["t"][hit & ~tie], ho["t"][hit & ~tie])
+
+
+def test_tlas_refit_rejects_changed_instances(scene):
+    osc, ctx = scene
+    inst = _instances(10, seed=9)
+    ctx.build_tlas(inst)
+    with pytest.raises(rgrt.RgrtError):
+        ctx.refit_tlas(inst[:9])
+    other = inst.copy()
+    other["blas"][0] ^= 1
+    with pytest.raises(rgrt.RgrtError):
+        ctx.refit_tlas(other)
+    ctx.refit_tlas(inst)  # the context still works
