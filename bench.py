@@ -110,6 +110,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-rows", type=int, default=0, help="rows of the frame the CPU baseline traces (0 = auto)")
     ap.add_argument("--no-fxaa", action="store_true")
+    ap.add_argument("--mode", choices=("eager", "graph"), default="eager",
+                    help="frame submission: eager calls, or rgrt_frame as one hipGraph launch")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="run only the counter-calibration copy kernel over MIB MiB")
     args = ap.parse_args()
@@ -147,28 +149,36 @@ def main():
     ubo = setup.ubo()
     use_fxaa = not args.no_fxaa
 
-    def step():
+    def eager_step():
         ctx.build_tlas(instances)
         ctx.set_frame(ubo)
         ctx.render(use_fxaa)
 
-    for _ in range(args.warmup):
-        step()
-    ctx.synchronize()
-    rays_per_frame = ctx.stats()["rays"]
-    log(f"[bench] warmup done, rays/frame={rays_per_frame}")
+    def graph_step():
+        ctx.frame(ubo, instances, fxaa=use_fxaa, graph=True)
 
     ev = HipEvents()
     stream = rgrt.lib().rgrt_stream(ctx.h)
-    e0, e1 = ev.create(), ev.create()
-    ctx.synchronize()
-    ev.record(e0, stream)
-    for _ in range(args.steps):
-        step()
-    ev.record(e1, stream)
-    ms_total = ev.elapsed_ms(e0, e1)
-    ctx.synchronize()
-    ms_per_step = ms_total / args.steps
+
+    def timed(step, steps, warmup):
+        for _ in range(warmup):
+            step()
+        ctx.synchronize()
+        e0, e1 = ev.create(), ev.create()
+        ev.record(e0, stream)
+        for _ in range(steps):
+            step()
+        ev.record(e1, stream)
+        ms = ev.elapsed_ms(e0, e1) / steps
+        ctx.synchronize()
+        return ms
+
+    steps_of = {"eager": eager_step, "graph": graph_step}
+    other = "graph" if args.mode == "eager" else "eager"
+    ms_other = timed(steps_of[other], max(5, args.steps // 2), 2)
+    ms_per_step = timed(steps_of[args.mode], args.steps, args.warmup)
+    frame_ms = {args.mode: round(ms_per_step, 4), other: round(ms_other, 4)}
+    log(f"[bench] frame ms by submission mode: {frame_ms}")
     st = ctx.stats()
     rays = st["rays"]
     value = rays / (ms_per_step * 1e-3) / 1e6
@@ -296,7 +306,8 @@ def main():
                    "width": W, "height": H, "seed": args.seed,
                    "stage_ms": {k: round(ts[k], 4) for k in ("asBuildMs", "rtOnlyMs", "roughMs", "postprocMs", "rtTotalMs",
                                                              "traceKernelMs", "shadeKernelMs", "drainKernelMs")},
-                   "tuning": ctx.tuning(), "wavefront_pass_rays": passes.tolist()},
+                   "tuning": ctx.tuning(), "wavefront_pass_rays": passes.tolist(),
+                   "submission": args.mode, "frame_ms_by_submission": frame_ms},
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -254,6 +254,26 @@ rgrt_status rgrt_trace_rays_device(rgrt_ctx* ctx, const rgrt_ray* rays, uint32_t
  * postprocess, and FXAA followed by swap(base, final) when useFxaa != 0.
  * Asynchronous on the context stream. */
 rgrt_status rgrt_render(rgrt_ctx* ctx, int32_t useFxaa);
+/* rgrt_frame flags */
+#define RGRT_FRAME_FXAA 0x1u       /* FXAA + swap(base, final), as doRaytracing with m_useFXAA */
+#define RGRT_FRAME_REFIT_TLAS 0x2u /* refit the TLAS (rgrt_refit_tlas) instead of building it */
+#define RGRT_FRAME_GRAPH 0x4u      /* the whole frame as one hipGraph launch                   */
+
+/* One frame of RenderSystem::render (raygun/render/render_system.cpp:88-100):
+ * updateUniformBuffer (rgrt_set_frame), setupTopLevelAS (rgrt_build_tlas, or
+ * rgrt_refit_tlas with RGRT_FRAME_REFIT_TLAS) and doRaytracing (rgrt_render).
+ * With RGRT_FRAME_GRAPH the instance upload, the TLAS build, the wavefront
+ * passes, the drain and the post passes are captured into one hipGraph and
+ * launched at once. The graph is captured again every frame and the
+ * executable graph updated in place (hipGraphExecUpdate), so frame
+ * constants, instances and the render size may change between frames. In
+ * graph form the pass count is fixed (min(wavefrontPasses, recursion bound))
+ * and the drain always runs; the images equal the eager form's. The staging
+ * of a frame's instances waits until the previous frame's upload has run
+ * (one frame in flight, like the reference's per-frame fence).
+ * Asynchronous on the context stream. */
+rgrt_status rgrt_frame(rgrt_ctx* ctx, const rgrt_ubo* ubo, const rgrt_instance* instances, uint32_t numInstances,
+                       uint32_t flags);
 /* The traceRaysKHR part alone (writes base, normal, rough). */
 rgrt_status rgrt_trace(rgrt_ctx* ctx);
 /* The compute post part alone (reads base, normal, rough). */
@@ -281,6 +301,22 @@ typedef enum rgrt_image {
 rgrt_status rgrt_read_image(rgrt_ctx* ctx, rgrt_image which, void* dst, uint64_t bytes);
 /* Device pointer of an image (valid until the next resize / render swap). */
 rgrt_status rgrt_image_device_ptr(rgrt_ctx* ctx, rgrt_image which, void** dptr, uint64_t* bytes);
+/* 8-bit display formats for rgrt_read_image_rgba8 */
+typedef enum rgrt_format {
+    RGRT_FORMAT_RGBA8_UNORM = 0,
+    RGRT_FORMAT_BGRA8_UNORM = 1,
+    RGRT_FORMAT_RGBA8_SRGB = 2,
+    RGRT_FORMAT_BGRA8_SRGB = 3
+} rgrt_format;
+/* Replaces the blit of the result image into the swapchain image
+ * (raygun/render/render_system.cpp:102-147: nearest filter, same size, so a
+ * per-pixel format conversion) followed by a readback: an rgba16f image
+ * converted on the device to 4 bytes per pixel in the given format (clamp
+ * to [0, 1], NaN to 0, sRGB encoding of RGB for the _SRGB formats, round to
+ * nearest) and copied to host memory (synchronous). `bytes` must equal
+ * width * height * 4. The reference's surface format is the driver's first
+ * (raygun/vulkan_context.cpp:180), so which one it was is not fixed. */
+rgrt_status rgrt_read_image_rgba8(rgrt_ctx* ctx, rgrt_image which, rgrt_format format, void* dst, uint64_t bytes);
 /* Uploads base/normal/rough images (rgba16f) so the post passes can run on
  * given inputs. */
 rgrt_status rgrt_write_image(rgrt_ctx* ctx, rgrt_image which, const void* src, uint64_t bytes);
@@ -305,6 +341,8 @@ typedef struct rgrt_stats {
     float shadeKernelMs;  /* summed duration of the shading launches       */
     float drainKernelMs;  /* duration of the drain launch (0 if none)      */
     uint32_t drained;     /* 1 if the frame's tail ran in the drain kernel */
+    float frameMs;        /* TLAS + trace + post of the last rgrt_frame (or of
+                             build_tlas + render with RGRT_CFG_TIMING)       */
 } rgrt_stats;
 
 /* Launch-shape parameters (see rgrt_set_tuning). */

@@ -79,8 +79,12 @@ struct rgrt_ctx {
     uint32_t numInstances = 0;
     bool haveTlas = false;
     std::vector<uint32_t> tlasBlas; // BLAS per instance position of the last build
-    rgrt_instance* hInstStage = nullptr; // pinned
-    hipEvent_t stageDone = nullptr;
+    // Two pinned staging buffers used in turn, each free again once the
+    // upload from it has run (stageDone): the host prepares frame N + 1
+    // while the GPU still works on frame N.
+    rgrt_instance* hInstStage[2] = {nullptr, nullptr};
+    hipEvent_t stageDone[2] = {nullptr, nullptr};
+    int stageIdx = 0;
     rgrt_instance* dInstIn = nullptr;
     Node* tlasNodes = nullptr;
     Node4* tlasWide = nullptr;
@@ -144,6 +148,14 @@ struct rgrt_ctx {
     uint32_t lastTraceLaunches = 0;
     uint32_t lastWaves = 0;
     bool timedAs = false, timedRt = false, timedPost = false;
+
+    // rgrt_frame in graph form: the executable graph, updated every frame
+    hipGraphExec_t frameExec = nullptr;
+
+    uint8_t* dRgba8 = nullptr; // display-format readback
+    uint64_t rgba8Cap = 0;
+    hipEvent_t evFrameStart = nullptr, evFrameEnd = nullptr;
+    bool timedFrame = false;
 
     TraceScene scene() const
     {
@@ -285,12 +297,14 @@ void ensure_inst_cap(rgrt_ctx* c, uint32_t n)
 {
     if(n <= c->instCap) return;
     const uint32_t cap = std::max<uint32_t>(n, 64);
-    if(c->hInstStage) {
-        check(hipEventSynchronize(c->stageDone));
-        (void)hipHostFree(c->hInstStage);
-        c->hInstStage = nullptr;
+    for(int k = 0; k < 2; ++k) {
+        if(c->hInstStage[k]) {
+            check(hipEventSynchronize(c->stageDone[k]));
+            (void)hipHostFree(c->hInstStage[k]);
+            c->hInstStage[k] = nullptr;
+        }
+        check(hipHostMalloc((void**)&c->hInstStage[k], cap * sizeof(rgrt_instance), hipHostMallocDefault));
     }
-    check(hipHostMalloc((void**)&c->hInstStage, cap * sizeof(rgrt_instance), hipHostMallocDefault));
     dev_alloc(c->dInstIn, cap);
     dev_alloc(c->tlasNodes, cap);
     dev_alloc(c->tlasWide, cap);
@@ -329,14 +343,10 @@ void make_event(hipEvent_t& e)
     if(!e) check(hipEventCreate(&e));
 }
 
-rgrt_status do_trace(rgrt_ctx* c)
-{
-    if(!c->haveTlas) return fail(c, RGRT_ERR_NOT_READY, "rgrt_trace: no TLAS built");
-    if(!c->haveUbo) return fail(c, RGRT_ERR_NOT_READY, "rgrt_trace: no frame constants (rgrt_set_frame)");
-    if(!c->w || !c->h) return fail(c, RGRT_ERR_NOT_READY, "rgrt_trace: zero-sized render target");
-    hipStream_t s = c->stream;
-    const bool timing = (c->flags & RGRT_CFG_TIMING) != 0;
+static_assert(sizeof(rgrt_stats) == 80, "rgrt_stats layout (Python binding: records.STATS_FIELDS)");
 
+ShadeParams shade_params(const rgrt_ctx* c)
+{
     ShadeParams p;
     p.vb = c->vb;
     p.ib = c->ib;
@@ -353,6 +363,17 @@ rgrt_status do_trace(rgrt_ctx* c)
     p.img = c->img;
     p.keepF32 = (c->flags & RGRT_CFG_FP32_RADIANCE) ? 1 : 0;
     p.keepAov = (c->flags & RGRT_CFG_PRIMARY_AOV) ? 1 : 0;
+    return p;
+}
+
+rgrt_status do_trace(rgrt_ctx* c)
+{
+    if(!c->haveTlas) return fail(c, RGRT_ERR_NOT_READY, "rgrt_trace: no TLAS built");
+    if(!c->haveUbo) return fail(c, RGRT_ERR_NOT_READY, "rgrt_trace: no frame constants (rgrt_set_frame)");
+    if(!c->w || !c->h) return fail(c, RGRT_ERR_NOT_READY, "rgrt_trace: zero-sized render target");
+    hipStream_t s = c->stream;
+    const bool timing = (c->flags & RGRT_CFG_TIMING) != 0;
+    const ShadeParams p = shade_params(c);
 
     const uint32_t perSample = max_sequential_rays(c->ubo.maxRecursions);
     const uint32_t maxIters = perSample * (uint32_t)c->ubo.numSamples + 2;
@@ -434,6 +455,31 @@ rgrt_status do_trace(rgrt_ctx* c)
     return RGRT_OK;
 }
 
+// The trace of one frame as a fixed launch sequence (no host waits or event
+// records, so it can be captured into a graph): raygen, min(wavefrontPasses,
+// recursion bound) trace + shade passes, then the drain, which finishes every
+// pixel still in flight (with an empty queue it returns at once).
+void enqueue_trace_fixed(rgrt_ctx* c)
+{
+    hipStream_t s = c->stream;
+    const ShadeParams p = shade_params(c);
+    const uint32_t perSample = max_sequential_rays(c->ubo.maxRecursions);
+    const uint32_t maxIters = perSample * (uint32_t)c->ubo.numSamples + 2;
+    const uint32_t passes = std::min<uint32_t>(c->tuning.wavefrontPasses, maxIters);
+    const TraceScene sc = c->scene();
+    launch_raygen(s, p);
+    int cur = 0;
+    for(uint32_t it = 0; it < passes; ++it) {
+        launch_trace_queue_impl(s, sc, c->st, cur, c->traceGrid, c->spill, c->hostCountsDev + it);
+        launch_shade(s, p, cur, c->shadeGrid);
+        cur ^= 1;
+    }
+    launch_drain(s, p, sc, cur, c->drainGrid, c->spill);
+    c->lastTraceLaunches = passes;
+    c->lastDrained = true;
+    c->lastWaves = passes;
+}
+
 rgrt_status do_post(rgrt_ctx* c, int32_t useFxaa)
 {
     if(!c->haveUbo) return fail(c, RGRT_ERR_NOT_READY, "rgrt_postprocess: no frame constants");
@@ -458,7 +504,7 @@ rgrt_status do_post(rgrt_ctx* c, int32_t useFxaa)
 
 // Validates the instances, starts the timing event and copies them to the
 // device through the pinned staging buffer.
-rgrt_status stage_instances(rgrt_ctx* c, const rgrt_instance* in, uint32_t n, const char* who)
+rgrt_status validate_instances(rgrt_ctx* c, const rgrt_instance* in, uint32_t n, bool refit, const char* who)
 {
     if(n && !in) return fail(c, RGRT_ERR_INVALID_ARG, std::string(who) + ": null instances");
     for(uint32_t i = 0; i < n; ++i) {
@@ -467,18 +513,59 @@ rgrt_status stage_instances(rgrt_ctx* c, const rgrt_instance* in, uint32_t n, co
         if(in[i].customIndex >= n)
             return fail(c, RGRT_ERR_INVALID_ARG, std::string(who) + ": customIndex must index the offset table");
     }
+    if(refit) {
+        if(!c->haveTlas) return fail(c, RGRT_ERR_NOT_READY, std::string(who) + ": refit without a TLAS built");
+        if(n != c->numInstances)
+            return fail(c, RGRT_ERR_INVALID_ARG, std::string(who) + ": refit: instance count differs from the build");
+        for(uint32_t i = 0; i < n; ++i)
+            if(in[i].blas != c->tlasBlas[i])
+                return fail(c, RGRT_ERR_INVALID_ARG, std::string(who) + ": refit: an instance's BLAS differs from the build");
+    }
+    return RGRT_OK;
+}
+
+// Copies the instances into the pinned staging buffer once the previous
+// upload from it has completed.
+void stage_to_host(rgrt_ctx* c, const rgrt_instance* in, uint32_t n)
+{
+    ensure_inst_cap(c, std::max<uint32_t>(n, 1));
+    c->stageIdx ^= 1;
+    check(hipEventSynchronize(c->stageDone[c->stageIdx])); // the upload two frames back has left this buffer
+    if(n) std::memcpy(c->hInstStage[c->stageIdx], in, n * sizeof(rgrt_instance));
+}
+
+rgrt_status stage_instances(rgrt_ctx* c, const rgrt_instance* in, uint32_t n, bool refit, const char* who)
+{
+    const rgrt_status st = validate_instances(c, in, n, refit, who);
+    if(st != RGRT_OK) return st;
     hipStream_t s = c->stream;
     if(c->flags & RGRT_CFG_TIMING) {
         make_event(c->evAsStart);
         make_event(c->evAsEnd);
         check(hipEventRecord(c->evAsStart, s));
     }
-    ensure_inst_cap(c, std::max<uint32_t>(n, 1));
-    check(hipEventSynchronize(c->stageDone)); // previous frame's upload has left the staging buffer
-    if(n) std::memcpy(c->hInstStage, in, n * sizeof(rgrt_instance));
-    if(n) check(hipMemcpyAsync(c->dInstIn, c->hInstStage, n * sizeof(rgrt_instance), hipMemcpyHostToDevice, s));
-    check(hipEventRecord(c->stageDone, s));
+    stage_to_host(c, in, n);
+    if(n)
+        check(hipMemcpyAsync(c->dInstIn, c->hInstStage[c->stageIdx], n * sizeof(rgrt_instance), hipMemcpyHostToDevice, s));
+    check(hipEventRecord(c->stageDone[c->stageIdx], s));
     return RGRT_OK;
+}
+
+// The device work of a TLAS build or refit over the instances already in
+// dInstIn (one workgroup up to 1024 instances, the multi-kernel LBVH above);
+// the builder's scratch must be reserved for n beforehand.
+void enqueue_tlas(rgrt_ctx* c, uint32_t n, bool refit)
+{
+    hipStream_t s = c->stream;
+    if(launch_tlas_fused(s, c->dInstIn, n, c->dBlasRoot, c->dW2O, c->dShade, c->tlasNodes, c->tlasLeaf, c->tlasRoot,
+                         c->tlasWide, c->tlasWideq, c->tlasNumWide, c->dInst, refit))
+        return;
+    launch_inst_prep(s, c->dInstIn, n, c->dBlasRoot, c->builder.boxLo, c->builder.boxHi, c->dW2O, c->dShade);
+    if(refit) c->builder.refit(s, n, c->tlasNodes, c->tlasLeaf, c->tlasRoot);
+    else c->builder.build(s, n, c->tlasNodes, c->tlasLeaf, c->tlasRoot);
+    c->builder.collapse(s, n, c->tlasNodes, c->tlasWide, c->tlasNumWide);
+    c->builder.quantize(s, std::max<uint32_t>(n, 2) - 1, c->tlasWide, c->tlasNumWide, c->tlasWideq);
+    launch_inst_gather(s, c->dW2O, c->dInstIn, c->tlasLeaf, n, c->dInst);
 }
 
 void finish_tlas(rgrt_ctx* c, const rgrt_instance* in, uint32_t n)
@@ -514,7 +601,7 @@ rgrt_status rgrt_create(rgrt_ctx** out, const rgrt_config* cfg)
         check(hipGetDeviceProperties(&prop, c->device));
         c->numCUs = (uint32_t)std::max(1, prop.multiProcessorCount);
         c->apply_tuning();
-        check(hipEventCreateWithFlags(&c->stageDone, hipEventDisableTiming));
+        for(int k = 0; k < 2; ++k) check(hipEventCreateWithFlags(&c->stageDone[k], hipEventDisableTiming));
         if(cfg->width && cfg->height) alloc_images(c, cfg->width, cfg->height);
         return RGRT_OK;
     });
@@ -557,15 +644,19 @@ void rgrt_destroy(rgrt_ctx* c)
     dev_free(c->tlasRoot);
     dev_free(c->spill);
     dev_free(c->dRays);
+    dev_free(c->dRgba8);
     dev_free(c->dHits);
     c->builder.release();
-    if(c->hInstStage) (void)hipHostFree(c->hInstStage);
+    for(int k = 0; k < 2; ++k)
+        if(c->hInstStage[k]) (void)hipHostFree(c->hInstStage[k]);
     if(c->hostCounts) (void)hipHostFree(c->hostCounts);
     for(hipEvent_t e: c->iterEvents) (void)hipEventDestroy(e);
     for(hipEvent_t e: c->traceEvents) (void)hipEventDestroy(e);
     for(hipEvent_t e: c->shadeEvents) (void)hipEventDestroy(e);
-    for(hipEvent_t e: {c->evAsStart, c->evAsEnd, c->evRtStart, c->evRtOnlyEnd, c->evRoughEnd, c->evPostEnd, c->stageDone,
-                       c->evDrainStart, c->evDrainEnd})
+    if(c->frameExec) (void)hipGraphExecDestroy(c->frameExec);
+    for(hipEvent_t e: {c->evAsStart, c->evAsEnd, c->evRtStart, c->evRtOnlyEnd, c->evRoughEnd, c->evPostEnd, c->stageDone[0],
+                       c->stageDone[1],
+                       c->evDrainStart, c->evDrainEnd, c->evFrameStart, c->evFrameEnd})
         if(e) (void)hipEventDestroy(e);
     if(c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -764,18 +855,10 @@ rgrt_status rgrt_export_tlas_wide(rgrt_ctx* c, int32_t compressed, void* nodes, 
 rgrt_status rgrt_build_tlas(rgrt_ctx* c, const rgrt_instance* in, uint32_t n)
 {
     return guarded(c, [&]() {
-        const rgrt_status st = stage_instances(c, in, n, "build_tlas");
+        const rgrt_status st = stage_instances(c, in, n, false, "build_tlas");
         if(st != RGRT_OK) return st;
-        hipStream_t s = c->stream;
-        if(!launch_tlas_fused(s, c->dInstIn, n, c->dBlasRoot, c->dW2O, c->dShade, c->tlasNodes, c->tlasLeaf, c->tlasRoot,
-                              c->tlasWide, c->tlasWideq, c->tlasNumWide, c->dInst, false)) {
-            c->builder.reserve(std::max<uint32_t>(n, 1));
-            launch_inst_prep(s, c->dInstIn, n, c->dBlasRoot, c->builder.boxLo, c->builder.boxHi, c->dW2O, c->dShade);
-            c->builder.build(s, n, c->tlasNodes, c->tlasLeaf, c->tlasRoot);
-            c->builder.collapse(s, n, c->tlasNodes, c->tlasWide, c->tlasNumWide);
-            c->builder.quantize(s, std::max<uint32_t>(n, 2) - 1, c->tlasWide, c->tlasNumWide, c->tlasWideq);
-            launch_inst_gather(s, c->dW2O, c->dInstIn, c->tlasLeaf, n, c->dInst);
-        }
+        c->builder.reserve(std::max<uint32_t>(n, 1));
+        enqueue_tlas(c, n, false);
         finish_tlas(c, in, n);
         return RGRT_OK;
     });
@@ -784,23 +867,10 @@ rgrt_status rgrt_build_tlas(rgrt_ctx* c, const rgrt_instance* in, uint32_t n)
 rgrt_status rgrt_refit_tlas(rgrt_ctx* c, const rgrt_instance* in, uint32_t n)
 {
     return guarded(c, [&]() {
-        if(!c->haveTlas) return fail(c, RGRT_ERR_NOT_READY, "refit_tlas: no TLAS built");
-        if(n != c->numInstances) return fail(c, RGRT_ERR_INVALID_ARG, "refit_tlas: instance count differs from the build");
-        for(uint32_t i = 0; i < n; ++i)
-            if(in && in[i].blas != c->tlasBlas[i])
-                return fail(c, RGRT_ERR_INVALID_ARG, "refit_tlas: an instance's BLAS differs from the build");
-        const rgrt_status st = stage_instances(c, in, n, "refit_tlas");
+        const rgrt_status st = stage_instances(c, in, n, true, "refit_tlas");
         if(st != RGRT_OK) return st;
-        hipStream_t s = c->stream;
-        if(!launch_tlas_fused(s, c->dInstIn, n, c->dBlasRoot, c->dW2O, c->dShade, c->tlasNodes, c->tlasLeaf, c->tlasRoot,
-                              c->tlasWide, c->tlasWideq, c->tlasNumWide, c->dInst, true)) {
-            c->builder.reserve(std::max<uint32_t>(n, 1));
-            launch_inst_prep(s, c->dInstIn, n, c->dBlasRoot, c->builder.boxLo, c->builder.boxHi, c->dW2O, c->dShade);
-            c->builder.refit(s, n, c->tlasNodes, c->tlasLeaf, c->tlasRoot);
-            c->builder.collapse(s, n, c->tlasNodes, c->tlasWide, c->tlasNumWide);
-            c->builder.quantize(s, std::max<uint32_t>(n, 2) - 1, c->tlasWide, c->tlasNumWide, c->tlasWideq);
-            launch_inst_gather(s, c->dW2O, c->dInstIn, c->tlasLeaf, n, c->dInst);
-        }
+        c->builder.reserve(std::max<uint32_t>(n, 1));
+        enqueue_tlas(c, n, true);
         finish_tlas(c, in, n);
         return RGRT_OK;
     });
@@ -895,11 +965,95 @@ rgrt_status rgrt_postprocess(rgrt_ctx* c, int32_t useFxaa)
 rgrt_status rgrt_render(rgrt_ctx* c, int32_t useFxaa)
 {
     return guarded(c, [&]() {
+        c->timedFrame = false;
         c->timedRt = false;
         c->timedPost = false;
         rgrt_status st = do_trace(c);
         if(st != RGRT_OK) return st;
         return do_post(c, useFxaa);
+    });
+}
+
+rgrt_status rgrt_frame(rgrt_ctx* c, const rgrt_ubo* ubo, const rgrt_instance* in, uint32_t n, uint32_t flags)
+{
+    return guarded(c, [&]() {
+        if(flags & ~(RGRT_FRAME_FXAA | RGRT_FRAME_REFIT_TLAS | RGRT_FRAME_GRAPH))
+            return fail(c, RGRT_ERR_INVALID_ARG, "frame: unknown flags");
+        const bool fxaa = (flags & RGRT_FRAME_FXAA) != 0;
+        const bool refit = (flags & RGRT_FRAME_REFIT_TLAS) != 0;
+        rgrt_status st = rgrt_set_frame(c, ubo);
+        if(st != RGRT_OK) return st;
+        if(!(flags & RGRT_FRAME_GRAPH)) {
+            // eager: the three calls of RenderSystem::render
+            st = refit ? rgrt_refit_tlas(c, in, n) : rgrt_build_tlas(c, in, n);
+            if(st != RGRT_OK) return st;
+            return rgrt_render(c, fxaa ? 1 : 0);
+        }
+        st = validate_instances(c, in, n, refit, "frame");
+        if(st != RGRT_OK) return st;
+        if(!c->w || !c->h) return fail(c, RGRT_ERR_NOT_READY, "frame: zero-sized render target");
+        hipStream_t s = c->stream;
+        // everything that allocates or waits happens before the capture
+        stage_to_host(c, in, n);
+        c->builder.reserve(std::max<uint32_t>(n, 1));
+        const uint32_t maxIters = max_sequential_rays(c->ubo.maxRecursions) * (uint32_t)c->ubo.numSamples + 2;
+        ensure_host_counts(c, maxIters + 2);
+        const bool timing = (c->flags & RGRT_CFG_TIMING) != 0;
+        if(timing) {
+            make_event(c->evFrameStart);
+            make_event(c->evFrameEnd);
+        }
+        // the instance count and tables the captured kernels read
+        const uint32_t prevInstances = c->numInstances;
+        const bool prevHave = c->haveTlas;
+        c->numInstances = n;
+        c->haveTlas = true;
+        hipGraph_t g = nullptr;
+        check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        try {
+            if(n)
+                check(hipMemcpyAsync(c->dInstIn, c->hInstStage[c->stageIdx], n * sizeof(rgrt_instance), hipMemcpyHostToDevice,
+                                     s));
+            enqueue_tlas(c, n, refit);
+            enqueue_trace_fixed(c);
+            launch_post(s, c->img, c->ubo, fxaa ? 1 : 0, nullptr);
+        }
+        catch(...) {
+            (void)hipStreamEndCapture(s, &g);
+            if(g) (void)hipGraphDestroy(g);
+            c->numInstances = prevInstances;
+            c->haveTlas = prevHave;
+            throw;
+        }
+        check(hipStreamEndCapture(s, &g));
+        bool updated = false;
+        if(c->frameExec) {
+            hipGraphNode_t errNode = nullptr;
+            hipGraphExecUpdateResult res = hipGraphExecUpdateError;
+            updated = hipGraphExecUpdate(c->frameExec, g, &errNode, &res) == hipSuccess && res == hipGraphExecUpdateSuccess;
+            if(!updated) {
+                (void)hipGetLastError(); // a refused update leaves nothing pending
+                (void)hipGraphExecDestroy(c->frameExec);
+                c->frameExec = nullptr;
+            }
+        }
+        if(!updated) {
+            const hipError_t e = hipGraphInstantiate(&c->frameExec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            check(e);
+        }
+        else {
+            (void)hipGraphDestroy(g);
+        }
+        if(timing) check(hipEventRecord(c->evFrameStart, s));
+        check(hipGraphLaunch(c->frameExec, s));
+        if(timing) check(hipEventRecord(c->evFrameEnd, s));
+        check(hipEventRecord(c->stageDone[c->stageIdx], s)); // this staging buffer is free after this frame
+        if(fxaa) std::swap(c->img.base, c->img.final_); // raygun/render/raytracer.cpp:139
+        finish_tlas(c, in, n);
+        c->timedAs = c->timedRt = c->timedPost = false;
+        c->timedFrame = timing;
+        return RGRT_OK;
     });
 }
 
@@ -933,6 +1087,32 @@ rgrt_status rgrt_read_image(rgrt_ctx* c, rgrt_image which, void* dst, uint64_t b
         if(bytes != sz || !dst) return fail(c, RGRT_ERR_INVALID_ARG, "read_image: size mismatch");
         check(hipStreamSynchronize(c->stream));
         check(hipMemcpy(dst, p, sz, hipMemcpyDeviceToHost));
+        return traversal_errors(c);
+    });
+}
+
+rgrt_status rgrt_read_image_rgba8(rgrt_ctx* c, rgrt_image which, rgrt_format format, void* dst, uint64_t bytes)
+{
+    return guarded(c, [&]() {
+        if(which != RGRT_IMAGE_FINAL && which != RGRT_IMAGE_BASE && which != RGRT_IMAGE_NORMAL && which != RGRT_IMAGE_ROUGH &&
+           which != RGRT_IMAGE_ROUGH_COLORS_A && which != RGRT_IMAGE_ROUGH_COLORS_B)
+            return fail(c, RGRT_ERR_INVALID_ARG, "read_image_rgba8: not an rgba16f image");
+        if((uint32_t)format > (uint32_t)RGRT_FORMAT_BGRA8_SRGB) return fail(c, RGRT_ERR_INVALID_ARG, "read_image_rgba8: format");
+        void* p = nullptr;
+        uint64_t sz = 0;
+        if(!image_info(c, which, &p, &sz)) return fail(c, RGRT_ERR_INVALID_ARG, "read_image_rgba8: image not available");
+        const uint64_t npix = (uint64_t)c->w * c->h;
+        if(bytes != npix * 4 || !dst) return fail(c, RGRT_ERR_INVALID_ARG, "read_image_rgba8: size mismatch");
+        if(c->rgba8Cap < npix) {
+            dev_free(c->dRgba8);
+            dev_alloc(c->dRgba8, npix * 4);
+            c->rgba8Cap = npix;
+        }
+        const int bgra = (format == RGRT_FORMAT_BGRA8_UNORM || format == RGRT_FORMAT_BGRA8_SRGB) ? 1 : 0;
+        const int srgb = (format == RGRT_FORMAT_RGBA8_SRGB || format == RGRT_FORMAT_BGRA8_SRGB) ? 1 : 0;
+        launch_to_rgba8(c->stream, static_cast<const uint16_t*>(p), c->dRgba8, npix, bgra, srgb);
+        check(hipStreamSynchronize(c->stream));
+        check(hipMemcpy(dst, c->dRgba8, npix * 4, hipMemcpyDeviceToHost));
         return traversal_errors(c);
     });
 }
@@ -994,7 +1174,9 @@ rgrt_status rgrt_get_stats(rgrt_ctx* c, rgrt_stats* out)
             out->roughMs = elapsed(c->evRtOnlyEnd, c->evRoughEnd);
             out->postprocMs = elapsed(c->evRtOnlyEnd, c->evPostEnd);
             out->rtTotalMs = elapsed(c->evRtStart, c->evPostEnd);
+            if(c->timedAs) out->frameMs = elapsed(c->evAsStart, c->evPostEnd);
         }
+        if(c->timedFrame) out->frameMs = elapsed(c->evFrameStart, c->evFrameEnd);
         return RGRT_OK;
     });
 }

@@ -356,7 +356,49 @@ __global__ __launch_bounds__(256) void k_fxaa(const uint16_t* __restrict__ src, 
     reinterpret_cast<ushort4*>(out)[(size_t)y * W + x] = o;
 }
 
+// ---------------------------------------------------------------------------
+// Readback in a display format (the swapchain blit of
+// raygun/render/render_system.cpp:118-147 converts the rgba16f result to the
+// surface format with nearest filtering, i.e. per pixel)
+// ---------------------------------------------------------------------------
+// Vulkan's float -> unorm8 conversion: clamp to [0, 1] (NaN to 0), then the
+// nearest of the 256 levels (x * 255 rounded half to even).
+__device__ __forceinline__ uint32_t unorm8(float x)
+{
+    x = fminf(fmaxf(x, 0.0f), 1.0f);
+    return (uint32_t)rintf(x * 255.0f);
+}
+
+// sRGB transfer function (IEC 61966-2-1) of a linear value in [0, 1].
+__device__ __forceinline__ float srgb_encode(float l)
+{
+    l = fminf(fmaxf(l, 0.0f), 1.0f);
+    return l <= 0.0031308f ? 12.92f * l : 1.055f * powf(l, 1.0f / 2.4f) - 0.055f;
+}
+
+__global__ void k_to_rgba8(const uint16_t* __restrict__ src, uint32_t* __restrict__ dst, size_t npix, int bgra, int srgb)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if(i >= npix) return;
+    const ushort4 v = reinterpret_cast<const ushort4*>(src)[i];
+    float r = h2f(v.x), g = h2f(v.y), b = h2f(v.z);
+    const float a = h2f(v.w); // alpha is never sRGB-encoded
+    if(srgb) {
+        r = srgb_encode(r);
+        g = srgb_encode(g);
+        b = srgb_encode(b);
+    }
+    const uint32_t c0 = unorm8(bgra ? b : r), c2 = unorm8(bgra ? r : b);
+    dst[i] = c0 | (unorm8(g) << 8) | (c2 << 16) | (unorm8(a) << 24);
+}
+
 } // namespace
+
+void launch_to_rgba8(hipStream_t s, const uint16_t* src, uint8_t* dst, size_t npix, int bgra, int srgb)
+{
+    if(npix)
+        k_to_rgba8<<<(unsigned)((npix + 255) / 256), 256, 0, s>>>(src, reinterpret_cast<uint32_t*>(dst), npix, bgra, srgb);
+}
 
 // Post passes: rough + composite into `final_`, optional showAlpha, and
 // FXAA from `final_` into `base`. `roughEnd` (may be null) is recorded

@@ -130,5 +130,7 @@ uint32_t drain_spill_words_per_thread();
 uint32_t drain_block_size();
 void launch_trace_rays(hipStream_t s, const TraceScene& sc, const rgrt_ray* rays, uint32_t n, uint32_t cullMask, rgrt_hit* hits);
 void launch_post(hipStream_t s, const Images& img, const rgrt_ubo& ubo, int useFxaa, hipEvent_t roughEnd);
+// rgba16f -> 8-bit RGBA / BGRA (optionally sRGB-encoded), 4 bytes per pixel
+void launch_to_rgba8(hipStream_t s, const uint16_t* src, uint8_t* dst, size_t npix, int bgra, int srgb);
 
 } // namespace rgrt

@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 
 namespace raygun {
@@ -466,11 +467,18 @@ TopLevelAS::TopLevelAS(rgrt_ctx* ctx, const Scene& scene) : m_instances(collectI
     ok(ctx, rgrt_build_tlas(ctx, m_instances.data(), (uint32_t)m_instances.size()), "TopLevelAS");
 }
 
+bool TopLevelAS::sameLayout(const std::vector<rgrt_instance>& a, const std::vector<rgrt_instance>& b)
+{
+    if(a.size() != b.size()) return false;
+    for(size_t i = 0; i < a.size(); ++i)
+        if(a[i].blas != b[i].blas) return false;
+    return true;
+}
+
 bool TopLevelAS::update(rgrt_ctx* ctx, const Scene& scene)
 {
     std::vector<rgrt_instance> next = collectInstances(scene);
-    bool same = next.size() == m_instances.size();
-    for(size_t i = 0; same && i < next.size(); ++i) same = next[i].blas == m_instances[i].blas;
+    const bool same = sameLayout(next, m_instances);
     m_instances = std::move(next);
     const uint32_t n = (uint32_t)m_instances.size();
     if(same) ok(ctx, rgrt_refit_tlas(ctx, m_instances.data(), n), "TopLevelAS::update");
@@ -500,6 +508,17 @@ void Raytracer::updateRenderTarget(const rgrt_ubo& uniformBuffer) { ok(m_ctx, rg
 rgrt_image Raytracer::doRaytracing()
 {
     ok(m_ctx, rgrt_render(m_ctx, useFXAA ? 1 : 0), "doRaytracing");
+    return RGRT_IMAGE_FINAL;
+}
+
+rgrt_image Raytracer::doFrame(const Scene& scene, const rgrt_ubo& uniformBuffer, bool graph)
+{
+    std::vector<rgrt_instance> inst = TopLevelAS::collectInstances(scene);
+    const bool refit = refitTopLevelAS && m_topLevelAS && TopLevelAS::sameLayout(inst, m_topLevelAS->instances());
+    const uint32_t flags =
+        (useFXAA ? RGRT_FRAME_FXAA : 0u) | (refit ? RGRT_FRAME_REFIT_TLAS : 0u) | (graph ? RGRT_FRAME_GRAPH : 0u);
+    ok(m_ctx, rgrt_frame(m_ctx, &uniformBuffer, inst.data(), (uint32_t)inst.size(), flags), "doFrame");
+    m_topLevelAS = std::make_unique<TopLevelAS>(std::move(inst));
     return RGRT_IMAGE_FINAL;
 }
 
@@ -585,7 +604,16 @@ rgrt_ubo RenderSystem::updateUniformBuffer(const Camera& camera, int32_t maxRecu
 {
     const rgrt_ubo u = cameraUbo(camera, m_width, m_height, lightDir, numSamples, maxRecursions);
     m_raytracer->updateRenderTarget(u);
+    m_maxRecursions = maxRecursions;
+    m_numSamples = numSamples;
+    m_lightDir = lightDir;
     return u;
+}
+
+rgrt_image RenderSystem::renderFrame(const Scene& scene, bool graph)
+{
+    const rgrt_ubo u = cameraUbo(*scene.camera, m_width, m_height, m_lightDir, m_numSamples, m_maxRecursions);
+    return m_raytracer->doFrame(scene, u, graph);
 }
 
 rgrt_image RenderSystem::render(const Scene& scene)
@@ -599,6 +627,29 @@ std::vector<uint16_t> RenderSystem::readImage(rgrt_image which) const
     std::vector<uint16_t> out((size_t)m_width * m_height * 4);
     ok(m_ctx, rgrt_read_image(m_ctx, which, out.data(), out.size() * 2), "readImage");
     return out;
+}
+
+std::vector<uint8_t> RenderSystem::readImageRGBA8(rgrt_image which, rgrt_format format) const
+{
+    std::vector<uint8_t> out((size_t)m_width * m_height * 4);
+    ok(m_ctx, rgrt_read_image_rgba8(m_ctx, which, format, out.data(), out.size()), "readImageRGBA8");
+    return out;
+}
+
+void writePPM(const std::string& path, const std::vector<uint8_t>& rgba, uint32_t width, uint32_t height)
+{
+    if(rgba.size() != (size_t)width * height * 4) throw Error(RGRT_ERR_INVALID_ARG, "writePPM: size mismatch");
+    std::FILE* f = std::fopen(path.c_str(), "wb");
+    if(!f) throw Error(RGRT_ERR_INVALID_ARG, "writePPM: cannot open " + path);
+    std::fprintf(f, "P6\n%u %u\n255\n", width, height);
+    std::vector<uint8_t> row((size_t)width * 3);
+    bool good = true;
+    for(uint32_t y = 0; y < height && good; ++y) {
+        for(uint32_t x = 0; x < width; ++x)
+            for(int c = 0; c < 3; ++c) row[3 * x + c] = rgba[((size_t)y * width + x) * 4 + c];
+        good = std::fwrite(row.data(), 1, row.size(), f) == row.size();
+    }
+    if(std::fclose(f) != 0 || !good) throw Error(RGRT_ERR_INVALID_ARG, "writePPM: write failed for " + path);
 }
 
 } // namespace render

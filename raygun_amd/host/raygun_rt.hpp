@@ -233,6 +233,10 @@ struct BottomLevelAS {
 // transform's first three rows and its offset-table entry.
 struct TopLevelAS {
     TopLevelAS(rgrt_ctx* ctx, const Scene& scene);
+    // a TLAS the context built from these instances in another call (rgrt_frame)
+    explicit TopLevelAS(std::vector<rgrt_instance> built) : m_instances(std::move(built)) {}
+    // same instance count and BLAS per position: a refit suffices
+    static bool sameLayout(const std::vector<rgrt_instance>& a, const std::vector<rgrt_instance>& b);
     static std::vector<rgrt_instance> collectInstances(const Scene& scene);
     const std::vector<rgrt_instance>& instances() const { return m_instances; }
     // Per-frame update without a rebuild (no counterpart in the reference):
@@ -255,6 +259,9 @@ struct Raytracer {
     // trace + rough prepare + 10 x (H, V) blur + postprocess + FXAA + swap;
     // returns the image the reference's selectResultImage() would return
     rgrt_image doRaytracing();
+    // setupTopLevelAS + doRaytracing with the frame constants as one
+    // rgrt_frame call, launched as one hipGraph when graph is set
+    rgrt_image doFrame(const Scene& scene, const rgrt_ubo& uniformBuffer, bool graph);
 
     bool useFXAA = true; // m_useFXAA (raytracer.hpp:80)
     // The reference rebuilds the TLAS every frame; with refitTopLevelAS set,
@@ -282,7 +289,14 @@ struct RenderSystem {
                                  const vec3& lightDir = {0.4, -0.6, -0.8});
     // per frame: TLAS + doRaytracing (render_system.cpp:88-100)
     rgrt_image render(const Scene& scene);
+    // the whole frame of render_system.cpp:88-100 (frame constants from the
+    // scene camera with the settings of the last updateUniformBuffer, TLAS,
+    // trace, post) as one rgrt_frame call; graph: one hipGraph launch
+    rgrt_image renderFrame(const Scene& scene, bool graph = true);
     std::vector<uint16_t> readImage(rgrt_image which) const; // rgba16f bits
+    // the swapchain blit's conversion (render_system.cpp:102-147) + readback:
+    // 4 bytes per pixel in the given 8-bit format
+    std::vector<uint8_t> readImageRGBA8(rgrt_image which, rgrt_format format = RGRT_FORMAT_RGBA8_UNORM) const;
 
     Raytracer& raytracer() { return *m_raytracer; }
     rgrt_ctx* context() const { return m_ctx; }
@@ -293,11 +307,16 @@ struct RenderSystem {
     rgrt_ctx* m_ctx = nullptr;
     uint32_t m_width, m_height;
     std::unique_ptr<Raytracer> m_raytracer;
+    int32_t m_maxRecursions = 5, m_numSamples = 1;
+    vec3 m_lightDir = {0.4, -0.6, -0.8};
 };
 
 // packing without a context: fills the buffer refs, returns the buffers
 void packModels(const std::vector<std::shared_ptr<Model>>& models, std::vector<rgrt_vertex>& vertices,
                 std::vector<uint32_t>& indices, std::vector<rgrt_material>& materials);
+
+// Binary PPM (P6) of 4-byte RGBA pixels (alpha dropped).
+void writePPM(const std::string& path, const std::vector<uint8_t>& rgba, uint32_t width, uint32_t height);
 
 // UBO of a frame (render_system.cpp:235-268; matrices column-major)
 rgrt_ubo cameraUbo(const Camera& camera, uint32_t width, uint32_t height, const vec3& lightDir, int32_t numSamples,

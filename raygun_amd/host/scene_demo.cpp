@@ -8,6 +8,11 @@
 //   scene_demo render W H OUT          render one frame; write the final image
 //                                      (rgba16f) to OUT and the frame constants
 //                                      to OUT.ubo
+//   scene_demo frames W H N OUT.ppm   N animated frames through rgrt_frame
+//                                      (one hipGraph per frame, TLAS refit);
+//                                      the last one read back as 8-bit RGBA
+//                                      and written as PPM, the same frames
+//                                      traced eagerly compared byte for byte
 //   scene_demo animate FRAMES          run FRAMES frames of the per-frame update
 //                                      (a scale animation on cube2, the group
 //                                      turning, cube4 moving) and print the
@@ -230,6 +235,35 @@ int main(int argc, char** argv)
             std::printf("],\n \"finished\": %d}\n", finished);
             return 0;
         }
+        if(mode == "frames" && argc == 6) {
+            const uint32_t W = (uint32_t)std::atoi(argv[2]), H = (uint32_t)std::atoi(argv[3]);
+            const int frames = std::atoi(argv[4]);
+            Scene g = demoScene(true), e = demoScene(true);
+            render::RenderSystem rg(W, H), re(W, H);
+            rg.setupModelBuffers(g);
+            re.setupModelBuffers(e);
+            rg.raytracer().refitTopLevelAS = true;
+            rg.updateUniformBuffer(*g.camera, 3, 1); // the frame settings renderFrame uses
+            std::vector<uint8_t> last;
+            for(int f = 0; f < frames; ++f) {
+                for(Scene* sc: {&g, &e}) {
+                    sc->update(0.25);
+                    findEntity(*sc->root, "group")->rotate(vec3{0.0, 0.1, 0.0});
+                    findEntity(*sc->root, "cube4")->move({0.05, 0.0, 0.0});
+                }
+                rg.renderFrame(g, /*graph=*/true);
+                re.updateUniformBuffer(*e.camera, 3, 1);
+                re.render(e);
+                last = rg.readImageRGBA8(RGRT_IMAGE_FINAL);
+                if(last != re.readImageRGBA8(RGRT_IMAGE_FINAL)) {
+                    std::fprintf(stderr, "frame %d: graph and eager frames differ\n", f);
+                    return 1;
+                }
+            }
+            render::writePPM(argv[5], last, W, H);
+            std::printf("%d frames %ux%u, graph == eager\n", frames, W, H);
+            return 0;
+        }
         if(mode == "render" && argc == 5) {
             const uint32_t W = (uint32_t)std::atoi(argv[2]), H = (uint32_t)std::atoi(argv[3]);
             render::RenderSystem rs(W, H);
@@ -242,7 +276,7 @@ int main(int argc, char** argv)
             std::printf("rendered %ux%u, %zu instances\n", W, H, render::TopLevelAS::collectInstances(s).size());
             return 0;
         }
-        std::fprintf(stderr, "usage: scene_demo collect | render W H OUT | animate FRAMES\n");
+        std::fprintf(stderr, "usage: scene_demo collect | render W H OUT | animate FRAMES | frames W H N OUT.ppm\n");
         return 2;
     }
     catch(const Error& e) {

@@ -58,6 +58,16 @@ class Raytracer:
         self.ctx.render(self.use_fxaa)
         return rgrt.IMAGE_FINAL
 
+    def doFrame(self, scene, ubo, graph=True):
+        """setupTopLevelAS + doRaytracing with the frame constants as one
+        rgrt_frame call (one hipGraph launch when graph is set)."""
+        inst = scene.collect_instances()
+        blas = inst["blas"].copy()
+        refit = self.refit_tlas and self._last_blas is not None and np.array_equal(blas, self._last_blas)
+        self.ctx.frame(ubo, inst, fxaa=self.use_fxaa, refit=refit, graph=graph)
+        self._last_blas = blas
+        return rgrt.IMAGE_FINAL
+
 
 class RenderSystem:
     def __init__(self, width, height, flags=0, device=0):
@@ -83,7 +93,15 @@ class RenderSystem:
         ubo = camera_ubo(camera, self.width, self.height, light_dir=light_dir, num_samples=num_samples,
                          max_recursions=max_recursions, fade=fade)
         self.raytracer.updateRenderTarget(ubo)
+        self._frame_settings = dict(light_dir=light_dir, num_samples=num_samples, max_recursions=max_recursions, fade=fade)
         return ubo
+
+    def render_frame(self, scene, graph=True):
+        """The whole frame (frame constants from the scene camera with the
+        settings of the last updateUniformBuffer, TLAS, trace, post) as one
+        rgrt_frame call."""
+        ubo = camera_ubo(scene.camera, self.width, self.height, **getattr(self, "_frame_settings", {}))
+        return self.raytracer.doFrame(scene, ubo, graph)
 
     def render(self, scene, ubo=None):
         self.raytracer.setupTopLevelAS(scene)

@@ -48,7 +48,7 @@ STATS_FIELDS = [
     ("pad_", "<u4"),
     ("rays", "<u8"), ("skyLookups", "<u8"), ("zeroDirRays", "<u8"),
     ("waves", "<u4"), ("traceLaunches", "<u4"), ("traceKernelMs", "<f4"), ("shadeKernelMs", "<f4"),
-    ("drainKernelMs", "<f4"), ("drained", "<u4"),
+    ("drainKernelMs", "<f4"), ("drained", "<u4"), ("frameMs", "<f4"), ("pad2_", "<u4"),
 ]
 
 TUNING = np.dtype([("wavefrontPasses", "<u4"), ("traceBlocksPerCU", "<u4"), ("shadeBlocksPerCU", "<u4"),

@@ -17,6 +17,13 @@ LIB_PATH = os.environ.get("RGRT_LIBRARY") or os.path.join(HERE, "librgrt.so")
 CFG_FP32_RADIANCE = 0x1
 CFG_PRIMARY_AOV = 0x2
 CFG_TIMING = 0x4
+FRAME_FXAA = 0x1
+FRAME_REFIT_TLAS = 0x2
+FRAME_GRAPH = 0x4
+FORMAT_RGBA8_UNORM = 0
+FORMAT_BGRA8_UNORM = 1
+FORMAT_RGBA8_SRGB = 2
+FORMAT_BGRA8_SRGB = 3
 
 IMAGE_FINAL, IMAGE_BASE, IMAGE_NORMAL, IMAGE_ROUGH, IMAGE_ROUGH_TRANSITIONS, IMAGE_ROUGH_COLORS_A, IMAGE_ROUGH_COLORS_B, \
     IMAGE_BASE_F32, IMAGE_NORMAL_F32, IMAGE_ROUGH_F32, IMAGE_PRIMARY_HIT = range(11)
@@ -29,8 +36,8 @@ EXPORTS = [
     "rgrt_upload_geometry", "rgrt_upload_materials", "rgrt_update_vertices",
     "rgrt_build_blas", "rgrt_refit_blas", "rgrt_blas_info", "rgrt_export_blas",
     "rgrt_build_tlas", "rgrt_refit_tlas", "rgrt_export_tlas", "rgrt_set_frame", "rgrt_resize",
-    "rgrt_trace_rays", "rgrt_trace_rays_device", "rgrt_render", "rgrt_trace", "rgrt_postprocess",
-    "rgrt_read_image", "rgrt_image_device_ptr", "rgrt_write_image", "rgrt_get_stats", "rgrt_get_pass_counts",
+    "rgrt_trace_rays", "rgrt_trace_rays_device", "rgrt_render", "rgrt_frame", "rgrt_trace", "rgrt_postprocess",
+    "rgrt_read_image", "rgrt_read_image_rgba8", "rgrt_image_device_ptr", "rgrt_write_image", "rgrt_get_stats", "rgrt_get_pass_counts",
     "rgrt_set_tuning", "rgrt_get_tuning", "rgrt_export_blas_wide", "rgrt_export_tlas_wide", "rgrt_version",
 ]
 
@@ -51,7 +58,7 @@ class _BlasDesc(ctypes.Structure):
 
 
 STATS = np.dtype(STATS_FIELDS)
-assert STATS.itemsize == 72
+assert STATS.itemsize == 80
 
 _lib = None
 
@@ -85,9 +92,11 @@ def lib():
             "rgrt_trace_rays": (i32, [P, P, u32, u32, P]),
             "rgrt_trace_rays_device": (i32, [P, P, u32, u32, P]),
             "rgrt_render": (i32, [P, i32]),
+            "rgrt_frame": (i32, [P, P, P, u32, u32]),
             "rgrt_trace": (i32, [P]),
             "rgrt_postprocess": (i32, [P, i32]),
             "rgrt_read_image": (i32, [P, i32, P, u64]),
+            "rgrt_read_image_rgba8": (i32, [P, i32, i32, P, u64]),
             "rgrt_image_device_ptr": (i32, [P, i32, ctypes.POINTER(P), ctypes.POINTER(u64)]),
             "rgrt_write_image": (i32, [P, i32, P, u64]),
             "rgrt_get_stats": (i32, [P, P]),
@@ -238,6 +247,24 @@ class Context:
 
     def trace_rays_device(self, rays_ptr, n, cull_mask, hits_ptr):
         self._ok(self._L.rgrt_trace_rays_device(self.h, rays_ptr, n, cull_mask, hits_ptr))
+
+    def read_image_rgba8(self, which=None, fmt=None):
+        """An rgba16f image converted on the device to 8 bits per channel
+        (the swapchain blit's conversion): (height, width, 4) uint8."""
+        which = IMAGE_FINAL if which is None else which
+        fmt = FORMAT_RGBA8_UNORM if fmt is None else fmt
+        out = np.zeros((self.height, self.width, 4), np.uint8)
+        self._ok(self._L.rgrt_read_image_rgba8(self.h, which, fmt, _p(out), out.nbytes))
+        return out
+
+    def frame(self, ubo, instances, fxaa=True, refit=False, graph=False):
+        """One frame: frame constants, TLAS build (or refit) and render; with
+        graph=True captured and launched as one hipGraph."""
+        u = np.ascontiguousarray(ubo, UBO)
+        inst = np.ascontiguousarray(instances, INSTANCE)
+        flags = (FRAME_FXAA if fxaa else 0) | (FRAME_REFIT_TLAS if refit else 0) | (FRAME_GRAPH if graph else 0)
+        self._ok(self._L.rgrt_frame(self.h, _p(u), _p(inst), len(inst), flags))
+        self._ninst = len(inst)
 
     def render(self, use_fxaa=True):
         self._ok(self._L.rgrt_render(self.h, int(bool(use_fxaa))))
