@@ -254,6 +254,26 @@ rgrt_status rgrt_trace_rays_device(rgrt_ctx* ctx, const rgrt_ray* rays, uint32_t
  * postprocess, and FXAA followed by swap(base, final) when useFxaa != 0.
  * Asynchronous on the context stream. */
 rgrt_status rgrt_render(rgrt_ctx* ctx, int32_t useFxaa);
+/* Sampling of rgrt_trace. RGRT_SAMPLING_WHITTED is the reference's
+ * deterministic tracer (the SSAA offsets of raygen.h:37-67). With
+ * RGRT_SAMPLING_STOCHASTIC (no counterpart in the reference) every sample
+ * jitters its pixel position uniformly and every reflection ray of a material
+ * with roughness > 0 is spread around the mirror direction: dir =
+ * normalize(reflect + roughness * s), s uniform in [-1, 1]^3, kept only above
+ * the surface. The random numbers are integer hashes of (seed, pixel, sample,
+ * hit position), so a frame is reproducible. BASE then holds the mean of the
+ * per-sample composite mix(base, rough, reflectContribution) with alpha 0, so
+ * the post passes reduce to clamp + luma (+ FXAA): the glossy spread replaces
+ * the screen-space roughness blur. FEATURES is written for the denoiser. */
+typedef struct rgrt_sampling {
+    uint32_t mode; /* RGRT_SAMPLING_* */
+    uint32_t seed;
+    uint32_t pad[2];
+} rgrt_sampling;
+#define RGRT_SAMPLING_WHITTED 0u
+#define RGRT_SAMPLING_STOCHASTIC 1u
+rgrt_status rgrt_set_sampling(rgrt_ctx* ctx, const rgrt_sampling* sampling);
+
 /* rgrt_frame flags */
 #define RGRT_FRAME_FXAA 0x1u       /* FXAA + swap(base, final), as doRaytracing with m_useFXAA */
 #define RGRT_FRAME_REFIT_TLAS 0x2u /* refit the TLAS (rgrt_refit_tlas) instead of building it */
@@ -294,7 +314,11 @@ typedef enum rgrt_image {
     RGRT_IMAGE_BASE_F32 = 7,     /* rgba32f, RGRT_CFG_FP32_RADIANCE */
     RGRT_IMAGE_NORMAL_F32 = 8,   /* rgba32f, RGRT_CFG_FP32_RADIANCE */
     RGRT_IMAGE_ROUGH_F32 = 9,    /* rgba32f, RGRT_CFG_FP32_RADIANCE */
-    RGRT_IMAGE_PRIMARY_HIT = 10  /* rgrt_hit per pixel, RGRT_CFG_PRIMARY_AOV */
+    RGRT_IMAGE_PRIMARY_HIT = 10, /* rgrt_hit per pixel, RGRT_CFG_PRIMARY_AOV */
+    RGRT_IMAGE_FEATURES = 11     /* 2 x rgba16f per pixel, stochastic sampling:
+                                    (albedo.rgb, roughness) of sample 0's primary
+                                    hit, (mean primary t, variance of the mean
+                                    luminance, 0, 0); zero on a miss */
 } rgrt_image;
 
 /* Copies an image to host memory (synchronous); `bytes` must equal its size. */

@@ -53,6 +53,8 @@ def lib():
         L.orc_blas_morton.argtypes = [P, i32, P]
         L.orc_build_tlas.restype = i32
         L.orc_build_tlas.argtypes = [P, P, u32]
+        L.orc_render_sampled.restype = i32
+        L.orc_render_sampled.argtypes = [P, P, u32, u32, u32, u32, P, P, P, P, P, P, P, i32]
         L.orc_refit_tlas.restype = i32
         L.orc_refit_tlas.argtypes = [P, P, u32]
         L.orc_tlas_num_nodes.restype = u32
@@ -207,6 +209,24 @@ class OracleScene:
         if rc != 0:
             raise RuntimeError("oracle render failed (no TLAS?)")
         out["rays"], out["sky"], out["zero_dir"] = (int(x) for x in stats)
+        return out
+
+    def render_sampled(self, ubo, width, height, sampling=1, seed=0, threads=None):
+        """Render with the sampling mode (1: stochastic): base16 / normal16 /
+        rough16 / features16 (H x W x 8) as uint16 fp16 bits, base32, tie."""
+        W, H = width, height
+        out = {k: np.zeros((H, W, 4), np.uint16) for k in ("base16", "normal16", "rough16")}
+        out["base32"] = np.zeros((H, W, 4), np.float32)
+        out["features16"] = np.zeros((H, W, 8), np.uint16)
+        out["tie"] = np.zeros((H, W), np.uint8)
+        stats = np.zeros(1, np.uint64)
+        ubo = np.ascontiguousarray(ubo)
+        rc = lib().orc_render_sampled(self.h, _p(ubo), W, H, sampling, seed, _p(out["base16"]), _p(out["normal16"]),
+                                      _p(out["rough16"]), _p(out["base32"]), _p(out["features16"]), _p(out["tie"]),
+                                      _p(stats), threads or default_threads())
+        if rc != 0:
+            raise RuntimeError("oracle render failed (no TLAS?)")
+        out["rays"] = int(stats[0])
         return out
 
     def render_counts(self, ubo, width, height, rows, threads=None):

@@ -1102,10 +1102,38 @@ struct Payload { // payload.h:29-39
     int recDepth;
 };
 
+// Random numbers of the stochastic sampling mode (no counterpart in the
+// reference): the "lowbias32" finaliser of C. Wellons' hash-prospector
+// (published constants 0x7feb352d, 0x846ca68b, shifts 16/15/16), keyed by
+// (seed, image pixel, sample) and, for glossy reflections, the bits of the
+// hit position; [0, 1) floats from the top 24 bits.
+inline uint32_t hash32(uint32_t x)
+{
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+inline uint32_t sample_key(uint32_t seed, uint32_t pixel, uint32_t sample)
+{
+    return hash32(seed ^ hash32(pixel ^ hash32(sample ^ 0x9e3779b9u)));
+}
+inline float u01(uint32_t h) { return (float)(h >> 8) * (1.0f / 16777216.0f); }
+inline uint32_t fbits(float f)
+{
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    return u;
+}
+
 struct Ctx {
     const orc_scene* s;
     const UBO* ubo;
     V3 lightDir;
+    uint32_t sampling = 0, seed = 0, key = 0; // stochastic sampling mode
+    float primAlbedo[3] = {0, 0, 0}, primRough = 0; // sample 0's primary hit material
     uint64_t rays = 0, sky = 0, zeroDir = 0;
     uint64_t nodes = 0, tris = 0, hits = 0; // traversal work of the traced rays
     uint64_t instLeaves = 0, blasEntries = 0;
@@ -1201,6 +1229,16 @@ void trace_ray(Ctx& c, Payload& P, uint32_t cullMask, int missIndex, V3 O, float
     }
     if(primary && c.recordPrimary) {
         c.primary = hit ? HitOut{h.t, h.u, h.v, c.s->inst[h.inst].rec.customIndex, h.prim} : HitOut{0, 0, 0, NO_HIT, 0};
+        // denoiser features of the stochastic mode: the hit's material (vertex 0's matIndex)
+        c.primAlbedo[0] = c.primAlbedo[1] = c.primAlbedo[2] = c.primRough = 0.0f;
+        if(hit) {
+            const orc_scene& s = *c.s;
+            const Instance& table = s.inst[s.inst[h.inst].rec.customIndex].rec;
+            const Vertex& v0 = s.vb[table.vertexBufferOffset + s.ib[table.indexBufferOffset + 3 * h.prim]];
+            const Material& gm = s.mats[table.materialBufferOffset + v0.matIndex];
+            for(int k = 0; k < 3; ++k) c.primAlbedo[k] = gm.diffuse[k];
+            c.primRough = gm.roughness;
+        }
     }
     if(hit) {
         closest_hit(c, P, h, O, D);
@@ -1348,6 +1386,14 @@ void closest_hit(Ctx& c, Payload& P, const Hit& h, V3 O, V3 D) // closesthit.rch
     float reflectDepth = 0.0f;
     if(P.recDepth < ubo.maxRecursions && mat.reflectivity > 0.0f) {
         V3 dir = reflect(D, vnW);
+        if(c.sampling == 1 && mat.roughness > 0.0f) {
+            // glossy spread: normalize(r + roughness * s), s uniform in [-1, 1]^3, kept above the surface
+            const uint32_t hh = hash32(c.key ^ hash32(fbits(origin.x) ^ hash32(fbits(origin.y) ^ hash32(fbits(origin.z)))));
+            const V3 sp = v3(u01(hash32(hh ^ 1u)) * 2.0f - 1.0f, u01(hash32(hh ^ 2u)) * 2.0f - 1.0f,
+                             u01(hash32(hh ^ 3u)) * 2.0f - 1.0f);
+            const V3 g = normalize(dir + sp * mat.roughness);
+            if(dot(g, vnW) > 0.0f) dir = g;
+        }
         P.recDepth += (int)mat.rayConsumption;
         P.refDepth += hitT;
         trace_ray(c, P, 0xff, 0, origin, tmin, dir, tmax);
@@ -1415,8 +1461,13 @@ const float kAAOffsets[9][8][2] = { // raygen.h:37-67
      {-0.4375f, -0.0625f}, {0.1875f, 0.4375f}, {0.4375f, -0.4375f}},
 };
 
-// raygen.h:69-115: returns the three rows of the mat3x4 result.
-void raygen(Ctx& c, uint32_t px, uint32_t py, uint32_t W, uint32_t H, float out[3][4])
+// raygen.h:69-115: returns the three rows of the mat3x4 result. With the
+// stochastic sampling mode the pixel offsets are hashed instead of the SSAA
+// table, the colour summed is each sample's composite mix(hitValue,
+// roughValue.rgb, reflectContribution) with alpha 0, and feat receives the
+// denoiser features (sample 0's primary material, mean primary t, variance
+// of the mean luminance).
+void raygen(Ctx& c, uint32_t px, uint32_t py, uint32_t W, uint32_t H, float out[3][4], float feat[2][4] = nullptr)
 {
     const UBO& ubo = *c.ubo;
     const int numSamples = ubo.numSamples;
@@ -1430,10 +1481,17 @@ void raygen(Ctx& c, uint32_t px, uint32_t py, uint32_t W, uint32_t H, float out[
     V3 origin = v3(org4[0], org4[1], org4[2]);
     const float tmin = 0.001f, tmax = 10000.0f;
 
+    float lum2 = 0.0f;
     for(int i = 0; i < numSamples; ++i) {
         const float* off = kAAOffsets[std::min(numSamples, 8)][i % 8];
-        float pcx = ((float)px + 0.5f) + off[0];
-        float pcy = ((float)py + 0.5f) + off[1];
+        float offx = off[0], offy = off[1];
+        if(c.sampling == 1) {
+            c.key = sample_key(c.seed, py * W + px, (uint32_t)i);
+            offx = u01(hash32(c.key ^ 0x68bc21ebu)) - 0.5f;
+            offy = u01(hash32(c.key ^ 0x02e5be93u)) - 0.5f;
+        }
+        float pcx = ((float)px + 0.5f) + offx;
+        float pcy = ((float)py + 0.5f) + offy;
         float ux = pcx / (float)W, uy = pcy / (float)H;
         float dx = ux * 2.0f - 1.0f, dy = uy * 2.0f - 1.0f;
         const float t4[4] = {dx, dy, 1.0f, 1.0f};
@@ -1457,7 +1515,15 @@ void raygen(Ctx& c, uint32_t px, uint32_t py, uint32_t W, uint32_t H, float out[
         trace_ray(c, P, 0xff, 0, origin, tmin, direction, tmax, true);
         c.recordPrimary = false;
 
-        color = color + P.hitValue;
+        if(c.sampling == 1) {
+            const V3 col = mix(P.hitValue, v3(P.roughValue[0], P.roughValue[1], P.roughValue[2]), P.reflectContribution);
+            const float l = (0.299f * col.x + 0.587f * col.y) + 0.114f * col.z;
+            lum2 = lum2 + l * l;
+            color = color + col;
+        }
+        else {
+            color = color + P.hitValue;
+        }
         normal = normal + P.normal;
         for(int k = 0; k < 4; ++k) rough[k] += P.roughValue[k];
         reflectContrib += P.reflectContribution;
@@ -1467,12 +1533,20 @@ void raygen(Ctx& c, uint32_t px, uint32_t py, uint32_t W, uint32_t H, float out[
     out[0][0] = color.x / n;
     out[0][1] = color.y / n;
     out[0][2] = color.z / n;
-    out[0][3] = reflectContrib / n;
+    out[0][3] = c.sampling == 1 ? 0.0f : reflectContrib / n;
     out[1][0] = normal.x / n;
     out[1][1] = normal.y / n;
     out[1][2] = normal.z / n;
     out[1][3] = (std::log(depth) * 0.25f) / n;
     for(int k = 0; k < 4; ++k) out[2][k] = rough[k] / n;
+    if(feat) {
+        for(int k = 0; k < 3; ++k) feat[0][k] = c.primAlbedo[k];
+        feat[0][3] = c.primRough;
+        const float lm = (0.299f * out[0][0] + 0.587f * out[0][1]) + 0.114f * out[0][2];
+        feat[1][0] = depth / n;
+        feat[1][1] = std::fmax(lum2 / n - lm * lm, 0.0f) / n;
+        feat[1][2] = feat[1][3] = 0.0f;
+    }
 }
 
 // --------------------------------------------------------------------------
@@ -1978,6 +2052,50 @@ int32_t orc_render(orc_scene* s, const void* uboPtr, uint32_t W, uint32_t H, uin
             stats[1] += sky[t];
             stats[2] += zd[t];
         }
+    }
+    return 0;
+}
+
+// orc_render with the sampling mode (0 Whitted, 1 stochastic) and the
+// denoiser feature image (2 x rgba16f per pixel) of the stochastic mode.
+int32_t orc_render_sampled(orc_scene* s, const void* uboPtr, uint32_t W, uint32_t H, uint32_t sampling, uint32_t seed,
+                           uint16_t* base16, uint16_t* normal16, uint16_t* rough16, float* base32, uint16_t* features16,
+                           uint8_t* tie, uint64_t* stats, int32_t threads)
+{
+    if(!s->hasTlas) return -1;
+    UBO ubo;
+    std::memcpy(&ubo, uboPtr, sizeof(UBO));
+    const int nt = std::max(1, (int)threads);
+    std::vector<uint64_t> rays(nt, 0);
+    parallel_rows(0, (int)H, nt, [&](int y, int t) {
+        Ctx c;
+        c.s = s;
+        c.ubo = &ubo;
+        c.lightDir = v3(ubo.lightDir[0], ubo.lightDir[1], ubo.lightDir[2]);
+        c.sampling = sampling;
+        c.seed = seed;
+        for(uint32_t x = 0; x < W; ++x) {
+            c.tie = false;
+            float out[3][4], feat[2][4];
+            raygen(c, x, (uint32_t)y, W, H, out, feat);
+            size_t p = (size_t)y * W + x;
+            for(int k = 0; k < 4; ++k) {
+                if(base16) base16[4 * p + k] = f32_to_f16(out[0][k]);
+                if(normal16) normal16[4 * p + k] = f32_to_f16(out[1][k]);
+                if(rough16) rough16[4 * p + k] = f32_to_f16(out[2][k]);
+                if(base32) base32[4 * p + k] = out[0][k];
+                if(features16) {
+                    features16[8 * p + k] = f32_to_f16(feat[0][k]);
+                    features16[8 * p + 4 + k] = f32_to_f16(feat[1][k]);
+                }
+            }
+            if(tie) tie[p] = c.tie;
+        }
+        rays[t] += c.rays;
+    });
+    if(stats) {
+        stats[0] = 0;
+        for(int t = 0; t < nt; ++t) stats[0] += rays[t];
     }
     return 0;
 }

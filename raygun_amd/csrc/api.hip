@@ -108,6 +108,7 @@ struct rgrt_ctx {
     size_t spillBytes = 0;
     uint32_t traceGrid = 0, shadeGrid = 0, drainGrid = 0;
     rgrt_tuning tuning{6, 6, 8, 4, 2}; // measured on config 2 (1080p, 2M triangles)
+    rgrt_sampling sampling{RGRT_SAMPLING_WHITTED, 0u, {0u, 0u}};
     hipEvent_t evDrainStart = nullptr, evDrainEnd = nullptr;
     bool lastDrained = false;
 
@@ -222,7 +223,9 @@ void free_images(rgrt_ctx* c)
     dev_free(m.normal32);
     dev_free(m.rough32);
     dev_free(m.primary);
+    dev_free(m.features);
     WavefrontState& s = c->st;
+    dev_free(s.lum2);
     dev_free(s.payload);
     dev_free(s.accum);
     dev_free(s.meta);
@@ -254,6 +257,8 @@ void alloc_images(rgrt_ctx* c, uint32_t w, uint32_t h)
         dev_alloc(m.rough32, npix * 4);
     }
     if(c->flags & RGRT_CFG_PRIMARY_AOV) dev_alloc(m.primary, npix);
+    dev_alloc(m.features, npix * 8);
+    check(hipMemsetAsync(m.features, 0, npix * 16, c->stream));
     check(hipMemsetAsync(m.base, 0, npix * 8, c->stream));
     check(hipMemsetAsync(m.normal, 0, npix * 8, c->stream));
     check(hipMemsetAsync(m.rough, 0, npix * 8, c->stream));
@@ -263,6 +268,7 @@ void alloc_images(rgrt_ctx* c, uint32_t w, uint32_t h)
     dev_alloc(s.payload, npix * kPayloadWords);
     dev_alloc(s.accum, npix * kAccumWords);
     dev_alloc(s.meta, npix);
+    dev_alloc(s.lum2, npix);
     dev_alloc(s.frames, npix * kFrameStride * kMaxFrames);
     dev_alloc(s.rays[0], npix * kRayWords);
     dev_alloc(s.rays[1], npix * kRayWords);
@@ -363,6 +369,8 @@ ShadeParams shade_params(const rgrt_ctx* c)
     p.img = c->img;
     p.keepF32 = (c->flags & RGRT_CFG_FP32_RADIANCE) ? 1 : 0;
     p.keepAov = (c->flags & RGRT_CFG_PRIMARY_AOV) ? 1 : 0;
+    p.sampling = c->sampling.mode;
+    p.seed = c->sampling.seed;
     return p;
 }
 
@@ -905,6 +913,15 @@ rgrt_status rgrt_set_frame(rgrt_ctx* c, const rgrt_ubo* ubo)
     });
 }
 
+rgrt_status rgrt_set_sampling(rgrt_ctx* c, const rgrt_sampling* smp)
+{
+    return guarded(c, [&]() {
+        if(!smp || smp->mode > RGRT_SAMPLING_STOCHASTIC) return fail(c, RGRT_ERR_INVALID_ARG, "set_sampling: mode");
+        c->sampling = *smp;
+        return RGRT_OK;
+    });
+}
+
 rgrt_status rgrt_resize(rgrt_ctx* c, uint32_t w, uint32_t h)
 {
     return guarded(c, [&]() {
@@ -1073,6 +1090,7 @@ static bool image_info(rgrt_ctx* c, rgrt_image which, void** ptr, uint64_t* byte
     case RGRT_IMAGE_NORMAL_F32: *ptr = m.normal32; *bytes = npix * 16; break;
     case RGRT_IMAGE_ROUGH_F32: *ptr = m.rough32; *bytes = npix * 16; break;
     case RGRT_IMAGE_PRIMARY_HIT: *ptr = m.primary; *bytes = npix * sizeof(rgrt_hit); break;
+    case RGRT_IMAGE_FEATURES: *ptr = m.features; *bytes = npix * 16; break;
     default: return false;
     }
     return *ptr != nullptr;

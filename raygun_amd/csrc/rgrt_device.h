@@ -150,6 +150,40 @@ __device__ __forceinline__ F3 normalize(F3 a)
 __device__ __forceinline__ float distance(F3 a, F3 b) { return length(a - b); }
 __device__ __forceinline__ float mixf(float x, float y, float a) { return (x * (1.0f - a)) + (y * a); }
 __device__ __forceinline__ F3 mix(F3 x, F3 y, float a) { return {mixf(x.x, y.x, a), mixf(x.y, y.y, a), mixf(x.z, y.z, a)}; }
+
+// ---------------------------------------------------------------------------
+// Random numbers of the stochastic sampling mode: the "lowbias32" integer
+// finaliser of C. Wellons' hash-prospector, keyed by (seed, image pixel,
+// sample) and, for glossy reflections, the hit position's bits. Integer only,
+// so the oracle reproduces every number bit for bit.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t hash32(uint32_t x)
+{
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+__device__ __forceinline__ uint32_t sample_key(uint32_t seed, uint32_t pixel, uint32_t sample)
+{
+    return hash32(seed ^ hash32(pixel ^ hash32(sample ^ 0x9e3779b9u)));
+}
+// [0, 1) in steps of 2^-24
+__device__ __forceinline__ float u01(uint32_t h) { return (float)(h >> 8) * (1.0f / 16777216.0f); }
+
+// Glossy reflection: the mirror direction r spread by roughness * s, s
+// uniform in [-1, 1]^3 from the hit position's hash; r itself if the spread
+// direction falls below the surface (normal n).
+__device__ __forceinline__ F3 glossy_dir(uint32_t key, F3 origin, F3 r, F3 n, float roughness)
+{
+    const uint32_t h =
+        hash32(key ^ hash32(__float_as_uint(origin.x) ^ hash32(__float_as_uint(origin.y) ^ hash32(__float_as_uint(origin.z)))));
+    const F3 sp = f3(u01(hash32(h ^ 1u)) * 2.0f - 1.0f, u01(hash32(h ^ 2u)) * 2.0f - 1.0f, u01(hash32(h ^ 3u)) * 2.0f - 1.0f);
+    const F3 g = normalize(r + sp * roughness);
+    return dot(g, n) > 0.0f ? g : r;
+}
 __device__ __forceinline__ float clampf(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }
 __device__ __forceinline__ float glsl_mod(float x, float y) { return x - y * floorf(x / y); }
 __device__ __forceinline__ F3 reflect(F3 I, F3 N)

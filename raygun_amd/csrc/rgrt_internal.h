@@ -88,6 +88,7 @@ struct WavefrontState {
     float* rays[2] = {nullptr, nullptr}; // [kRayWords][npix]
     float* hits = nullptr;    // [kHitWords][npix]
     uint32_t* counters = nullptr; // [0..1] queue counts, [2..3] fetch, [4..11] stats (u64 pairs), [12] traversal errors
+    float* lum2 = nullptr;        // [npix]: sum of squared sample luminances (stochastic sampling)
 };
 
 struct Images {
@@ -103,6 +104,7 @@ struct Images {
     float* normal32 = nullptr;
     float* rough32 = nullptr;
     rgrt_hit* primary = nullptr;
+    uint16_t* features = nullptr; // [npix][2][4] fp16 (stochastic sampling)
 };
 
 // Arguments of the shading kernels
@@ -120,6 +122,8 @@ struct ShadeParams {
     Images img;
     int32_t keepF32;
     int32_t keepAov;
+    uint32_t sampling; // RGRT_SAMPLING_*
+    uint32_t seed;
 };
 
 void launch_raygen(hipStream_t s, const ShadeParams& p);

@@ -100,6 +100,7 @@ struct Ctx {
     const ShadeParams& p;
     F3 L;
     int maxRec;
+    uint32_t key; // sample key of the ray being shaded (stochastic sampling)
 };
 
 // ---------------------------------------------------------------------------
@@ -160,15 +161,22 @@ __device__ __forceinline__ void init_payload(Payload& P) // raygen.h:92-100
 
 // raygen.h:78-90: origin = viewInverse * (0,0,0,1); direction =
 // viewInverse * (normalize((projInverse * (d, 1, 1)).xyz), 0).
+// With stochastic sampling the pixel offset is uniform in [-0.5, 0.5)^2
+// from the sample key instead of the SSAA table.
 __device__ __forceinline__ void primary_ray(const rgrt_ubo& u, uint32_t px, uint32_t py, uint32_t W, uint32_t H, int sample,
-                                            OutRay& r)
+                                            uint32_t sampling, uint32_t seed, OutRay& r)
 {
     const float* vi = u.viewInverse;
     const float* pi = u.projInverse;
     const int n = u.numSamples;
     const int row = n < 8 ? n : 8;
-    const float offx = kAAOffsets[row][sample & 7][0];
-    const float offy = kAAOffsets[row][sample & 7][1];
+    float offx = kAAOffsets[row][sample & 7][0];
+    float offy = kAAOffsets[row][sample & 7][1];
+    if(sampling == RGRT_SAMPLING_STOCHASTIC) {
+        const uint32_t key = sample_key(seed, py * W + px, (uint32_t)sample);
+        offx = u01(hash32(key ^ 0x68bc21ebu)) - 0.5f;
+        offy = u01(hash32(key ^ 0x02e5be93u)) - 0.5f;
+    }
     const float pcx = ((float)px + 0.5f) + offx;
     const float pcy = ((float)py + 0.5f) + offy;
     const float ux = pcx / (float)W, uy = pcy / (float)H;
@@ -302,7 +310,9 @@ __device__ int generic_reflection(const Ctx& c, Frame& F, Payload& P, OutRay& ou
     F.reflectColor = f3(1.0f);
     F.reflectDepth = 0.0f;
     if(P.recDepth < c.maxRec && F.reflectivity > 0.0f) {
-        const F3 dir = reflect(F.dir, F.vn);
+        F3 dir = reflect(F.dir, F.vn);
+        if(c.p.sampling == RGRT_SAMPLING_STOCHASTIC && F.roughness > 0.0f)
+            dir = glossy_dir(c.key, F.origin, dir, F.vn, F.roughness);
         P.recDepth += F.rayConsumption;
         P.refDepth += F.t;
         set_ray(out, F.origin, 0.01f, dir, 1000.0f, 0xffu, 0u);
@@ -332,6 +342,22 @@ __device__ void grid_effect(float refDepth, float hitT, Frame& F, F3 pos) // clo
 }
 
 // closesthit.rchit:93-204 up to the first nested trace.
+// Material of a hit (closesthit.rchit:95-110: offset table of the instance's
+// customIndex, vertex 0's matIndex), with the gathers clamped to the buffers.
+__device__ const rgrt_material& hit_material(const ShadeParams& p, uint32_t instPos, uint32_t prim)
+{
+    const InstanceShade& is = p.inst[min(instPos, p.numInstances - 1)];
+    const rgrt_instance_offsets table = p.inst[min(is.customIndex, p.numInstances - 1)].offsets;
+    const uint64_t lastIndex = p.numIndices ? p.numIndices - 1 : 0;
+    const uint64_t lastVertex = p.numVertices ? p.numVertices - 1 : 0;
+    const uint64_t ib0 = (uint64_t)table.indexBufferOffset + 3ull * prim;
+    const uint32_t i0 = p.ib[min(ib0, lastIndex)];
+    const uint64_t vi0 = (uint64_t)table.vertexBufferOffset + i0;
+    const rgrt_vertex& v0 = p.vb[min(vi0, lastVertex)];
+    const uint32_t mi = table.materialBufferOffset + v0.matIndex;
+    return p.mats[p.numMaterials ? min(mi, p.numMaterials - 1) : 0u];
+}
+
 __device__ int closest_hit(const Ctx& c, Payload& P, uint32_t instPos, uint32_t prim, float hitT, float u, float v, F3 O, F3 D,
                            Frame& F, OutRay& out)
 {
@@ -506,13 +532,22 @@ __device__ __forceinline__ void write_rgba16(uint16_t* img, uint32_t pix, float 
 
 // raygen.h:105-114 accumulation and raygen.rgen:35-38 stores.
 // Accumulator record: hitValue, normal, rv[4], reflectContribution, depth.
+// With stochastic sampling the colour accumulated is the sample's composite
+// mix(hitValue, roughValue.rgb, reflectContribution) (what the post pass
+// would form from this one sample), and its squared luminance is summed too.
 __device__ void accumulate(const ShadeParams& p, uint32_t pix, const Payload& P)
 {
     float4* a = reinterpret_cast<float4*>(p.st.accum) + (size_t)pix * 3;
     float4 a0 = a[0], a1 = a[1], a2 = a[2];
-    a0.x = a0.x + P.hitValue.x;
-    a0.y = a0.y + P.hitValue.y;
-    a0.z = a0.z + P.hitValue.z;
+    F3 col = P.hitValue;
+    if(p.sampling == RGRT_SAMPLING_STOCHASTIC) {
+        col = mix(P.hitValue, f3(P.rv[0], P.rv[1], P.rv[2]), P.reflectContribution);
+        const float l = (0.299f * col.x + 0.587f * col.y) + 0.114f * col.z;
+        p.st.lum2[pix] = p.st.lum2[pix] + l * l;
+    }
+    a0.x = a0.x + col.x;
+    a0.y = a0.y + col.y;
+    a0.z = a0.z + col.z;
     a0.w = a0.w + P.normal.x;
     a1.x = a1.x + P.normal.y;
     a1.y = a1.y + P.normal.z;
@@ -551,7 +586,8 @@ __device__ void finalize(const ShadeParams& p, uint32_t slot)
     const float4 a0 = av[0], a1 = av[1], a2 = av[2];
     const float a[12] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w, a2.x, a2.y, a2.z, a2.w};
     const float ns = (float)p.ubo.numSamples;
-    const float r0[4] = {a[0] / ns, a[1] / ns, a[2] / ns, a[10] / ns};
+    const bool stochastic = p.sampling == RGRT_SAMPLING_STOCHASTIC;
+    const float r0[4] = {a[0] / ns, a[1] / ns, a[2] / ns, stochastic ? 0.0f : a[10] / ns};
     const float r1[4] = {a[3] / ns, a[4] / ns, a[5] / ns, (logf(a[11]) * 0.25f) / ns};
     const float r2[4] = {a[6] / ns, a[7] / ns, a[8] / ns, a[9] / ns};
     uint32_t px, py;
@@ -560,6 +596,11 @@ __device__ void finalize(const ShadeParams& p, uint32_t slot)
     write_rgba16(p.img.base, pix, r0[0], r0[1], r0[2], r0[3]);
     write_rgba16(p.img.normal, pix, r1[0], r1[1], r1[2], r1[3]);
     write_rgba16(p.img.rough, pix, r2[0], r2[1], r2[2], r2[3]);
+    if(stochastic) {
+        const float lm = (0.299f * r0[0] + 0.587f * r0[1]) + 0.114f * r0[2];
+        const float var = fmaxf(p.st.lum2[slot] / ns - lm * lm, 0.0f) / ns;
+        write_rgba16(p.img.features, 2 * pix + 1, a[11] / ns, var, 0.0f, 0.0f);
+    }
     if(p.keepF32) {
         reinterpret_cast<float4*>(p.img.base32)[pix] = make_float4(r0[0], r0[1], r0[2], r0[3]);
         reinterpret_cast<float4*>(p.img.normal32)[pix] = make_float4(r1[0], r1[1], r1[2], r1[3]);
@@ -599,12 +640,13 @@ __global__ __launch_bounds__(kShadeBlock) void k_raygen(ShadeParams p)
     uint32_t px, py;
     slot_pixel(pix, p.width, p.height, px, py);
     OutRay r;
-    primary_ray(p.ubo, px, py, p.width, p.height, 0, r);
+    primary_ray(p.ubo, px, py, p.width, p.height, 0, p.sampling, p.seed, r);
     Payload P;
     init_payload(P);
     store_payload(p, pix, P);
     float4* a = reinterpret_cast<float4*>(p.st.accum) + (size_t)pix * 3;
     for(int k = 0; k < 3; ++k) a[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if(p.sampling == RGRT_SAMPLING_STOCHASTIC) p.st.lum2[pix] = 0.0f;
     p.st.meta[pix] = 0u;
     write_ray(p.st.rays[0], p.st.npix, pix, r, pix, kFlagPrimary | (0xffu << kMaskShift));
 }
@@ -613,10 +655,29 @@ __global__ __launch_bounds__(kShadeBlock) void k_raygen(ShadeParams p)
 // continuations it returns into. Returns true with `out` set when the pixel
 // issues its next ray; false once its last sample is accumulated and the
 // outputs are written.
-__device__ bool advance_pixel(const Ctx& c, uint32_t pix, uint32_t flags, F3 O, F3 D, const HitRec& h, Payload& P, int& sp,
+__device__ bool advance_pixel(const Ctx& c0, uint32_t pix, uint32_t flags, F3 O, F3 D, const HitRec& h, Payload& P, int& sp,
                               int& sample, OutRay& out)
 {
-    const ShadeParams& p = c.p;
+    const ShadeParams& p = c0.p;
+    Ctx c{c0.p, c0.L, c0.maxRec, 0u};
+    uint32_t imgPix = 0;
+    if(p.sampling == RGRT_SAMPLING_STOCHASTIC) {
+        uint32_t px, py;
+        slot_pixel(pix, p.width, p.height, px, py);
+        imgPix = py * p.width + px;
+        c.key = sample_key(p.seed, imgPix, (uint32_t)sample);
+        if(flags & kFlagPrimary) { // denoiser features of sample 0's primary hit
+            float f[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+            if(h.inst != kNoHit) {
+                const rgrt_material& m = hit_material(p, h.inst, h.prim);
+                f[0] = m.diffuse[0];
+                f[1] = m.diffuse[1];
+                f[2] = m.diffuse[2];
+                f[3] = m.roughness;
+            }
+            write_rgba16(p.img.features, 2 * imgPix, f[0], f[1], f[2], f[3]);
+        }
+    }
     unsigned long long* skyCount = reinterpret_cast<unsigned long long*>(p.st.counters + 6);
     unsigned long long* zeroCount = reinterpret_cast<unsigned long long*>(p.st.counters + 8);
     if((flags & kFlagPrimary) && p.keepAov) {
@@ -669,7 +730,8 @@ __device__ bool advance_pixel(const Ctx& c, uint32_t pix, uint32_t flags, F3 O, 
                 init_payload(P);
                 uint32_t px, py;
                 slot_pixel(pix, p.width, p.height, px, py);
-                primary_ray(p.ubo, px, py, p.width, p.height, sample, out);
+                primary_ray(p.ubo, px, py, p.width, p.height, sample, p.sampling, p.seed, out);
+                if(p.sampling == RGRT_SAMPLING_STOCHASTIC) c.key = sample_key(p.seed, imgPix, (uint32_t)sample);
                 step = EMIT;
                 continue;
             }
@@ -703,7 +765,7 @@ __global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(RGR
     uint32_t* outCount = p.st.counters + (1 - cur);
     const int lane = threadIdx.x & 63;
     const uint64_t ltMask = lane == 0 ? 0ull : ((~0ull) >> (64 - lane));
-    const Ctx c{p, f3(p.ubo.lightDir[0], p.ubo.lightDir[1], p.ubo.lightDir[2]), p.ubo.maxRecursions};
+    const Ctx c{p, f3(p.ubo.lightDir[0], p.ubo.lightDir[1], p.ubo.lightDir[2]), p.ubo.maxRecursions, 0u};
     const uint32_t stride = gridDim.x * blockDim.x;
 
     for(uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < count; base += stride) {
@@ -758,7 +820,7 @@ __global__ __launch_bounds__(kDrainBlock) __attribute__((amdgpu_waves_per_eu(RGR
     uint32_t* fetch = p.st.counters + 2 + cur;
     const int lane = threadIdx.x & 63;
     const uint64_t ltMask = lane == 0 ? 0ull : ((~0ull) >> (64 - lane));
-    const Ctx c{p, f3(p.ubo.lightDir[0], p.ubo.lightDir[1], p.ubo.lightDir[2]), p.ubo.maxRecursions};
+    const Ctx c{p, f3(p.ubo.lightDir[0], p.ubo.lightDir[1], p.ubo.lightDir[2]), p.ubo.maxRecursions, 0u};
     const uint32_t gtid = blockIdx.x * kDrainBlock + threadIdx.x;
     TraversalStack<kDrainLdsStack, kDrainBlock, kDrainSpill> stack{ldsStack, spillBase + (size_t)gtid * kDrainSpill,
                                                                    (int)threadIdx.x};

@@ -26,7 +26,8 @@ FORMAT_RGBA8_SRGB = 2
 FORMAT_BGRA8_SRGB = 3
 
 IMAGE_FINAL, IMAGE_BASE, IMAGE_NORMAL, IMAGE_ROUGH, IMAGE_ROUGH_TRANSITIONS, IMAGE_ROUGH_COLORS_A, IMAGE_ROUGH_COLORS_B, \
-    IMAGE_BASE_F32, IMAGE_NORMAL_F32, IMAGE_ROUGH_F32, IMAGE_PRIMARY_HIT = range(11)
+    IMAGE_BASE_F32, IMAGE_NORMAL_F32, IMAGE_ROUGH_F32, IMAGE_PRIMARY_HIT, IMAGE_FEATURES = range(12)
+SAMPLING_WHITTED, SAMPLING_STOCHASTIC = 0, 1
 
 STATUS = {0: "OK", 1: "INVALID_ARG", 2: "OUT_OF_MEMORY", 3: "HIP", 4: "NOT_READY", 5: "NO_DEVICE"}
 
@@ -38,7 +39,7 @@ EXPORTS = [
     "rgrt_build_tlas", "rgrt_refit_tlas", "rgrt_export_tlas", "rgrt_set_frame", "rgrt_resize",
     "rgrt_trace_rays", "rgrt_trace_rays_device", "rgrt_render", "rgrt_frame", "rgrt_trace", "rgrt_postprocess",
     "rgrt_read_image", "rgrt_read_image_rgba8", "rgrt_image_device_ptr", "rgrt_write_image", "rgrt_get_stats", "rgrt_get_pass_counts",
-    "rgrt_set_tuning", "rgrt_get_tuning", "rgrt_export_blas_wide", "rgrt_export_tlas_wide", "rgrt_version",
+    "rgrt_set_tuning", "rgrt_get_tuning", "rgrt_set_sampling", "rgrt_export_blas_wide", "rgrt_export_tlas_wide", "rgrt_version",
 ]
 
 
@@ -102,6 +103,7 @@ def lib():
             "rgrt_get_stats": (i32, [P, P]),
             "rgrt_get_pass_counts": (i32, [P, P, u32, ctypes.POINTER(u32)]),
             "rgrt_set_tuning": (i32, [P, P]),
+            "rgrt_set_sampling": (i32, [P, P]),
             "rgrt_export_blas_wide": (i32, [P, u32, i32, P, u32, ctypes.POINTER(u32)]),
             "rgrt_export_tlas_wide": (i32, [P, i32, P, u32, ctypes.POINTER(u32)]),
             "rgrt_get_tuning": (i32, [P, P]),
@@ -248,6 +250,12 @@ class Context:
     def trace_rays_device(self, rays_ptr, n, cull_mask, hits_ptr):
         self._ok(self._L.rgrt_trace_rays_device(self.h, rays_ptr, n, cull_mask, hits_ptr))
 
+    def set_sampling(self, mode, seed=0):
+        """SAMPLING_WHITTED (the reference's tracer) or SAMPLING_STOCHASTIC
+        (jittered pixels, glossy reflections, denoiser features)."""
+        sm = np.array([mode, seed, 0, 0], np.uint32)
+        self._ok(self._L.rgrt_set_sampling(self.h, _p(sm)))
+
     def read_image_rgba8(self, which=None, fmt=None):
         """An rgba16f image converted on the device to 8 bits per channel
         (the swapchain blit's conversion): (height, width, 4) uint8."""
@@ -287,6 +295,8 @@ class Context:
             out = np.zeros((H, W, 4), np.float32)
         elif which == IMAGE_PRIMARY_HIT:
             out = np.zeros((H, W), HIT)
+        elif which == IMAGE_FEATURES:
+            out = np.zeros((H, W, 8), np.uint16)
         else:
             out = np.zeros((H, W, 4), np.uint16)
         self._ok(self._L.rgrt_read_image(self.h, which, _p(out), out.nbytes))
