@@ -274,6 +274,47 @@ typedef struct rgrt_sampling {
 #define RGRT_SAMPLING_STOCHASTIC 1u
 rgrt_status rgrt_set_sampling(rgrt_ctx* ctx, const rgrt_sampling* sampling);
 
+/* ------------------------------------------------------------------------ */
+/* Learned denoiser (no counterpart in the reference: it stands in for the   */
+/* screen-space roughness blur of raygun/compute/rough_blur.h and for high   */
+/* sample counts). A 3-level U-Net (16/32/64 channels, 3x3 convolutions,     */
+/* skip concatenation; 11 input channels: noisy RGB, albedo, normal,         */
+/* log depth, roughness) in BF16 on the matrix cores with FP32 accumulation, */
+/* trained on the device by its own forward and backward kernels (Adam).     */
+/* It denoises a frame of RGRT_SAMPLING_STOCHASTIC in place (BASE.rgb), then */
+/* rgrt_postprocess finishes it; 16x16 tiles where the network's change      */
+/* exceeds fallbackA * (noise variance of the tile) + fallbackB keep the     */
+/* traced frame. A denoiser belongs to one context and runs on its stream.   */
+/* ------------------------------------------------------------------------ */
+typedef struct rgrt_denoiser rgrt_denoiser;
+rgrt_status rgrt_denoiser_create(rgrt_ctx* ctx, uint32_t seed, rgrt_denoiser** out);
+void rgrt_denoiser_destroy(rgrt_denoiser* den);
+/* fp32 parameters: per layer the weights [9][cin][cout] then the biases */
+rgrt_status rgrt_denoiser_num_params(rgrt_denoiser* den, uint64_t* count);
+rgrt_status rgrt_denoiser_get_params(rgrt_denoiser* den, float* params, uint64_t count);
+rgrt_status rgrt_denoiser_set_params(rgrt_denoiser* den, const float* params, uint64_t count);
+/* One forward + backward pass on host data: x0 = n*h*w*16 bf16 bit patterns
+ * (the packed input), target = n*h*w*3 floats; writes the loss and, when the
+ * pointers are given, every parameter's gradient and the network output
+ * (n*h*w*16 bf16, the correction added to the noisy RGB). Synchronous. */
+rgrt_status rgrt_denoiser_loss_and_grads(rgrt_denoiser* den, const uint16_t* x0, const float* target, uint32_t n,
+                                         uint32_t height, uint32_t width, float* loss, float* grads, uint16_t* output);
+/* Training set in device memory: `count` frames of height x width. add_input
+ * packs the context's current stochastic frame into a slot; add_target
+ * (accumulate != 0: adds) weight * BASE.rgb of the current frame to the slot's
+ * target. */
+rgrt_status rgrt_denoiser_dataset_reserve(rgrt_denoiser* den, uint32_t count, uint32_t height, uint32_t width);
+rgrt_status rgrt_denoiser_dataset_add_input(rgrt_denoiser* den, uint32_t slot);
+rgrt_status rgrt_denoiser_dataset_add_target(rgrt_denoiser* den, uint32_t slot, float weight, int32_t accumulate);
+/* `steps` Adam steps (learning rate lr) on random batches of `batch` of the
+ * first `used` slots; *loss = mean loss of the last steps. Synchronous. */
+rgrt_status rgrt_denoiser_train(rgrt_denoiser* den, uint32_t steps, uint32_t batch, float lr, uint32_t seed, uint32_t used,
+                                float* loss);
+/* Denoises the context's current frame (asynchronous on the context stream).
+ * With fallbackTiles != NULL waits and reports how many of the *tiles fell
+ * back to the traced frame. */
+rgrt_status rgrt_denoise(rgrt_denoiser* den, float fallbackA, float fallbackB, uint32_t* fallbackTiles, uint32_t* tiles);
+
 /* rgrt_frame flags */
 #define RGRT_FRAME_FXAA 0x1u       /* FXAA + swap(base, final), as doRaytracing with m_useFXAA */
 #define RGRT_FRAME_REFIT_TLAS 0x2u /* refit the TLAS (rgrt_refit_tlas) instead of building it */

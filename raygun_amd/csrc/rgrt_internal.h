@@ -126,6 +126,71 @@ struct ShadeParams {
     uint32_t seed;
 };
 
+// ---------------------------------------------------------------------------
+// Learned denoiser (denoise.hip)
+// ---------------------------------------------------------------------------
+class Denoiser {
+  public:
+    static constexpr int kNumLayers = 11;
+    // activation and gradient tensors (bf16 NHWC), in workspace order
+    enum Buf {
+        X0, A1, S1, P1, A2, S2, P2, A3, B3, C2, A4, U2, C1, A5, U1, OUT, // forward
+        DZO, GA, GC1, GU2, GS1X, GC2, GB3, GS2X, GL2A, GP2, GL1A, GP1, GS1, GB, GS2, // backward
+        kNumBufs
+    };
+    static constexpr int kNumFwdBufs = OUT + 1;
+
+    explicit Denoiser(uint32_t seed);
+    ~Denoiser();
+    Denoiser(const Denoiser&) = delete;
+    Denoiser& operator=(const Denoiser&) = delete;
+
+    size_t num_params() const { return numParams; }
+    void set_params(hipStream_t s, const float* host);
+    void get_params(hipStream_t s, float* host);
+    // forward + loss + gradients on host inputs (x0: N*H*W*16 bf16 bits,
+    // target: N*H*W*3 fp32); returns the loss, gradients to gradHost
+    float loss_and_grads(hipStream_t s, const uint16_t* x0Host, const float* targetHost, int N, int H, int W,
+                         float* gradHost);
+    void read_output(hipStream_t s, uint16_t* host); // o of the last forward pass
+    // training set in device memory: packed inputs and fp32 targets
+    void dataset_reserve(uint32_t count, int H, int W);
+    void dataset_add_input(hipStream_t s, const Images& img, uint32_t slot);
+    void dataset_add_target(hipStream_t s, const Images& img, uint32_t slot, float weight, bool accumulate);
+    // steps of Adam on random batches of the first `used` samples; returns the mean loss of the last steps
+    float train(hipStream_t s, uint32_t steps, uint32_t batch, float lr, uint32_t seed, uint32_t used);
+    // denoises img.base in place (rgb; fallback tiles keep the traced frame); returns the tile count
+    uint32_t denoise(hipStream_t s, Images& img, float fallbackA, float fallbackB);
+    uint32_t count_fallback(hipStream_t s, uint32_t tiles);
+
+  private:
+    void reserve(int N, int H, int W, bool training);
+    void prep_weights(hipStream_t s);
+    void forward(hipStream_t s, int N, int H, int W);
+    void backward(hipStream_t s, const float* target, int N, int H, int W);
+    void adam_step(hipStream_t s, float lr);
+
+    size_t numParams = 0;
+    size_t wOff[kNumLayers], bOff[kNumLayers], wbfOff[kNumLayers];
+    float *params = nullptr, *grads = nullptr, *adamM = nullptr, *adamV = nullptr;
+    uint16_t *wFwd = nullptr, *wBwd = nullptr;
+    bool weightsReady = false;
+    uint32_t stepCount = 0;
+    void* ws = nullptr;
+    size_t wsBytes = 0;
+    uint16_t* buf[kNumBufs] = {};
+    float* partial = nullptr;
+    float* lossPart = nullptr;
+    uint8_t* fallback = nullptr;
+    int wsN = 0, wsH = 0, wsW = 0;
+    bool wsTraining = false;
+    bool ds = false;
+    uint16_t* dsX = nullptr;
+    float* dsT = nullptr;
+    uint32_t dsCount = 0;
+    int dsH = 0, dsW = 0;
+};
+
 void launch_raygen(hipStream_t s, const ShadeParams& p);
 void launch_trace_queue(hipStream_t s, const TraceScene& sc, const WavefrontState& st, int cur, uint32_t gridBlocks);
 void launch_shade(hipStream_t s, const ShadeParams& p, int cur, uint32_t gridBlocks);

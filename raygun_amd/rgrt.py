@@ -39,7 +39,10 @@ EXPORTS = [
     "rgrt_build_tlas", "rgrt_refit_tlas", "rgrt_export_tlas", "rgrt_set_frame", "rgrt_resize",
     "rgrt_trace_rays", "rgrt_trace_rays_device", "rgrt_render", "rgrt_frame", "rgrt_trace", "rgrt_postprocess",
     "rgrt_read_image", "rgrt_read_image_rgba8", "rgrt_image_device_ptr", "rgrt_write_image", "rgrt_get_stats", "rgrt_get_pass_counts",
-    "rgrt_set_tuning", "rgrt_get_tuning", "rgrt_set_sampling", "rgrt_export_blas_wide", "rgrt_export_tlas_wide", "rgrt_version",
+    "rgrt_set_tuning", "rgrt_get_tuning", "rgrt_set_sampling", "rgrt_denoiser_create", "rgrt_denoiser_destroy",
+    "rgrt_denoiser_num_params", "rgrt_denoiser_get_params", "rgrt_denoiser_set_params", "rgrt_denoiser_loss_and_grads",
+    "rgrt_denoiser_dataset_reserve", "rgrt_denoiser_dataset_add_input", "rgrt_denoiser_dataset_add_target",
+    "rgrt_denoiser_train", "rgrt_denoise", "rgrt_export_blas_wide", "rgrt_export_tlas_wide", "rgrt_version",
 ]
 
 
@@ -108,6 +111,17 @@ def lib():
             "rgrt_export_tlas_wide": (i32, [P, i32, P, u32, ctypes.POINTER(u32)]),
             "rgrt_get_tuning": (i32, [P, P]),
             "rgrt_version": (ctypes.c_char_p, []),
+            "rgrt_denoiser_create": (i32, [P, u32, ctypes.POINTER(P)]),
+            "rgrt_denoiser_destroy": (None, [P]),
+            "rgrt_denoiser_num_params": (i32, [P, ctypes.POINTER(u64)]),
+            "rgrt_denoiser_get_params": (i32, [P, P, u64]),
+            "rgrt_denoiser_set_params": (i32, [P, P, u64]),
+            "rgrt_denoiser_loss_and_grads": (i32, [P, P, P, u32, u32, u32, ctypes.POINTER(ctypes.c_float), P, P]),
+            "rgrt_denoiser_dataset_reserve": (i32, [P, u32, u32, u32]),
+            "rgrt_denoiser_dataset_add_input": (i32, [P, u32]),
+            "rgrt_denoiser_dataset_add_target": (i32, [P, u32, ctypes.c_float, i32]),
+            "rgrt_denoiser_train": (i32, [P, u32, u32, ctypes.c_float, u32, u32, ctypes.POINTER(ctypes.c_float)]),
+            "rgrt_denoise": (i32, [P, ctypes.c_float, ctypes.c_float, ctypes.POINTER(u32), ctypes.POINTER(u32)]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -338,3 +352,72 @@ class Context:
 
 def version():
     return lib().rgrt_version().decode()
+
+
+class Denoiser:
+    """The learned denoiser of one context (rgrt_denoiser_*)."""
+
+    def __init__(self, ctx, seed=0):
+        self.ctx = ctx
+        self._L = lib()
+        self.h = ctypes.c_void_p()
+        ctx._ok(self._L.rgrt_denoiser_create(ctx.h, seed, ctypes.byref(self.h)))
+        n = ctypes.c_uint64()
+        ctx._ok(self._L.rgrt_denoiser_num_params(self.h, ctypes.byref(n)))
+        self.num_params = n.value
+
+    def close(self):
+        if self.h:
+            self._L.rgrt_denoiser_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def get_params(self):
+        p = np.zeros(self.num_params, np.float32)
+        self.ctx._ok(self._L.rgrt_denoiser_get_params(self.h, _p(p), p.size))
+        return p
+
+    def set_params(self, params):
+        p = np.ascontiguousarray(params, np.float32)
+        self.ctx._ok(self._L.rgrt_denoiser_set_params(self.h, _p(p), p.size))
+
+    def loss_and_grads(self, x0_bits, target, with_output=False):
+        """x0_bits: (n, h, w, 16) uint16 bf16 patterns; target: (n, h, w, 3)."""
+        x = np.ascontiguousarray(x0_bits, np.uint16)
+        t = np.ascontiguousarray(target, np.float32)
+        n, hh, ww, _ = x.shape
+        g = np.zeros(self.num_params, np.float32)
+        out = np.zeros((n, hh, ww, 16), np.uint16) if with_output else None
+        loss = ctypes.c_float()
+        self.ctx._ok(self._L.rgrt_denoiser_loss_and_grads(self.h, _p(x), _p(t), n, hh, ww, ctypes.byref(loss), _p(g),
+                                                          _p(out)))
+        return (loss.value, g, out) if with_output else (loss.value, g)
+
+    def dataset_reserve(self, count, height, width):
+        self.ctx._ok(self._L.rgrt_denoiser_dataset_reserve(self.h, count, height, width))
+
+    def dataset_add_input(self, slot):
+        self.ctx._ok(self._L.rgrt_denoiser_dataset_add_input(self.h, slot))
+
+    def dataset_add_target(self, slot, weight, accumulate):
+        self.ctx._ok(self._L.rgrt_denoiser_dataset_add_target(self.h, slot, weight, int(bool(accumulate))))
+
+    def train(self, steps, batch, lr, seed, used):
+        loss = ctypes.c_float()
+        self.ctx._ok(self._L.rgrt_denoiser_train(self.h, steps, batch, lr, seed, used, ctypes.byref(loss)))
+        return loss.value
+
+    def denoise(self, fallback_a=8.0, fallback_b=2e-4, count=False):
+        """Denoises the context's current frame; with count=True returns
+        (fallback tiles, tiles)."""
+        if not count:
+            self.ctx._ok(self._L.rgrt_denoise(self.h, fallback_a, fallback_b, None, None))
+            return None
+        f, t = ctypes.c_uint32(), ctypes.c_uint32()
+        self.ctx._ok(self._L.rgrt_denoise(self.h, fallback_a, fallback_b, ctypes.byref(f), ctypes.byref(t)))
+        return f.value, t.value
