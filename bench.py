@@ -39,32 +39,6 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-class HipEvents:
-    """hipEvent pairs through libamdhip64 (events on the library's stream)."""
-
-    def __init__(self):
-        self.L = ctypes.CDLL("libamdhip64.so")
-        self.L.hipEventCreate.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
-        self.L.hipEventRecord.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
-        self.L.hipEventSynchronize.argtypes = [ctypes.c_void_p]
-        self.L.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_void_p, ctypes.c_void_p]
-        self.L.hipEventDestroy.argtypes = [ctypes.c_void_p]
-
-    def create(self):
-        e = ctypes.c_void_p()
-        assert self.L.hipEventCreate(ctypes.byref(e)) == 0
-        return e
-
-    def record(self, e, stream):
-        assert self.L.hipEventRecord(e, ctypes.c_void_p(stream)) == 0
-
-    def elapsed_ms(self, a, b):
-        assert self.L.hipEventSynchronize(b) == 0
-        ms = ctypes.c_float()
-        assert self.L.hipEventElapsedTime(ctypes.byref(ms), a, b) == 0
-        return ms.value
-
-
 def measured_traffic(kernel, rays_per_launch):
     """HBM bytes per launch of `kernel` from the committed counter summary
     (FETCH_SIZE + WRITE_SIZE per traced ray, scaled by the copy-kernel
@@ -99,6 +73,8 @@ def host_cores():
 
 
 def main():
+    from raygun_amd.hiptime import HipEvents
+
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)

@@ -335,6 +335,28 @@ rgrt_status rgrt_denoise(rgrt_denoiser* den, float fallbackA, float fallbackB, u
  * Asynchronous on the context stream. */
 rgrt_status rgrt_frame(rgrt_ctx* ctx, const rgrt_ubo* ubo, const rgrt_instance* instances, uint32_t numInstances,
                        uint32_t flags);
+/* ------------------------------------------------------------------------ */
+/* Batches of scenes (no counterpart in the reference, which renders one    */
+/* scene): the frames of several contexts — scenes of any size and render   */
+/* resolution on one device — captured into one hipGraph with one branch per */
+/* scene, so the GPU runs the scenes' kernels side by side, and launched at  */
+/* once. Each frame is what rgrt_frame(..., RGRT_FRAME_GRAPH | flags) does   */
+/* for its context; the graph is captured again every call and updated in   */
+/* place. Work issued later on any of the contexts runs after the batch.    */
+/* ------------------------------------------------------------------------ */
+typedef struct rgrt_batch rgrt_batch;
+typedef struct rgrt_frame_desc {
+    const rgrt_ubo* ubo;
+    const rgrt_instance* instances;
+    uint32_t numInstances;
+    uint32_t flags; /* RGRT_FRAME_FXAA | RGRT_FRAME_REFIT_TLAS */
+} rgrt_frame_desc;
+rgrt_status rgrt_batch_create(rgrt_ctx* const* ctxs, uint32_t count, rgrt_batch** out);
+void rgrt_batch_destroy(rgrt_batch* batch);
+const char* rgrt_batch_last_error(const rgrt_batch* batch);
+/* descs[i] is the frame of ctxs[i]. Asynchronous on the first context's stream. */
+rgrt_status rgrt_batch_frame(rgrt_batch* batch, const rgrt_frame_desc* descs);
+
 /* The traceRaysKHR part alone (writes base, normal, rough). */
 rgrt_status rgrt_trace(rgrt_ctx* ctx);
 /* The compute post part alone (reads base, normal, rough). */

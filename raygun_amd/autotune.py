@@ -1,0 +1,47 @@
+"""Launch-shape autotuner: picks rgrt_tuning (wavefront passes before the
+drain, persistent grid sizes per CU, host lag) for one scene by timing its
+frames with hipEvents. Coordinate descent: each parameter in turn takes the
+value from its candidate list that gives the fastest frame with the others
+fixed, for `rounds` rounds. Every candidate is a documented launch shape, so
+a sweep only changes timing, never results (the tests render the same frame
+under different tunings and compare).
+
+No counterpart in the reference (its pipeline's launch shape is the
+driver's).
+"""
+from .hiptime import HipEvents, time_ms
+
+CANDIDATES = {
+    "wavefrontPasses": [2, 4, 6, 8, 12],
+    "traceBlocksPerCU": [4, 6, 8],
+    "shadeBlocksPerCU": [4, 8, 12],
+    "drainBlocksPerCU": [2, 4, 8],
+    "hostLag": [1, 2, 4],
+}
+
+
+def tune(ctx, frame, candidates=None, rounds=1, reps=2, log=None):
+    """frame(): enqueues one frame on ctx. Returns (best tuning, best ms,
+    table of (tuning, ms)). Leaves ctx set to the best tuning."""
+    cand = candidates or CANDIDATES
+    ev = HipEvents()
+    best = dict(ctx.tuning())
+    ctx.set_tuning(**best)
+    best_ms = time_ms(ctx, frame, reps=reps, warmup=1, events=ev)
+    table = [(dict(best), best_ms)]
+    for _ in range(rounds):
+        for name, values in cand.items():
+            for v in values:
+                if v == best[name]:
+                    continue
+                t = dict(best)
+                t[name] = v
+                ctx.set_tuning(**t)
+                ms = time_ms(ctx, frame, reps=reps, warmup=1, events=ev)
+                table.append((t, ms))
+                if log:
+                    log(f"[autotune] {t} {ms:.3f} ms")
+                if ms < best_ms:
+                    best, best_ms = t, ms
+            ctx.set_tuning(**best)
+    return best, best_ms, table

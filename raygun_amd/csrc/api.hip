@@ -991,6 +991,74 @@ rgrt_status rgrt_render(rgrt_ctx* c, int32_t useFxaa)
     });
 }
 
+} // extern "C"
+
+namespace {
+
+// Graph form of a frame, in three parts: everything that validates,
+// allocates or waits (before a capture), the captured work on the context's
+// stream, and the bookkeeping after the launch.
+rgrt_status frame_prepare(rgrt_ctx* c, const rgrt_ubo* ubo, const rgrt_instance* in, uint32_t n, uint32_t flags)
+{
+    if(flags & ~(RGRT_FRAME_FXAA | RGRT_FRAME_REFIT_TLAS | RGRT_FRAME_GRAPH))
+        return fail(c, RGRT_ERR_INVALID_ARG, "frame: unknown flags");
+    rgrt_status st = rgrt_set_frame(c, ubo);
+    if(st != RGRT_OK) return st;
+    st = validate_instances(c, in, n, (flags & RGRT_FRAME_REFIT_TLAS) != 0, "frame");
+    if(st != RGRT_OK) return st;
+    if(!c->w || !c->h) return fail(c, RGRT_ERR_NOT_READY, "frame: zero-sized render target");
+    stage_to_host(c, in, n);
+    c->builder.reserve(std::max<uint32_t>(n, 1));
+    const uint32_t maxIters = max_sequential_rays(c->ubo.maxRecursions) * (uint32_t)c->ubo.numSamples + 2;
+    ensure_host_counts(c, maxIters + 2);
+    return RGRT_OK;
+}
+
+void frame_enqueue(rgrt_ctx* c, uint32_t n, uint32_t flags)
+{
+    hipStream_t s = c->stream;
+    if(n)
+        check(hipMemcpyAsync(c->dInstIn, c->hInstStage[c->stageIdx], n * sizeof(rgrt_instance), hipMemcpyHostToDevice, s));
+    enqueue_tlas(c, n, (flags & RGRT_FRAME_REFIT_TLAS) != 0);
+    enqueue_trace_fixed(c);
+    launch_post(s, c->img, c->ubo, (flags & RGRT_FRAME_FXAA) ? 1 : 0, nullptr);
+}
+
+void frame_finish(rgrt_ctx* c, const rgrt_instance* in, uint32_t n, uint32_t flags)
+{
+    if(flags & RGRT_FRAME_FXAA) std::swap(c->img.base, c->img.final_); // raygun/render/raytracer.cpp:139
+    finish_tlas(c, in, n);
+    c->timedAs = c->timedRt = c->timedPost = false;
+}
+
+// Instantiates g into *exec, or updates *exec in place when the topology
+// allows; destroys g.
+void update_exec(hipGraphExec_t* exec, hipGraph_t g)
+{
+    bool updated = false;
+    if(*exec) {
+        hipGraphNode_t errNode = nullptr;
+        hipGraphExecUpdateResult res = hipGraphExecUpdateError;
+        updated = hipGraphExecUpdate(*exec, g, &errNode, &res) == hipSuccess && res == hipGraphExecUpdateSuccess;
+        if(!updated) {
+            (void)hipGetLastError(); // a refused update leaves nothing pending
+            (void)hipGraphExecDestroy(*exec);
+            *exec = nullptr;
+        }
+    }
+    if(!updated) {
+        const hipError_t e = hipGraphInstantiate(exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        check(e);
+        return;
+    }
+    (void)hipGraphDestroy(g);
+}
+
+} // namespace
+
+extern "C" {
+
 rgrt_status rgrt_frame(rgrt_ctx* c, const rgrt_ubo* ubo, const rgrt_instance* in, uint32_t n, uint32_t flags)
 {
     return guarded(c, [&]() {
@@ -998,23 +1066,17 @@ rgrt_status rgrt_frame(rgrt_ctx* c, const rgrt_ubo* ubo, const rgrt_instance* in
             return fail(c, RGRT_ERR_INVALID_ARG, "frame: unknown flags");
         const bool fxaa = (flags & RGRT_FRAME_FXAA) != 0;
         const bool refit = (flags & RGRT_FRAME_REFIT_TLAS) != 0;
-        rgrt_status st = rgrt_set_frame(c, ubo);
-        if(st != RGRT_OK) return st;
         if(!(flags & RGRT_FRAME_GRAPH)) {
             // eager: the three calls of RenderSystem::render
+            rgrt_status st = rgrt_set_frame(c, ubo);
+            if(st != RGRT_OK) return st;
             st = refit ? rgrt_refit_tlas(c, in, n) : rgrt_build_tlas(c, in, n);
             if(st != RGRT_OK) return st;
             return rgrt_render(c, fxaa ? 1 : 0);
         }
-        st = validate_instances(c, in, n, refit, "frame");
+        const rgrt_status st = frame_prepare(c, ubo, in, n, flags);
         if(st != RGRT_OK) return st;
-        if(!c->w || !c->h) return fail(c, RGRT_ERR_NOT_READY, "frame: zero-sized render target");
         hipStream_t s = c->stream;
-        // everything that allocates or waits happens before the capture
-        stage_to_host(c, in, n);
-        c->builder.reserve(std::max<uint32_t>(n, 1));
-        const uint32_t maxIters = max_sequential_rays(c->ubo.maxRecursions) * (uint32_t)c->ubo.numSamples + 2;
-        ensure_host_counts(c, maxIters + 2);
         const bool timing = (c->flags & RGRT_CFG_TIMING) != 0;
         if(timing) {
             make_event(c->evFrameStart);
@@ -1028,12 +1090,7 @@ rgrt_status rgrt_frame(rgrt_ctx* c, const rgrt_ubo* ubo, const rgrt_instance* in
         hipGraph_t g = nullptr;
         check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
         try {
-            if(n)
-                check(hipMemcpyAsync(c->dInstIn, c->hInstStage[c->stageIdx], n * sizeof(rgrt_instance), hipMemcpyHostToDevice,
-                                     s));
-            enqueue_tlas(c, n, refit);
-            enqueue_trace_fixed(c);
-            launch_post(s, c->img, c->ubo, fxaa ? 1 : 0, nullptr);
+            frame_enqueue(c, n, flags);
         }
         catch(...) {
             (void)hipStreamEndCapture(s, &g);
@@ -1043,35 +1100,149 @@ rgrt_status rgrt_frame(rgrt_ctx* c, const rgrt_ubo* ubo, const rgrt_instance* in
             throw;
         }
         check(hipStreamEndCapture(s, &g));
-        bool updated = false;
-        if(c->frameExec) {
-            hipGraphNode_t errNode = nullptr;
-            hipGraphExecUpdateResult res = hipGraphExecUpdateError;
-            updated = hipGraphExecUpdate(c->frameExec, g, &errNode, &res) == hipSuccess && res == hipGraphExecUpdateSuccess;
-            if(!updated) {
-                (void)hipGetLastError(); // a refused update leaves nothing pending
-                (void)hipGraphExecDestroy(c->frameExec);
-                c->frameExec = nullptr;
-            }
-        }
-        if(!updated) {
-            const hipError_t e = hipGraphInstantiate(&c->frameExec, g, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(g);
-            check(e);
-        }
-        else {
-            (void)hipGraphDestroy(g);
-        }
+        update_exec(&c->frameExec, g);
         if(timing) check(hipEventRecord(c->evFrameStart, s));
         check(hipGraphLaunch(c->frameExec, s));
         if(timing) check(hipEventRecord(c->evFrameEnd, s));
         check(hipEventRecord(c->stageDone[c->stageIdx], s)); // this staging buffer is free after this frame
-        if(fxaa) std::swap(c->img.base, c->img.final_); // raygun/render/raytracer.cpp:139
-        finish_tlas(c, in, n);
-        c->timedAs = c->timedRt = c->timedPost = false;
+        frame_finish(c, in, n, flags);
         c->timedFrame = timing;
         return RGRT_OK;
     });
+}
+
+} // extern "C"
+
+struct rgrt_batch {
+    std::vector<rgrt_ctx*> ctxs;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t fork = nullptr, done = nullptr;
+    std::vector<hipEvent_t> join;
+    std::string err;
+};
+
+extern "C" {
+
+rgrt_status rgrt_batch_create(rgrt_ctx* const* ctxs, uint32_t n, rgrt_batch** out)
+{
+    if(!out || !ctxs || n == 0) return RGRT_ERR_INVALID_ARG;
+    *out = nullptr;
+    for(uint32_t i = 0; i < n; ++i) {
+        if(!ctxs[i]) return RGRT_ERR_INVALID_ARG;
+        if(ctxs[i]->device != ctxs[0]->device) return RGRT_ERR_INVALID_ARG;
+        for(uint32_t j = 0; j < i; ++j)
+            if(ctxs[j] == ctxs[i]) return RGRT_ERR_INVALID_ARG;
+    }
+    auto* b = new rgrt_batch();
+    b->ctxs.assign(ctxs, ctxs + n);
+    try {
+        check(hipSetDevice(ctxs[0]->device));
+        check(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
+        check(hipEventCreateWithFlags(&b->done, hipEventDisableTiming));
+        b->join.resize(n, nullptr);
+        for(auto& e: b->join) check(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
+    catch(...) {
+        rgrt_batch_destroy(b);
+        return RGRT_ERR_HIP;
+    }
+    *out = b;
+    return RGRT_OK;
+}
+
+void rgrt_batch_destroy(rgrt_batch* b)
+{
+    if(!b) return;
+    for(rgrt_ctx* c: b->ctxs)
+        if(c && c->stream) (void)hipStreamSynchronize(c->stream);
+    if(b->exec) (void)hipGraphExecDestroy(b->exec);
+    for(hipEvent_t e: b->join)
+        if(e) (void)hipEventDestroy(e);
+    if(b->fork) (void)hipEventDestroy(b->fork);
+    if(b->done) (void)hipEventDestroy(b->done);
+    delete b;
+}
+
+const char* rgrt_batch_last_error(const rgrt_batch* b) { return b ? b->err.c_str() : "null batch"; }
+
+rgrt_status rgrt_batch_frame(rgrt_batch* b, const rgrt_frame_desc* descs)
+{
+    if(!b || !descs) return RGRT_ERR_INVALID_ARG;
+    const uint32_t n = (uint32_t)b->ctxs.size();
+    // validation and staging, per context, before anything is captured
+    for(uint32_t i = 0; i < n; ++i) {
+        rgrt_ctx* c = b->ctxs[i];
+        const rgrt_frame_desc& d = descs[i];
+        const rgrt_status st =
+            guarded(c, [&]() { return frame_prepare(c, d.ubo, d.instances, d.numInstances, d.flags | RGRT_FRAME_GRAPH); });
+        if(st != RGRT_OK) {
+            b->err = "scene " + std::to_string(i) + ": " + c->err;
+            return st;
+        }
+    }
+    rgrt_ctx* c0 = b->ctxs[0];
+    hipStream_t s0 = c0->stream;
+    std::vector<uint32_t> prevN(n);
+    std::vector<bool> prevHave(n);
+    for(uint32_t i = 0; i < n; ++i) {
+        prevN[i] = b->ctxs[i]->numInstances;
+        prevHave[i] = b->ctxs[i]->haveTlas;
+        b->ctxs[i]->numInstances = descs[i].numInstances;
+        b->ctxs[i]->haveTlas = true;
+    }
+    try {
+        check(hipSetDevice(c0->device));
+        // one graph, one branch per scene: the scenes' streams fork from and
+        // join into the first scene's stream inside the capture
+        hipGraph_t g = nullptr;
+        check(hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal));
+        uint32_t joined = 0;
+        try {
+            check(hipEventRecord(b->fork, s0));
+            for(uint32_t i = 0; i < n; ++i) {
+                rgrt_ctx* c = b->ctxs[i];
+                if(i) check(hipStreamWaitEvent(c->stream, b->fork, 0));
+                frame_enqueue(c, descs[i].numInstances, descs[i].flags);
+                if(i) {
+                    check(hipEventRecord(b->join[i], c->stream));
+                    check(hipStreamWaitEvent(s0, b->join[i], 0));
+                }
+                joined = i + 1;
+            }
+        }
+        catch(...) {
+            // join whatever branches were opened, then end the capture
+            for(uint32_t i = 1; i < n && i <= joined; ++i) {
+                (void)hipEventRecord(b->join[i], b->ctxs[i]->stream);
+                (void)hipStreamWaitEvent(s0, b->join[i], 0);
+            }
+            (void)hipStreamEndCapture(s0, &g);
+            if(g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        check(hipStreamEndCapture(s0, &g));
+        update_exec(&b->exec, g);
+        check(hipGraphLaunch(b->exec, s0));
+        check(hipEventRecord(b->done, s0));
+        for(uint32_t i = 0; i < n; ++i) {
+            rgrt_ctx* c = b->ctxs[i];
+            // the staging buffers are free and the scenes' later work is
+            // ordered once the batch has run
+            check(hipEventRecord(c->stageDone[c->stageIdx], s0));
+            if(i) check(hipStreamWaitEvent(c->stream, b->done, 0));
+            frame_finish(c, descs[i].instances, descs[i].numInstances, descs[i].flags);
+            c->timedFrame = false;
+        }
+    }
+    catch(const std::exception& e) {
+        for(uint32_t i = 0; i < n; ++i) {
+            b->ctxs[i]->numInstances = prevN[i];
+            b->ctxs[i]->haveTlas = prevHave[i];
+        }
+        b->err = e.what();
+        return RGRT_ERR_HIP;
+    }
+    return RGRT_OK;
 }
 
 static bool image_info(rgrt_ctx* c, rgrt_image which, void** ptr, uint64_t* bytes)

@@ -1,0 +1,54 @@
+"""hipEvent timing on a context's stream through libamdhip64 (ctypes)."""
+import ctypes
+
+
+class HipEvents:
+    """hipEvent pairs: create(), record(event, stream), elapsed_ms(a, b)."""
+
+    def __init__(self):
+        self.L = ctypes.CDLL("libamdhip64.so")
+        self.L.hipEventCreate.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
+        self.L.hipEventRecord.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        self.L.hipEventSynchronize.argtypes = [ctypes.c_void_p]
+        self.L.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_void_p, ctypes.c_void_p]
+        self.L.hipEventDestroy.argtypes = [ctypes.c_void_p]
+
+    def create(self):
+        e = ctypes.c_void_p()
+        if self.L.hipEventCreate(ctypes.byref(e)) != 0:
+            raise RuntimeError("hipEventCreate failed")
+        return e
+
+    def record(self, e, stream):
+        if self.L.hipEventRecord(e, ctypes.c_void_p(stream)) != 0:
+            raise RuntimeError("hipEventRecord failed")
+
+    def elapsed_ms(self, a, b):
+        if self.L.hipEventSynchronize(b) != 0:
+            raise RuntimeError("hipEventSynchronize failed")
+        ms = ctypes.c_float()
+        if self.L.hipEventElapsedTime(ctypes.byref(ms), a, b) != 0:
+            raise RuntimeError("hipEventElapsedTime failed")
+        return ms.value
+
+    def destroy(self, e):
+        self.L.hipEventDestroy(e)
+
+
+def time_ms(ctx, fn, reps=3, warmup=1, events=None):
+    """Mean milliseconds of fn() between hipEvents on ctx's stream."""
+    from . import rgrt
+    ev = events or HipEvents()
+    stream = rgrt.lib().rgrt_stream(ctx.h)
+    for _ in range(warmup):
+        fn()
+    a, b = ev.create(), ev.create()
+    ctx.synchronize()
+    ev.record(a, stream)
+    for _ in range(reps):
+        fn()
+    ev.record(b, stream)
+    ms = ev.elapsed_ms(a, b) / reps
+    ev.destroy(a)
+    ev.destroy(b)
+    return ms

@@ -42,7 +42,8 @@ EXPORTS = [
     "rgrt_set_tuning", "rgrt_get_tuning", "rgrt_set_sampling", "rgrt_denoiser_create", "rgrt_denoiser_destroy",
     "rgrt_denoiser_num_params", "rgrt_denoiser_get_params", "rgrt_denoiser_set_params", "rgrt_denoiser_loss_and_grads",
     "rgrt_denoiser_dataset_reserve", "rgrt_denoiser_dataset_add_input", "rgrt_denoiser_dataset_add_target",
-    "rgrt_denoiser_train", "rgrt_denoise", "rgrt_export_blas_wide", "rgrt_export_tlas_wide", "rgrt_version",
+    "rgrt_denoiser_train", "rgrt_denoise", "rgrt_batch_create", "rgrt_batch_destroy", "rgrt_batch_last_error",
+    "rgrt_batch_frame", "rgrt_export_blas_wide", "rgrt_export_tlas_wide", "rgrt_version",
 ]
 
 
@@ -122,6 +123,10 @@ def lib():
             "rgrt_denoiser_dataset_add_target": (i32, [P, u32, ctypes.c_float, i32]),
             "rgrt_denoiser_train": (i32, [P, u32, u32, ctypes.c_float, u32, u32, ctypes.POINTER(ctypes.c_float)]),
             "rgrt_denoise": (i32, [P, ctypes.c_float, ctypes.c_float, ctypes.POINTER(u32), ctypes.POINTER(u32)]),
+            "rgrt_batch_create": (i32, [P, u32, ctypes.POINTER(P)]),
+            "rgrt_batch_destroy": (None, [P]),
+            "rgrt_batch_last_error": (ctypes.c_char_p, [P]),
+            "rgrt_batch_frame": (i32, [P, P]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -421,3 +426,44 @@ class Denoiser:
         f, t = ctypes.c_uint32(), ctypes.c_uint32()
         self.ctx._ok(self._L.rgrt_denoise(self.h, fallback_a, fallback_b, ctypes.byref(f), ctypes.byref(t)))
         return f.value, t.value
+
+
+class _FrameDesc(ctypes.Structure):
+    _fields_ = [("ubo", ctypes.c_void_p), ("instances", ctypes.c_void_p), ("numInstances", ctypes.c_uint32),
+                ("flags", ctypes.c_uint32)]
+
+
+class Batch:
+    """Frames of several contexts as one hipGraph (rgrt_batch_*)."""
+
+    def __init__(self, contexts):
+        self._L = lib()
+        self.contexts = list(contexts)
+        arr = (ctypes.c_void_p * len(self.contexts))(*[c.h.value for c in self.contexts])
+        self.h = ctypes.c_void_p()
+        st = self._L.rgrt_batch_create(arr, len(self.contexts), ctypes.byref(self.h))
+        if st != 0:
+            raise RgrtError(st, "rgrt_batch_create failed")
+
+    def close(self):
+        if self.h:
+            self._L.rgrt_batch_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def frame(self, frames):
+        """frames: per context (ubo, instances, fxaa, refit)."""
+        keep = []
+        descs = (_FrameDesc * len(self.contexts))()
+        for d, (ubo, inst, fxaa, refit) in zip(descs, frames):
+            u = np.ascontiguousarray(ubo, UBO)
+            i = np.ascontiguousarray(inst, INSTANCE)
+            keep += [u, i]
+            d.ubo = u.ctypes.data
+            d.instances = i.ctypes.data if len(i) else None
+            d.numInstances = len(i)
+            d.flags = (FRAME_FXAA if fxaa else 0) | (FRAME_REFIT_TLAS if refit else 0)
+        st = self._L.rgrt_batch_frame(self.h, descs)
+        if st != 0:
+            raise RgrtError(st, self._L.rgrt_batch_last_error(self.h).decode())
+        for c, (_, inst, _, _) in zip(self.contexts, frames):
+            c._ninst = len(inst)
