@@ -88,6 +88,8 @@ def main():
     ap.add_argument("--no-fxaa", action="store_true")
     ap.add_argument("--mode", choices=("eager", "graph"), default="eager",
                     help="frame submission: eager calls, or rgrt_frame as one hipGraph launch")
+    ap.add_argument("--batch-scenes", type=int, default=32,
+                    help="scenes of the batched workload measured after the headline (0 = skip)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="run only the counter-calibration copy kernel over MIB MiB")
     args = ap.parse_args()
@@ -288,7 +290,61 @@ def main():
         "cpu_baseline": cpu,
     }
     ctx.close()
+    if args.batch_scenes:
+        try:
+            result["batched"] = bench_batched(args.batch_scenes, max(3, args.steps // 4))
+        except Exception as e:  # the headline stands without the batched figures
+            log(f"[bench] batched workload failed: {e!r}")
     print(json.dumps(result), flush=True)
+
+
+def bench_batched(num_scenes, frames):
+    """Config 3: num_scenes scenes of log-normal size (10k-4M triangles) and
+    mixed resolution, animated; per-scene autotuned launch shapes; batched
+    frames (one hipGraph, a branch per scene) against the same frames
+    submitted scene by scene."""
+    from raygun_amd import autotune, batched, rgrt
+    from raygun_amd.hiptime import HipEvents
+    t0 = time.time()
+    w = batched.BatchWorkload(num_scenes=num_scenes, seed=3)
+    log(f"[bench] batched workload: {num_scenes} scenes, {sum(w.info['triangles'])} triangles, "
+        f"{w.info['pixels']} pixels, built in {time.time() - t0:.1f}s")
+    try:
+        cand = {"wavefrontPasses": [2, 4, 6, 8], "traceBlocksPerCU": [4, 6, 8]}
+        t1 = time.time()
+        tunings = []
+        for sc in w.scenes:
+            inst = sc.instances(0.0)
+            best, ms, _ = autotune.tune(sc.ctx, lambda: sc.ctx.frame(sc.ubo, inst, fxaa=True), candidates=cand, reps=1)
+            tunings.append(best)
+        log(f"[bench] autotuned {num_scenes} scenes in {time.time() - t1:.1f}s")
+        ev = HipEvents()
+        stream0 = rgrt.lib().rgrt_stream(w.scenes[0].ctx.h)
+        w.frame_batched(0.0)  # first frame builds the TLASes
+        w.synchronize()
+        a, b = ev.create(), ev.create()
+        ev.record(a, stream0)
+        for f in range(frames):
+            w.frame_batched(0.05 * (f + 1))
+        ev.record(b, stream0)
+        ms_batched = ev.elapsed_ms(a, b) / frames
+        w.synchronize()
+        rays = w.rays()
+        t2 = time.perf_counter()
+        for f in range(frames):
+            w.frame_sequential(0.05 * (f + 1 + frames))
+        w.synchronize()
+        ms_seq_wall = (time.perf_counter() - t2) * 1e3 / frames
+        return {"workload": f"config3: {num_scenes} scenes, triangles log-normal in [10k, 4M], resolutions 320x180.."
+                            "1920x1080, animated instances (TLAS refit), one scene in four deforming (BLAS refit)",
+                "scenes": num_scenes, "triangles": w.info["triangles"], "resolutions": w.info["resolutions"],
+                "pixels_per_frame": w.info["pixels"], "rays_per_frame": rays, "frames": frames,
+                "ms_per_batched_frame": round(ms_batched, 3),
+                "mrays_per_s_batched": round(rays / (ms_batched * 1e-3) / 1e6, 1),
+                "ms_per_sequential_frame_wall": round(ms_seq_wall, 3),
+                "autotuned": tunings}
+    finally:
+        w.close()
 
 
 if __name__ == "__main__":
