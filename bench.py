@@ -86,8 +86,11 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-rows", type=int, default=0, help="rows of the frame the CPU baseline traces (0 = auto)")
     ap.add_argument("--no-fxaa", action="store_true")
-    ap.add_argument("--mode", choices=("eager", "graph"), default="eager",
+    ap.add_argument("--mode", choices=("eager", "graph"), default="graph",
                     help="frame submission: eager calls, or rgrt_frame as one hipGraph launch")
+    ap.add_argument("--in-flight", type=int, choices=(1, 2), default=2,
+                    help="frames in flight: 2 alternates frames between two contexts sharing the scene "
+                         "(rgrt_create_shared), so one frame's tail overlaps the next frame's start")
     ap.add_argument("--batch-scenes", type=int, default=32,
                     help="scenes of the batched workload measured after the headline (0 = skip)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -137,26 +140,53 @@ def main():
 
     ev = HipEvents()
     stream = rgrt.lib().rgrt_stream(ctx.h)
+    # the second frame in flight: same scene, its own TLAS, images and stream
+    ctx2 = ctx.shared()
+    stream2 = rgrt.lib().rgrt_stream(ctx2.h)
 
-    def timed(step, steps, warmup):
-        for _ in range(warmup):
-            step()
+    def eager_step2():
+        ctx2.build_tlas(instances)
+        ctx2.set_frame(ubo)
+        ctx2.render(use_fxaa)
+
+    def graph_step2():
+        ctx2.frame(ubo, instances, fxaa=use_fxaa, graph=True)
+
+    def timed(steps_pair, steps, warmup):
+        """Mean ms per frame of `steps` frames, frame k on steps_pair[k % len];
+        hipEvents on the first context's stream bracket every frame (the
+        second stream waits for the opening event; the closing time is the
+        later of the two streams' closing events)."""
+        n = len(steps_pair)
+        for k in range(warmup):
+            steps_pair[k % n]()
         ctx.synchronize()
-        e0, e1 = ev.create(), ev.create()
+        ctx2.synchronize()
+        e0, e1, e2 = ev.create(), ev.create(), ev.create()
         ev.record(e0, stream)
-        for _ in range(steps):
-            step()
+        ev.wait(stream2, e0)
+        for k in range(steps):
+            steps_pair[k % n]()
         ev.record(e1, stream)
-        ms = ev.elapsed_ms(e0, e1) / steps
+        ev.record(e2, stream2)
+        ms = max(ev.elapsed_ms(e0, e1), ev.elapsed_ms(e0, e2)) / steps
         ctx.synchronize()
+        ctx2.synchronize()
+        for e in (e0, e1, e2):
+            ev.destroy(e)
         return ms
 
-    steps_of = {"eager": eager_step, "graph": graph_step}
-    other = "graph" if args.mode == "eager" else "eager"
-    ms_other = timed(steps_of[other], max(5, args.steps // 2), 2)
-    ms_per_step = timed(steps_of[args.mode], args.steps, args.warmup)
-    frame_ms = {args.mode: round(ms_per_step, 4), other: round(ms_other, 4)}
+    variants = {"eager": [eager_step], "graph": [graph_step],
+                "eager_2_in_flight": [eager_step, eager_step2], "graph_2_in_flight": [graph_step, graph_step2]}
+    chosen = args.mode if args.in_flight == 1 else f"{args.mode}_2_in_flight"
+    frame_ms = {}
+    for name, fns in variants.items():
+        if name != chosen:
+            frame_ms[name] = round(timed(fns, max(6, args.steps // 2), 2), 4)
+    ms_per_step = timed(variants[chosen], args.steps, args.warmup)
+    frame_ms[chosen] = round(ms_per_step, 4)
     log(f"[bench] frame ms by submission mode: {frame_ms}")
+    ctx2.close()
     st = ctx.stats()
     rays = st["rays"]
     value = rays / (ms_per_step * 1e-3) / 1e6
@@ -285,7 +315,7 @@ def main():
                    "stage_ms": {k: round(ts[k], 4) for k in ("asBuildMs", "rtOnlyMs", "roughMs", "postprocMs", "rtTotalMs",
                                                              "traceKernelMs", "shadeKernelMs", "drainKernelMs")},
                    "tuning": ctx.tuning(), "wavefront_pass_rays": passes.tolist(),
-                   "submission": args.mode, "frame_ms_by_submission": frame_ms},
+                   "submission": chosen, "frames_in_flight": args.in_flight, "frame_ms_by_submission": frame_ms},
         "roofline": roof,
         "cpu_baseline": cpu,
     }

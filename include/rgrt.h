@@ -178,6 +178,19 @@ typedef struct rgrt_config {
 /* Replaces the ray-tracing parts of VulkanContext() and Raytracer()
  * (raygun/vulkan_context.cpp:48-69, raygun/render/raytracer.cpp:38-52). */
 rgrt_status rgrt_create(rgrt_ctx** out, const rgrt_config* cfg);
+/* No counterpart in the reference (it renders one frame at a time; its
+ * frames in flight are swapchain images over one scene). A second context on
+ * the same device that shares the parent's geometry, materials and BLASes
+ * (handles are the same in both) with its own TLAS, images, wavefront state
+ * and stream: frames rendered alternately on the two overlap on the GPU.
+ * Every call that changes the shared scene (upload_geometry,
+ * upload_materials, update_vertices, build_blas, refit_blas) first waits
+ * for the work queued on every sharing context; new geometry drops every
+ * sharing context's TLAS. Calls on sharing contexts must not run
+ * concurrently from several threads while one changes the scene. The scene
+ * is freed with the last context that holds it; contexts may be destroyed
+ * in any order. */
+rgrt_status rgrt_create_shared(rgrt_ctx* parent, const rgrt_config* cfg, rgrt_ctx** out);
 void rgrt_destroy(rgrt_ctx* ctx);
 const char* rgrt_last_error(const rgrt_ctx* ctx);
 /* Stream all work of the context is ordered on (hipStream_t as void*). */

@@ -1,6 +1,9 @@
 // api.hip — the C ABI (include/rgrt.h) over the HIP kernels.
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -44,6 +47,56 @@ void dev_alloc(T*& p, size_t count)
     check(hipMalloc((void**)&p, std::max<size_t>(count, 1) * sizeof(T)));
 }
 
+} // namespace
+struct rgrt_ctx;
+namespace {
+
+// Geometry, materials and BLASes. Contexts created with rgrt_create_shared
+// share one SceneData (each keeps its own TLAS, images, wavefront state and
+// stream, so their frames overlap on the GPU); a change of the scene first
+// waits for every sharing context's stream.
+struct SceneData {
+    rgrt_vertex* vb = nullptr;
+    uint64_t nv = 0;
+    uint32_t* ib = nullptr;
+    uint64_t ni = 0;
+    rgrt_material* mats = nullptr;
+    uint32_t nm = 0;
+    std::vector<BlasRec> blas;
+    DevBlas* dBlasTable = nullptr;
+    float4* dBlasRoot = nullptr;
+    uint32_t blasCap = 0;
+    std::mutex m;
+    std::vector<rgrt_ctx*> users; // the contexts sharing it
+
+    ~SceneData()
+    {
+        for(auto& b: blas) {
+            dev_free(b.nodes);
+            dev_free(b.wide);
+            dev_free(b.wideq);
+            dev_free(b.numWide);
+            dev_free(b.tris);
+            dev_free(b.leafPrim);
+        }
+        dev_free(vb);
+        dev_free(ib);
+        dev_free(mats);
+        dev_free(dBlasTable);
+        dev_free(dBlasRoot);
+    }
+    void add_user(rgrt_ctx* c)
+    {
+        std::lock_guard<std::mutex> g(m);
+        users.push_back(c);
+    }
+    void remove_user(rgrt_ctx* c)
+    {
+        std::lock_guard<std::mutex> g(m);
+        users.erase(std::remove(users.begin(), users.end(), c), users.end());
+    }
+};
+
 // Bound on the rays one pixel can trace one after another for one sample
 // when every material reflects and refracts with rayConsumption 1:
 // B(r) = (R - r) shadow-chain rays + 2 * (1 + B(r + 1)), B(R) = 0.
@@ -63,17 +116,7 @@ struct rgrt_ctx {
     std::string err;
     uint32_t numCUs = 256;
 
-    rgrt_vertex* vb = nullptr;
-    uint64_t nv = 0;
-    uint32_t* ib = nullptr;
-    uint64_t ni = 0;
-    rgrt_material* mats = nullptr;
-    uint32_t nm = 0;
-
-    std::vector<BlasRec> blas;
-    DevBlas* dBlasTable = nullptr;
-    float4* dBlasRoot = nullptr;
-    uint32_t blasCap = 0;
+    std::shared_ptr<SceneData> sd; // geometry, materials, BLASes (possibly shared)
 
     uint32_t instCap = 0;
     uint32_t numInstances = 0;
@@ -163,10 +206,10 @@ struct rgrt_ctx {
         TraceScene sc;
         sc.tlasNodes = tlasWideq;
         sc.instances = dInst;
-        sc.blas = dBlasTable;
+        sc.blas = sd->dBlasTable;
         sc.numInstances = haveTlas ? numInstances : 0;
         sc.tlasNodeCap = std::max<uint32_t>(sc.numInstances, 2u) - 1u;
-        sc.numBlas = (uint32_t)blas.size();
+        sc.numBlas = (uint32_t)sd->blas.size();
         sc.errors = st.counters + kTraversalErrorWord;
         return sc;
     }
@@ -178,6 +221,21 @@ rgrt_status fail(rgrt_ctx* c, rgrt_status code, const std::string& msg)
 {
     if(c) c->err = msg;
     return code;
+}
+
+// Before the scene data changes: the work queued on every context that
+// shares it (frames in flight included) has finished.
+void scene_quiesce(rgrt_ctx* c)
+{
+    std::lock_guard<std::mutex> g(c->sd->m);
+    for(rgrt_ctx* u: c->sd->users) check(hipStreamSynchronize(u->stream));
+}
+
+// New geometry invalidates every BLAS, so every sharing context's TLAS.
+void scene_drop_tlases(rgrt_ctx* c)
+{
+    std::lock_guard<std::mutex> g(c->sd->m);
+    for(rgrt_ctx* u: c->sd->users) u->haveTlas = false;
 }
 
 // The traversal kernels flag tree or table indices out of range (a corrupt
@@ -281,22 +339,22 @@ void alloc_images(rgrt_ctx* c, uint32_t w, uint32_t h)
 
 void ensure_blas_cap(rgrt_ctx* c, uint32_t n)
 {
-    if(n <= c->blasCap) return;
-    const uint32_t cap = std::max<uint32_t>(n, std::max<uint32_t>(16, c->blasCap * 2));
+    if(n <= c->sd->blasCap) return;
+    const uint32_t cap = std::max<uint32_t>(n, std::max<uint32_t>(16, c->sd->blasCap * 2));
     DevBlas* t = nullptr;
     float4* r = nullptr;
     check(hipMalloc((void**)&t, cap * sizeof(DevBlas)));
     check(hipMalloc((void**)&r, cap * 2 * sizeof(float4)));
-    if(c->blasCap) {
-        check(hipMemcpyAsync(t, c->dBlasTable, c->blasCap * sizeof(DevBlas), hipMemcpyDeviceToDevice, c->stream));
-        check(hipMemcpyAsync(r, c->dBlasRoot, c->blasCap * 2 * sizeof(float4), hipMemcpyDeviceToDevice, c->stream));
+    if(c->sd->blasCap) {
+        check(hipMemcpyAsync(t, c->sd->dBlasTable, c->sd->blasCap * sizeof(DevBlas), hipMemcpyDeviceToDevice, c->stream));
+        check(hipMemcpyAsync(r, c->sd->dBlasRoot, c->sd->blasCap * 2 * sizeof(float4), hipMemcpyDeviceToDevice, c->stream));
         check(hipStreamSynchronize(c->stream));
     }
-    dev_free(c->dBlasTable);
-    dev_free(c->dBlasRoot);
-    c->dBlasTable = t;
-    c->dBlasRoot = r;
-    c->blasCap = cap;
+    dev_free(c->sd->dBlasTable);
+    dev_free(c->sd->dBlasRoot);
+    c->sd->dBlasTable = t;
+    c->sd->dBlasRoot = r;
+    c->sd->blasCap = cap;
 }
 
 void ensure_inst_cap(rgrt_ctx* c, uint32_t n)
@@ -354,14 +412,14 @@ static_assert(sizeof(rgrt_stats) == 80, "rgrt_stats layout (Python binding: reco
 ShadeParams shade_params(const rgrt_ctx* c)
 {
     ShadeParams p;
-    p.vb = c->vb;
-    p.ib = c->ib;
-    p.mats = c->mats;
+    p.vb = c->sd->vb;
+    p.ib = c->sd->ib;
+    p.mats = c->sd->mats;
     p.inst = c->dShade;
     p.numInstances = c->numInstances;
-    p.numVertices = c->nv;
-    p.numIndices = c->ni;
-    p.numMaterials = c->nm;
+    p.numVertices = c->sd->nv;
+    p.numIndices = c->sd->ni;
+    p.numMaterials = c->sd->nm;
     p.ubo = c->ubo;
     p.width = c->w;
     p.height = c->h;
@@ -516,7 +574,7 @@ rgrt_status validate_instances(rgrt_ctx* c, const rgrt_instance* in, uint32_t n,
 {
     if(n && !in) return fail(c, RGRT_ERR_INVALID_ARG, std::string(who) + ": null instances");
     for(uint32_t i = 0; i < n; ++i) {
-        if(in[i].blas >= c->blas.size() || !c->blas[in[i].blas].valid)
+        if(in[i].blas >= c->sd->blas.size() || !c->sd->blas[in[i].blas].valid)
             return fail(c, RGRT_ERR_INVALID_ARG, std::string(who) + ": instance references an invalid BLAS");
         if(in[i].customIndex >= n)
             return fail(c, RGRT_ERR_INVALID_ARG, std::string(who) + ": customIndex must index the offset table");
@@ -565,10 +623,10 @@ rgrt_status stage_instances(rgrt_ctx* c, const rgrt_instance* in, uint32_t n, bo
 void enqueue_tlas(rgrt_ctx* c, uint32_t n, bool refit)
 {
     hipStream_t s = c->stream;
-    if(launch_tlas_fused(s, c->dInstIn, n, c->dBlasRoot, c->dW2O, c->dShade, c->tlasNodes, c->tlasLeaf, c->tlasRoot,
+    if(launch_tlas_fused(s, c->dInstIn, n, c->sd->dBlasRoot, c->dW2O, c->dShade, c->tlasNodes, c->tlasLeaf, c->tlasRoot,
                          c->tlasWide, c->tlasWideq, c->tlasNumWide, c->dInst, refit))
         return;
-    launch_inst_prep(s, c->dInstIn, n, c->dBlasRoot, c->builder.boxLo, c->builder.boxHi, c->dW2O, c->dShade);
+    launch_inst_prep(s, c->dInstIn, n, c->sd->dBlasRoot, c->builder.boxLo, c->builder.boxHi, c->dW2O, c->dShade);
     if(refit) c->builder.refit(s, n, c->tlasNodes, c->tlasLeaf, c->tlasRoot);
     else c->builder.build(s, n, c->tlasNodes, c->tlasLeaf, c->tlasRoot);
     c->builder.collapse(s, n, c->tlasNodes, c->tlasWide, c->tlasNumWide);
@@ -602,6 +660,8 @@ rgrt_status rgrt_create(rgrt_ctx** out, const rgrt_config* cfg)
     auto* c = new rgrt_ctx();
     c->device = cfg->device;
     c->flags = cfg->flags;
+    c->sd = std::make_shared<SceneData>();
+    c->sd->add_user(c);
     rgrt_status st = guarded(c, [&]() {
         check(hipSetDevice(c->device));
         check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
@@ -621,25 +681,30 @@ rgrt_status rgrt_create(rgrt_ctx** out, const rgrt_config* cfg)
     return RGRT_OK;
 }
 
+rgrt_status rgrt_create_shared(rgrt_ctx* parent, const rgrt_config* cfg, rgrt_ctx** out)
+{
+    if(!out) return RGRT_ERR_INVALID_ARG;
+    *out = nullptr;
+    if(!parent || !cfg) return RGRT_ERR_INVALID_ARG;
+    if(cfg->device != parent->device) return fail(parent, RGRT_ERR_INVALID_ARG, "create_shared: other device");
+    rgrt_ctx* c = nullptr;
+    const rgrt_status st = rgrt_create(&c, cfg);
+    if(st != RGRT_OK) return st;
+    c->sd->remove_user(c);
+    c->sd = parent->sd;
+    c->sd->add_user(c);
+    *out = c;
+    return RGRT_OK;
+}
+
 void rgrt_destroy(rgrt_ctx* c)
 {
     if(!c) return;
     (void)hipSetDevice(c->device);
     if(c->stream) (void)hipStreamSynchronize(c->stream);
     free_images(c);
-    for(auto& b: c->blas) {
-        dev_free(b.nodes);
-        dev_free(b.wide);
-        dev_free(b.wideq);
-        dev_free(b.numWide);
-        dev_free(b.tris);
-        dev_free(b.leafPrim);
-    }
-    dev_free(c->vb);
-    dev_free(c->ib);
-    dev_free(c->mats);
-    dev_free(c->dBlasTable);
-    dev_free(c->dBlasRoot);
+    c->sd->remove_user(c);
+    c->sd.reset(); // the last context sharing the scene frees it
     dev_free(c->dInstIn);
     dev_free(c->tlasNodes);
     dev_free(c->tlasWide);
@@ -687,15 +752,15 @@ rgrt_status rgrt_upload_geometry(rgrt_ctx* c, const rgrt_vertex* vertices, uint6
     return guarded(c, [&]() {
         if((nv && !vertices) || (ni && !indices) || ni % 3) return fail(c, RGRT_ERR_INVALID_ARG, "upload_geometry: bad arrays");
         check(hipSetDevice(c->device));
-        check(hipStreamSynchronize(c->stream));
-        dev_alloc(c->vb, nv);
-        dev_alloc(c->ib, ni);
-        if(nv) check(hipMemcpy(c->vb, vertices, nv * sizeof(rgrt_vertex), hipMemcpyHostToDevice));
-        if(ni) check(hipMemcpy(c->ib, indices, ni * 4, hipMemcpyHostToDevice));
-        c->nv = nv;
-        c->ni = ni;
-        for(auto& b: c->blas) b.valid = false; // geometry changed: every BLAS must be rebuilt
-        c->haveTlas = false;
+        scene_quiesce(c);
+        dev_alloc(c->sd->vb, nv);
+        dev_alloc(c->sd->ib, ni);
+        if(nv) check(hipMemcpy(c->sd->vb, vertices, nv * sizeof(rgrt_vertex), hipMemcpyHostToDevice));
+        if(ni) check(hipMemcpy(c->sd->ib, indices, ni * 4, hipMemcpyHostToDevice));
+        c->sd->nv = nv;
+        c->sd->ni = ni;
+        for(auto& b: c->sd->blas) b.valid = false; // geometry changed: every BLAS must be rebuilt
+        scene_drop_tlases(c);
         return RGRT_OK;
     });
 }
@@ -703,9 +768,10 @@ rgrt_status rgrt_upload_geometry(rgrt_ctx* c, const rgrt_vertex* vertices, uint6
 rgrt_status rgrt_update_vertices(rgrt_ctx* c, uint64_t first, const rgrt_vertex* vertices, uint64_t count)
 {
     return guarded(c, [&]() {
-        if(first + count > c->nv || (count && !vertices)) return fail(c, RGRT_ERR_INVALID_ARG, "update_vertices: range");
+        if(first + count > c->sd->nv || (count && !vertices)) return fail(c, RGRT_ERR_INVALID_ARG, "update_vertices: range");
+        scene_quiesce(c);
         if(count)
-            check(hipMemcpyAsync(c->vb + first, vertices, count * sizeof(rgrt_vertex), hipMemcpyHostToDevice, c->stream));
+            check(hipMemcpyAsync(c->sd->vb + first, vertices, count * sizeof(rgrt_vertex), hipMemcpyHostToDevice, c->stream));
         check(hipStreamSynchronize(c->stream));
         return RGRT_OK;
     });
@@ -715,10 +781,10 @@ rgrt_status rgrt_upload_materials(rgrt_ctx* c, const rgrt_material* materials, u
 {
     return guarded(c, [&]() {
         if(n && !materials) return fail(c, RGRT_ERR_INVALID_ARG, "upload_materials: null");
-        check(hipStreamSynchronize(c->stream));
-        dev_alloc(c->mats, n);
-        if(n) check(hipMemcpy(c->mats, materials, n * sizeof(rgrt_material), hipMemcpyHostToDevice));
-        c->nm = n;
+        scene_quiesce(c);
+        dev_alloc(c->sd->mats, n);
+        if(n) check(hipMemcpy(c->sd->mats, materials, n * sizeof(rgrt_material), hipMemcpyHostToDevice));
+        c->sd->nm = n;
         return RGRT_OK;
     });
 }
@@ -729,18 +795,19 @@ rgrt_status rgrt_build_blas(rgrt_ctx* c, const rgrt_blas_desc* d, rgrt_blas* out
         if(!d || !out) return fail(c, RGRT_ERR_INVALID_ARG, "build_blas: null");
         if(d->indexOffsetBytes % 4) return fail(c, RGRT_ERR_INVALID_ARG, "build_blas: index offset not 4-aligned");
         const uint64_t io = d->indexOffsetBytes / 4;
-        if(io + 3ull * d->numFaces > c->ni) return fail(c, RGRT_ERR_INVALID_ARG, "build_blas: index range outside the buffer");
-        if((uint64_t)d->firstVertex + d->maxVertex > c->nv)
+        if(io + 3ull * d->numFaces > c->sd->ni) return fail(c, RGRT_ERR_INVALID_ARG, "build_blas: index range outside the buffer");
+        if((uint64_t)d->firstVertex + d->maxVertex > c->sd->nv)
             return fail(c, RGRT_ERR_INVALID_ARG, "build_blas: vertex range outside the buffer");
         if(d->numFaces && d->maxVertex == 0) return fail(c, RGRT_ERR_INVALID_ARG, "build_blas: maxVertex is 0");
         // every index must address a vertex of the range
         std::vector<uint32_t> idx(3ull * d->numFaces);
-        if(!idx.empty()) check(hipMemcpy(idx.data(), c->ib + io, idx.size() * 4, hipMemcpyDeviceToHost));
+        if(!idx.empty()) check(hipMemcpy(idx.data(), c->sd->ib + io, idx.size() * 4, hipMemcpyDeviceToHost));
         for(uint32_t v: idx)
             if(v >= d->maxVertex) return fail(c, RGRT_ERR_INVALID_ARG, "build_blas: index >= maxVertex");
 
+        scene_quiesce(c);
         hipStream_t s = c->stream;
-        const uint32_t id = (uint32_t)c->blas.size();
+        const uint32_t id = (uint32_t)c->sd->blas.size();
         ensure_blas_cap(c, id + 1);
         BlasRec b;
         b.desc = *d;
@@ -752,16 +819,16 @@ rgrt_status rgrt_build_blas(rgrt_ctx* c, const rgrt_blas_desc* d, rgrt_blas* out
         dev_alloc(b.tris, n);
         dev_alloc(b.leafPrim, n);
         c->builder.reserve(std::max<uint32_t>(n, 1));
-        launch_tri_boxes(s, c->vb, c->ib, d->firstVertex, (uint32_t)io, n, c->builder.boxLo, c->builder.boxHi);
-        c->builder.build(s, n, b.nodes, b.leafPrim, c->dBlasRoot + 2 * id);
+        launch_tri_boxes(s, c->sd->vb, c->sd->ib, d->firstVertex, (uint32_t)io, n, c->builder.boxLo, c->builder.boxHi);
+        c->builder.build(s, n, b.nodes, b.leafPrim, c->sd->dBlasRoot + 2 * id);
         c->builder.collapse(s, n, b.nodes, b.wide, b.numWide);
         c->builder.quantize(s, std::max<uint32_t>(n, 2) - 1, b.wide, b.numWide, b.wideq);
-        launch_tri_gather(s, c->vb, c->ib, d->firstVertex, (uint32_t)io, b.leafPrim, n, b.tris);
+        launch_tri_gather(s, c->sd->vb, c->sd->ib, d->firstVertex, (uint32_t)io, b.leafPrim, n, b.tris);
         DevBlas db{b.wideq, b.tris, n, 0};
-        check(hipMemcpyAsync(c->dBlasTable + id, &db, sizeof(DevBlas), hipMemcpyHostToDevice, s));
+        check(hipMemcpyAsync(c->sd->dBlasTable + id, &db, sizeof(DevBlas), hipMemcpyHostToDevice, s));
         check(hipStreamSynchronize(s)); // the reference blocks on a fence (raytracer.cpp:71-73)
         b.valid = true;
-        c->blas.push_back(b);
+        c->sd->blas.push_back(b);
         *out = id;
         return RGRT_OK;
     });
@@ -770,17 +837,18 @@ rgrt_status rgrt_build_blas(rgrt_ctx* c, const rgrt_blas_desc* d, rgrt_blas* out
 rgrt_status rgrt_refit_blas(rgrt_ctx* c, rgrt_blas id)
 {
     return guarded(c, [&]() {
-        if(id >= c->blas.size() || !c->blas[id].valid) return fail(c, RGRT_ERR_INVALID_ARG, "refit_blas: bad handle");
-        BlasRec& b = c->blas[id];
+        if(id >= c->sd->blas.size() || !c->sd->blas[id].valid) return fail(c, RGRT_ERR_INVALID_ARG, "refit_blas: bad handle");
+        scene_quiesce(c);
+        BlasRec& b = c->sd->blas[id];
         hipStream_t s = c->stream;
         const uint32_t n = b.desc.numFaces;
         const uint32_t io = b.desc.indexOffsetBytes / 4;
         c->builder.reserve(std::max<uint32_t>(n, 1));
-        launch_tri_boxes(s, c->vb, c->ib, b.desc.firstVertex, io, n, c->builder.boxLo, c->builder.boxHi);
-        c->builder.refit(s, n, b.nodes, b.leafPrim, c->dBlasRoot + 2 * id);
+        launch_tri_boxes(s, c->sd->vb, c->sd->ib, b.desc.firstVertex, io, n, c->builder.boxLo, c->builder.boxHi);
+        c->builder.refit(s, n, b.nodes, b.leafPrim, c->sd->dBlasRoot + 2 * id);
         c->builder.collapse(s, n, b.nodes, b.wide, b.numWide); // wide topology follows the new areas
         c->builder.quantize(s, std::max<uint32_t>(n, 2) - 1, b.wide, b.numWide, b.wideq);
-        launch_tri_gather(s, c->vb, c->ib, b.desc.firstVertex, io, b.leafPrim, n, b.tris);
+        launch_tri_gather(s, c->sd->vb, c->sd->ib, b.desc.firstVertex, io, b.leafPrim, n, b.tris);
         check(hipStreamSynchronize(s));
         return RGRT_OK;
     });
@@ -789,13 +857,13 @@ rgrt_status rgrt_refit_blas(rgrt_ctx* c, rgrt_blas id)
 rgrt_status rgrt_blas_info(rgrt_ctx* c, rgrt_blas id, uint32_t* numNodes, uint32_t* numLeaves, float rootLo[3], float rootHi[3])
 {
     return guarded(c, [&]() {
-        if(id >= c->blas.size() || !c->blas[id].valid) return fail(c, RGRT_ERR_INVALID_ARG, "blas_info: bad handle");
-        const uint32_t n = c->blas[id].desc.numFaces;
+        if(id >= c->sd->blas.size() || !c->sd->blas[id].valid) return fail(c, RGRT_ERR_INVALID_ARG, "blas_info: bad handle");
+        const uint32_t n = c->sd->blas[id].desc.numFaces;
         if(numNodes) *numNodes = std::max<uint32_t>(n, 2) - 1;
         if(numLeaves) *numLeaves = n;
         float4 r[2];
         check(hipStreamSynchronize(c->stream));
-        check(hipMemcpy(r, c->dBlasRoot + 2 * id, sizeof(r), hipMemcpyDeviceToHost));
+        check(hipMemcpy(r, c->sd->dBlasRoot + 2 * id, sizeof(r), hipMemcpyDeviceToHost));
         if(rootLo) {
             rootLo[0] = r[0].x;
             rootLo[1] = r[0].y;
@@ -813,8 +881,8 @@ rgrt_status rgrt_blas_info(rgrt_ctx* c, rgrt_blas id, uint32_t* numNodes, uint32
 rgrt_status rgrt_export_blas(rgrt_ctx* c, rgrt_blas id, rgrt_bvh_node* nodes, uint32_t nodeCap, uint32_t* leafPrim, uint32_t leafCap)
 {
     return guarded(c, [&]() {
-        if(id >= c->blas.size() || !c->blas[id].valid) return fail(c, RGRT_ERR_INVALID_ARG, "export_blas: bad handle");
-        const BlasRec& b = c->blas[id];
+        if(id >= c->sd->blas.size() || !c->sd->blas[id].valid) return fail(c, RGRT_ERR_INVALID_ARG, "export_blas: bad handle");
+        const BlasRec& b = c->sd->blas[id];
         const uint32_t nn = std::max<uint32_t>(b.desc.numFaces, 2) - 1;
         if((nodes && nodeCap < nn) || (leafPrim && leafCap < b.desc.numFaces))
             return fail(c, RGRT_ERR_INVALID_ARG, "export_blas: capacity");
@@ -829,15 +897,15 @@ rgrt_status rgrt_export_blas_wide(rgrt_ctx* c, rgrt_blas id, int32_t compressed,
                                   uint32_t* numNodes)
 {
     return guarded(c, [&]() {
-        if(id >= c->blas.size() || !c->blas[id].valid) return fail(c, RGRT_ERR_INVALID_ARG, "export_blas_wide: bad handle");
+        if(id >= c->sd->blas.size() || !c->sd->blas[id].valid) return fail(c, RGRT_ERR_INVALID_ARG, "export_blas_wide: bad handle");
         check(hipStreamSynchronize(c->stream));
         uint32_t nw = 0;
-        check(hipMemcpy(&nw, c->blas[id].numWide, 4, hipMemcpyDeviceToHost));
+        check(hipMemcpy(&nw, c->sd->blas[id].numWide, 4, hipMemcpyDeviceToHost));
         if(numNodes) *numNodes = nw;
         if(nodes) {
             if(capacity < nw) return fail(c, RGRT_ERR_INVALID_ARG, "export_blas_wide: capacity");
-            if(compressed) check(hipMemcpy(nodes, c->blas[id].wideq, nw * sizeof(Node4q), hipMemcpyDeviceToHost));
-            else check(hipMemcpy(nodes, c->blas[id].wide, nw * sizeof(Node4), hipMemcpyDeviceToHost));
+            if(compressed) check(hipMemcpy(nodes, c->sd->blas[id].wideq, nw * sizeof(Node4q), hipMemcpyDeviceToHost));
+            else check(hipMemcpy(nodes, c->sd->blas[id].wide, nw * sizeof(Node4), hipMemcpyDeviceToHost));
         }
         return RGRT_OK;
     });

@@ -12,6 +12,7 @@ class HipEvents:
         self.L.hipEventSynchronize.argtypes = [ctypes.c_void_p]
         self.L.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_void_p, ctypes.c_void_p]
         self.L.hipEventDestroy.argtypes = [ctypes.c_void_p]
+        self.L.hipStreamWaitEvent.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint]
 
     def create(self):
         e = ctypes.c_void_p()
@@ -22,6 +23,11 @@ class HipEvents:
     def record(self, e, stream):
         if self.L.hipEventRecord(e, ctypes.c_void_p(stream)) != 0:
             raise RuntimeError("hipEventRecord failed")
+
+    def wait(self, stream, e):
+        """Work queued on stream after this call starts after event e."""
+        if self.L.hipStreamWaitEvent(ctypes.c_void_p(stream), e, 0) != 0:
+            raise RuntimeError("hipStreamWaitEvent failed")
 
     def elapsed_ms(self, a, b):
         if self.L.hipEventSynchronize(b) != 0:

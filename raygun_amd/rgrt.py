@@ -33,7 +33,7 @@ STATUS = {0: "OK", 1: "INVALID_ARG", 2: "OUT_OF_MEMORY", 3: "HIP", 4: "NOT_READY
 
 # Every symbol include/rgrt.h declares.
 EXPORTS = [
-    "rgrt_create", "rgrt_destroy", "rgrt_last_error", "rgrt_stream", "rgrt_synchronize",
+    "rgrt_create", "rgrt_create_shared", "rgrt_destroy", "rgrt_last_error", "rgrt_stream", "rgrt_synchronize",
     "rgrt_upload_geometry", "rgrt_upload_materials", "rgrt_update_vertices",
     "rgrt_build_blas", "rgrt_refit_blas", "rgrt_blas_info", "rgrt_export_blas",
     "rgrt_build_tlas", "rgrt_refit_tlas", "rgrt_export_tlas", "rgrt_set_frame", "rgrt_resize",
@@ -78,6 +78,7 @@ def lib():
         P, u32, u64, i32 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int32
         sig = {
             "rgrt_create": (i32, [ctypes.POINTER(P), ctypes.POINTER(_Config)]),
+            "rgrt_create_shared": (i32, [P, ctypes.POINTER(_Config), ctypes.POINTER(P)]),
             "rgrt_destroy": (None, [P]),
             "rgrt_last_error": (ctypes.c_char_p, [P]),
             "rgrt_stream": (P, [P]),
@@ -143,15 +144,27 @@ def _p(a):
 class Context:
     """One rgrt context: device buffers, BLASes, the TLAS and the images."""
 
-    def __init__(self, width=0, height=0, flags=0, device=0):
+    def __init__(self, width=0, height=0, flags=0, device=0, share=None):
+        """share: another Context whose geometry, materials and BLASes this
+        one uses too (rgrt_create_shared); its TLAS, images and stream are
+        its own, so frames on the two overlap on the GPU."""
         self._L = lib()
         self.h = ctypes.c_void_p()
         cfg = _Config(device, width, height, flags)
-        st = self._L.rgrt_create(ctypes.byref(self.h), ctypes.byref(cfg))
+        if share is None:
+            st = self._L.rgrt_create(ctypes.byref(self.h), ctypes.byref(cfg))
+            self._blas_faces = []
+        else:
+            st = self._L.rgrt_create_shared(share.h, ctypes.byref(cfg), ctypes.byref(self.h))
+            self._blas_faces = share._blas_faces
         if st != 0:
-            raise RgrtError(st, "rgrt_create failed")
-        self.width, self.height, self.flags = width, height, flags
-        self._blas_faces = []
+            raise RgrtError(st, "rgrt_create_shared failed" if share is not None else "rgrt_create failed")
+        self.width, self.height, self.flags, self.device = width, height, flags, device
+
+    def shared(self, width=None, height=None, flags=None):
+        """A new context sharing this one's scene."""
+        return Context(self.width if width is None else width, self.height if height is None else height,
+                       self.flags if flags is None else flags, self.device, share=self)
 
     def close(self):
         if getattr(self, "h", None) and self.h.value:
