@@ -33,6 +33,9 @@ TRI_BYTES = 48    # rgrt::Tri
 RAY_BYTES = 40    # queue entry: origin, tmin, dir, tmax, pixel, flags
 HIT_BYTES = 20    # t, u, v, instance, primitive
 GATHER_BYTES = 184  # closest-hit gathers: 12 B indices + 96 B vertices + 64 B material + 12 B offsets
+# launch shapes the autotuner tries for the headline pipeline (coordinate descent)
+TUNE_CANDIDATES = {"wavefrontPasses": [6, 8, 10, 12, 14], "traceBlocksPerCU": [4, 6, 8],
+                   "drainBlocksPerCU": [2, 4, 8], "shadeBlocksPerCU": [4, 8, 12]}
 
 
 def log(msg):
@@ -88,8 +91,10 @@ def main():
     ap.add_argument("--no-fxaa", action="store_true")
     ap.add_argument("--mode", choices=("eager", "graph"), default="graph",
                     help="frame submission: eager calls, or rgrt_frame as one hipGraph launch")
-    ap.add_argument("--in-flight", type=int, choices=(1, 2), default=2,
-                    help="frames in flight: 2 alternates frames between two contexts sharing the scene "
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the library's default launch shape instead of autotuning it for the measured pipeline")
+    ap.add_argument("--in-flight", type=int, choices=(1, 2, 3, 4), default=3,
+                    help="frames in flight: N > 1 rotates frames over N contexts sharing the scene "
                          "(rgrt_create_shared), so one frame's tail overlaps the next frame's start")
     ap.add_argument("--batch-scenes", type=int, default=32,
                     help="scenes of the batched workload measured after the headline (0 = skip)")
@@ -105,7 +110,8 @@ def main():
               "--gpus above 1 is not supported", file=sys.stderr)
         sys.exit(2)
 
-    from raygun_amd import procgen, rgrt
+    from raygun_amd import autotune, procgen, rgrt
+    from raygun_amd.hiptime import frames_ms
     from raygun_amd.scene import pack_models
 
     t0 = time.time()
@@ -130,81 +136,60 @@ def main():
     ubo = setup.ubo()
     use_fxaa = not args.no_fxaa
 
-    def eager_step():
-        ctx.build_tlas(instances)
-        ctx.set_frame(ubo)
-        ctx.render(use_fxaa)
-
-    def graph_step():
-        ctx.frame(ubo, instances, fxaa=use_fxaa, graph=True)
-
     ev = HipEvents()
-    stream = rgrt.lib().rgrt_stream(ctx.h)
-    # the second frame in flight: same scene, its own TLAS, images and stream
-    ctx2 = ctx.shared()
-    stream2 = rgrt.lib().rgrt_stream(ctx2.h)
+    # further frames in flight: contexts on the same scene, each with its own
+    # TLAS, images, wavefront state and stream
+    ctxs = [ctx] + [ctx.shared() for _ in range(max(args.in_flight, 3) - 1)]
 
-    def eager_step2():
-        ctx2.build_tlas(instances)
-        ctx2.set_frame(ubo)
-        ctx2.render(use_fxaa)
+    def eager_on(c):
+        def step():
+            c.build_tlas(instances)
+            c.set_frame(ubo)
+            c.render(use_fxaa)
+        return step
 
-    def graph_step2():
-        ctx2.frame(ubo, instances, fxaa=use_fxaa, graph=True)
+    def graph_on(c):
+        return lambda: c.frame(ubo, instances, fxaa=use_fxaa, graph=True)
 
-    def timed(steps_pair, steps, warmup):
-        """Mean ms per frame of `steps` frames, frame k on steps_pair[k % len];
-        hipEvents on the first context's stream bracket every frame (the
-        second stream waits for the opening event; the closing time is the
-        later of the two streams' closing events)."""
-        n = len(steps_pair)
-        for k in range(warmup):
-            steps_pair[k % n]()
-        ctx.synchronize()
-        ctx2.synchronize()
-        e0, e1, e2 = ev.create(), ev.create(), ev.create()
-        ev.record(e0, stream)
-        ev.wait(stream2, e0)
-        for k in range(steps):
-            steps_pair[k % n]()
-        ev.record(e1, stream)
-        ev.record(e2, stream2)
-        ms = max(ev.elapsed_ms(e0, e1), ev.elapsed_ms(e0, e2)) / steps
-        ctx.synchronize()
-        ctx2.synchronize()
-        for e in (e0, e1, e2):
-            ev.destroy(e)
-        return ms
+    def timed(fns, steps, warmup):
+        return frames_ms(ctxs, fns, steps, warmup, ev)
 
-    variants = {"eager": [eager_step], "graph": [graph_step],
-                "eager_2_in_flight": [eager_step, eager_step2], "graph_2_in_flight": [graph_step, graph_step2]}
-    chosen = args.mode if args.in_flight == 1 else f"{args.mode}_2_in_flight"
+    def name_of(mode, n):
+        return mode if n == 1 else f"{mode}_{n}_in_flight"
+
+    variants = {}
+    for mode, make in (("eager", eager_on), ("graph", graph_on)):
+        for n in (1, 2) if mode == "eager" else (1, 2, 3):
+            variants[name_of(mode, n)] = [make(c) for c in ctxs[:n]]
+    chosen = name_of(args.mode, args.in_flight)
+    if chosen not in variants:
+        variants[chosen] = [(eager_on if args.mode == "eager" else graph_on)(c) for c in ctxs[:args.in_flight]]
+    tuned = None
+    if not args.no_autotune:
+        # launch shape for the measured pipeline, picked before the timed region
+        t_tune = time.time()
+        best, best_ms, table = autotune.tune(ctx, None, candidates=TUNE_CANDIDATES, contexts=ctxs,
+                                             timer=lambda: timed(variants[chosen], 8, 3))
+        tuned = {"tuning": best, "ms": round(best_ms, 4), "candidates_timed": len(table),
+                 "seconds": round(time.time() - t_tune, 1)}
+        log(f"[bench] autotuned {chosen}: {tuned}")
     frame_ms = {}
     for name, fns in variants.items():
         if name != chosen:
-            frame_ms[name] = round(timed(fns, max(6, args.steps // 2), 2), 4)
+            frame_ms[name] = round(timed(fns, max(6, args.steps // 2), 3), 4)
     ms_per_step = timed(variants[chosen], args.steps, args.warmup)
     frame_ms[chosen] = round(ms_per_step, 4)
     log(f"[bench] frame ms by submission mode: {frame_ms}")
-    ctx2.close()
+    for c in ctxs[1:]:
+        c.close()
     st = ctx.stats()
     rays = st["rays"]
     value = rays / (ms_per_step * 1e-3) / 1e6
     log(f"[bench] {args.steps} frames: {ms_per_step:.3f} ms/frame, {rays} rays/frame, {value:.1f} Mrays/s")
 
-    # per-stage and per-kernel timing of one extra frame (instrumented context
-    # flag set on a second context would rebuild everything; instead switch on
-    # the timing events by recreating nothing: a timed context is cheap to make)
-    tctx = rgrt.Context(W, H, rgrt.CFG_TIMING)
-    tctx.upload_geometry(vertices, indices)
-    tctx.upload_materials(materials)
-    for m in models:
-        m.mesh.blas = None
-    tb = {}
-    for m in models:
-        k = id(m.mesh)
-        if k not in tb:
-            tb[k] = tctx.build_blas(m.mesh.first_vertex, m.mesh.index_offset_bytes, m.mesh.num_faces(), len(m.mesh.vertices))
+    # per-stage and per-kernel timing of two more frames, alone on the GPU,
+    # on a context with the timing events switched on (sharing the scene)
+    tctx = ctx.shared(flags=rgrt.CFG_TIMING)
     tctx.set_tuning(**ctx.tuning())
     tctx.build_tlas(instances)
     tctx.set_frame(ubo)
@@ -315,7 +300,8 @@ def main():
                    "stage_ms": {k: round(ts[k], 4) for k in ("asBuildMs", "rtOnlyMs", "roughMs", "postprocMs", "rtTotalMs",
                                                              "traceKernelMs", "shadeKernelMs", "drainKernelMs")},
                    "tuning": ctx.tuning(), "wavefront_pass_rays": passes.tolist(),
-                   "submission": chosen, "frames_in_flight": args.in_flight, "frame_ms_by_submission": frame_ms},
+                   "submission": chosen, "frames_in_flight": args.in_flight, "frame_ms_by_submission": frame_ms,
+                   "autotune": tuned},
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -20,14 +20,26 @@ CANDIDATES = {
 }
 
 
-def tune(ctx, frame, candidates=None, rounds=1, reps=2, log=None):
+def tune(ctx, frame, candidates=None, rounds=1, reps=2, log=None, contexts=None, timer=None):
     """frame(): enqueues one frame on ctx. Returns (best tuning, best ms,
-    table of (tuning, ms)). Leaves ctx set to the best tuning."""
+    table of (tuning, ms)). Leaves ctx set to the best tuning.
+
+    For frames in flight on several contexts sharing a scene: `contexts`
+    lists every context the tuning is applied to, and timer() returns the
+    ms per frame of the whole pipeline under the current tuning (frame is
+    then unused)."""
     cand = candidates or CANDIDATES
     ev = HipEvents()
+    ctxs = contexts or [ctx]
+    measure = timer or (lambda: time_ms(ctx, frame, reps=reps, warmup=1, events=ev))
+
+    def apply(t):
+        for c in ctxs:
+            c.set_tuning(**t)
+
     best = dict(ctx.tuning())
-    ctx.set_tuning(**best)
-    best_ms = time_ms(ctx, frame, reps=reps, warmup=1, events=ev)
+    apply(best)
+    best_ms = measure()
     table = [(dict(best), best_ms)]
     for _ in range(rounds):
         for name, values in cand.items():
@@ -36,12 +48,12 @@ def tune(ctx, frame, candidates=None, rounds=1, reps=2, log=None):
                     continue
                 t = dict(best)
                 t[name] = v
-                ctx.set_tuning(**t)
-                ms = time_ms(ctx, frame, reps=reps, warmup=1, events=ev)
+                apply(t)
+                ms = measure()
                 table.append((t, ms))
                 if log:
                     log(f"[autotune] {t} {ms:.3f} ms")
                 if ms < best_ms:
                     best, best_ms = t, ms
-            ctx.set_tuning(**best)
+            apply(best)
     return best, best_ms, table

@@ -58,3 +58,35 @@ def time_ms(ctx, fn, reps=3, warmup=1, events=None):
     ev.destroy(a)
     ev.destroy(b)
     return ms
+
+
+def frames_ms(ctxs, fns, steps, warmup=1, events=None):
+    """Mean milliseconds per frame of `steps` frames, frame k enqueued by
+    fns[k % len(fns)] on ctxs[k % len(fns)] (frames in flight on several
+    contexts). hipEvents bracket the frames: every other stream waits for the
+    opening event on the first context's stream, and the closing time is the
+    latest of the streams' closing events."""
+    from . import rgrt
+    ev = events or HipEvents()
+    n = len(fns)
+    used = ctxs[:n]
+    streams = [rgrt.lib().rgrt_stream(c.h) for c in used]
+    for k in range(warmup):
+        fns[k % n]()
+    for c in used:
+        c.synchronize()
+    e0 = ev.create()
+    ends = [ev.create() for _ in range(n)]
+    ev.record(e0, streams[0])
+    for s in streams[1:]:
+        ev.wait(s, e0)
+    for k in range(steps):
+        fns[k % n]()
+    for e, s in zip(ends, streams):
+        ev.record(e, s)
+    ms = max(ev.elapsed_ms(e0, e) for e in ends) / steps
+    for c in used:
+        c.synchronize()
+    for e in [e0] + ends:
+        ev.destroy(e)
+    return ms
