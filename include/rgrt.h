@@ -185,8 +185,10 @@ rgrt_status rgrt_create(rgrt_ctx** out, const rgrt_config* cfg);
  * and stream: frames rendered alternately on the two overlap on the GPU.
  * Every call that changes the shared scene (upload_geometry,
  * upload_materials, update_vertices, build_blas, refit_blas) first waits
- * for the work queued on every sharing context; new geometry drops every
- * sharing context's TLAS. Calls on sharing contexts must not run
+ * on the host for the work queued on every sharing context;
+ * update_vertices_device and rebuild_blas are ordered against the sharing
+ * contexts' streams on the GPU instead. New geometry drops every sharing
+ * context's TLAS. Calls on sharing contexts must not run
  * concurrently from several threads while one changes the scene. The scene
  * is freed with the last context that holds it; contexts may be destroyed
  * in any order. */
@@ -207,6 +209,11 @@ rgrt_status rgrt_upload_materials(rgrt_ctx* ctx, const rgrt_material* materials,
 /* Rewrites vertex positions/normals of a range in place (deforming meshes);
  * follow with rgrt_refit_blas or rgrt_build_blas. */
 rgrt_status rgrt_update_vertices(rgrt_ctx* ctx, uint64_t firstVertex, const rgrt_vertex* vertices, uint64_t count);
+/* No counterpart in the reference. The same from device memory (vertices
+ * animated on the GPU): ordered on the context's stream, returns at once;
+ * contexts sharing the scene order their later work after it. */
+rgrt_status rgrt_update_vertices_device(rgrt_ctx* ctx, uint64_t firstVertex, const rgrt_vertex* deviceVertices,
+                                        uint64_t count);
 
 /* Replaces BottomLevelAS(cmd, mesh) (raygun/render/acceleration_structure.cpp:140-194).
  * LBVH build on the device; blocks until the build is complete, as the
@@ -215,6 +222,12 @@ rgrt_status rgrt_build_blas(rgrt_ctx* ctx, const rgrt_blas_desc* desc, rgrt_blas
 /* Refit of an existing BLAS after rgrt_update_vertices (no counterpart in
  * the reference, which never updates a BLAS): same topology, new boxes. */
 rgrt_status rgrt_refit_blas(rgrt_ctx* ctx, rgrt_blas blas);
+/* No counterpart in the reference. Full LBVH rebuild of an existing BLAS
+ * from its current vertices (deforming meshes), into its own buffers: the
+ * handle stays valid. Ordered on the context's stream and returns at once
+ * (no host wait); contexts sharing the scene order their later work after
+ * it. Rebuild the TLAS (or refit it) afterwards for the new root box. */
+rgrt_status rgrt_rebuild_blas(rgrt_ctx* ctx, rgrt_blas blas);
 /* Number of internal nodes / leaves and the root box of a BLAS. */
 rgrt_status rgrt_blas_info(rgrt_ctx* ctx, rgrt_blas blas, uint32_t* numNodes, uint32_t* numLeaves, float rootLo[3],
                            float rootHi[3]);

@@ -7,7 +7,10 @@ Workload (seeded): triangle counts log-normal, clipped to [10k, 4M]; each
 scene a mesh (UV sphere or torus of that size) in 1-6 instances over a ground
 plane, config-2 style materials, a resolution from 320x180 to 1920x1080. Per
 frame every instance turns about its vertical axis (the TLAS is refitted);
-one scene in four deforms its mesh (new vertex positions, BLAS refitted).
+one scene in four deforms its mesh: its vertices are replaced by the next of
+a cycle of poses held in device memory (as a GPU skinning pass would leave
+them) and its BLAS gets a full LBVH rebuild, both ordered on the scene's
+stream with no host wait (rgrt_update_vertices_device, rgrt_rebuild_blas).
 No counterpart in the reference, which renders one scene.
 """
 import math
@@ -15,10 +18,12 @@ import math
 import numpy as np
 
 from . import procgen, rgrt
+from .hiptime import DeviceBuffer
 from .records import INSTANCE, default_material
 from .scene import Camera, Model, camera_ubo, pack_models, quat_to_mat3
 
 RESOLUTIONS = [(320, 180), (480, 270), (640, 360), (960, 540), (1280, 720), (1920, 1080)]
+POSES = 4  # deformation poses per deforming mesh, cycled frame by frame
 
 
 def scene_sizes(num_scenes, seed, lo=10_000, hi=4_000_000, sigma=1.2):
@@ -70,6 +75,12 @@ class BatchScene:
         self.ubo = camera_ubo(cam, self.width, self.height, max_recursions=max_recursions)
         self.num_instances = n + 1
         self.rays = 0
+        self.poses = []
+        if self.deforms:
+            for k in range(POSES):
+                buf = DeviceBuffer(self.base_vertices.nbytes)
+                buf.upload(self.pose(2.0 * math.pi * k / POSES))
+                self.poses.append(buf)
 
     def instances(self, t):
         inst = np.zeros(self.num_instances, INSTANCE)
@@ -90,17 +101,26 @@ class BatchScene:
         inst["flags"] = 1
         return inst
 
-    def deform(self, t):
-        """New vertex positions (a radial wobble) and a BLAS refit."""
+    def pose(self, phase):
+        """The mesh's vertices under a radial wobble of the given phase."""
         v = self.base_vertices.copy()
         p = v["position"].astype(np.float64)
-        w = 1.0 + 0.05 * np.sin(3.0 * p[:, 1] + 2.0 * t)
+        w = 1.0 + 0.05 * np.sin(3.0 * p[:, 1] + phase)
         v["position"] = (p * w[:, None]).astype(np.float32)
-        self.ctx.update_vertices(self.mesh.first_vertex, v)
-        self.ctx.refit_blas(self.mesh_blas)
+        return v
+
+    def deform(self, frame):
+        """Pose frame % POSES into the vertex buffer and a full BLAS rebuild,
+        both queued on the scene's stream."""
+        buf = self.poses[frame % POSES]
+        self.ctx.update_vertices_device(self.mesh.first_vertex, buf.address, len(self.base_vertices))
+        self.ctx.rebuild_blas(self.mesh_blas)
 
     def close(self):
         self.ctx.close()
+        for b in self.poses:
+            b.close()
+        self.poses = []
 
 
 class BatchWorkload:
@@ -122,14 +142,14 @@ class BatchWorkload:
     def frame_batched(self, t):
         for s in self.scenes:
             if s.deforms and self.frames_done:
-                s.deform(t)
+                s.deform(self.frames_done)
         self.batch.frame(self.descs(t))
         self.frames_done += 1
 
     def frame_sequential(self, t):
         for s, (ubo, inst, fxaa, refit) in zip(self.scenes, self.descs(t)):
             if s.deforms and self.frames_done:
-                s.deform(t)
+                s.deform(self.frames_done)
             s.ctx.frame(ubo, inst, fxaa=fxaa, refit=refit, graph=False)
         self.frames_done += 1
 

@@ -127,6 +127,7 @@ struct rgrt_ctx {
     // while the GPU still works on frame N.
     rgrt_instance* hInstStage[2] = {nullptr, nullptr};
     hipEvent_t stageDone[2] = {nullptr, nullptr};
+    hipEvent_t sceneEv = nullptr; // orders in-place scene writes against sharing contexts
     int stageIdx = 0;
     rgrt_instance* dInstIn = nullptr;
     Node* tlasNodes = nullptr;
@@ -229,6 +230,38 @@ void scene_quiesce(rgrt_ctx* c)
 {
     std::lock_guard<std::mutex> g(c->sd->m);
     for(rgrt_ctx* u: c->sd->users) check(hipStreamSynchronize(u->stream));
+}
+
+// In-place writes of the scene (vertex updates, BLAS rebuilds) are ordered
+// on the GPU rather than on the host: they start after the work already
+// queued on every sharing context, and work queued later on those contexts
+// starts after them. No allocation changes, so nothing waits on the host.
+void scene_write_begin(rgrt_ctx* c)
+{
+    std::lock_guard<std::mutex> g(c->sd->m);
+    for(rgrt_ctx* u: c->sd->users)
+        if(u != c) {
+            check(hipEventRecord(u->sceneEv, u->stream));
+            check(hipStreamWaitEvent(c->stream, u->sceneEv, 0));
+        }
+}
+
+void scene_write_end(rgrt_ctx* c)
+{
+    std::lock_guard<std::mutex> g(c->sd->m);
+    if(c->sd->users.size() < 2) return;
+    check(hipEventRecord(c->sceneEv, c->stream));
+    for(rgrt_ctx* u: c->sd->users)
+        if(u != c) check(hipStreamWaitEvent(u->stream, c->sceneEv, 0));
+}
+
+// Grows the LBVH builder's scratch for n primitives; earlier work queued on
+// the context may still use the old scratch, so it finishes first.
+void reserve_builder(rgrt_ctx* c, uint32_t n)
+{
+    if(n <= c->builder.capacity) return;
+    check(hipStreamSynchronize(c->stream));
+    c->builder.reserve(n);
 }
 
 // New geometry invalidates every BLAS, so every sharing context's TLAS.
@@ -360,6 +393,7 @@ void ensure_blas_cap(rgrt_ctx* c, uint32_t n)
 void ensure_inst_cap(rgrt_ctx* c, uint32_t n)
 {
     if(n <= c->instCap) return;
+    check(hipStreamSynchronize(c->stream)); // queued frames still use the old buffers
     const uint32_t cap = std::max<uint32_t>(n, 64);
     for(int k = 0; k < 2; ++k) {
         if(c->hInstStage[k]) {
@@ -670,6 +704,7 @@ rgrt_status rgrt_create(rgrt_ctx** out, const rgrt_config* cfg)
         c->numCUs = (uint32_t)std::max(1, prop.multiProcessorCount);
         c->apply_tuning();
         for(int k = 0; k < 2; ++k) check(hipEventCreateWithFlags(&c->stageDone[k], hipEventDisableTiming));
+        check(hipEventCreateWithFlags(&c->sceneEv, hipEventDisableTiming));
         if(cfg->width && cfg->height) alloc_images(c, cfg->width, cfg->height);
         return RGRT_OK;
     });
@@ -728,7 +763,7 @@ void rgrt_destroy(rgrt_ctx* c)
     for(hipEvent_t e: c->shadeEvents) (void)hipEventDestroy(e);
     if(c->frameExec) (void)hipGraphExecDestroy(c->frameExec);
     for(hipEvent_t e: {c->evAsStart, c->evAsEnd, c->evRtStart, c->evRtOnlyEnd, c->evRoughEnd, c->evPostEnd, c->stageDone[0],
-                       c->stageDone[1],
+                       c->stageDone[1], c->sceneEv,
                        c->evDrainStart, c->evDrainEnd, c->evFrameStart, c->evFrameEnd})
         if(e) (void)hipEventDestroy(e);
     if(c->stream) (void)hipStreamDestroy(c->stream);
@@ -777,6 +812,19 @@ rgrt_status rgrt_update_vertices(rgrt_ctx* c, uint64_t first, const rgrt_vertex*
     });
 }
 
+rgrt_status rgrt_update_vertices_device(rgrt_ctx* c, uint64_t first, const rgrt_vertex* vertices, uint64_t count)
+{
+    return guarded(c, [&]() {
+        if(first + count > c->sd->nv || (count && !vertices))
+            return fail(c, RGRT_ERR_INVALID_ARG, "update_vertices_device: range");
+        if(!count) return RGRT_OK;
+        scene_write_begin(c);
+        check(hipMemcpyAsync(c->sd->vb + first, vertices, count * sizeof(rgrt_vertex), hipMemcpyDeviceToDevice, c->stream));
+        scene_write_end(c);
+        return RGRT_OK;
+    });
+}
+
 rgrt_status rgrt_upload_materials(rgrt_ctx* c, const rgrt_material* materials, uint32_t n)
 {
     return guarded(c, [&]() {
@@ -818,7 +866,7 @@ rgrt_status rgrt_build_blas(rgrt_ctx* c, const rgrt_blas_desc* d, rgrt_blas* out
         dev_alloc(b.numWide, 1);
         dev_alloc(b.tris, n);
         dev_alloc(b.leafPrim, n);
-        c->builder.reserve(std::max<uint32_t>(n, 1));
+        reserve_builder(c, std::max<uint32_t>(n, 1));
         launch_tri_boxes(s, c->sd->vb, c->sd->ib, d->firstVertex, (uint32_t)io, n, c->builder.boxLo, c->builder.boxHi);
         c->builder.build(s, n, b.nodes, b.leafPrim, c->sd->dBlasRoot + 2 * id);
         c->builder.collapse(s, n, b.nodes, b.wide, b.numWide);
@@ -843,7 +891,7 @@ rgrt_status rgrt_refit_blas(rgrt_ctx* c, rgrt_blas id)
         hipStream_t s = c->stream;
         const uint32_t n = b.desc.numFaces;
         const uint32_t io = b.desc.indexOffsetBytes / 4;
-        c->builder.reserve(std::max<uint32_t>(n, 1));
+        reserve_builder(c, std::max<uint32_t>(n, 1));
         launch_tri_boxes(s, c->sd->vb, c->sd->ib, b.desc.firstVertex, io, n, c->builder.boxLo, c->builder.boxHi);
         c->builder.refit(s, n, b.nodes, b.leafPrim, c->sd->dBlasRoot + 2 * id);
         c->builder.collapse(s, n, b.nodes, b.wide, b.numWide); // wide topology follows the new areas
@@ -854,10 +902,32 @@ rgrt_status rgrt_refit_blas(rgrt_ctx* c, rgrt_blas id)
     });
 }
 
+rgrt_status rgrt_rebuild_blas(rgrt_ctx* c, rgrt_blas id)
+{
+    return guarded(c, [&]() {
+        if(id >= c->sd->blas.size() || !c->sd->blas[id].valid) return fail(c, RGRT_ERR_INVALID_ARG, "rebuild_blas: bad handle");
+        BlasRec& b = c->sd->blas[id];
+        hipStream_t s = c->stream;
+        const uint32_t n = b.desc.numFaces;
+        const uint32_t io = b.desc.indexOffsetBytes / 4;
+        reserve_builder(c, std::max<uint32_t>(n, 1));
+        scene_write_begin(c);
+        // the same kernels as rgrt_build_blas, into the BLAS's own buffers
+        launch_tri_boxes(s, c->sd->vb, c->sd->ib, b.desc.firstVertex, io, n, c->builder.boxLo, c->builder.boxHi);
+        c->builder.build(s, n, b.nodes, b.leafPrim, c->sd->dBlasRoot + 2 * id);
+        c->builder.collapse(s, n, b.nodes, b.wide, b.numWide);
+        c->builder.quantize(s, std::max<uint32_t>(n, 2) - 1, b.wide, b.numWide, b.wideq);
+        launch_tri_gather(s, c->sd->vb, c->sd->ib, b.desc.firstVertex, io, b.leafPrim, n, b.tris);
+        scene_write_end(c);
+        return RGRT_OK;
+    });
+}
+
 rgrt_status rgrt_blas_info(rgrt_ctx* c, rgrt_blas id, uint32_t* numNodes, uint32_t* numLeaves, float rootLo[3], float rootHi[3])
 {
     return guarded(c, [&]() {
         if(id >= c->sd->blas.size() || !c->sd->blas[id].valid) return fail(c, RGRT_ERR_INVALID_ARG, "blas_info: bad handle");
+        check(hipStreamSynchronize(c->stream)); // a rebuild may still be queued
         const uint32_t n = c->sd->blas[id].desc.numFaces;
         if(numNodes) *numNodes = std::max<uint32_t>(n, 2) - 1;
         if(numLeaves) *numLeaves = n;
@@ -882,6 +952,7 @@ rgrt_status rgrt_export_blas(rgrt_ctx* c, rgrt_blas id, rgrt_bvh_node* nodes, ui
 {
     return guarded(c, [&]() {
         if(id >= c->sd->blas.size() || !c->sd->blas[id].valid) return fail(c, RGRT_ERR_INVALID_ARG, "export_blas: bad handle");
+        check(hipStreamSynchronize(c->stream)); // a rebuild may still be queued
         const BlasRec& b = c->sd->blas[id];
         const uint32_t nn = std::max<uint32_t>(b.desc.numFaces, 2) - 1;
         if((nodes && nodeCap < nn) || (leafPrim && leafCap < b.desc.numFaces))
@@ -898,6 +969,7 @@ rgrt_status rgrt_export_blas_wide(rgrt_ctx* c, rgrt_blas id, int32_t compressed,
 {
     return guarded(c, [&]() {
         if(id >= c->sd->blas.size() || !c->sd->blas[id].valid) return fail(c, RGRT_ERR_INVALID_ARG, "export_blas_wide: bad handle");
+        check(hipStreamSynchronize(c->stream)); // a rebuild may still be queued
         check(hipStreamSynchronize(c->stream));
         uint32_t nw = 0;
         check(hipMemcpy(&nw, c->sd->blas[id].numWide, 4, hipMemcpyDeviceToHost));
@@ -933,7 +1005,7 @@ rgrt_status rgrt_build_tlas(rgrt_ctx* c, const rgrt_instance* in, uint32_t n)
     return guarded(c, [&]() {
         const rgrt_status st = stage_instances(c, in, n, false, "build_tlas");
         if(st != RGRT_OK) return st;
-        c->builder.reserve(std::max<uint32_t>(n, 1));
+        reserve_builder(c, std::max<uint32_t>(n, 1));
         enqueue_tlas(c, n, false);
         finish_tlas(c, in, n);
         return RGRT_OK;
@@ -945,7 +1017,7 @@ rgrt_status rgrt_refit_tlas(rgrt_ctx* c, const rgrt_instance* in, uint32_t n)
     return guarded(c, [&]() {
         const rgrt_status st = stage_instances(c, in, n, true, "refit_tlas");
         if(st != RGRT_OK) return st;
-        c->builder.reserve(std::max<uint32_t>(n, 1));
+        reserve_builder(c, std::max<uint32_t>(n, 1));
         enqueue_tlas(c, n, true);
         finish_tlas(c, in, n);
         return RGRT_OK;
@@ -1076,7 +1148,7 @@ rgrt_status frame_prepare(rgrt_ctx* c, const rgrt_ubo* ubo, const rgrt_instance*
     if(st != RGRT_OK) return st;
     if(!c->w || !c->h) return fail(c, RGRT_ERR_NOT_READY, "frame: zero-sized render target");
     stage_to_host(c, in, n);
-    c->builder.reserve(std::max<uint32_t>(n, 1));
+    reserve_builder(c, std::max<uint32_t>(n, 1));
     const uint32_t maxIters = max_sequential_rays(c->ubo.maxRecursions) * (uint32_t)c->ubo.numSamples + 2;
     ensure_host_counts(c, maxIters + 2);
     return RGRT_OK;
@@ -1185,7 +1257,7 @@ struct rgrt_batch {
     std::vector<rgrt_ctx*> ctxs;
     hipGraphExec_t exec = nullptr;
     hipEvent_t fork = nullptr, done = nullptr;
-    std::vector<hipEvent_t> join;
+    std::vector<hipEvent_t> join, pre;
     std::string err;
 };
 
@@ -1209,6 +1281,8 @@ rgrt_status rgrt_batch_create(rgrt_ctx* const* ctxs, uint32_t n, rgrt_batch** ou
         check(hipEventCreateWithFlags(&b->done, hipEventDisableTiming));
         b->join.resize(n, nullptr);
         for(auto& e: b->join) check(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        b->pre.resize(n, nullptr);
+        for(auto& e: b->pre) check(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     }
     catch(...) {
         rgrt_batch_destroy(b);
@@ -1225,6 +1299,8 @@ void rgrt_batch_destroy(rgrt_batch* b)
         if(c && c->stream) (void)hipStreamSynchronize(c->stream);
     if(b->exec) (void)hipGraphExecDestroy(b->exec);
     for(hipEvent_t e: b->join)
+        if(e) (void)hipEventDestroy(e);
+    for(hipEvent_t e: b->pre)
         if(e) (void)hipEventDestroy(e);
     if(b->fork) (void)hipEventDestroy(b->fork);
     if(b->done) (void)hipEventDestroy(b->done);
@@ -1260,6 +1336,12 @@ rgrt_status rgrt_batch_frame(rgrt_batch* b, const rgrt_frame_desc* descs)
     }
     try {
         check(hipSetDevice(c0->device));
+        // work the scenes' own streams already hold (vertex updates, BLAS
+        // rebuilds) comes before the batch
+        for(uint32_t i = 1; i < n; ++i) {
+            check(hipEventRecord(b->pre[i], b->ctxs[i]->stream));
+            check(hipStreamWaitEvent(s0, b->pre[i], 0));
+        }
         // one graph, one branch per scene: the scenes' streams fork from and
         // join into the first scene's stream inside the capture
         hipGraph_t g = nullptr;

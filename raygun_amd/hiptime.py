@@ -90,3 +90,36 @@ def frames_ms(ctxs, fns, steps, warmup=1, events=None):
     for e in [e0] + ends:
         ev.destroy(e)
     return ms
+
+
+class DeviceBuffer:
+    """Device memory from hipMalloc (for data the library reads from device
+    pointers, e.g. animated vertices); freed by close()."""
+
+    def __init__(self, nbytes):
+        self.L = ctypes.CDLL("libamdhip64.so")
+        self.L.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+        self.L.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+        self.L.hipFree.argtypes = [ctypes.c_void_p]
+        self.ptr = ctypes.c_void_p()
+        self.nbytes = int(nbytes)
+        if self.L.hipMalloc(ctypes.byref(self.ptr), max(self.nbytes, 1)) != 0:
+            raise RuntimeError("hipMalloc failed")
+
+    def upload(self, array):
+        """Copies the bytes of a C-contiguous host array (blocking)."""
+        import numpy as np
+        a = np.ascontiguousarray(array)
+        if a.nbytes > self.nbytes:
+            raise ValueError("array larger than the buffer")
+        if self.L.hipMemcpy(self.ptr, a.ctypes.data_as(ctypes.c_void_p), a.nbytes, 1) != 0:  # hipMemcpyHostToDevice
+            raise RuntimeError("hipMemcpy failed")
+
+    @property
+    def address(self):
+        return self.ptr.value
+
+    def close(self):
+        if self.ptr.value:
+            self.L.hipFree(self.ptr)
+            self.ptr = ctypes.c_void_p()

@@ -34,8 +34,8 @@ STATUS = {0: "OK", 1: "INVALID_ARG", 2: "OUT_OF_MEMORY", 3: "HIP", 4: "NOT_READY
 # Every symbol include/rgrt.h declares.
 EXPORTS = [
     "rgrt_create", "rgrt_create_shared", "rgrt_destroy", "rgrt_last_error", "rgrt_stream", "rgrt_synchronize",
-    "rgrt_upload_geometry", "rgrt_upload_materials", "rgrt_update_vertices",
-    "rgrt_build_blas", "rgrt_refit_blas", "rgrt_blas_info", "rgrt_export_blas",
+    "rgrt_upload_geometry", "rgrt_upload_materials", "rgrt_update_vertices", "rgrt_update_vertices_device",
+    "rgrt_build_blas", "rgrt_refit_blas", "rgrt_rebuild_blas", "rgrt_blas_info", "rgrt_export_blas",
     "rgrt_build_tlas", "rgrt_refit_tlas", "rgrt_export_tlas", "rgrt_set_frame", "rgrt_resize",
     "rgrt_trace_rays", "rgrt_trace_rays_device", "rgrt_render", "rgrt_frame", "rgrt_trace", "rgrt_postprocess",
     "rgrt_read_image", "rgrt_read_image_rgba8", "rgrt_image_device_ptr", "rgrt_write_image", "rgrt_get_stats", "rgrt_get_pass_counts",
@@ -86,6 +86,8 @@ def lib():
             "rgrt_upload_geometry": (i32, [P, P, u64, P, u64]),
             "rgrt_upload_materials": (i32, [P, P, u32]),
             "rgrt_update_vertices": (i32, [P, u64, P, u64]),
+            "rgrt_update_vertices_device": (i32, [P, u64, P, u64]),
+            "rgrt_rebuild_blas": (i32, [P, u32]),
             "rgrt_build_blas": (i32, [P, ctypes.POINTER(_BlasDesc), ctypes.POINTER(u32)]),
             "rgrt_refit_blas": (i32, [P, u32]),
             "rgrt_blas_info": (i32, [P, u32, ctypes.POINTER(u32), ctypes.POINTER(u32), P, P]),
@@ -212,6 +214,15 @@ class Context:
 
     def refit_blas(self, b):
         self._ok(self._L.rgrt_refit_blas(self.h, b))
+
+    def update_vertices_device(self, first, device_ptr, count):
+        """count vertices (32 B each) from device memory at device_ptr (an
+        int address, e.g. a torch tensor's data_ptr()); stream-ordered."""
+        self._ok(self._L.rgrt_update_vertices_device(self.h, first, ctypes.c_void_p(device_ptr), count))
+
+    def rebuild_blas(self, b):
+        """Full LBVH rebuild of BLAS b from the current vertices; stream-ordered."""
+        self._ok(self._L.rgrt_rebuild_blas(self.h, b))
 
     def blas_info(self, b):
         nn, nl = ctypes.c_uint32(), ctypes.c_uint32()
