@@ -364,11 +364,22 @@ rgrt_status rgrt_frame(rgrt_ctx* ctx, const rgrt_ubo* ubo, const rgrt_instance* 
 /* ------------------------------------------------------------------------ */
 /* Batches of scenes (no counterpart in the reference, which renders one    */
 /* scene): the frames of several contexts — scenes of any size and render   */
-/* resolution on one device — captured into one hipGraph with one branch per */
-/* scene, so the GPU runs the scenes' kernels side by side, and launched at  */
-/* once. Each frame is what rgrt_frame(..., RGRT_FRAME_GRAPH | flags) does   */
-/* for its context; the graph is captured again every call and updated in   */
-/* place. Work issued later on any of the contexts runs after the batch.    */
+/* resolution on one device — submitted together so the GPU runs the        */
+/* scenes' kernels side by side. Each frame is what                          */
+/* rgrt_frame(..., RGRT_FRAME_GRAPH | flags) does for its context, captured  */
+/* again every call and updated in place. Two submission modes:             */
+/*  RGRT_BATCH_GRAPH_PER_SCENE (default): each scene's frame is one hipGraph */
+/*   launched on its own context's stream after that stream's earlier work   */
+/*   (vertex updates, BLAS rebuilds, the scene's previous frame) and nothing */
+/*   else, so a scene's next frame may start while slower scenes still run. */
+/*  RGRT_BATCH_ONE_GRAPH: one hipGraph with a branch per scene, launched on  */
+/*   the first context's stream after the work queued on every scene's      */
+/*   stream; work issued later on any of the contexts runs after the batch. */
+/* In both modes the batch's own stream (rgrt_batch_stream) is ordered after */
+/* every scene of every batch frame submitted so far.                        */
+/* How far the scenes overlap depends on the hardware queues the HIP runtime */
+/* gives the process (GPU_MAX_HW_QUEUES, 4 by default): streams beyond that  */
+/* share queues.                                                              */
 /* ------------------------------------------------------------------------ */
 typedef struct rgrt_batch rgrt_batch;
 typedef struct rgrt_frame_desc {
@@ -380,8 +391,13 @@ typedef struct rgrt_frame_desc {
 rgrt_status rgrt_batch_create(rgrt_ctx* const* ctxs, uint32_t count, rgrt_batch** out);
 void rgrt_batch_destroy(rgrt_batch* batch);
 const char* rgrt_batch_last_error(const rgrt_batch* batch);
-/* descs[i] is the frame of ctxs[i]. Asynchronous on the first context's stream. */
+#define RGRT_BATCH_GRAPH_PER_SCENE 0u
+#define RGRT_BATCH_ONE_GRAPH 1u
+rgrt_status rgrt_batch_set_mode(rgrt_batch* batch, uint32_t mode);
+/* descs[i] is the frame of ctxs[i]. Asynchronous. */
 rgrt_status rgrt_batch_frame(rgrt_batch* batch, const rgrt_frame_desc* descs);
+/* The batch's stream (hipStream_t as void*): waits for each batch frame. */
+void* rgrt_batch_stream(rgrt_batch* batch);
 
 /* The traceRaysKHR part alone (writes base, normal, rough). */
 rgrt_status rgrt_trace(rgrt_ctx* ctx);

@@ -1,7 +1,7 @@
 """Batched scenes (config 3): many independent scenes of uneven size and
-resolution, animated per frame, rendered together as one hipGraph with one
-branch per scene (rgrt_batch_frame), each scene's launch shape picked by the
-autotuner.
+resolution, animated per frame, rendered together (rgrt_batch_frame: a
+hipGraph per scene on its own stream, or one hipGraph with a branch per
+scene), each scene's launch shape picked by the autotuner.
 
 Workload (seeded): triangle counts log-normal, clipped to [10k, 4M]; each
 scene a mesh (UV sphere or torus of that size) in 1-6 instances over a ground
@@ -124,12 +124,13 @@ class BatchScene:
 
 
 class BatchWorkload:
-    def __init__(self, num_scenes=32, seed=3, lo=10_000, hi=4_000_000, resolutions=RESOLUTIONS):
+    def __init__(self, num_scenes=32, seed=3, lo=10_000, hi=4_000_000, resolutions=RESOLUTIONS,
+                 mode=rgrt.BATCH_GRAPH_PER_SCENE):
         rng = np.random.default_rng(seed + 17)
         sizes = scene_sizes(num_scenes, seed, lo, hi)
         self.scenes = [BatchScene(k, seed * 1000 + k, sizes[k], resolutions[int(rng.integers(len(resolutions)))])
                        for k in range(num_scenes)]
-        self.batch = rgrt.Batch([s.ctx for s in self.scenes])
+        self.batch = rgrt.Batch([s.ctx for s in self.scenes], mode)
         self.frames_done = 0
         self.info = {"scenes": num_scenes, "triangles": [s.mesh.num_faces() for s in self.scenes],
                      "resolutions": [[s.width, s.height] for s in self.scenes],

@@ -1195,6 +1195,42 @@ void update_exec(hipGraphExec_t* exec, hipGraph_t g)
     (void)hipGraphDestroy(g);
 }
 
+// A prepared frame (frame_prepare) captured on the context's stream and
+// launched there as one hipGraph.
+void launch_frame_graph(rgrt_ctx* c, const rgrt_instance* in, uint32_t n, uint32_t flags)
+{
+    hipStream_t s = c->stream;
+    const bool timing = (c->flags & RGRT_CFG_TIMING) != 0;
+    if(timing) {
+        make_event(c->evFrameStart);
+        make_event(c->evFrameEnd);
+    }
+    // the instance count and tables the captured kernels read
+    const uint32_t prevInstances = c->numInstances;
+    const bool prevHave = c->haveTlas;
+    c->numInstances = n;
+    c->haveTlas = true;
+    hipGraph_t g = nullptr;
+    check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    try {
+        frame_enqueue(c, n, flags);
+    }
+    catch(...) {
+        (void)hipStreamEndCapture(s, &g);
+        if(g) (void)hipGraphDestroy(g);
+        c->numInstances = prevInstances;
+        c->haveTlas = prevHave;
+        throw;
+    }
+    check(hipStreamEndCapture(s, &g));
+    update_exec(&c->frameExec, g);
+    if(timing) check(hipEventRecord(c->evFrameStart, s));
+    check(hipGraphLaunch(c->frameExec, s));
+    if(timing) check(hipEventRecord(c->evFrameEnd, s));
+    check(hipEventRecord(c->stageDone[c->stageIdx], s)); // this staging buffer is free after this frame
+    frame_finish(c, in, n, flags);
+    c->timedFrame = timing;
+}
 } // namespace
 
 extern "C" {
@@ -1216,37 +1252,7 @@ rgrt_status rgrt_frame(rgrt_ctx* c, const rgrt_ubo* ubo, const rgrt_instance* in
         }
         const rgrt_status st = frame_prepare(c, ubo, in, n, flags);
         if(st != RGRT_OK) return st;
-        hipStream_t s = c->stream;
-        const bool timing = (c->flags & RGRT_CFG_TIMING) != 0;
-        if(timing) {
-            make_event(c->evFrameStart);
-            make_event(c->evFrameEnd);
-        }
-        // the instance count and tables the captured kernels read
-        const uint32_t prevInstances = c->numInstances;
-        const bool prevHave = c->haveTlas;
-        c->numInstances = n;
-        c->haveTlas = true;
-        hipGraph_t g = nullptr;
-        check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        try {
-            frame_enqueue(c, n, flags);
-        }
-        catch(...) {
-            (void)hipStreamEndCapture(s, &g);
-            if(g) (void)hipGraphDestroy(g);
-            c->numInstances = prevInstances;
-            c->haveTlas = prevHave;
-            throw;
-        }
-        check(hipStreamEndCapture(s, &g));
-        update_exec(&c->frameExec, g);
-        if(timing) check(hipEventRecord(c->evFrameStart, s));
-        check(hipGraphLaunch(c->frameExec, s));
-        if(timing) check(hipEventRecord(c->evFrameEnd, s));
-        check(hipEventRecord(c->stageDone[c->stageIdx], s)); // this staging buffer is free after this frame
-        frame_finish(c, in, n, flags);
-        c->timedFrame = timing;
+        launch_frame_graph(c, in, n, flags);
         return RGRT_OK;
     });
 }
@@ -1258,6 +1264,8 @@ struct rgrt_batch {
     hipGraphExec_t exec = nullptr;
     hipEvent_t fork = nullptr, done = nullptr;
     std::vector<hipEvent_t> join, pre;
+    uint32_t mode = RGRT_BATCH_GRAPH_PER_SCENE;
+    hipStream_t stream = nullptr; // ordered after every scene of each batch frame
     std::string err;
 };
 
@@ -1283,6 +1291,7 @@ rgrt_status rgrt_batch_create(rgrt_ctx* const* ctxs, uint32_t n, rgrt_batch** ou
         for(auto& e: b->join) check(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         b->pre.resize(n, nullptr);
         for(auto& e: b->pre) check(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        check(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
     }
     catch(...) {
         rgrt_batch_destroy(b);
@@ -1302,12 +1311,25 @@ void rgrt_batch_destroy(rgrt_batch* b)
         if(e) (void)hipEventDestroy(e);
     for(hipEvent_t e: b->pre)
         if(e) (void)hipEventDestroy(e);
+    if(b->stream) {
+        (void)hipStreamSynchronize(b->stream);
+        (void)hipStreamDestroy(b->stream);
+    }
     if(b->fork) (void)hipEventDestroy(b->fork);
     if(b->done) (void)hipEventDestroy(b->done);
     delete b;
 }
 
 const char* rgrt_batch_last_error(const rgrt_batch* b) { return b ? b->err.c_str() : "null batch"; }
+
+void* rgrt_batch_stream(rgrt_batch* b) { return b ? (void*)b->stream : nullptr; }
+
+rgrt_status rgrt_batch_set_mode(rgrt_batch* b, uint32_t mode)
+{
+    if(!b || (mode != RGRT_BATCH_GRAPH_PER_SCENE && mode != RGRT_BATCH_ONE_GRAPH)) return RGRT_ERR_INVALID_ARG;
+    b->mode = mode;
+    return RGRT_OK;
+}
 
 rgrt_status rgrt_batch_frame(rgrt_batch* b, const rgrt_frame_desc* descs)
 {
@@ -1326,6 +1348,27 @@ rgrt_status rgrt_batch_frame(rgrt_batch* b, const rgrt_frame_desc* descs)
     }
     rgrt_ctx* c0 = b->ctxs[0];
     hipStream_t s0 = c0->stream;
+    if(b->mode == RGRT_BATCH_GRAPH_PER_SCENE) {
+        // each scene's frame is one hipGraph on its own stream, after that
+        // stream's earlier work only (no barrier between batch frames: a
+        // scene's next frame starts while slower scenes still run); the
+        // batch's stream then waits for all of them
+        uint32_t i = 0;
+        try {
+            check(hipSetDevice(c0->device));
+            for(; i < n; ++i) {
+                rgrt_ctx* c = b->ctxs[i];
+                launch_frame_graph(c, descs[i].instances, descs[i].numInstances, descs[i].flags | RGRT_FRAME_GRAPH);
+                check(hipEventRecord(b->join[i], c->stream));
+                check(hipStreamWaitEvent(b->stream, b->join[i], 0));
+            }
+        }
+        catch(const std::exception& e) {
+            b->err = "scene " + std::to_string(i) + ": " + e.what();
+            return RGRT_ERR_HIP;
+        }
+        return RGRT_OK;
+    }
     std::vector<uint32_t> prevN(n);
     std::vector<bool> prevHave(n);
     for(uint32_t i = 0; i < n; ++i) {
@@ -1374,6 +1417,7 @@ rgrt_status rgrt_batch_frame(rgrt_batch* b, const rgrt_frame_desc* descs)
         update_exec(&b->exec, g);
         check(hipGraphLaunch(b->exec, s0));
         check(hipEventRecord(b->done, s0));
+        check(hipStreamWaitEvent(b->stream, b->done, 0));
         for(uint32_t i = 0; i < n; ++i) {
             rgrt_ctx* c = b->ctxs[i];
             // the staging buffers are free and the scenes' later work is

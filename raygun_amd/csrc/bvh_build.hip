@@ -302,7 +302,8 @@ __device__ __forceinline__ int karras_delta(const uint32_t* k, int n, int i, int
 
 // Children of internal node i (Karras 2012): refs >= 0 are internal nodes,
 // ~k the leaf at sorted position k; gamma is the split position.
-__device__ __forceinline__ void karras_node(const uint32_t* keys, int n, int i, int& left, int& right, int& gamma)
+__device__ __forceinline__ void karras_node(const uint32_t* keys, int n, int i, int& left, int& right, int& gamma, int& rlo,
+                                            int& rhi)
 {
     const int d = (karras_delta(keys, n, i, i + 1) - karras_delta(keys, n, i, i - 1)) >= 0 ? 1 : -1;
     const int dmin = karras_delta(keys, n, i, i - d);
@@ -323,14 +324,18 @@ __device__ __forceinline__ void karras_node(const uint32_t* keys, int n, int i, 
     const int lo = min(i, j), hi = max(i, j);
     left = (lo == gamma) ? ~gamma : gamma;
     right = (hi == gamma + 1) ? ~(gamma + 1) : gamma + 1;
+    rlo = lo;
+    rhi = hi;
 }
 
-__global__ void k_karras(const uint32_t* __restrict__ keys, int n, Node* __restrict__ nodes, uint32_t* __restrict__ leafParent)
+__global__ void k_karras(const uint32_t* __restrict__ keys, int n, Node* __restrict__ nodes, uint32_t* __restrict__ leafParent,
+                         uint2* __restrict__ range)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if(i >= n - 1) return;
-    int left, right, gamma;
-    karras_node(keys, n, i, left, right, gamma);
+    int left, right, gamma, lo, hi;
+    karras_node(keys, n, i, left, right, gamma, lo, hi);
+    range[i] = make_uint2((uint32_t)lo, (uint32_t)hi);
     Node& nd = nodes[i];
     nd.child0 = left;
     nd.child1 = right;
@@ -351,9 +356,22 @@ __global__ void k_fill_u32(uint32_t* p, uint32_t n, uint32_t v)
 // ---------------------------------------------------------------------------
 // 6. bottom-up refit
 // ---------------------------------------------------------------------------
+// The walk from every leaf to the root combines two child boxes per node.
+// A single walk across the whole grid would need a device-wide release and
+// acquire per node (each XCD has its own L2), which costs more than the work
+// itself. The leaves are therefore cut into chunks of kRefitChunk
+// consecutive leaves: a workgroup finishes every node whose leaf range lies
+// inside its chunk with LDS counters and LDS copies of the child boxes, and
+// records each arrival at a node reaching outside the chunk. One workgroup
+// then finishes those upper nodes from the recorded arrivals (workgroup-scope
+// ordering only). A node's box is always fminf/fmaxf(slot 0, slot 1), so the
+// result does not depend on the order of arrival.
+constexpr int kRefitChunk = 256;
+constexpr int kRefitTopThreads = 1024;
+
 __device__ __forceinline__ void write_slot(Node* nd, int slot, const float lo[3], const float hi[3])
 {
-    volatile float* f = reinterpret_cast<volatile float*>(nd);
+    float* f = reinterpret_cast<float*>(nd);
     // lo[slot] at float offset 3*slot, hi[slot] at 6 + 3*slot
     for(int c = 0; c < 3; ++c) {
         f[3 * slot + c] = lo[c];
@@ -361,11 +379,21 @@ __device__ __forceinline__ void write_slot(Node* nd, int slot, const float lo[3]
     }
 }
 
-__global__ void k_refit(const float4* __restrict__ boxLo, const float4* __restrict__ boxHi,
-                        const uint32_t* __restrict__ leafPrim, const uint32_t* __restrict__ leafParent, Node* nodes,
-                        uint32_t* counters, int n, float4* rootBox)
+// counters[n - 1] counts the recorded arrivals, exits[] holds their nodes.
+__global__ __launch_bounds__(kRefitChunk) void k_refit_chunk(const float4* __restrict__ boxLo, const float4* __restrict__ boxHi,
+                                                             const uint32_t* __restrict__ leafPrim,
+                                                             const uint32_t* __restrict__ leafParent,
+                                                             const uint2* __restrict__ range, Node* nodes, uint32_t* counters,
+                                                             uint32_t* exits, int n, float4* rootBox)
 {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ uint32_t cnt[kRefitChunk];
+    __shared__ float box[kRefitChunk][12]; // node - c0: slot 0 lo, slot 1 lo, slot 0 hi, slot 1 hi
+    const int t = threadIdx.x;
+    const int c0 = blockIdx.x * kRefitChunk;
+    const int c1 = min(c0 + kRefitChunk, n) - 1; // last leaf of the chunk
+    cnt[t] = 0u;
+    __syncthreads();
+    const int k = c0 + t;
     if(k >= n) return;
     const uint32_t prim = leafPrim[k];
     const float4 a = boxLo[prim], b = boxHi[prim];
@@ -377,13 +405,23 @@ __global__ void k_refit(const float4* __restrict__ boxLo, const float4* __restri
     for(int level = 0; level < 128; ++level) {
         if(p >= (uint32_t)(n - 1)) return;
         Node* nd = nodes + p;
-        const int slot = (reinterpret_cast<volatile int*>(nd)[12] == child) ? 0 : 1;
+        const int slot = nd->child0 == child ? 0 : 1;
+        const uint2 r = range[p];
         write_slot(nd, slot, lo, hi);
-        __threadfence();
-        const uint32_t prev = atomicAdd(&counters[p], 1u);
-        if(prev == 0) return; // the sibling's thread finishes this node
-        __threadfence();
-        const volatile float* f = reinterpret_cast<const volatile float*>(nd);
+        const int q = (int)p - c0; // a node lies inside its own leaf range
+        if((int)r.x < c0 || (int)r.y > c1 || q < 0 || q >= kRefitChunk) {
+            // p reaches outside the chunk: the top pass takes it from here
+            const uint32_t e = atomicAdd(&counters[n - 1], 1u);
+            if(e < (uint32_t)n) exits[e] = p;
+            return;
+        }
+        float* f = box[q];
+        for(int c = 0; c < 3; ++c) {
+            f[3 * slot + c] = lo[c];
+            f[6 + 3 * slot + c] = hi[c];
+        }
+        const uint32_t prev = __hip_atomic_fetch_add(&cnt[q], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if(prev == 0u) return; // the sibling's thread finishes this node
         for(int c = 0; c < 3; ++c) {
             lo[c] = fminf(f[c], f[3 + c]);
             hi[c] = fmaxf(f[6 + c], f[9 + c]);
@@ -394,8 +432,52 @@ __global__ void k_refit(const float4* __restrict__ boxLo, const float4* __restri
             return;
         }
         child = (int)p;
-        p = reinterpret_cast<volatile uint32_t*>(nd)[14];
+        p = nd->parent;
     }
+}
+
+// The nodes above the chunks, from the recorded arrivals, in one workgroup.
+__global__ __launch_bounds__(kRefitTopThreads) void k_refit_top(Node* nodes, uint32_t* counters, const uint32_t* __restrict__ exits,
+                                                                int n, float4* rootBox)
+{
+    const uint32_t numExits = min(counters[n - 1], (uint32_t)n);
+    for(uint32_t e = threadIdx.x; e < numExits; e += kRefitTopThreads) {
+        uint32_t p = exits[e];
+        for(int level = 0; level < 128; ++level) {
+            if(p >= (uint32_t)(n - 1)) break;
+            Node* nd = nodes + p;
+            const uint32_t prev = __hip_atomic_fetch_add(&counters[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if(prev == 0u) break; // the other child's arrival finishes this node
+            const float* f = reinterpret_cast<const float*>(nd);
+            float lo[3], hi[3];
+            for(int c = 0; c < 3; ++c) {
+                lo[c] = fminf(f[c], f[3 + c]);
+                hi[c] = fmaxf(f[6 + c], f[9 + c]);
+            }
+            if(p == 0) {
+                rootBox[0] = make_float4(lo[0], lo[1], lo[2], 0.0f);
+                rootBox[1] = make_float4(hi[0], hi[1], hi[2], 0.0f);
+                break;
+            }
+            const uint32_t q = nd->parent;
+            if(q >= (uint32_t)(n - 1)) break;
+            Node* pn = nodes + q;
+            write_slot(pn, pn->child0 == (int32_t)p ? 0 : 1, lo, hi);
+            p = q;
+        }
+    }
+}
+
+// Leaf range of every internal node of an existing tree: the leftmost leaf
+// below child 0 and the rightmost below child 1.
+__global__ void k_node_ranges(const Node* __restrict__ nodes, int n, uint2* __restrict__ range)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if(i >= n - 1) return;
+    int32_t a = nodes[i].child0, b = nodes[i].child1;
+    for(int level = 0; level < 128 && a >= 0 && a < n - 1; ++level) a = nodes[a].child0;
+    for(int level = 0; level < 128 && b >= 0 && b < n - 1; ++level) b = nodes[b].child1;
+    range[i] = make_uint2(a < 0 ? (uint32_t)~a : 0u, b < 0 ? (uint32_t)~b : (uint32_t)(n - 1));
 }
 
 // Trees of 0 or 1 primitives: one node with an empty second child.
@@ -771,7 +853,8 @@ __global__ void k_inst_gather(const float* __restrict__ w2o, const rgrt_instance
 // records), with the intermediate arrays in LDS and one launch instead of
 // about twenty. The sort ranks each key by counting the keys that precede
 // it in (key, index) order, which is the order the stable radix sort
-// produces; the refit combines child boxes in the same order as k_refit.
+// produces; the refit combines child boxes in the same order as
+// k_refit_chunk / k_refit_top (fminf/fmaxf of slot 0 and slot 1).
 // The trees are therefore identical to the multi-kernel path's.
 constexpr int kFusedTlasMax = kCollapseThreads;
 
@@ -887,7 +970,8 @@ __global__ __launch_bounds__(kCollapseThreads) void k_tlas_fused(
         }
         else {
             int gamma;
-            karras_node(keys, n, tid, left, right, gamma);
+            int rlo, rhi;
+            karras_node(keys, n, tid, left, right, gamma, rlo, rhi);
         }
         child0[tid] = left;
         child1[tid] = right;
@@ -983,16 +1067,18 @@ void LbvhBuilder::reserve(uint32_t n)
     check(hipMalloc(&leafParent, cap * 4ull));
     check(hipMalloc(&counters, cap * 4ull));
     check(hipMalloc(&sceneBox, 64));
+    check(hipMalloc(&nodeRange, cap * 8ull));
     capacity = cap;
 }
 
 void LbvhBuilder::release()
 {
     for(void* p: {(void*)keysA, (void*)keysB, (void*)valsA, (void*)valsB, (void*)hist, (void*)boxLo, (void*)boxHi,
-                  (void*)leafParent, (void*)counters, (void*)sceneBox})
+                  (void*)leafParent, (void*)counters, (void*)sceneBox, (void*)nodeRange})
         if(p) (void)hipFree(p);
     keysA = keysB = valsA = valsB = hist = leafParent = counters = sceneBox = nullptr;
     boxLo = boxHi = nullptr;
+    nodeRange = nullptr;
     capacity = 0;
 }
 
@@ -1019,10 +1105,19 @@ void LbvhBuilder::build(hipStream_t s, uint32_t n, Node* nodes, uint32_t* leafPr
         std::swap(vin, vout);
     }
     // after 4 passes the sorted data is back in keysA / valsA
-    k_karras<<<blocks_for(n - 1, 256), 256, 0, s>>>(kin, (int)n, nodes, leafParent);
-    k_fill_u32<<<blocks_for(n - 1, 256), 256, 0, s>>>(counters, n - 1, 0u);
-    k_refit<<<blocks_for(n, 256), 256, 0, s>>>(boxLo, boxHi, vin, leafParent, nodes, counters, (int)n, rootBox);
+    k_karras<<<blocks_for(n - 1, 256), 256, 0, s>>>(kin, (int)n, nodes, leafParent, nodeRange);
+    refit_boxes(s, n, vin, nodes, rootBox);
     check(hipMemcpyAsync(leafPrimOut, vin, n * 4ull, hipMemcpyDeviceToDevice, s));
+}
+
+void LbvhBuilder::refit_boxes(hipStream_t s, uint32_t n, const uint32_t* leafPrim, Node* nodes, float4* rootBox)
+{
+    // counters[0, n-1) per node, counters[n-1] the arrivals the top pass
+    // takes over; keysB (free after the sort) holds those arrivals
+    k_fill_u32<<<blocks_for(n, 256), 256, 0, s>>>(counters, n, 0u);
+    k_refit_chunk<<<blocks_for(n, kRefitChunk), kRefitChunk, 0, s>>>(boxLo, boxHi, leafPrim, leafParent, nodeRange, nodes,
+                                                                     counters, keysB, (int)n, rootBox);
+    if(n > (uint32_t)kRefitChunk) k_refit_top<<<1, kRefitTopThreads, 0, s>>>(nodes, counters, keysB, (int)n, rootBox);
 }
 
 // Refit with the existing topology: new primitive boxes, same leaf order.
@@ -1032,10 +1127,10 @@ void LbvhBuilder::refit(hipStream_t s, uint32_t n, Node* nodes, const uint32_t* 
         k_small_tree<<<1, 64, 0, s>>>(boxLo, boxHi, (int)n, nodes, rootBox);
         return;
     }
-    // leafParent from the existing nodes
+    // leafParent and the leaf ranges from the existing nodes
     recompute_leaf_parents(s, n, nodes);
-    k_fill_u32<<<blocks_for(n - 1, 256), 256, 0, s>>>(counters, n - 1, 0u);
-    k_refit<<<blocks_for(n, 256), 256, 0, s>>>(boxLo, boxHi, leafPrim, leafParent, nodes, counters, (int)n, rootBox);
+    k_node_ranges<<<blocks_for(n - 1, 256), 256, 0, s>>>(nodes, (int)n, nodeRange);
+    refit_boxes(s, n, leafPrim, nodes, rootBox);
 }
 
 namespace {

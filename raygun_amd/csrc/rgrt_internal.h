@@ -31,12 +31,15 @@ struct LbvhBuilder {
     uint32_t* leafParent = nullptr;
     uint32_t* counters = nullptr;
     uint32_t* sceneBox = nullptr;
+    uint2* nodeRange = nullptr; // first and last leaf of every internal node
 
     void reserve(uint32_t n);
     void release();
     void build(hipStream_t s, uint32_t n, Node* nodes, uint32_t* leafPrimOut, float4* rootBox);
     void refit(hipStream_t s, uint32_t n, Node* nodes, const uint32_t* leafPrim, float4* rootBox);
     void recompute_leaf_parents(hipStream_t s, uint32_t n, const Node* nodes);
+    // Bottom-up boxes of a tree whose leafParent and nodeRange are set.
+    void refit_boxes(hipStream_t s, uint32_t n, const uint32_t* leafPrim, Node* nodes, float4* rootBox);
     // Collapses the binary tree of n leaves into 4-wide nodes (breadth-first);
     // writes the wide node count to *numWide (device).
     void collapse(hipStream_t s, uint32_t n, const Node* nodes, Node4* wide, uint32_t* numWide);

@@ -28,6 +28,7 @@ FORMAT_BGRA8_SRGB = 3
 IMAGE_FINAL, IMAGE_BASE, IMAGE_NORMAL, IMAGE_ROUGH, IMAGE_ROUGH_TRANSITIONS, IMAGE_ROUGH_COLORS_A, IMAGE_ROUGH_COLORS_B, \
     IMAGE_BASE_F32, IMAGE_NORMAL_F32, IMAGE_ROUGH_F32, IMAGE_PRIMARY_HIT, IMAGE_FEATURES = range(12)
 SAMPLING_WHITTED, SAMPLING_STOCHASTIC = 0, 1
+BATCH_GRAPH_PER_SCENE, BATCH_ONE_GRAPH = 0, 1
 
 STATUS = {0: "OK", 1: "INVALID_ARG", 2: "OUT_OF_MEMORY", 3: "HIP", 4: "NOT_READY", 5: "NO_DEVICE"}
 
@@ -42,7 +43,7 @@ EXPORTS = [
     "rgrt_set_tuning", "rgrt_get_tuning", "rgrt_set_sampling", "rgrt_denoiser_create", "rgrt_denoiser_destroy",
     "rgrt_denoiser_num_params", "rgrt_denoiser_get_params", "rgrt_denoiser_set_params", "rgrt_denoiser_loss_and_grads",
     "rgrt_denoiser_dataset_reserve", "rgrt_denoiser_dataset_add_input", "rgrt_denoiser_dataset_add_target",
-    "rgrt_denoiser_train", "rgrt_denoise", "rgrt_batch_create", "rgrt_batch_destroy", "rgrt_batch_last_error",
+    "rgrt_denoiser_train", "rgrt_denoise", "rgrt_batch_create", "rgrt_batch_destroy", "rgrt_batch_last_error", "rgrt_batch_set_mode", "rgrt_batch_stream",
     "rgrt_batch_frame", "rgrt_export_blas_wide", "rgrt_export_tlas_wide", "rgrt_version",
 ]
 
@@ -130,6 +131,8 @@ def lib():
             "rgrt_batch_destroy": (None, [P]),
             "rgrt_batch_last_error": (ctypes.c_char_p, [P]),
             "rgrt_batch_frame": (i32, [P, P]),
+            "rgrt_batch_set_mode": (i32, [P, u32]),
+            "rgrt_batch_stream": (P, [P]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -458,9 +461,11 @@ class _FrameDesc(ctypes.Structure):
 
 
 class Batch:
-    """Frames of several contexts as one hipGraph (rgrt_batch_*)."""
+    """Frames of several contexts submitted together (rgrt_batch_*): a
+    hipGraph per scene on its own stream (BATCH_GRAPH_PER_SCENE) or one
+    hipGraph with a branch per scene (BATCH_ONE_GRAPH)."""
 
-    def __init__(self, contexts):
+    def __init__(self, contexts, mode=BATCH_GRAPH_PER_SCENE):
         self._L = lib()
         self.contexts = list(contexts)
         arr = (ctypes.c_void_p * len(self.contexts))(*[c.h.value for c in self.contexts])
@@ -468,6 +473,21 @@ class Batch:
         st = self._L.rgrt_batch_create(arr, len(self.contexts), ctypes.byref(self.h))
         if st != 0:
             raise RgrtError(st, "rgrt_batch_create failed")
+        self.set_mode(mode)
+
+    def stream(self):
+        """The batch's stream (int address): ordered after every batch frame."""
+        return self._L.rgrt_batch_stream(self.h)
+
+    def synchronize(self):
+        for c in self.contexts:
+            c.synchronize()
+
+    def set_mode(self, mode):
+        st = self._L.rgrt_batch_set_mode(self.h, mode)
+        if st != 0:
+            raise RgrtError(st, "rgrt_batch_set_mode failed")
+        self.mode = mode
 
     def close(self):
         if self.h:

@@ -1,5 +1,6 @@
-"""Batched scenes (rgrt_batch_frame, one hipGraph with a branch per scene)
-against the same scenes rendered one by one with the eager calls: identical
+"""Batched scenes (rgrt_batch_frame: a hipGraph per scene on its own stream,
+or one hipGraph with a branch per scene) against the same scenes rendered one
+by one with the eager calls: identical
 final images per scene over animated frames (TLAS refits, a deforming mesh
 refitted), a refused frame naming its scene, and the autotuner changing only
 timing, never a frame."""
@@ -13,13 +14,14 @@ pytestmark = pytest.mark.gpu
 RES = [(160, 90), (96, 64), (200, 100), (64, 64)]
 
 
-def _workload():
-    return batched.BatchWorkload(num_scenes=4, seed=5, lo=2_000, hi=30_000, resolutions=RES)
+def _workload(mode=rgrt.BATCH_GRAPH_PER_SCENE):
+    return batched.BatchWorkload(num_scenes=4, seed=5, lo=2_000, hi=30_000, resolutions=RES, mode=mode)
 
 
-def test_batched_frames_equal_sequential_frames():
-    print("[case] batch 4 scenes x 3 frames", flush=True)
-    a, b = _workload(), _workload()
+@pytest.mark.parametrize("mode", [rgrt.BATCH_GRAPH_PER_SCENE, rgrt.BATCH_ONE_GRAPH])
+def test_batched_frames_equal_sequential_frames(mode):
+    print(f"[case] batch 4 scenes x 3 frames mode {mode}", flush=True)
+    a, b = _workload(mode), _workload()
     try:
         assert a.info == b.info
         for f in range(3):
@@ -35,9 +37,10 @@ def test_batched_frames_equal_sequential_frames():
         b.close()
 
 
-def test_batch_refuses_a_bad_scene_and_recovers():
-    print("[case] batch error path", flush=True)
-    w = _workload()
+@pytest.mark.parametrize("mode", [rgrt.BATCH_GRAPH_PER_SCENE, rgrt.BATCH_ONE_GRAPH])
+def test_batch_refuses_a_bad_scene_and_recovers(mode):
+    print(f"[case] batch error path mode {mode}", flush=True)
+    w = _workload(mode)
     try:
         w.frame_batched(0.0)
         descs = w.descs(0.1)
