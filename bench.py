@@ -96,6 +96,10 @@ def main():
                          "scene streams side by side, which 4 hardware queues would serialise")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the library's default launch shape instead of autotuning it for the measured pipeline")
+    ap.add_argument("--tuning", type=str, default="",
+                    help="launch shape as JSON (rgrt_tuning fields), used instead of autotuning (for profiling runs)")
+    ap.add_argument("--only-chosen", action="store_true",
+                    help="time only the chosen submission mode (for profiling runs)")
     ap.add_argument("--in-flight", type=int, choices=(1, 2, 3, 4), default=3,
                     help="frames in flight: N > 1 rotates frames over N contexts sharing the scene "
                          "(rgrt_create_shared), so one frame's tail overlaps the next frame's start")
@@ -170,7 +174,13 @@ def main():
     if chosen not in variants:
         variants[chosen] = [(eager_on if args.mode == "eager" else graph_on)(c) for c in ctxs[:args.in_flight]]
     tuned = None
-    if not args.no_autotune:
+    if args.tuning:
+        t = dict(ctx.tuning())
+        t.update(json.loads(args.tuning))
+        for c in ctxs:
+            c.set_tuning(**t)
+        tuned = {"tuning": t, "given": True}
+    elif not args.no_autotune:
         # launch shape for the measured pipeline, picked before the timed region
         t_tune = time.time()
         best, best_ms, table = autotune.tune(ctx, None, candidates=TUNE_CANDIDATES, contexts=ctxs,
@@ -180,7 +190,7 @@ def main():
         log(f"[bench] autotuned {chosen}: {tuned}")
     frame_ms = {}
     for name, fns in variants.items():
-        if name != chosen:
+        if name != chosen and not args.only_chosen:
             frame_ms[name] = round(timed(fns, max(6, args.steps // 2), 3), 4)
     ms_per_step = timed(variants[chosen], args.steps, args.warmup)
     frame_ms[chosen] = round(ms_per_step, 4)
