@@ -33,6 +33,9 @@ TRI_BYTES = 48    # rgrt::Tri
 RAY_BYTES = 40    # queue entry: origin, tmin, dir, tmax, pixel, flags
 HIT_BYTES = 20    # t, u, v, instance, primitive
 GATHER_BYTES = 184  # closest-hit gathers: 12 B indices + 96 B vertices + 64 B material + 12 B offsets
+# config 5 (4 samples per pixel, depth 6): far longer per-pixel ray chains
+CONFIG5_CANDIDATES = {"wavefrontPasses": [12, 24, 32, 48, 64], "traceBlocksPerCU": [4, 6, 8],
+                      "drainBlocksPerCU": [2, 4, 8]}
 # launch shapes the autotuner tries for the headline pipeline (coordinate descent)
 TUNE_CANDIDATES = {"wavefrontPasses": [6, 8, 10, 12, 14], "traceBlocksPerCU": [4, 6, 8],
                    "drainBlocksPerCU": [2, 4, 8], "shadeBlocksPerCU": [4, 8, 12]}
@@ -105,6 +108,8 @@ def main():
                          "(rgrt_create_shared), so one frame's tail overlaps the next frame's start")
     ap.add_argument("--batch-scenes", type=int, default=32,
                     help="scenes of the batched workload measured after the headline (0 = skip)")
+    ap.add_argument("--no-config5", action="store_true",
+                    help="skip the config-5 measurement (3840x2160, 16M unique triangles, 4 spp, depth 6) after the headline")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="run only the counter-calibration copy kernel over MIB MiB")
     args = ap.parse_args()
@@ -321,12 +326,69 @@ def main():
         "cpu_baseline": cpu,
     }
     ctx.close()
+    if not args.no_config5:
+        try:
+            result["config5"] = bench_config5(max(4, args.steps // 6))
+        except Exception as e:  # the headline stands without it
+            log(f"[bench] config5 failed: {e!r}")
     if args.batch_scenes:
         try:
             result["batched"] = bench_batched(args.batch_scenes, max(3, args.steps // 4))
         except Exception as e:  # the headline stands without the batched figures
             log(f"[bench] batched workload failed: {e!r}")
     print(json.dumps(result), flush=True)
+
+
+def bench_config5(frames, in_flight=2):
+    """Config 5's plain tracer: 3840x2160, 8 seeded displaced meshes of ~2M
+    triangles (16M unique), 256 instances, one in five refractive,
+    maxRecursions 6, 4 samples per pixel (the SSAA table), FXAA; frames as
+    hipGraphs, `in_flight` frames in flight on contexts sharing the scene.
+    The learned denoiser and the batch with 8 lower-resolution scenes that
+    config 5 also names are not part of this figure."""
+    from raygun_amd import autotune, procgen, rgrt
+    from raygun_amd.hiptime import HipEvents, frames_ms
+    from raygun_amd.scene import pack_models
+    t0 = time.time()
+    setup = procgen.config5()
+    models = setup.scene.models()
+    v, i, m = pack_models(models)
+    log(f"[bench] config5 scene generated in {time.time() - t0:.1f}s: {setup.info}")
+    ctx = rgrt.Context(setup.width, setup.height, 0)
+    ctxs = [ctx]
+    try:
+        ctx.upload_geometry(v, i)
+        ctx.upload_materials(m)
+        t1 = time.time()
+        blas_of = {}
+        for mo in models:
+            k = id(mo.mesh)
+            if k not in blas_of:
+                blas_of[k] = ctx.build_blas(mo.mesh.first_vertex, mo.mesh.index_offset_bytes, mo.mesh.num_faces(),
+                                            len(mo.mesh.vertices))
+            mo.blas = blas_of[k]
+        build_s = time.time() - t1
+        ctxs += [ctx.shared() for _ in range(in_flight - 1)]
+        inst = setup.scene.collect_instances()
+        ubo = setup.ubo()
+        fns = [(lambda c=c: c.frame(ubo, inst, fxaa=True, graph=True)) for c in ctxs]
+        ev = HipEvents()
+        t2 = time.time()
+        best, best_ms, table = autotune.tune(ctx, None, candidates=CONFIG5_CANDIDATES, contexts=ctxs,
+                                             timer=lambda: frames_ms(ctxs, fns, 3, 1, ev))
+        log(f"[bench] config5 autotuned in {time.time() - t2:.1f}s: {best} {best_ms:.2f} ms")
+        ms = frames_ms(ctxs, fns, frames, 2, ev)
+        rays = ctx.stats()["rays"]
+        out = {"workload": "config5: 3840x2160, 8 displaced meshes (16M unique triangles), 256 instances, 20% refractive "
+                           "(IOR U[1.3,1.7]), maxRecursions 6, 4 spp (SSAA table), FXAA; plain tracer only",
+               "unique_triangles": setup.info["unique_triangles"], "instanced_triangles": setup.info["instanced_triangles"],
+               "blas_build_s": round(build_s, 3), "frames": frames, "frames_in_flight": in_flight, "tuning": best,
+               "rays_per_frame": rays, "ms_per_frame": round(ms, 3), "mrays_per_s": round(rays / (ms * 1e-3) / 1e6, 1)}
+        log(f"[bench] config5: {out}")
+        return out
+    finally:
+        for c in ctxs:
+            c.close()
 
 
 def bench_batched(num_scenes, frames):

@@ -14,6 +14,10 @@ Scenes:
   noise-displaced sphere, displaced torus) of ~400k triangles each (2.0M
   unique BLAS triangles), 64 TRS instances uniformly placed in a 100^3 box,
   per-instance materials, 1920x1080, maxRecursions 4.
+- ``config5``: 8 seeded noise-displaced meshes (grids, spheres, tori) of ~2M
+  triangles each (16M unique BLAS triangles), 256 TRS instances in a 200^3
+  box, one instance in five refractive with IOR U[1.3, 1.7], 3840x2160,
+  maxRecursions 6, 4 samples per pixel.
 """
 import math
 
@@ -259,3 +263,58 @@ def config2(width=1920, height=1080, seed=2, tris_per_mesh=400_000, num_instance
                             "instances": num_instances,
                             "reflective": sum(1 for r, _ in mats_info if r > 0),
                             "transparent": sum(1 for _, t in mats_info if t > 0)})
+
+
+def config5_meshes(seed=5, tris_per_mesh=2_000_000):
+    """Eight seeded noise-displaced meshes of ~tris_per_mesh triangles each:
+    displaced grids, bumpy spheres and tori, three of each kind but one."""
+    rng = np.random.default_rng(seed)
+    S = int(round(math.sqrt(tris_per_mesh)))
+    S += S % 2
+    V = int(round(math.sqrt(tris_per_mesh / 8)))
+    n = int(round(math.sqrt(tris_per_mesh / 2)))
+    meshes = []
+    for k in range(8):
+        noise = ValueNoise(int(rng.integers(1 << 30)), octaves=3, base_freq=float(rng.uniform(1.5, 4.0)))
+        amp = float(rng.uniform(0.1, 0.35))
+        kind = k % 3
+        if kind == 0:
+            meshes.append(displaced_grid(n, 2.0, amp, noise=noise))
+        elif kind == 1:
+            meshes.append(uv_sphere(S, S // 2, 1.0, noise=lambda d, f=noise, a=amp: a * f(d)))
+        else:
+            meshes.append(torus(4 * V, V, 1.0, 0.4, noise=lambda d, f=noise, a=amp: a * f(d)))
+    return meshes
+
+
+def config5(width=3840, height=2160, seed=5, tris_per_mesh=2_000_000, num_instances=256, meshes=None, num_samples=4):
+    """256 TRS instances of 8 seeded displaced meshes in a 200^3 box; one
+    instance in five refractive (transparency U[0.5, 0.95], IOR U[1.3, 1.7]),
+    maxRecursions 6, num_samples samples per pixel."""
+    rng = np.random.default_rng(seed + 1000)
+    meshes = meshes if meshes is not None else config5_meshes(seed, tris_per_mesh)
+    sc = Scene()
+    sc.camera = Camera(fov_deg=60.0)
+    sc.camera.look_at((0.0, 10.0, 130.0), (0.0, 0.0, -40.0))
+    refractive = 0
+    for i in range(num_instances):
+        mesh = meshes[int(rng.integers(len(meshes)))]
+        is_refr = rng.uniform() < 0.2
+        refractive += int(is_refr)
+        mat = default_material(
+            diffuse=tuple(rng.uniform(0.0, 1.0, 3)),
+            roughness=float(rng.uniform(0.0, 1.0)),
+            ior=float(rng.uniform(1.3, 1.7)) if is_refr else 1.0,
+            reflectivity=float(rng.uniform(0.3, 1.0)) if rng.uniform() < 0.3 else 0.0,
+            transparency=float(rng.uniform(0.5, 0.95)) if is_refr else 0.0,
+            effectId=1 if rng.uniform() < 0.1 else 0,
+        )
+        model = Model(mesh, np.array([mat]))
+        t = Transform(position=rng.uniform(-100.0, 100.0, 3), rotation=random_quat(rng),
+                      scaling=np.full(3, rng.uniform(8.0, 20.0)))
+        sc.root.add_child(Entity(f"inst{i}", transform=t, model=model))
+    unique = sum(m.num_faces() for m in meshes)
+    inst_tris = sum(e.model.mesh.num_faces() for e in sc.root.children)
+    return SceneSetup(sc, width, height, max_recursions=6, num_samples=num_samples, name="config5", seed=seed,
+                      info={"unique_triangles": unique, "instanced_triangles": inst_tris, "instances": num_instances,
+                            "refractive": refractive})

@@ -75,3 +75,21 @@ def test_gpu_matches_golden_fixtures():
     assert np.array_equal(hits["primitive"][hit], ref["primitive"][hit])
     assert np.array_equal(hits["t"][hit], ref["t"][hit])
     ctx.close()
+
+
+def test_config5_scene_family():
+    """Config 5's generator: 8 meshes, 256 instances, one in five refractive
+    with IOR in [1.3, 1.7], maxRecursions 6, 4 samples; seeded."""
+    from raygun_amd import procgen
+    a = procgen.config5(64, 36, tris_per_mesh=2_000)
+    b = procgen.config5(64, 36, tris_per_mesh=2_000)
+    assert a.info == b.info
+    assert a.max_recursions == 6 and a.num_samples == 4
+    assert a.info["instances"] == 256 and 30 <= a.info["refractive"] <= 75
+    mats = [e.model.materials[0] for e in a.scene.root.children]
+    for m in mats:
+        if m["transparency"] > 0:
+            assert 1.3 <= m["ior"] <= 1.7
+    assert len({id(e.model.mesh) for e in a.scene.root.children}) == 8
+    ubo = a.ubo()
+    assert int(ubo["maxRecursions"]) == 6 and int(ubo["numSamples"]) == 4
