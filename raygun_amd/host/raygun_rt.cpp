@@ -565,7 +565,8 @@ rgrt_ubo cameraUbo(const Camera& camera, uint32_t width, uint32_t height, const 
     return u;
 }
 
-RenderSystem::RenderSystem(uint32_t width, uint32_t height, uint32_t flags, int32_t device) : m_width(width), m_height(height)
+RenderSystem::RenderSystem(uint32_t width, uint32_t height, uint32_t flags, int32_t device)
+    : m_width(width), m_height(height), m_flags(flags), m_device(device)
 {
     rgrt_config cfg{device, width, height, flags};
     const rgrt_status s = rgrt_create(&m_ctx, &cfg);
@@ -578,12 +579,32 @@ RenderSystem::RenderSystem(uint32_t width, uint32_t height, uint32_t flags, int3
         throw Error(s, msg);
     }
     m_raytracer = std::make_unique<Raytracer>(m_ctx);
+    m_last = m_ctx;
 }
 
 RenderSystem::~RenderSystem()
 {
+    m_extraRt.clear();
+    for(rgrt_ctx* c: m_extra) rgrt_destroy(c);
     m_raytracer.reset();
     if(m_ctx) rgrt_destroy(m_ctx);
+}
+
+void RenderSystem::setFramesInFlight(uint32_t n)
+{
+    if(n < 1) throw Error(RGRT_ERR_INVALID_ARG, "setFramesInFlight: n must be at least 1");
+    m_extraRt.clear();
+    for(rgrt_ctx* c: m_extra) rgrt_destroy(c);
+    m_extra.clear();
+    m_last = m_ctx;
+    m_frame = 0;
+    rgrt_config cfg{m_device, m_width, m_height, m_flags};
+    for(uint32_t k = 1; k < n; ++k) {
+        rgrt_ctx* c = nullptr;
+        ok(m_ctx, rgrt_create_shared(m_ctx, &cfg, &c), "setFramesInFlight");
+        m_extra.push_back(c);
+        m_extraRt.push_back(std::make_unique<Raytracer>(c));
+    }
 }
 
 void RenderSystem::setupModelBuffers(const Scene& scene)
@@ -594,6 +615,8 @@ void RenderSystem::setupModelBuffers(const Scene& scene)
     const auto models = scene.models();
     for(const auto& m: models)
         if(m->mesh) m->mesh->bottomLevelAS.reset(); // buffers change: trees are rebuilt
+    m_raytracer->resetTopLevelAS(); // new geometry drops every sharing context's TLAS
+    for(auto& rt: m_extraRt) rt->resetTopLevelAS();
     packModels(models, vertices, indices, materials);
     ok(m_ctx, rgrt_upload_geometry(m_ctx, vertices.data(), vertices.size(), indices.data(), indices.size()), "setupModelBuffers");
     ok(m_ctx, rgrt_upload_materials(m_ctx, materials.data(), (uint32_t)materials.size()), "setupModelBuffers");
@@ -613,26 +636,33 @@ rgrt_ubo RenderSystem::updateUniformBuffer(const Camera& camera, int32_t maxRecu
 rgrt_image RenderSystem::renderFrame(const Scene& scene, bool graph)
 {
     const rgrt_ubo u = cameraUbo(*scene.camera, m_width, m_height, m_lightDir, m_numSamples, m_maxRecursions);
-    return m_raytracer->doFrame(scene, u, graph);
+    const size_t k = (size_t)(m_frame++ % framesInFlight());
+    Raytracer& rt = k == 0 ? *m_raytracer : *m_extraRt[k - 1];
+    rt.useFXAA = m_raytracer->useFXAA; // the first ray tracer's settings hold for all
+    rt.refitTopLevelAS = m_raytracer->refitTopLevelAS;
+    const rgrt_image img = rt.doFrame(scene, u, graph);
+    m_last = rt.context();
+    return img;
 }
 
 rgrt_image RenderSystem::render(const Scene& scene)
 {
     m_raytracer->setupTopLevelAS(scene);
+    m_last = m_ctx;
     return m_raytracer->doRaytracing();
 }
 
 std::vector<uint16_t> RenderSystem::readImage(rgrt_image which) const
 {
     std::vector<uint16_t> out((size_t)m_width * m_height * 4);
-    ok(m_ctx, rgrt_read_image(m_ctx, which, out.data(), out.size() * 2), "readImage");
+    ok(m_last, rgrt_read_image(m_last, which, out.data(), out.size() * 2), "readImage");
     return out;
 }
 
 std::vector<uint8_t> RenderSystem::readImageRGBA8(rgrt_image which, rgrt_format format) const
 {
     std::vector<uint8_t> out((size_t)m_width * m_height * 4);
-    ok(m_ctx, rgrt_read_image_rgba8(m_ctx, which, format, out.data(), out.size()), "readImageRGBA8");
+    ok(m_last, rgrt_read_image_rgba8(m_last, which, format, out.data(), out.size()), "readImageRGBA8");
     return out;
 }
 

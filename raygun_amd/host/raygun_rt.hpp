@@ -267,6 +267,9 @@ struct Raytracer {
     // The reference rebuilds the TLAS every frame; with refitTopLevelAS set,
     // frames that only move instances refit it (TopLevelAS::update).
     bool refitTopLevelAS = false;
+    // new geometry drops the context's TLAS: the next frame builds one
+    void resetTopLevelAS() { m_topLevelAS.reset(); }
+    rgrt_ctx* context() const { return m_ctx; }
 
   private:
     rgrt_ctx* m_ctx;
@@ -293,6 +296,13 @@ struct RenderSystem {
     // scene camera with the settings of the last updateUniformBuffer, TLAS,
     // trace, post) as one rgrt_frame call; graph: one hipGraph launch
     rgrt_image renderFrame(const Scene& scene, bool graph = true);
+    // Frames in flight (no counterpart in the reference, which renders one
+    // frame at a time): n - 1 more contexts sharing the scene
+    // (rgrt_create_shared); renderFrame then rotates over the n contexts, so
+    // a frame starts while the previous ones still run. readImage and
+    // readImageRGBA8 read the most recent renderFrame's images.
+    void setFramesInFlight(uint32_t n);
+    uint32_t framesInFlight() const { return 1u + (uint32_t)m_extra.size(); }
     std::vector<uint16_t> readImage(rgrt_image which) const; // rgba16f bits
     // the swapchain blit's conversion (render_system.cpp:102-147) + readback:
     // 4 bytes per pixel in the given 8-bit format
@@ -305,8 +315,13 @@ struct RenderSystem {
 
   private:
     rgrt_ctx* m_ctx = nullptr;
-    uint32_t m_width, m_height;
+    uint32_t m_width, m_height, m_flags;
+    int32_t m_device;
     std::unique_ptr<Raytracer> m_raytracer;
+    std::vector<rgrt_ctx*> m_extra;                    // frames in flight beyond the first
+    std::vector<std::unique_ptr<Raytracer>> m_extraRt; // one per extra context
+    uint64_t m_frame = 0;
+    rgrt_ctx* m_last = nullptr; // context of the most recent renderFrame / render
     int32_t m_maxRecursions = 5, m_numSamples = 1;
     vec3 m_lightDir = {0.4, -0.6, -0.8};
 };

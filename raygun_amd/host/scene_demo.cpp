@@ -8,11 +8,14 @@
 //   scene_demo render W H OUT          render one frame; write the final image
 //                                      (rgba16f) to OUT and the frame constants
 //                                      to OUT.ubo
-//   scene_demo frames W H N OUT.ppm   N animated frames through rgrt_frame
+//   scene_demo frames W H N OUT.ppm [IN_FLIGHT]
+//                                      N animated frames through rgrt_frame
 //                                      (one hipGraph per frame, TLAS refit);
 //                                      the last one read back as 8-bit RGBA
 //                                      and written as PPM, the same frames
-//                                      traced eagerly compared byte for byte
+//                                      traced eagerly compared byte for byte;
+//                                      IN_FLIGHT contexts sharing the scene
+//                                      take the frames in turn
 //   scene_demo animate FRAMES          run FRAMES frames of the per-frame update
 //                                      (a scale animation on cube2, the group
 //                                      turning, cube4 moving) and print the
@@ -235,11 +238,13 @@ int main(int argc, char** argv)
             std::printf("],\n \"finished\": %d}\n", finished);
             return 0;
         }
-        if(mode == "frames" && argc == 6) {
+        if(mode == "frames" && (argc == 6 || argc == 7)) {
             const uint32_t W = (uint32_t)std::atoi(argv[2]), H = (uint32_t)std::atoi(argv[3]);
             const int frames = std::atoi(argv[4]);
+            const uint32_t inFlight = argc == 7 ? (uint32_t)std::atoi(argv[6]) : 1u;
             Scene g = demoScene(true), e = demoScene(true);
             render::RenderSystem rg(W, H), re(W, H);
+            rg.setFramesInFlight(inFlight);
             rg.setupModelBuffers(g);
             re.setupModelBuffers(e);
             rg.raytracer().refitTopLevelAS = true;
@@ -261,7 +266,7 @@ int main(int argc, char** argv)
                 }
             }
             render::writePPM(argv[5], last, W, H);
-            std::printf("%d frames %ux%u, graph == eager\n", frames, W, H);
+            std::printf("%d frames %ux%u, %u in flight, graph == eager\n", frames, W, H, inFlight);
             return 0;
         }
         if(mode == "render" && argc == 5) {
@@ -276,7 +281,7 @@ int main(int argc, char** argv)
             std::printf("rendered %ux%u, %zu instances\n", W, H, render::TopLevelAS::collectInstances(s).size());
             return 0;
         }
-        std::fprintf(stderr, "usage: scene_demo collect | render W H OUT | animate FRAMES | frames W H N OUT.ppm\n");
+        std::fprintf(stderr, "usage: scene_demo collect | render W H OUT | animate FRAMES | frames W H N OUT.ppm [IN_FLIGHT]\n");
         return 2;
     }
     catch(const Error& e) {
