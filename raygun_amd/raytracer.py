@@ -74,11 +74,32 @@ class RenderSystem:
         self.ctx = rgrt.Context(width, height, flags, device)
         self.raytracer = Raytracer(self.ctx)
         self.width, self.height = width, height
+        self._extra = []  # (context, Raytracer) per frame in flight beyond the first
+        self._frame = 0
+        self._last = self.ctx
+
+    def set_frames_in_flight(self, n):
+        """n - 1 more contexts sharing the scene (rgrt_create_shared):
+        render_frame then rotates over the n contexts, so a frame starts
+        while the previous ones still run; read() reads the latest frame.
+        No counterpart in the reference, which renders one frame at a time."""
+        if n < 1:
+            raise ValueError("frames in flight must be at least 1")
+        for c, _ in self._extra:
+            c.close()
+        self._extra = []
+        for _ in range(n - 1):
+            c = self.ctx.shared()
+            self._extra.append((c, Raytracer(c)))
+        self._frame = 0
+        self._last = self.ctx
 
     def setupModelBuffers(self, scene):
         models = scene.models()
         for m in models:
             m.mesh.blas = None
+        for rt in [self.raytracer] + [r for _, r in self._extra]:
+            rt._last_blas = None  # new geometry drops every sharing context's TLAS
         vertices, indices, materials = pack_models(models)
         self.ctx.upload_geometry(vertices, indices)
         self.ctx.upload_materials(materials)
@@ -101,16 +122,28 @@ class RenderSystem:
         settings of the last updateUniformBuffer, TLAS, trace, post) as one
         rgrt_frame call."""
         ubo = camera_ubo(scene.camera, self.width, self.height, **getattr(self, "_frame_settings", {}))
-        return self.raytracer.doFrame(scene, ubo, graph)
+        k = self._frame % (1 + len(self._extra))
+        self._frame += 1
+        ctx, rt = (self.ctx, self.raytracer) if k == 0 else self._extra[k - 1]
+        rt.use_fxaa, rt.refit_tlas = self.raytracer.use_fxaa, self.raytracer.refit_tlas
+        self._last = ctx
+        return rt.doFrame(scene, ubo, graph)
 
     def render(self, scene, ubo=None):
         self.raytracer.setupTopLevelAS(scene)
         if ubo is not None:
             self.raytracer.updateRenderTarget(ubo)
+        self._last = self.ctx
         return self.raytracer.doRaytracing()
 
     def read(self, which=rgrt.IMAGE_FINAL):
-        return self.ctx.read_image(which)
+        return self._last.read_image(which)
+
+    def close(self):
+        for c, _ in self._extra:
+            c.close()
+        self._extra = []
+        self.ctx.close()
 
 
 def half_to_float(img16):
