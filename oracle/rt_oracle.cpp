@@ -832,9 +832,9 @@ void ray_pre(RayPre& r, const float o[3], const float d[3])
     r.kx = kx;
     r.ky = ky;
     r.kz = kz;
-    r.Sx = d[kx] / d[kz];
-    r.Sy = d[ky] / d[kz];
-    r.Sz = 1.0f / d[kz];
+    r.Sz = 1.0f / d[kz]; // one division; the shears reuse it (as the HIP traversal does)
+    r.Sx = d[kx] * r.Sz;
+    r.Sy = d[ky] * r.Sz;
 }
 
 // Slab test in the fused form t = fma(bound, 1/d, -o/d); the far distance

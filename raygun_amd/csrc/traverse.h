@@ -35,11 +35,13 @@ struct HitRec {
 };
 
 // Reciprocal for the slab test: a zero component becomes +-1e-20 so the
-// fused form lo*inv - o*inv stays finite.
+// fused form lo*inv - o*inv stays finite. The box test only culls, so the
+// hardware reciprocal (v_rcp_f32, within 1 ulp) is enough; the far distance
+// is widened for its error (qbox_enter).
 __device__ __forceinline__ float safe_rcp(float d)
 {
     const float a = fabsf(d) > 1e-20f ? d : copysignf(1e-20f, d);
-    return 1.0f / a;
+    return __builtin_amdgcn_rcpf(a);
 }
 
 struct BoxRay {
@@ -73,7 +75,8 @@ __device__ __forceinline__ void cswap(float& ta, int32_t& ra, float& tb, int32_t
 // Entry distance of child c of a compressed node, +inf on a miss. A plane at
 // origin + q * 2^e lies at t = q * (2^e / d) + (origin - o) / d: one fused
 // multiply-add per plane (the box test only culls). The far distance is
-// widened by 1 + 2*gamma(3) (Ize 2013) to stay conservative.
+// widened by 1 + 2*gamma(3) (Ize 2013) plus twice the reciprocal's 1-ulp
+// error (near and far may come from different axes): 1 + 14 * 2^-24.
 __device__ __forceinline__ float qbox_enter(uint32_t wx, uint32_t hxw, uint32_t wy, uint32_t hyw, uint32_t wz, uint32_t hzw, int c,
                                             float ax, float bx, float ay, float by, float az, float bz, float tmin, float tmax)
 {
@@ -83,7 +86,7 @@ __device__ __forceinline__ float qbox_enter(uint32_t wx, uint32_t hxw, uint32_t 
     const float z0 = __builtin_fmaf((float)((wz >> sh) & 0xffu), az, bz), z1 = __builtin_fmaf((float)((hzw >> sh) & 0xffu), az, bz);
     const float n = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), tmin));
     const float f = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tmax));
-    return n <= f * 1.0000004f ? n : INFINITY;
+    return n <= f * 1.0000008f ? n : INFINITY;
 }
 
 // Visits a compressed wide node (64 bytes, four 16-byte loads): tests the
@@ -157,9 +160,9 @@ __device__ __forceinline__ void wt_setup(WtRay& w, float ox, float oy, float oz,
     w.ky = ky;
     w.kz = kz;
     const float dkx = sel3(dx, dy, dz, kx), dky = sel3(dx, dy, dz, ky);
-    w.Sx = dkx / dkz;
-    w.Sy = dky / dkz;
-    w.Sz = 1.0f / dkz;
+    w.Sz = 1.0f / dkz; // one division; the shears reuse it
+    w.Sx = dkx * w.Sz;
+    w.Sy = dky * w.Sz;
     w.okx = sel3(ox, oy, oz, kx);
     w.oky = sel3(ox, oy, oz, ky);
     w.okz = sel3(ox, oy, oz, kz);
