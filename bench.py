@@ -34,7 +34,7 @@ RAY_BYTES = 40    # queue entry: origin, tmin, dir, tmax, pixel, flags
 HIT_BYTES = 20    # t, u, v, instance, primitive
 GATHER_BYTES = 184  # closest-hit gathers: 12 B indices + 96 B vertices + 64 B material + 12 B offsets
 # config 5 (4 samples per pixel, depth 6): far longer per-pixel ray chains
-CONFIG5_CANDIDATES = {"wavefrontPasses": [12, 24, 32, 48, 64], "traceBlocksPerCU": [4, 6, 8],
+CONFIG5_CANDIDATES = {"wavefrontPasses": [24, 32, 48, 64], "traceBlocksPerCU": [4, 6, 8],
                       "drainBlocksPerCU": [2, 4, 8]}
 # launch shapes the autotuner tries for the headline pipeline (coordinate descent)
 TUNE_CANDIDATES = {"wavefrontPasses": [6, 8, 10, 12, 14], "traceBlocksPerCU": [4, 6, 8],
