@@ -110,6 +110,9 @@ def main():
                     help="scenes of the batched workload measured after the headline (0 = skip)")
     ap.add_argument("--no-config5", action="store_true",
                     help="skip the config-5 measurement (3840x2160, 16M unique triangles, 4 spp, depth 6) after the headline")
+    ap.add_argument("--profile-workload", choices=("config5", "batched"), default="",
+                    help="run only that secondary workload and print its JSON object (for profiler runs; "
+                         "no headline line)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="run only the counter-calibration copy kernel over MIB MiB")
     args = ap.parse_args()
@@ -117,6 +120,13 @@ def main():
     os.environ["GPU_MAX_HW_QUEUES"] = str(max(1, min(args.hw_queues, 32)))
     if args.copy_probe:
         copy_probe(args.copy_probe)
+        return
+    if args.profile_workload:
+        if args.profile_workload == "config5":
+            out = bench_config5(max(2, args.steps))
+        else:
+            out = bench_batched(args.batch_scenes or 32, max(2, args.steps))
+        print(json.dumps(out), flush=True)
         return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
