@@ -490,6 +490,15 @@ typedef struct rgrt_tuning {
  * 1..32 blocks per CU or a hostLag outside 1..16 are rejected. */
 rgrt_status rgrt_set_tuning(rgrt_ctx* ctx, const rgrt_tuning* t);
 rgrt_status rgrt_get_tuning(rgrt_ctx* ctx, rgrt_tuning* t);
+/* No counterpart in the reference (its raygen loops over a pixel's samples,
+ * raygen.h:83-112). With numSamples > 1 the samples of a pixel are
+ * independent (each starts from a fresh payload), so up to maxGroups of
+ * them are traced side by side in separate wavefront slots; each sample's
+ * record is kept and the records are summed in sample order afterwards, so
+ * the images equal those of one sample at a time bit for bit. 0 (default):
+ * as many as fit 2^25 slots (~1.26 KB of device memory each); 1: one
+ * sample at a time; at most 32. */
+rgrt_status rgrt_set_sample_groups(rgrt_ctx* ctx, uint32_t maxGroups);
 
 rgrt_status rgrt_get_stats(rgrt_ctx* ctx, rgrt_stats* out);
 /* Rays traced in each wavefront pass of the last rgrt_trace / rgrt_render

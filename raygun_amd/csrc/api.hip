@@ -180,6 +180,9 @@ struct rgrt_ctx {
     rgrt_ray* dRays = nullptr;
     rgrt_hit* dHits = nullptr;
     uint32_t rayCap = 0;
+    size_t slotCap = 0;       // wavefront slots the slot arrays hold
+    size_t sampleCap = 0;     // per-sample records the sample buffers hold
+    uint32_t sampleGroups = 0; // most sample groups side by side (0: as many as fit)
 
     uint32_t* hostCounts = nullptr; // pinned, device-visible
     uint32_t* hostCountsDev = nullptr;
@@ -324,6 +327,10 @@ void free_images(rgrt_ctx* c)
     dev_free(s.rays[0]);
     dev_free(s.rays[1]);
     dev_free(s.hits);
+    dev_free(s.sampleAcc);
+    dev_free(s.sampleLum2);
+    c->slotCap = 0;
+    c->sampleCap = 0;
     dev_free(s.counters);
     s.npix = 0;
 }
@@ -356,6 +363,10 @@ void alloc_images(rgrt_ctx* c, uint32_t w, uint32_t h)
     check(hipMemsetAsync(m.final_, 0, npix * 8, c->stream));
     WavefrontState& s = c->st;
     s.npix = (uint32_t)npix;
+    s.nimg = (uint32_t)npix;
+    s.groups = 1;
+    s.numSamples = 1;
+    c->slotCap = npix;
     dev_alloc(s.payload, npix * kPayloadWords);
     dev_alloc(s.accum, npix * kAccumWords);
     dev_alloc(s.meta, npix);
@@ -466,6 +477,54 @@ ShadeParams shade_params(const rgrt_ctx* c)
     return p;
 }
 
+// Wavefront slots at most (≈ 1.26 KB of state each: payload, frames, queues).
+constexpr uint64_t kMaxSlots = 1ull << 25;
+
+// Sample groups for the frame constants in c->ubo: with several samples per
+// pixel, up to numSamples of a pixel's samples run in separate slots side by
+// side (each has its own payload, so the order within a sample is kept and
+// the samples' records are summed in sample order afterwards), as many as
+// kMaxSlots and the context's sampleGroups allow. The slot arrays grow here,
+// before anything is enqueued or captured.
+void prepare_sample_groups(rgrt_ctx* c)
+{
+    WavefrontState& s = c->st;
+    const uint32_t n = (uint32_t)std::max(1, c->ubo.numSamples);
+    uint32_t g = 1;
+    if(n > 1 && c->sampleGroups != 1 && s.nimg) {
+        g = (uint32_t)std::min<uint64_t>(n, std::max<uint64_t>(1, kMaxSlots / s.nimg));
+        if(c->sampleGroups > 1) g = std::min(g, c->sampleGroups);
+    }
+    const size_t slots = (size_t)s.nimg * g;
+    if(slots > c->slotCap) {
+        check(hipStreamSynchronize(c->stream)); // queued frames still use the old arrays
+        dev_alloc(s.payload, slots * kPayloadWords);
+        dev_alloc(s.meta, slots);
+        dev_alloc(s.frames, slots * kFrameStride * kMaxFrames);
+        dev_alloc(s.rays[0], slots * kRayWords);
+        dev_alloc(s.rays[1], slots * kRayWords);
+        dev_alloc(s.hits, slots * kHitWords);
+        c->slotCap = slots;
+    }
+    if(g > 1 && (size_t)n * s.nimg > c->sampleCap) {
+        check(hipStreamSynchronize(c->stream));
+        dev_alloc(s.sampleAcc, (size_t)n * s.nimg * kAccumWords);
+        dev_alloc(s.sampleLum2, (size_t)n * s.nimg);
+        c->sampleCap = (size_t)n * s.nimg;
+    }
+    s.npix = (uint32_t)slots;
+    s.groups = g;
+    s.numSamples = n;
+}
+
+// Bound on the passes until every slot is done: the rays of one slot's
+// samples one after another, plus raygen's pass and one to spare.
+uint32_t chain_iters(const rgrt_ctx* c)
+{
+    const uint32_t perSlot = (c->st.numSamples + c->st.groups - 1) / c->st.groups;
+    return max_sequential_rays(c->ubo.maxRecursions) * perSlot + 2;
+}
+
 rgrt_status do_trace(rgrt_ctx* c)
 {
     if(!c->haveTlas) return fail(c, RGRT_ERR_NOT_READY, "rgrt_trace: no TLAS built");
@@ -473,10 +532,10 @@ rgrt_status do_trace(rgrt_ctx* c)
     if(!c->w || !c->h) return fail(c, RGRT_ERR_NOT_READY, "rgrt_trace: zero-sized render target");
     hipStream_t s = c->stream;
     const bool timing = (c->flags & RGRT_CFG_TIMING) != 0;
+    prepare_sample_groups(c);
     const ShadeParams p = shade_params(c);
 
-    const uint32_t perSample = max_sequential_rays(c->ubo.maxRecursions);
-    const uint32_t maxIters = perSample * (uint32_t)c->ubo.numSamples + 2;
+    const uint32_t maxIters = chain_iters(c);
     ensure_host_counts(c, maxIters + 2);
     const uint32_t lag = c->tuning.hostLag;
     const int ring = (int)std::max<uint32_t>(8u, lag + 2u);
@@ -547,6 +606,7 @@ rgrt_status do_trace(rgrt_ctx* c)
         check(hipMemcpy(&remaining, c->st.counters + cur, 4, hipMemcpyDeviceToHost));
         if(remaining) return fail(c, RGRT_ERR_HIP, "rgrt_trace: ray queue did not drain within the recursion bound");
     }
+    launch_finalize_samples(s, p);
     if(timing) check(hipEventRecord(c->evRtOnlyEnd, s));
     c->lastTraceLaunches = traceLaunches;
     c->lastDrained = drained;
@@ -562,9 +622,8 @@ rgrt_status do_trace(rgrt_ctx* c)
 void enqueue_trace_fixed(rgrt_ctx* c)
 {
     hipStream_t s = c->stream;
-    const ShadeParams p = shade_params(c);
-    const uint32_t perSample = max_sequential_rays(c->ubo.maxRecursions);
-    const uint32_t maxIters = perSample * (uint32_t)c->ubo.numSamples + 2;
+    const ShadeParams p = shade_params(c); // sample groups prepared by frame_prepare
+    const uint32_t maxIters = chain_iters(c);
     const uint32_t passes = std::min<uint32_t>(c->tuning.wavefrontPasses, maxIters);
     const TraceScene sc = c->scene();
     launch_raygen(s, p);
@@ -575,6 +634,7 @@ void enqueue_trace_fixed(rgrt_ctx* c)
         cur ^= 1;
     }
     launch_drain(s, p, sc, cur, c->drainGrid, c->spill);
+    launch_finalize_samples(s, p);
     c->lastTraceLaunches = passes;
     c->lastDrained = true;
     c->lastWaves = passes;
@@ -1149,8 +1209,8 @@ rgrt_status frame_prepare(rgrt_ctx* c, const rgrt_ubo* ubo, const rgrt_instance*
     if(!c->w || !c->h) return fail(c, RGRT_ERR_NOT_READY, "frame: zero-sized render target");
     stage_to_host(c, in, n);
     reserve_builder(c, std::max<uint32_t>(n, 1));
-    const uint32_t maxIters = max_sequential_rays(c->ubo.maxRecursions) * (uint32_t)c->ubo.numSamples + 2;
-    ensure_host_counts(c, maxIters + 2);
+    prepare_sample_groups(c);
+    ensure_host_counts(c, chain_iters(c) + 2);
     return RGRT_OK;
 }
 
@@ -1716,6 +1776,15 @@ rgrt_status rgrt_set_tuning(rgrt_ctx* c, const rgrt_tuning* t)
         if(t->hostLag < 1 || t->hostLag > 16) return fail(c, RGRT_ERR_INVALID_ARG, "set_tuning: hostLag outside 1..16");
         c->tuning = *t;
         c->apply_tuning();
+        return RGRT_OK;
+    });
+}
+
+rgrt_status rgrt_set_sample_groups(rgrt_ctx* c, uint32_t maxGroups)
+{
+    return guarded(c, [&]() {
+        if(maxGroups > 32) return fail(c, RGRT_ERR_INVALID_ARG, "set_sample_groups: more than 32");
+        c->sampleGroups = maxGroups;
         return RGRT_OK;
     });
 }

@@ -81,7 +81,15 @@ constexpr uint32_t kFlagPrimary = 2u;   // primary ray of sample 0 (AOV record)
 constexpr uint32_t kMaskShift = 8u;     // cull mask in bits 8..15
 
 struct WavefrontState {
-    uint32_t npix = 0;
+    uint32_t npix = 0; // wavefront slots: image pixels x sample groups
+    uint32_t nimg = 0; // image pixels
+    // Samples of a pixel traced side by side: slot g * nimg + i runs samples
+    // g, g + groups, ... of the pixel in tile order i and stores each
+    // sample's record; k_finalize_samples sums them in sample order.
+    uint32_t groups = 1;
+    uint32_t numSamples = 1;     // of the frame being traced
+    float* sampleAcc = nullptr;  // [numSamples][nimg][kAccumWords] (groups > 1)
+    float* sampleLum2 = nullptr; // [numSamples][nimg] (groups > 1, stochastic sampling)
     // Per-slot records are contiguous (a lane reads its slot's record with
     // 16-byte loads, and consecutive lanes read consecutive records).
     float* payload = nullptr; // [npix][kPayloadWords]
@@ -197,6 +205,7 @@ class Denoiser {
 void launch_raygen(hipStream_t s, const ShadeParams& p);
 void launch_trace_queue(hipStream_t s, const TraceScene& sc, const WavefrontState& st, int cur, uint32_t gridBlocks);
 void launch_shade(hipStream_t s, const ShadeParams& p, int cur, uint32_t gridBlocks);
+void launch_finalize_samples(hipStream_t s, const ShadeParams& p);
 void launch_drain(hipStream_t s, const ShadeParams& p, const TraceScene& sc, int cur, uint32_t gridBlocks, int32_t* spill);
 uint32_t drain_spill_words_per_thread();
 uint32_t drain_block_size();

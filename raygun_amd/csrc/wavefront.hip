@@ -580,6 +580,30 @@ __device__ __forceinline__ void slot_pixel(uint32_t slot, uint32_t W, uint32_t H
     py = y0 + i / tw;
 }
 
+// Tile-order pixel index of a wavefront slot (slots of sample group g
+// follow those of group g - 1).
+__device__ __forceinline__ uint32_t slot_image(const ShadeParams& p, uint32_t slot)
+{
+    return p.st.groups == 1u ? slot : slot % p.st.nimg;
+}
+
+// Record of one finished sample of tile-order pixel i (sample groups > 1):
+// the values accumulate() would add, summed later in sample order.
+__device__ void store_sample(const ShadeParams& p, uint32_t i, int sample, const Payload& P)
+{
+    F3 col = P.hitValue;
+    const size_t at = (size_t)sample * p.st.nimg + i;
+    if(p.sampling == RGRT_SAMPLING_STOCHASTIC) {
+        col = mix(P.hitValue, f3(P.rv[0], P.rv[1], P.rv[2]), P.reflectContribution);
+        const float l = (0.299f * col.x + 0.587f * col.y) + 0.114f * col.z;
+        p.st.sampleLum2[at] = l * l;
+    }
+    float4* r = reinterpret_cast<float4*>(p.st.sampleAcc) + at * 3;
+    r[0] = make_float4(col.x, col.y, col.z, P.normal.x);
+    r[1] = make_float4(P.normal.y, P.normal.z, P.rv[0], P.rv[1]);
+    r[2] = make_float4(P.rv[2], P.rv[3], P.reflectContribution, P.depth);
+}
+
 __device__ void finalize(const ShadeParams& p, uint32_t slot)
 {
     const float4* av = reinterpret_cast<const float4*>(p.st.accum) + (size_t)slot * 3;
@@ -637,18 +661,59 @@ __global__ __launch_bounds__(kShadeBlock) void k_raygen(ShadeParams p)
         for(int k = 4; k < 12; ++k) c[k] = 0; // ray / sky / zero-dir counters (u64)
     }
     if(pix >= p.st.npix) return;
+    // slot `pix` starts sample g (its group) of tile-order pixel i
+    const uint32_t g = p.st.groups == 1u ? 0u : pix / p.st.nimg;
+    const uint32_t i = pix - g * p.st.nimg;
     uint32_t px, py;
-    slot_pixel(pix, p.width, p.height, px, py);
+    slot_pixel(i, p.width, p.height, px, py);
     OutRay r;
-    primary_ray(p.ubo, px, py, p.width, p.height, 0, p.sampling, p.seed, r);
+    primary_ray(p.ubo, px, py, p.width, p.height, (int)g, p.sampling, p.seed, r);
     Payload P;
     init_payload(P);
     store_payload(p, pix, P);
-    float4* a = reinterpret_cast<float4*>(p.st.accum) + (size_t)pix * 3;
-    for(int k = 0; k < 3; ++k) a[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    if(p.sampling == RGRT_SAMPLING_STOCHASTIC) p.st.lum2[pix] = 0.0f;
-    p.st.meta[pix] = 0u;
-    write_ray(p.st.rays[0], p.st.npix, pix, r, pix, kFlagPrimary | (0xffu << kMaskShift));
+    if(g == 0u) {
+        float4* a = reinterpret_cast<float4*>(p.st.accum) + (size_t)i * 3;
+        for(int k = 0; k < 3; ++k) a[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if(p.sampling == RGRT_SAMPLING_STOCHASTIC) p.st.lum2[i] = 0.0f;
+    }
+    p.st.meta[pix] = g << 8;
+    // only sample 0's primary ray carries the primary-hit flag (AOV, features)
+    write_ray(p.st.rays[0], p.st.npix, pix, r, pix, (g == 0u ? kFlagPrimary : 0u) | (0xffu << kMaskShift));
+}
+
+// Sample groups > 1: every pixel's sample records summed in sample order
+// (the additions accumulate() makes one sample at a time), then the outputs.
+__global__ __launch_bounds__(kShadeBlock) void k_finalize_samples(ShadeParams p)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if(i >= p.st.nimg) return;
+    float4 a0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), a1 = a0, a2 = a0;
+    float l2 = 0.0f;
+    const bool stochastic = p.sampling == RGRT_SAMPLING_STOCHASTIC;
+    for(uint32_t s = 0; s < p.st.numSamples; ++s) {
+        const size_t at = (size_t)s * p.st.nimg + i;
+        const float4* r = reinterpret_cast<const float4*>(p.st.sampleAcc) + at * 3;
+        const float4 r0 = r[0], r1 = r[1], r2 = r[2];
+        a0.x = a0.x + r0.x;
+        a0.y = a0.y + r0.y;
+        a0.z = a0.z + r0.z;
+        a0.w = a0.w + r0.w;
+        a1.x = a1.x + r1.x;
+        a1.y = a1.y + r1.y;
+        a1.z = a1.z + r1.z;
+        a1.w = a1.w + r1.w;
+        a2.x = a2.x + r2.x;
+        a2.y = a2.y + r2.y;
+        a2.z = a2.z + r2.z;
+        a2.w = a2.w + r2.w;
+        if(stochastic) l2 = l2 + p.st.sampleLum2[at];
+    }
+    float4* a = reinterpret_cast<float4*>(p.st.accum) + (size_t)i * 3;
+    a[0] = a0;
+    a[1] = a1;
+    a[2] = a2;
+    if(stochastic) p.st.lum2[i] = l2;
+    finalize(p, i);
 }
 
 // One closest-hit or miss stage for a traced ray of pixel `pix`, then the
@@ -660,10 +725,11 @@ __device__ bool advance_pixel(const Ctx& c0, uint32_t pix, uint32_t flags, F3 O,
 {
     const ShadeParams& p = c0.p;
     Ctx c{c0.p, c0.L, c0.maxRec, 0u};
+    const uint32_t tile = slot_image(p, pix); // the pixel in tile order
     uint32_t imgPix = 0;
     if(p.sampling == RGRT_SAMPLING_STOCHASTIC) {
         uint32_t px, py;
-        slot_pixel(pix, p.width, p.height, px, py);
+        slot_pixel(tile, p.width, p.height, px, py);
         imgPix = py * p.width + px;
         c.key = sample_key(p.seed, imgPix, (uint32_t)sample);
         if(flags & kFlagPrimary) { // denoiser features of sample 0's primary hit
@@ -689,7 +755,7 @@ __device__ bool advance_pixel(const Ctx& c0, uint32_t pix, uint32_t flags, F3 O,
         a.instance = hit ? p.inst[min(h.inst, p.numInstances - 1)].customIndex : kNoHit;
         a.primitive = hit ? h.prim : 0u;
         uint32_t px, py;
-        slot_pixel(pix, p.width, p.height, px, py);
+        slot_pixel(tile, p.width, p.height, px, py);
         p.img.primary[py * p.width + px] = a;
     }
     Frame F;
@@ -724,18 +790,21 @@ __device__ bool advance_pixel(const Ctx& c0, uint32_t pix, uint32_t flags, F3 O,
         }
         // step == RET: the innermost call returned
         if(sp == 0) {
-            accumulate(p, pix, P);
-            sample++;
+            if(p.st.groups == 1u)
+                accumulate(p, pix, P);
+            else
+                store_sample(p, tile, sample, P);
+            sample += (int)p.st.groups;
             if(sample < p.ubo.numSamples) {
                 init_payload(P);
                 uint32_t px, py;
-                slot_pixel(pix, p.width, p.height, px, py);
+                slot_pixel(tile, p.width, p.height, px, py);
                 primary_ray(p.ubo, px, py, p.width, p.height, sample, p.sampling, p.seed, out);
                 if(p.sampling == RGRT_SAMPLING_STOCHASTIC) c.key = sample_key(p.seed, imgPix, (uint32_t)sample);
                 step = EMIT;
                 continue;
             }
-            finalize(p, pix);
+            if(p.st.groups == 1u) finalize(p, pix); // otherwise k_finalize_samples does
             return false;
         }
         sp--;
@@ -893,6 +962,11 @@ void launch_raygen(hipStream_t s, const ShadeParams& p)
 {
     const uint32_t blocks = (p.st.npix + kShadeBlock - 1) / kShadeBlock;
     k_raygen<<<blocks, kShadeBlock, 0, s>>>(p);
+}
+
+void launch_finalize_samples(hipStream_t s, const ShadeParams& p)
+{
+    if(p.st.groups > 1u) k_finalize_samples<<<(p.st.nimg + kShadeBlock - 1) / kShadeBlock, kShadeBlock, 0, s>>>(p);
 }
 
 void launch_shade(hipStream_t s, const ShadeParams& p, int cur, uint32_t gridBlocks)

@@ -40,7 +40,7 @@ EXPORTS = [
     "rgrt_build_tlas", "rgrt_refit_tlas", "rgrt_export_tlas", "rgrt_set_frame", "rgrt_resize",
     "rgrt_trace_rays", "rgrt_trace_rays_device", "rgrt_render", "rgrt_frame", "rgrt_trace", "rgrt_postprocess",
     "rgrt_read_image", "rgrt_read_image_rgba8", "rgrt_image_device_ptr", "rgrt_write_image", "rgrt_get_stats", "rgrt_get_pass_counts",
-    "rgrt_set_tuning", "rgrt_get_tuning", "rgrt_set_sampling", "rgrt_denoiser_create", "rgrt_denoiser_destroy",
+    "rgrt_set_tuning", "rgrt_get_tuning", "rgrt_set_sample_groups", "rgrt_set_sampling", "rgrt_denoiser_create", "rgrt_denoiser_destroy",
     "rgrt_denoiser_num_params", "rgrt_denoiser_get_params", "rgrt_denoiser_set_params", "rgrt_denoiser_loss_and_grads",
     "rgrt_denoiser_dataset_reserve", "rgrt_denoiser_dataset_add_input", "rgrt_denoiser_dataset_add_target",
     "rgrt_denoiser_train", "rgrt_denoise", "rgrt_batch_create", "rgrt_batch_destroy", "rgrt_batch_last_error", "rgrt_batch_set_mode", "rgrt_batch_stream",
@@ -111,6 +111,7 @@ def lib():
             "rgrt_get_stats": (i32, [P, P]),
             "rgrt_get_pass_counts": (i32, [P, P, u32, ctypes.POINTER(u32)]),
             "rgrt_set_tuning": (i32, [P, P]),
+            "rgrt_set_sample_groups": (i32, [P, u32]),
             "rgrt_set_sampling": (i32, [P, P]),
             "rgrt_export_blas_wide": (i32, [P, u32, i32, P, u32, ctypes.POINTER(u32)]),
             "rgrt_export_tlas_wide": (i32, [P, i32, P, u32, ctypes.POINTER(u32)]),
@@ -295,6 +296,10 @@ class Context:
 
     def trace_rays_device(self, rays_ptr, n, cull_mask, hits_ptr):
         self._ok(self._L.rgrt_trace_rays_device(self.h, rays_ptr, n, cull_mask, hits_ptr))
+
+    def set_sample_groups(self, max_groups):
+        """Samples of a pixel traced side by side (0: as many as fit, 1: one at a time)."""
+        self._ok(self._L.rgrt_set_sample_groups(self.h, max_groups))
 
     def set_sampling(self, mode, seed=0):
         """SAMPLING_WHITTED (the reference's tracer) or SAMPLING_STOCHASTIC
