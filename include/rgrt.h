@@ -491,7 +491,7 @@ typedef struct rgrt_tuning {
 rgrt_status rgrt_set_tuning(rgrt_ctx* ctx, const rgrt_tuning* t);
 rgrt_status rgrt_get_tuning(rgrt_ctx* ctx, rgrt_tuning* t);
 /* No counterpart in the reference (its raygen loops over a pixel's samples,
- * raygen.h:83-112). With numSamples > 1 the samples of a pixel are
+ * raygen.h:84-112). With numSamples > 1 the samples of a pixel are
  * independent (each starts from a fresh payload), so up to maxGroups of
  * them are traced side by side in separate wavefront slots; each sample's
  * record is kept and the records are summed in sample order afterwards, so
