@@ -130,7 +130,7 @@ def main():
         return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this build runs on one GPU only (the frame is one GPU's share); "
+        print("bench.py: measures one GPU only (the frame is one GPU's share); "
               "--gpus above 1 is not supported", file=sys.stderr)
         sys.exit(2)
 
