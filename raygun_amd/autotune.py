@@ -43,8 +43,9 @@ def tune(ctx, frame, candidates=None, rounds=1, reps=2, log=None, contexts=None,
     table = [(dict(best), best_ms)]
     for _ in range(rounds):
         for name, values in cand.items():
+            current = best[name]  # already timed with the other parameters as they are
             for v in values:
-                if v == best[name]:
+                if v == current:
                     continue
                 t = dict(best)
                 t[name] = v
