@@ -15,7 +15,14 @@ constexpr int kSpill = 368; // with the 16 LDS entries: the 381-entry worst case
 
 // Wavefront trace pass: rays [0, count) of queue `cur` -> hit records.
 // Waves fetch 64 rays at a time from a global counter (persistent grid).
-__global__ __launch_bounds__(kTraceBlock) void k_trace_queue(TraceScene sc, const float* __restrict__ rays,
+// Waves per SIMD the traversal pass is compiled for: the register budget is
+// 512 / waves VGPRs (6 fits its working set without spilling; more waves
+// hide more of the node-fetch latency).
+#ifndef RGRT_TRACE_WAVES
+#define RGRT_TRACE_WAVES 6
+#endif
+
+__global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(RGRT_TRACE_WAVES))) void k_trace_queue(TraceScene sc, const float* __restrict__ rays,
                                                               float* __restrict__ hits, uint32_t npix,
                                                               uint32_t* __restrict__ counters, int cur,
                                                               int32_t* __restrict__ spillBase, uint32_t* hostSlot)
