@@ -561,6 +561,43 @@ __device__ __forceinline__ WideEntry pick(bool c, const WideEntry& a, const Wide
     return r;
 }
 
+// The children of a wide node rooted at binary node r: its binary node's two
+// children, then repeatedly the internal child of largest surface area
+// (the first on equal areas) opened in place, until there are four or only
+// leaves. Returns how many of S[0..4) are used.
+template <class Src>
+__device__ __forceinline__ int wide_children(const Src& src, int r, WideEntry S[4])
+{
+    const WideEntry e0 = src.slot(r, 0), e1 = src.slot(r, 1);
+    const bool h0 = e0.ref != kEmptyChild, h1 = e1.ref != kEmptyChild;
+    S[0] = pick(h0, e0, e1);
+    S[1] = e1;
+    int ns = (int)h0 + (int)h1;
+    while(ns < 4) {
+        int j = -1, e = 0;
+        float best = -1.0f;
+#pragma unroll
+        for(int q = 0; q < 4; ++q) {
+            if(q < ns && S[q].ref >= 0) {
+                const float a = half_area(S[q].lo, S[q].hi);
+                if(a > best) {
+                    best = a;
+                    j = q;
+                    e = S[q].ref;
+                }
+            }
+        }
+        if(j < 0) break;
+        const WideEntry a0 = src.slot(e, 0), a1 = src.slot(e, 1);
+#pragma unroll
+        for(int q = 3; q >= 1; --q) S[q] = pick(q > j + 1 && q <= ns, S[q - 1], S[q]);
+#pragma unroll
+        for(int q = 0; q < 4; ++q) S[q] = pick(q == j, a0, pick(q == j + 1, a1, S[q]));
+        ++ns;
+    }
+    return ns;
+}
+
 // One workgroup of kCollapseThreads walks the binary tree breadth-first. A
 // wide node starts from its binary node's two children and opens the
 // internal child of largest surface area (the first on equal areas) until it
@@ -589,36 +626,7 @@ __device__ int collapse_bfs(const Src src, Node4* __restrict__ out, Node4q* __re
             const bool valid = k < fcount;
             WideEntry S[4];
             int ns = 0;
-            if(valid) {
-                const int r = front[k];
-                const WideEntry e0 = src.slot(r, 0), e1 = src.slot(r, 1);
-                const bool h0 = e0.ref != kEmptyChild, h1 = e1.ref != kEmptyChild;
-                S[0] = pick(h0, e0, e1);
-                S[1] = e1;
-                ns = (int)h0 + (int)h1;
-                while(ns < 4) {
-                    int j = -1, e = 0;
-                    float best = -1.0f;
-#pragma unroll
-                    for(int q = 0; q < 4; ++q) {
-                        if(q < ns && S[q].ref >= 0) {
-                            const float a = half_area(S[q].lo, S[q].hi);
-                            if(a > best) {
-                                best = a;
-                                j = q;
-                                e = S[q].ref;
-                            }
-                        }
-                    }
-                    if(j < 0) break;
-                    const WideEntry a0 = src.slot(e, 0), a1 = src.slot(e, 1);
-#pragma unroll
-                    for(int q = 3; q >= 1; --q) S[q] = pick(q > j + 1 && q <= ns, S[q - 1], S[q]);
-#pragma unroll
-                    for(int q = 0; q < 4; ++q) S[q] = pick(q == j, a0, pick(q == j + 1, a1, S[q]));
-                    ++ns;
-                }
-            }
+            if(valid) ns = wide_children(src, front[k], S);
             int cnt = 0;
 #pragma unroll
             for(int q = 0; q < 4; ++q) cnt += (q < ns && S[q].ref >= 0) ? 1 : 0;
@@ -684,12 +692,133 @@ __device__ int collapse_bfs(const Src src, Node4* __restrict__ out, Node4q* __re
     return base;
 }
 
-__global__ __launch_bounds__(kCollapseThreads) void k_collapse(const Node* __restrict__ nodes, Node4* __restrict__ out,
-                                                               int32_t* frontA, int32_t* frontB, uint32_t* numWide)
+// The collapse of a tree in HBM in two steps. Which children a wide node
+// gets depends only on the subtree below its binary node, so the first step
+// works that out for every binary node at once (k_wide_candidates, one
+// thread per node: candidate wide nodes whose internal child refs are still
+// binary node indices). The breadth-first numbering then needs one 128-byte
+// load per wide node instead of a chain of dependent node reads
+// (k_collapse_numbered, one workgroup); the result is the wide tree
+// collapse_bfs builds, node for node.
+__global__ void k_wide_candidates(const Node* __restrict__ nodes, int numInternal, Node4* __restrict__ cand)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if(r >= numInternal) return;
+    WideEntry S[4];
+    const int ns = wide_children(GlobalNodes{nodes}, r, S);
+    Node4 w;
+#pragma unroll
+    for(int q = 0; q < 4; ++q) {
+        const bool used = q < ns;
+        w.lox[q] = used ? S[q].lo[0] : INFINITY;
+        w.loy[q] = used ? S[q].lo[1] : INFINITY;
+        w.loz[q] = used ? S[q].lo[2] : INFINITY;
+        w.hix[q] = used ? S[q].hi[0] : -INFINITY;
+        w.hiy[q] = used ? S[q].hi[1] : -INFINITY;
+        w.hiz[q] = used ? S[q].hi[2] : -INFINITY;
+        w.child[q] = used ? S[q].ref : kEmptyChild;
+        w.pad[q] = 0;
+    }
+    cand[r] = w;
+}
+
+constexpr int kCollapseItems = 4; // frontier entries per thread and chunk (consecutive)
+
+__global__ __launch_bounds__(kCollapseThreads) void k_collapse_numbered(const Node4* __restrict__ cand, int numInternal,
+                                                                        Node4* __restrict__ out, int32_t* frontA,
+                                                                        int32_t* frontB, uint32_t* numWide)
 {
     __shared__ int32_t waveTot[kCollapseThreads / 64];
-    const int base = collapse_bfs(GlobalNodes{nodes}, out, nullptr, frontA, frontB, waveTot);
-    if(threadIdx.x == 0) *numWide = (uint32_t)base;
+    constexpr int kWavesPerBlock = kCollapseThreads / 64;
+    constexpr int kChunk = kCollapseThreads * kCollapseItems;
+    // a chunk's candidates and renumbered child refs, for the coalesced copy
+    __shared__ int32_t chunkNode[kChunk];
+    __shared__ u32x4 chunkRefs[kChunk];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    int32_t* front = frontA;
+    int32_t* next = frontB;
+    if(tid == 0) front[0] = 0;
+    __syncthreads();
+    int fcount = 1, base = 0;
+    for(int level = 0; level < 128 && fcount > 0; ++level) {
+        int nextCount = 0;
+        for(int chunk = 0; chunk < fcount; chunk += kChunk) {
+            // thread tid takes frontier entries k0 .. k0 + kCollapseItems - 1 (in order)
+            const int k0 = chunk + tid * kCollapseItems;
+            int r[kCollapseItems];
+            u32x4 ch[kCollapseItems];
+            int cnt = 0;
+            // the entries' binary nodes first, then their candidates' child refs
+            // (each group of loads issued back to back)
+#pragma unroll
+            for(int j = 0; j < kCollapseItems; ++j) {
+                const int k = k0 + j;
+                const int f = k < fcount ? front[k] : -1;
+                r[j] = k < fcount ? (f < numInternal && f >= 0 ? f : 0) : -1;
+            }
+#pragma unroll
+            for(int j = 0; j < kCollapseItems; ++j) {
+                ch[j] = u32x4{0x80000000u, 0x80000000u, 0x80000000u, 0x80000000u}; // kEmptyChild
+                if(r[j] >= 0) ch[j] = ld_global16(reinterpret_cast<const u32x4*>(cand + r[j]) + 6);
+            }
+#pragma unroll
+            for(int j = 0; j < kCollapseItems; ++j)
+                cnt += ((int)ch[j].x >= 0) + ((int)ch[j].y >= 0) + ((int)ch[j].z >= 0) + ((int)ch[j].w >= 0);
+            int incl = cnt;
+#pragma unroll
+            for(int off = 1; off < 64; off <<= 1) {
+                const int t = __shfl_up(incl, off);
+                if(lane >= off) incl += t;
+            }
+            if(lane == 63) waveTot[wave] = incl;
+            __syncthreads();
+            int excl = incl - cnt, total = 0;
+#pragma unroll
+            for(int wv = 0; wv < kWavesPerBlock; ++wv) {
+                const int t = waveTot[wv];
+                excl += wv < wave ? t : 0;
+                total += t;
+            }
+            // internal children in entry and slot order get consecutive next-level numbers
+            int at = nextCount + excl;
+#pragma unroll
+            for(int j = 0; j < kCollapseItems; ++j) {
+                if(r[j] < 0) continue;
+                const uint32_t c0 = ch[j].x, c1 = ch[j].y, c2 = ch[j].z, c3 = ch[j].w;
+                const int i0 = at, i1 = i0 + ((int)c0 >= 0), i2 = i1 + ((int)c1 >= 0), i3 = i2 + ((int)c2 >= 0);
+                at = i3 + ((int)c3 >= 0);
+                if((int)c0 >= 0) next[i0] = (int32_t)c0;
+                if((int)c1 >= 0) next[i1] = (int32_t)c1;
+                if((int)c2 >= 0) next[i2] = (int32_t)c2;
+                if((int)c3 >= 0) next[i3] = (int32_t)c3;
+                chunkNode[tid * kCollapseItems + j] = r[j];
+                chunkRefs[tid * kCollapseItems + j] = u32x4{(int)c0 >= 0 ? (uint32_t)(base + fcount + i0) : c0,
+                                                            (int)c1 >= 0 ? (uint32_t)(base + fcount + i1) : c1,
+                                                            (int)c2 >= 0 ? (uint32_t)(base + fcount + i2) : c2,
+                                                            (int)c3 >= 0 ? (uint32_t)(base + fcount + i3) : c3};
+            }
+            __syncthreads(); // chunkNode / chunkRefs complete
+            // copy the chunk's nodes to their places: consecutive lanes move
+            // consecutive 16-byte pieces (8 per node), so a wave writes 1 KB at once
+            const int items = min(kChunk, fcount - chunk);
+            for(int piece = tid; piece < items * 8; piece += kCollapseThreads) {
+                const int it = piece >> 3, q = piece & 7;
+                const u32x4 v = q == 6 ? chunkRefs[it] : ld_global16(reinterpret_cast<const u32x4*>(cand + chunkNode[it]) + q);
+                reinterpret_cast<u32x4*>(out + base + chunk + it)[q] = v;
+            }
+            nextCount += total;
+            __syncthreads(); // waveTot[], the chunk arrays and next[] settle before the next chunk / level
+        }
+        base += fcount;
+        fcount = nextCount;
+        int32_t* t = front;
+        front = next;
+        next = t;
+        __threadfence_block();
+        __syncthreads();
+    }
+    if(tid == 0) *numWide = (uint32_t)base;
 }
 
 // ---------------------------------------------------------------------------
@@ -1068,17 +1197,19 @@ void LbvhBuilder::reserve(uint32_t n)
     check(hipMalloc(&counters, cap * 4ull));
     check(hipMalloc(&sceneBox, 64));
     check(hipMalloc(&nodeRange, cap * 8ull));
+    check(hipMalloc(&wideTmp, cap * sizeof(Node4)));
     capacity = cap;
 }
 
 void LbvhBuilder::release()
 {
     for(void* p: {(void*)keysA, (void*)keysB, (void*)valsA, (void*)valsB, (void*)hist, (void*)boxLo, (void*)boxHi,
-                  (void*)leafParent, (void*)counters, (void*)sceneBox, (void*)nodeRange})
+                  (void*)leafParent, (void*)counters, (void*)sceneBox, (void*)nodeRange, (void*)wideTmp})
         if(p) (void)hipFree(p);
     keysA = keysB = valsA = valsB = hist = leafParent = counters = sceneBox = nullptr;
     boxLo = boxHi = nullptr;
     nodeRange = nullptr;
+    wideTmp = nullptr;
     capacity = 0;
 }
 
@@ -1148,9 +1279,10 @@ void LbvhBuilder::collapse(hipStream_t s, uint32_t n, const Node* nodes, Node4* 
 {
     // the sort's second key/value buffers are free after the build: they
     // hold the two breadth-first frontiers (at most n - 1 entries each)
-    (void)n;
-    k_collapse<<<1, kCollapseThreads, 0, s>>>(nodes, wide, reinterpret_cast<int32_t*>(keysB), reinterpret_cast<int32_t*>(valsB),
-                                              numWide);
+    const int numInternal = (int)std::max<uint32_t>(n, 2) - 1;
+    k_wide_candidates<<<blocks_for(numInternal, 256), 256, 0, s>>>(nodes, numInternal, wideTmp);
+    k_collapse_numbered<<<1, kCollapseThreads, 0, s>>>(wideTmp, numInternal, wide, reinterpret_cast<int32_t*>(keysB),
+                                                       reinterpret_cast<int32_t*>(valsB), numWide);
 }
 
 void LbvhBuilder::quantize(hipStream_t s, uint32_t capacity, const Node4* wide, const uint32_t* numWide, Node4q* out)

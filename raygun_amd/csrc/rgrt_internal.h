@@ -32,6 +32,7 @@ struct LbvhBuilder {
     uint32_t* counters = nullptr;
     uint32_t* sceneBox = nullptr;
     uint2* nodeRange = nullptr; // first and last leaf of every internal node
+    Node4* wideTmp = nullptr;   // candidate wide node of every binary node (collapse)
 
     void reserve(uint32_t n);
     void release();
