@@ -366,7 +366,7 @@ __global__ void k_fill_u32(uint32_t* p, uint32_t n, uint32_t v)
 // then finishes those upper nodes from the recorded arrivals (workgroup-scope
 // ordering only). A node's box is always fminf/fmaxf(slot 0, slot 1), so the
 // result does not depend on the order of arrival.
-constexpr int kRefitChunk = 256;
+constexpr int kRefitChunk = 1024;
 constexpr int kRefitTopThreads = 1024;
 
 __device__ __forceinline__ void write_slot(Node* nd, int slot, const float lo[3], const float hi[3])
