@@ -206,28 +206,77 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_hist(const uint32_t* __re
     hist[threadIdx.x * numBlocks + blockIdx.x] = cnt[threadIdx.x];
 }
 
-// Exclusive scan of `total` counters in one workgroup of 1024 threads.
-__global__ __launch_bounds__(1024) void k_sort_scan(uint32_t* __restrict__ hist, uint32_t total)
+// Exclusive prefix sum of the sort histogram over many workgroups (it has
+// ~n/8 entries for n keys): kScanBlocks blocks each total one contiguous
+// segment, one block scans the totals, then every block rescans its segment
+// from its offset, 256 consecutive entries at a time (coalesced).
+constexpr int kScanBlocks = 256;
+constexpr int kScanThreads = 256;
+
+__device__ __forceinline__ uint32_t block_sum256(uint32_t v, uint32_t* red)
 {
-    __shared__ uint32_t part[1024];
-    const uint32_t per = (total + 1023) / 1024;
-    const uint32_t b = threadIdx.x * per;
-    const uint32_t e = min(total, b + per);
-    uint32_t s = 0;
-    for(uint32_t i = b; i < e; ++i) s += hist[i];
-    part[threadIdx.x] = s;
+    for(int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if(lane == 0) red[wave] = v;
     __syncthreads();
-    for(uint32_t off = 1; off < 1024; off <<= 1) {
-        const uint32_t v = threadIdx.x >= off ? part[threadIdx.x - off] : 0u;
+    const uint32_t t = red[0] + red[1] + red[2] + red[3];
+    __syncthreads();
+    return t;
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const uint32_t* __restrict__ hist, uint32_t total,
+                                                              uint32_t* __restrict__ partial)
+{
+    __shared__ uint32_t red[kScanThreads / 64];
+    const uint32_t seg = (total + kScanBlocks - 1) / kScanBlocks;
+    const uint32_t b = blockIdx.x * seg, e = min(total, b + seg);
+    uint32_t s = 0;
+    for(uint32_t i = b + threadIdx.x; i < e; i += kScanThreads) s += hist[i];
+    s = block_sum256(s, red);
+    if(threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_partials(uint32_t* __restrict__ partial)
+{
+    __shared__ uint32_t part[kScanBlocks];
+    const uint32_t v = partial[threadIdx.x];
+    part[threadIdx.x] = v;
+    __syncthreads();
+    for(uint32_t off = 1; off < kScanBlocks; off <<= 1) {
+        const uint32_t t = threadIdx.x >= off ? part[threadIdx.x - off] : 0u;
         __syncthreads();
-        part[threadIdx.x] += v;
+        part[threadIdx.x] += t;
         __syncthreads();
     }
-    uint32_t run = part[threadIdx.x] - s; // exclusive prefix of this thread's chunk
-    for(uint32_t i = b; i < e; ++i) {
-        const uint32_t c = hist[i];
-        hist[i] = run;
-        run += c;
+    partial[threadIdx.x] = part[threadIdx.x] - v; // exclusive
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_apply(uint32_t* __restrict__ hist, uint32_t total,
+                                                             const uint32_t* __restrict__ partial)
+{
+    __shared__ uint32_t waveTot[kScanThreads / 64];
+    const uint32_t seg = (total + kScanBlocks - 1) / kScanBlocks;
+    const uint32_t b = blockIdx.x * seg, e = min(total, b + seg);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t carry = partial[blockIdx.x];
+    for(uint32_t base = b; base < e; base += kScanThreads) {
+        const uint32_t i = base + threadIdx.x;
+        const uint32_t c = i < e ? hist[i] : 0u;
+        uint32_t incl = c;
+        for(int off = 1; off < 64; off <<= 1) {
+            const uint32_t t = __shfl_up(incl, off);
+            if(lane >= off) incl += t;
+        }
+        if(lane == 63) waveTot[wave] = incl;
+        __syncthreads();
+        uint32_t before = 0, all = 0;
+        for(int w = 0; w < kScanThreads / 64; ++w) {
+            before += w < wave ? waveTot[w] : 0u;
+            all += waveTot[w];
+        }
+        if(i < e) hist[i] = carry + before + incl - c;
+        carry += all;
+        __syncthreads();
     }
 }
 
@@ -1230,7 +1279,10 @@ void LbvhBuilder::build(hipStream_t s, uint32_t n, Node* nodes, uint32_t* leafPr
     uint32_t *kin = keysA, *vin = valsA, *kout = keysB, *vout = valsB;
     for(int shift = 0; shift < 32; shift += 8) {
         k_sort_hist<<<nb, kSortThreads, 0, s>>>(kin, n, shift, hist, nb);
-        k_sort_scan<<<1, 1024, 0, s>>>(hist, 256u * nb);
+        // counters[] is free until the refit: it holds the segment totals
+        k_scan_reduce<<<kScanBlocks, kScanThreads, 0, s>>>(hist, 256u * nb, counters);
+        k_scan_partials<<<1, kScanThreads, 0, s>>>(counters);
+        k_scan_apply<<<kScanBlocks, kScanThreads, 0, s>>>(hist, 256u * nb, counters);
         k_sort_scatter<<<nb, kSortThreads, 0, s>>>(kin, vin, kout, vout, n, shift, hist, nb);
         std::swap(kin, kout);
         std::swap(vin, vout);
