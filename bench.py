@@ -110,6 +110,8 @@ def main():
                     help="scenes of the batched workload measured after the headline (0 = skip)")
     ap.add_argument("--no-config5", action="store_true",
                     help="skip the config-5 measurement (3840x2160, 16M unique triangles, 4 spp, depth 6) after the headline")
+    ap.add_argument("--config5-in-flight", type=int, choices=(1, 2, 3, 4, 5, 6), default=4,
+                    help="frames in flight for the config-5 measurement")
     ap.add_argument("--profile-workload", choices=("config5", "batched"), default="",
                     help="run only that secondary workload and print its JSON object (for profiler runs; "
                          "no headline line)")
@@ -123,7 +125,7 @@ def main():
         return
     if args.profile_workload:
         if args.profile_workload == "config5":
-            out = bench_config5(max(2, args.steps))
+            out = bench_config5(max(2, args.steps), args.config5_in_flight)
         else:
             out = bench_batched(args.batch_scenes or 32, max(2, args.steps))
         print(json.dumps(out), flush=True)
@@ -343,7 +345,7 @@ def main():
     ctx.close()
     if not args.no_config5:
         try:
-            result["config5"] = bench_config5(max(4, args.steps // 6))
+            result["config5"] = bench_config5(max(4, args.steps // 6), args.config5_in_flight)
         except Exception as e:  # the headline stands without it
             log(f"[bench] config5 failed: {e!r}")
     if args.batch_scenes:
@@ -354,7 +356,7 @@ def main():
     print(json.dumps(result), flush=True)
 
 
-def bench_config5(frames, in_flight=2):
+def bench_config5(frames, in_flight=4):
     """Config 5's plain tracer: 3840x2160, 8 seeded displaced meshes of ~2M
     triangles (16M unique), 256 instances, one in five refractive,
     maxRecursions 6, 4 samples per pixel (the SSAA table), FXAA; frames as
