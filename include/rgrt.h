@@ -338,7 +338,9 @@ rgrt_status rgrt_denoiser_train(rgrt_denoiser* den, uint32_t steps, uint32_t bat
                                 float* loss);
 /* Denoises the context's current frame (asynchronous on the context stream).
  * With fallbackTiles != NULL waits and reports how many of the *tiles fell
- * back to the traced frame. */
+ * back to the traced frame. RGRT_ERR_NOT_READY until the network has
+ * parameters from rgrt_denoiser_set_params or rgrt_denoiser_train (a fresh
+ * denoiser holds only its random initialisation). */
 rgrt_status rgrt_denoise(rgrt_denoiser* den, float fallbackA, float fallbackB, uint32_t* fallbackTiles, uint32_t* tiles);
 
 /* rgrt_frame flags */

@@ -1665,152 +1665,113 @@ void tex_sample(const Img16& img, float px, float py, int ox, int oy, float out[
     }
 }
 
-inline float luma_at(const Img16& img, float px, float py, int ox = 0, int oy = 0)
+// FXAA, quality variant, with Raygun's settings (fxaa.comp:29-36: subpixel
+// quality 1.0, edge threshold 0.063, edge threshold minimum 0.0312; preset
+// 39's strides 1,1,1,1,1,1.5,2,2,2,2,4,8 from fxaa.h:487-501; green as luma,
+// fxaa.h:201), restated from the published algorithm (Lottes, "FXAA", NVIDIA
+// white paper 2009; FXAA 3.11 quality): contrast gate on the 4-neighbour
+// luma range; edge orientation from 1-2-1 weighted second differences of
+// the 3x3 luma; the neighbour across the edge with the larger luma step
+// picks the side and the reference luma (mean of the two); a walk along the
+// half-pixel line in both directions until the luma leaves the reference by
+// a quarter of that step; a shift across the edge of 0.5 - near / span when
+// the nearer end turns away from the pixel, bounded below by the squared
+// smoothstep of the 3x3 mean's contrast; resample there, alpha = luma.
+// Walk distances are exact texel counts (the strides' running sums).
+struct FxaaNeighbourhood {
+    // luma[r][c] for rows y-1..y+1 and columns x-1..x+1
+    float luma[3][3];
+    float centre[4];
+};
+
+float sample_green(const Img16& img, float u, float v, int ox, int oy)
 {
     float c[4];
-    tex_sample(img, px, py, ox, oy, c);
-    return c[1]; // FXAA_GREEN_AS_LUMA = 1 (fxaa.h:201)
+    tex_sample(img, u, v, ox, oy, c);
+    return c[1];
 }
 
-// FXAA 3.11 "quality" on the PC path, preset 39, subpix 1.0, edge
-// threshold 0.063, edge threshold min 0.0312 (fxaa.comp:33-34), early exit
-// on, green as luma, no gather4.
+float walk_reach(int k)
+{
+    static const float reach[12] = {1.0f, 2.0f, 3.0f, 4.0f, 5.0f, 6.5f, 8.5f, 10.5f, 12.5f, 14.5f, 18.5f, 26.5f};
+    return reach[k];
+}
+
 void fxaa_pixel(const Img16& tex, int x, int y, float out[4])
 {
-    const float kSteps[12] = {1.0f, 1.0f, 1.0f, 1.0f, 1.0f, 1.5f, 2.0f, 2.0f, 2.0f, 2.0f, 4.0f, 8.0f};
-    const float subpix = 1.0f, edgeThreshold = 0.063f, edgeThresholdMin = 0.0312f;
-    const float rcpX = 1.0f / (float)tex.W, rcpY = 1.0f / (float)tex.H;
-    float posMx = ((float)x + 0.5f) / (float)tex.W;
-    float posMy = ((float)y + 0.5f) / (float)tex.H;
-
-    float rgbyM[4];
-    tex_sample(tex, posMx, posMy, 0, 0, rgbyM);
-    const float lumaM = rgbyM[1];
-    float lumaS = luma_at(tex, posMx, posMy, 0, 1);
-    float lumaE = luma_at(tex, posMx, posMy, 1, 0);
-    float lumaN = luma_at(tex, posMx, posMy, 0, -1);
-    float lumaW = luma_at(tex, posMx, posMy, -1, 0);
-
-    float maxSM = std::fmax(lumaS, lumaM), minSM = std::fmin(lumaS, lumaM);
-    float maxESM = std::fmax(lumaE, maxSM), minESM = std::fmin(lumaE, minSM);
-    float maxWN = std::fmax(lumaN, lumaW), minWN = std::fmin(lumaN, lumaW);
-    float rangeMax = std::fmax(maxWN, maxESM), rangeMin = std::fmin(minWN, minESM);
-    float rangeMaxScaled = rangeMax * edgeThreshold;
-    float range = rangeMax - rangeMin;
-    float rangeMaxClamped = std::fmax(edgeThresholdMin, rangeMaxScaled);
-    if(range < rangeMaxClamped) {
-        for(int k = 0; k < 4; ++k) out[k] = rgbyM[k];
+    const float u0 = ((float)x + 0.5f) / (float)tex.W, v0 = ((float)y + 0.5f) / (float)tex.H;
+    const float texel[2] = {1.0f / (float)tex.W, 1.0f / (float)tex.H};
+    FxaaNeighbourhood nb;
+    tex_sample(tex, u0, v0, 0, 0, nb.centre);
+    nb.luma[1][1] = nb.centre[1];
+    // the four edge neighbours first: the gate needs only them
+    nb.luma[0][1] = sample_green(tex, u0, v0, 0, -1);
+    nb.luma[2][1] = sample_green(tex, u0, v0, 0, 1);
+    nb.luma[1][0] = sample_green(tex, u0, v0, -1, 0);
+    nb.luma[1][2] = sample_green(tex, u0, v0, 1, 0);
+    const float c = nb.luma[1][1];
+    const float n = nb.luma[0][1], s = nb.luma[2][1], w = nb.luma[1][0], e = nb.luma[1][2];
+    const float hi = std::fmax(std::fmax(std::fmax(n, s), std::fmax(w, e)), c);
+    const float lo = std::fmin(std::fmin(std::fmin(n, s), std::fmin(w, e)), c);
+    const float range = hi - lo;
+    if(range < std::fmax(0.0312f, hi * 0.063f)) {
+        for(int k = 0; k < 4; ++k) out[k] = nb.centre[k];
         return;
     }
-    float lumaNW = luma_at(tex, posMx, posMy, -1, -1);
-    float lumaSE = luma_at(tex, posMx, posMy, 1, 1);
-    float lumaNE = luma_at(tex, posMx, posMy, 1, -1);
-    float lumaSW = luma_at(tex, posMx, posMy, -1, 1);
+    for(int r = 0; r < 3; r += 2)
+        for(int q = 0; q < 3; q += 2) nb.luma[r][q] = sample_green(tex, u0, v0, q - 1, r - 1);
+    const float nw = nb.luma[0][0], ne = nb.luma[0][2], sw = nb.luma[2][0], se = nb.luma[2][2];
 
-    float lumaNS = lumaN + lumaS, lumaWE = lumaW + lumaE;
-    float subpixRcpRange = 1.0f / range;
-    float subpixNSWE = lumaNS + lumaWE;
-    float edgeHorz1 = (-2.0f * lumaM) + lumaNS;
-    float edgeVert1 = (-2.0f * lumaM) + lumaWE;
-    float lumaNESE = lumaNE + lumaSE, lumaNWNE = lumaNW + lumaNE;
-    float edgeHorz2 = (-2.0f * lumaE) + lumaNESE;
-    float edgeVert2 = (-2.0f * lumaN) + lumaNWNE;
-    float lumaNWSW = lumaNW + lumaSW, lumaSWSE = lumaSW + lumaSE;
-    float edgeHorz4 = (std::fabs(edgeHorz1) * 2.0f) + std::fabs(edgeHorz2);
-    float edgeVert4 = (std::fabs(edgeVert1) * 2.0f) + std::fabs(edgeVert2);
-    float edgeHorz3 = (-2.0f * lumaW) + lumaNWSW;
-    float edgeVert3 = (-2.0f * lumaS) + lumaSWSE;
-    float edgeHorz = std::fabs(edgeHorz3) + edgeHorz4;
-    float edgeVert = std::fabs(edgeVert3) + edgeVert4;
-    float subpixNWSWNESE = lumaNWSW + lumaNESE;
-    float lengthSign = rcpX;
-    bool horzSpan = edgeHorz >= edgeVert;
-    float subpixA = subpixNSWE * 2.0f + subpixNWSWNESE;
-    if(!horzSpan) {
-        lumaN = lumaW;
-        lumaS = lumaE;
-    }
-    if(horzSpan) lengthSign = rcpY;
-    float subpixB = (subpixA * (float)(1.0 / 12.0)) - lumaM;
-    float gradientN = lumaN - lumaM, gradientS = lumaS - lumaM;
-    float lumaNN = lumaN + lumaM, lumaSS = lumaS + lumaM;
-    bool pairN = std::fabs(gradientN) >= std::fabs(gradientS);
-    float gradient = std::fmax(std::fabs(gradientN), std::fabs(gradientS));
-    if(pairN) lengthSign = -lengthSign;
-    float subpixC = clampf(std::fabs(subpixB) * subpixRcpRange, 0.0f, 1.0f);
+    // |second difference| of a line of three lumas
+    auto curv = [](float a, float b, float d) { return std::fabs((a + d) - 2.0f * b); };
+    const float rowsChange = 2.0f * curv(n, c, s) + (curv(nw, w, sw) + curv(ne, e, se));
+    const float colsChange = 2.0f * curv(w, c, e) + (curv(nw, n, ne) + curv(sw, s, se));
+    const int alongAxis = rowsChange >= colsChange ? 0 : 1; // 0: the edge runs along x
+    const int acrossAxis = 1 - alongAxis;
 
-    float posBx = posMx, posBy = posMy;
-    float offNPx = (!horzSpan) ? 0.0f : rcpX;
-    float offNPy = (horzSpan) ? 0.0f : rcpY;
-    if(!horzSpan) posBx += lengthSign * 0.5f;
-    if(horzSpan) posBy += lengthSign * 0.5f;
+    const float mean3x3 = (2.0f * ((n + s) + (w + e)) + ((nw + ne) + (sw + se))) / 12.0f;
+    const float t = clampf(std::fabs(mean3x3 - c) / range, 0.0f, 1.0f);
+    const float smooth = (t * t) * (3.0f - 2.0f * t);
+    const float subpixelShift = (smooth * smooth) * 1.0f;
 
-    float posNx = posBx - offNPx * kSteps[0], posNy = posBy - offNPy * kSteps[0];
-    float posPx = posBx + offNPx * kSteps[0], posPy = posBy + offNPy * kSteps[0];
-    float subpixD = ((-2.0f) * subpixC) + 3.0f;
-    float lumaEndN = luma_at(tex, posNx, posNy);
-    float subpixE = subpixC * subpixC;
-    float lumaEndP = luma_at(tex, posPx, posPy);
-    if(!pairN) lumaNN = lumaSS;
-    float gradientScaled = (gradient * 1.0f) / 4.0f;
-    float lumaMM = lumaM - lumaNN * 0.5f;
-    float subpixF = subpixD * subpixE;
-    bool lumaMLTZero = lumaMM < 0.0f;
-    lumaEndN -= lumaNN * 0.5f;
-    lumaEndP -= lumaNN * 0.5f;
-    bool doneN = std::fabs(lumaEndN) >= gradientScaled;
-    bool doneP = std::fabs(lumaEndP) >= gradientScaled;
-    if(!doneN) {
-        posNx -= offNPx * kSteps[1];
-        posNy -= offNPy * kSteps[1];
-    }
-    bool doneNP = (!doneN) || (!doneP);
-    if(!doneP) {
-        posPx += offNPx * kSteps[1];
-        posPy += offNPy * kSteps[1];
-    }
-    for(int step = 2; step < 12 && doneNP; ++step) {
-        if(!doneN) lumaEndN = luma_at(tex, posNx, posNy);
-        if(!doneP) lumaEndP = luma_at(tex, posPx, posPy);
-        if(!doneN) lumaEndN = lumaEndN - lumaNN * 0.5f;
-        if(!doneP) lumaEndP = lumaEndP - lumaNN * 0.5f;
-        doneN = std::fabs(lumaEndN) >= gradientScaled;
-        doneP = std::fabs(lumaEndP) >= gradientScaled;
-        if(!doneN) {
-            posNx -= offNPx * kSteps[step];
-            posNy -= offNPy * kSteps[step];
-        }
-        doneNP = (!doneN) || (!doneP);
-        if(!doneP) {
-            posPx += offNPx * kSteps[step];
-            posPy += offNPy * kSteps[step];
+    // the two neighbours across the edge: (before, after) along the across axis
+    const float before = alongAxis == 0 ? n : w, after = alongAxis == 0 ? s : e;
+    const float dBefore = before - c, dAfter = after - c;
+    const bool sideBefore = std::fabs(dBefore) >= std::fabs(dAfter);
+    const float acrossStep = sideBefore ? -texel[acrossAxis] : texel[acrossAxis];
+    const float reference = (c + (sideBefore ? before : after)) * 0.5f;
+    const float stopAt = std::fmax(std::fabs(dBefore), std::fabs(dAfter)) * 0.25f;
+    const bool pixelDarker = (c - reference) < 0.0f;
+
+    float start[2] = {u0, v0};
+    start[acrossAxis] = start[acrossAxis] + acrossStep * 0.5f;
+    float reach[2], endDelta[2];
+    for(int side = 0; side < 2; ++side) { // side 0 walks toward -along, side 1 toward +along
+        const float dir = side == 0 ? -texel[alongAxis] : texel[alongAxis];
+        reach[side] = walk_reach(11);
+        endDelta[side] = 0.0f;
+        for(int k = 0; k < 11; ++k) {
+            float p[2] = {start[0], start[1]};
+            p[alongAxis] = p[alongAxis] + dir * walk_reach(k);
+            endDelta[side] = sample_green(tex, p[0], p[1], 0, 0) - reference;
+            if(std::fabs(endDelta[side]) >= stopAt) {
+                reach[side] = walk_reach(k);
+                break;
+            }
         }
     }
-    float dstN = posMx - posNx, dstP = posPx - posMx;
-    if(!horzSpan) {
-        dstN = posMy - posNy;
-        dstP = posPy - posMy;
-    }
-    bool goodSpanN = (lumaEndN < 0.0f) != lumaMLTZero;
-    float spanLength = dstP + dstN;
-    bool goodSpanP = (lumaEndP < 0.0f) != lumaMLTZero;
-    float spanLengthRcp = 1.0f / spanLength;
-    bool directionN = dstN < dstP;
-    float dst = std::fmin(dstN, dstP);
-    bool goodSpan = directionN ? goodSpanN : goodSpanP;
-    float subpixG = subpixF * subpixF;
-    float pixelOffset = (dst * (-spanLengthRcp)) + 0.5f;
-    float subpixH = subpixG * subpix;
-    float pixelOffsetGood = goodSpan ? pixelOffset : 0.0f;
-    float pixelOffsetSubpix = std::fmax(pixelOffsetGood, subpixH);
-    if(!horzSpan) posMx += pixelOffsetSubpix * lengthSign;
-    if(horzSpan) posMy += pixelOffsetSubpix * lengthSign;
-    float c[4];
-    tex_sample(tex, posMx, posMy, 0, 0, c);
-    out[0] = c[0];
-    out[1] = c[1];
-    out[2] = c[2];
-    out[3] = lumaM;
+    const int nearer = reach[0] < reach[1] ? 0 : 1;
+    const bool opposite = (endDelta[nearer] < 0.0f) != pixelDarker;
+    const float edgeShift = opposite ? 0.5f - reach[nearer] / (reach[0] + reach[1]) : 0.0f;
+    const float shift = std::fmax(edgeShift, subpixelShift);
+    float fin[2] = {u0, v0};
+    fin[acrossAxis] = fin[acrossAxis] + shift * acrossStep;
+    float col[4];
+    tex_sample(tex, fin[0], fin[1], 0, 0, col);
+    out[0] = col[0];
+    out[1] = col[1];
+    out[2] = col[2];
+    out[3] = c;
 }
 
 template<typename F>

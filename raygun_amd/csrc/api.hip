@@ -1670,6 +1670,10 @@ rgrt_status rgrt_denoise(rgrt_denoiser* den, float fallbackA, float fallbackB, u
             return fail(c, RGRT_ERR_INVALID_ARG, "denoise: render size must be a multiple of 4 in each direction");
         if(c->sampling.mode != RGRT_SAMPLING_STOCHASTIC)
             return fail(c, RGRT_ERR_NOT_READY, "denoise: the frame must be traced with RGRT_SAMPLING_STOCHASTIC");
+        if(!den->d->has_trained_params())
+            return fail(c, RGRT_ERR_NOT_READY,
+                        "denoise: the network has no trained parameters (load them with rgrt_denoiser_set_params or "
+                        "train it with rgrt_denoiser_train first)");
         const uint32_t nt = den->d->denoise(c->stream, c->img, fallbackA, fallbackB);
         if(tiles) *tiles = nt;
         if(fallbackTiles) *fallbackTiles = den->d->count_fallback(c->stream, nt);

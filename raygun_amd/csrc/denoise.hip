@@ -584,6 +584,7 @@ void Denoiser::set_params(hipStream_t s, const float* host)
     check(hipStreamSynchronize(s));
     stepCount = 0;
     weightsReady = false;
+    paramsLoaded = true;
 }
 
 void Denoiser::get_params(hipStream_t s, float* host)
@@ -835,6 +836,7 @@ float Denoiser::train(hipStream_t s, uint32_t steps, uint32_t batch, float lr, u
         (void)hipFree(tgt);
         throw;
     }
+    paramsLoaded = true;
     return accN ? (float)(acc / accN) : 0.0f;
 }
 

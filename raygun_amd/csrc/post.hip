@@ -7,9 +7,10 @@
 // launch: a 32x32 output tile is computed from a 52x52 region (10-pixel
 // halo, one pixel per H or V pass), the two colour ping-pong images live in
 // LDS as fp16 exactly as the reference's rgba16f images hold them between
-// passes, and transitions are stored R8_SNORM. k_fxaa is FXAA 3.11 quality,
-// preset 39 (fxaa.comp:29-36) with an emulated bilinear clamp-to-edge
-// sampler (8-bit sub-texel weights).
+// passes, and transitions are stored R8_SNORM. k_fxaa is FXAA (the quality
+// variant, Raygun's preset 39 settings, fxaa.comp:29-36) written from the
+// published algorithm, with an emulated bilinear clamp-to-edge sampler
+// (8-bit sub-texel weights).
 #pragma clang fp contract(off)
 
 #include "rgrt_internal.h"
@@ -170,7 +171,31 @@ __global__ void k_show_alpha(Images img)
 }
 
 // ---------------------------------------------------------------------------
-// FXAA
+// FXAA (Lottes, "FXAA" white paper, NVIDIA 2009, and the FXAA 3.11 quality
+// algorithm as published): Raygun runs it with preset 39, subpixel quality
+// 1.0, edge threshold 0.063, edge threshold minimum 0.0312 and the green
+// channel as luma (fxaa.comp:29-36; knobs fxaa.h:178-216, 487-501).
+//
+// Per pixel:
+//  1. contrast gate: luma of the pixel and its 4 neighbours; a pixel whose
+//     luma range is below max(0.0312, 0.063 * brightest) keeps its colour;
+//  2. orientation: the second differences across rows (weights 1-2-1 over
+//     the three columns) against those across columns; the larger says
+//     whether the edge runs horizontally or vertically;
+//  3. side: of the two neighbours across the edge, the one with the larger
+//     luma step; the edge lies half a pixel toward it, and its luma and the
+//     pixel's average to the edge's reference luma;
+//  4. walk: from that half-pixel point, along the edge in both directions
+//     with the preset's growing strides, sampling (bilinearly, so the two
+//     rows either side of the edge are averaged) until the luma leaves the
+//     reference by a quarter of the step found in 3;
+//  5. blend: if the nearer end turns the opposite way from the pixel, the
+//     pixel is moved across the edge by 0.5 - (distance to the nearer end)
+//     / (length of the span); a subpixel term from the 3x3 neighbourhood's
+//     mean luma (smoothstep of its contrast, squared) is the lower bound;
+//     the colour is resampled there, and alpha carries the pixel's luma.
+// Distances along the edge are kept as exact texel counts (the strides'
+// running sums), so the span ratio of 5 has no accumulated rounding.
 // ---------------------------------------------------------------------------
 struct Tex {
     const uint16_t* p;
@@ -203,157 +228,100 @@ __device__ __forceinline__ Px4 tex_sample(const Tex& t, float px, float py, int 
     return r;
 }
 
-// FXAA_GREEN_AS_LUMA = 1: luma is the green channel.
-__device__ __forceinline__ float luma_at(const Tex& t, float px, float py, int ox = 0, int oy = 0)
+// Running sums of preset 39's strides (1, 1, 1, 1, 1, 1.5, 2, 2, 2, 2, 4, 8):
+// the distance in texels from the half-pixel start after each stride.
+__constant__ float kWalkReach[12] = {1.0f, 2.0f, 3.0f, 4.0f, 5.0f, 6.5f, 8.5f, 10.5f, 12.5f, 14.5f, 18.5f, 26.5f};
+constexpr int kWalkProbes = 11; // samples per direction; the last stride moves without sampling
+
+struct EdgeEnd {
+    float reach; // texels from the start point
+    float delta; // luma there minus the reference luma (of the last sample taken)
+};
+
+// One direction of step 4: (su, sv) is the start point, (du, dv) one texel
+// along the edge in texture coordinates, signed for the direction.
+__device__ __forceinline__ EdgeEnd walk_edge(const Tex& t, float su, float sv, float du, float dv, float refLuma, float limit)
 {
-    return tex_sample(t, px, py, ox, oy).y;
+    EdgeEnd end{kWalkReach[kWalkProbes], 0.0f};
+    for(int k = 0; k < kWalkProbes; ++k) {
+        const float r = kWalkReach[k];
+        const float d = tex_sample(t, su + du * r, sv + dv * r, 0, 0).y - refLuma;
+        end.delta = d;
+        if(fabsf(d) >= limit) {
+            end.reach = r;
+            break;
+        }
+    }
+    return end;
 }
 
-__constant__ float kFxaaSteps[12] = {1.0f, 1.0f, 1.0f, 1.0f, 1.0f, 1.5f, 2.0f, 2.0f, 2.0f, 2.0f, 4.0f, 8.0f};
+__device__ __forceinline__ float second_diff(float a, float b, float c) { return fabsf((a + c) - 2.0f * b); }
 
 __global__ __launch_bounds__(256) void k_fxaa(const uint16_t* __restrict__ src, uint16_t* __restrict__ out, int W, int H)
 {
     const int x = blockIdx.x * 16 + (threadIdx.x & 15);
     const int y = blockIdx.y * 16 + (threadIdx.x >> 4);
     if(x >= W || y >= H) return;
-    const Tex tex{src, W, H};
-    const float subpix = 1.0f, edgeThreshold = 0.063f, edgeThresholdMin = 0.0312f;
-    const float rcpX = 1.0f / (float)W, rcpY = 1.0f / (float)H;
-    float posMx = ((float)x + 0.5f) / (float)W;
-    float posMy = ((float)y + 0.5f) / (float)H;
+    const Tex t{src, W, H};
+    const float texelU = 1.0f / (float)W, texelV = 1.0f / (float)H;
+    const float u0 = ((float)x + 0.5f) / (float)W, v0 = ((float)y + 0.5f) / (float)H;
+    ushort4* dst = reinterpret_cast<ushort4*>(out) + (size_t)y * W + x;
 
-    const Px4 rgbyM = tex_sample(tex, posMx, posMy, 0, 0);
-    const float lumaM = rgbyM.y;
-    float lumaS = luma_at(tex, posMx, posMy, 0, 1);
-    float lumaE = luma_at(tex, posMx, posMy, 1, 0);
-    float lumaN = luma_at(tex, posMx, posMy, 0, -1);
-    float lumaW = luma_at(tex, posMx, posMy, -1, 0);
-    const float maxSM = fmaxf(lumaS, lumaM), minSM = fminf(lumaS, lumaM);
-    const float maxESM = fmaxf(lumaE, maxSM), minESM = fminf(lumaE, minSM);
-    const float maxWN = fmaxf(lumaN, lumaW), minWN = fminf(lumaN, lumaW);
-    const float rangeMax = fmaxf(maxWN, maxESM), rangeMin = fminf(minWN, minESM);
-    const float rangeMaxScaled = rangeMax * edgeThreshold;
-    const float range = rangeMax - rangeMin;
-    const float rangeMaxClamped = fmaxf(edgeThresholdMin, rangeMaxScaled);
-    ushort4 o;
-    if(range < rangeMaxClamped) {
-        o.x = f2h(rgbyM.x);
-        o.y = f2h(rgbyM.y);
-        o.z = f2h(rgbyM.z);
-        o.w = f2h(rgbyM.w);
-        reinterpret_cast<ushort4*>(out)[(size_t)y * W + x] = o;
+    // 1. contrast gate
+    const Px4 centre = tex_sample(t, u0, v0, 0, 0);
+    const float lC = centre.y;
+    const float lUp = tex_sample(t, u0, v0, 0, -1).y, lDown = tex_sample(t, u0, v0, 0, 1).y;
+    const float lLeft = tex_sample(t, u0, v0, -1, 0).y, lRight = tex_sample(t, u0, v0, 1, 0).y;
+    const float brightest = fmaxf(fmaxf(fmaxf(lUp, lDown), fmaxf(lLeft, lRight)), lC);
+    const float darkest = fminf(fminf(fminf(lUp, lDown), fminf(lLeft, lRight)), lC);
+    const float contrast = brightest - darkest;
+    if(contrast < fmaxf(0.0312f, brightest * 0.063f)) {
+        *dst = make_ushort4(f2h(centre.x), f2h(centre.y), f2h(centre.z), f2h(centre.w));
         return;
     }
-    const float lumaNW = luma_at(tex, posMx, posMy, -1, -1);
-    const float lumaSE = luma_at(tex, posMx, posMy, 1, 1);
-    const float lumaNE = luma_at(tex, posMx, posMy, 1, -1);
-    const float lumaSW = luma_at(tex, posMx, posMy, -1, 1);
 
-    const float lumaNS = lumaN + lumaS, lumaWE = lumaW + lumaE;
-    const float subpixRcpRange = 1.0f / range;
-    const float subpixNSWE = lumaNS + lumaWE;
-    const float edgeHorz1 = (-2.0f * lumaM) + lumaNS;
-    const float edgeVert1 = (-2.0f * lumaM) + lumaWE;
-    const float lumaNESE = lumaNE + lumaSE, lumaNWNE = lumaNW + lumaNE;
-    const float edgeHorz2 = (-2.0f * lumaE) + lumaNESE;
-    const float edgeVert2 = (-2.0f * lumaN) + lumaNWNE;
-    const float lumaNWSW = lumaNW + lumaSW, lumaSWSE = lumaSW + lumaSE;
-    const float edgeHorz4 = (fabsf(edgeHorz1) * 2.0f) + fabsf(edgeHorz2);
-    const float edgeVert4 = (fabsf(edgeVert1) * 2.0f) + fabsf(edgeVert2);
-    const float edgeHorz3 = (-2.0f * lumaW) + lumaNWSW;
-    const float edgeVert3 = (-2.0f * lumaS) + lumaSWSE;
-    const float edgeHorz = fabsf(edgeHorz3) + edgeHorz4;
-    const float edgeVert = fabsf(edgeVert3) + edgeVert4;
-    const float subpixNWSWNESE = lumaNWSW + lumaNESE;
-    float lengthSign = rcpX;
-    const bool horzSpan = edgeHorz >= edgeVert;
-    const float subpixA = subpixNSWE * 2.0f + subpixNWSWNESE;
-    if(!horzSpan) {
-        lumaN = lumaW;
-        lumaS = lumaE;
-    }
-    if(horzSpan) lengthSign = rcpY;
-    const float subpixB = (subpixA * (float)(1.0 / 12.0)) - lumaM;
-    const float gradientN = lumaN - lumaM, gradientS = lumaS - lumaM;
-    float lumaNN = lumaN + lumaM;
-    const float lumaSS = lumaS + lumaM;
-    const bool pairN = fabsf(gradientN) >= fabsf(gradientS);
-    const float gradient = fmaxf(fabsf(gradientN), fabsf(gradientS));
-    if(pairN) lengthSign = -lengthSign;
-    const float subpixC = clampf(fabsf(subpixB) * subpixRcpRange, 0.0f, 1.0f);
+    // 2. orientation
+    const float lUL = tex_sample(t, u0, v0, -1, -1).y, lUR = tex_sample(t, u0, v0, 1, -1).y;
+    const float lDL = tex_sample(t, u0, v0, -1, 1).y, lDR = tex_sample(t, u0, v0, 1, 1).y;
+    const float acrossRows =
+        2.0f * second_diff(lUp, lC, lDown) + (second_diff(lUL, lLeft, lDL) + second_diff(lUR, lRight, lDR));
+    const float acrossCols =
+        2.0f * second_diff(lLeft, lC, lRight) + (second_diff(lUL, lUp, lUR) + second_diff(lDL, lDown, lDR));
+    const bool edgeRunsAlongX = acrossRows >= acrossCols;
 
-    float posBx = posMx, posBy = posMy;
-    const float offNPx = (!horzSpan) ? 0.0f : rcpX;
-    const float offNPy = (horzSpan) ? 0.0f : rcpY;
-    if(!horzSpan) posBx += lengthSign * 0.5f;
-    if(horzSpan) posBy += lengthSign * 0.5f;
+    // subpixel lower bound of step 5
+    const float ringMean = (2.0f * ((lUp + lDown) + (lLeft + lRight)) + ((lUL + lUR) + (lDL + lDR))) / 12.0f;
+    float sub = clampf(fabsf(ringMean - lC) / contrast, 0.0f, 1.0f);
+    sub = (sub * sub) * (3.0f - 2.0f * sub);
+    const float subShift = (sub * sub) * 1.0f;
 
-    float posNx = posBx - offNPx * kFxaaSteps[0], posNy = posBy - offNPy * kFxaaSteps[0];
-    float posPx = posBx + offNPx * kFxaaSteps[0], posPy = posBy + offNPy * kFxaaSteps[0];
-    const float subpixD = ((-2.0f) * subpixC) + 3.0f;
-    float lumaEndN = luma_at(tex, posNx, posNy);
-    const float subpixE = subpixC * subpixC;
-    float lumaEndP = luma_at(tex, posPx, posPy);
-    if(!pairN) lumaNN = lumaSS;
-    const float gradientScaled = (gradient * 1.0f) / 4.0f;
-    const float lumaMM = lumaM - lumaNN * 0.5f;
-    const float subpixF = subpixD * subpixE;
-    const bool lumaMLTZero = lumaMM < 0.0f;
-    lumaEndN -= lumaNN * 0.5f;
-    lumaEndP -= lumaNN * 0.5f;
-    bool doneN = fabsf(lumaEndN) >= gradientScaled;
-    bool doneP = fabsf(lumaEndP) >= gradientScaled;
-    if(!doneN) {
-        posNx -= offNPx * kFxaaSteps[1];
-        posNy -= offNPy * kFxaaSteps[1];
-    }
-    bool doneNP = (!doneN) || (!doneP);
-    if(!doneP) {
-        posPx += offNPx * kFxaaSteps[1];
-        posPy += offNPy * kFxaaSteps[1];
-    }
-    for(int step = 2; step < 12 && doneNP; ++step) {
-        if(!doneN) lumaEndN = luma_at(tex, posNx, posNy);
-        if(!doneP) lumaEndP = luma_at(tex, posPx, posPy);
-        if(!doneN) lumaEndN = lumaEndN - lumaNN * 0.5f;
-        if(!doneP) lumaEndP = lumaEndP - lumaNN * 0.5f;
-        doneN = fabsf(lumaEndN) >= gradientScaled;
-        doneP = fabsf(lumaEndP) >= gradientScaled;
-        if(!doneN) {
-            posNx -= offNPx * kFxaaSteps[step];
-            posNy -= offNPy * kFxaaSteps[step];
-        }
-        doneNP = (!doneN) || (!doneP);
-        if(!doneP) {
-            posPx += offNPx * kFxaaSteps[step];
-            posPy += offNPy * kFxaaSteps[step];
-        }
-    }
-    float dstN = posMx - posNx, dstP = posPx - posMx;
-    if(!horzSpan) {
-        dstN = posMy - posNy;
-        dstP = posPy - posMy;
-    }
-    const bool goodSpanN = (lumaEndN < 0.0f) != lumaMLTZero;
-    const float spanLength = dstP + dstN;
-    const bool goodSpanP = (lumaEndP < 0.0f) != lumaMLTZero;
-    const float spanLengthRcp = 1.0f / spanLength;
-    const bool directionN = dstN < dstP;
-    const float dst = fminf(dstN, dstP);
-    const bool goodSpan = directionN ? goodSpanN : goodSpanP;
-    const float subpixG = subpixF * subpixF;
-    const float pixelOffset = (dst * (-spanLengthRcp)) + 0.5f;
-    const float subpixH = subpixG * subpix;
-    const float pixelOffsetGood = goodSpan ? pixelOffset : 0.0f;
-    const float pixelOffsetSubpix = fmaxf(pixelOffsetGood, subpixH);
-    if(!horzSpan) posMx += pixelOffsetSubpix * lengthSign;
-    if(horzSpan) posMy += pixelOffsetSubpix * lengthSign;
-    const Px4 c = tex_sample(tex, posMx, posMy, 0, 0);
-    o.x = f2h(c.x);
-    o.y = f2h(c.y);
-    o.z = f2h(c.z);
-    o.w = f2h(lumaM);
-    reinterpret_cast<ushort4*>(out)[(size_t)y * W + x] = o;
+    // 3. side
+    const float lNeg = edgeRunsAlongX ? lUp : lLeft, lPos = edgeRunsAlongX ? lDown : lRight;
+    const float stepNeg = lNeg - lC, stepPos = lPos - lC;
+    const bool towardNeg = fabsf(stepNeg) >= fabsf(stepPos);
+    const float texelAcross = edgeRunsAlongX ? texelV : texelU;
+    const float across = towardNeg ? -texelAcross : texelAcross;
+    const float refLuma = (lC + (towardNeg ? lNeg : lPos)) * 0.5f;
+    const float limit = fmaxf(fabsf(stepNeg), fabsf(stepPos)) * 0.25f;
+    const bool centreBelow = (lC - refLuma) < 0.0f;
+
+    // 4. walk both ways from the half-pixel point on the edge
+    const float su = edgeRunsAlongX ? u0 : u0 + across * 0.5f;
+    const float sv = edgeRunsAlongX ? v0 + across * 0.5f : v0;
+    const float du = edgeRunsAlongX ? texelU : 0.0f, dv = edgeRunsAlongX ? 0.0f : texelV;
+    const EdgeEnd back = walk_edge(t, su, sv, -du, -dv, refLuma, limit);
+    const EdgeEnd fwd = walk_edge(t, su, sv, du, dv, refLuma, limit);
+
+    // 5. blend
+    const bool backNearer = back.reach < fwd.reach;
+    const EdgeEnd near = backNearer ? back : fwd;
+    const bool turnsAway = (near.delta < 0.0f) != centreBelow;
+    const float spanShift = turnsAway ? 0.5f - near.reach / (back.reach + fwd.reach) : 0.0f;
+    const float shift = fmaxf(spanShift, subShift);
+    const float fu = edgeRunsAlongX ? u0 : u0 + shift * across;
+    const float fv = edgeRunsAlongX ? v0 + shift * across : v0;
+    const Px4 c = tex_sample(t, fu, fv, 0, 0);
+    *dst = make_ushort4(f2h(c.x), f2h(c.y), f2h(c.z), f2h(lC));
 }
 
 // ---------------------------------------------------------------------------

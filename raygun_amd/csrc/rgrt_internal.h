@@ -158,6 +158,9 @@ class Denoiser {
     Denoiser& operator=(const Denoiser&) = delete;
 
     size_t num_params() const { return numParams; }
+    // true once parameters were loaded (set_params) or trained (train);
+    // the He-initialised network of a fresh denoiser is not a denoiser
+    bool has_trained_params() const { return paramsLoaded; }
     void set_params(hipStream_t s, const float* host);
     void get_params(hipStream_t s, float* host);
     // forward + loss + gradients on host inputs (x0: N*H*W*16 bf16 bits,
@@ -187,6 +190,7 @@ class Denoiser {
     float *params = nullptr, *grads = nullptr, *adamM = nullptr, *adamV = nullptr;
     uint16_t *wFwd = nullptr, *wBwd = nullptr;
     bool weightsReady = false;
+    bool paramsLoaded = false;
     uint32_t stepCount = 0;
     void* ws = nullptr;
     size_t wsBytes = 0;

@@ -615,6 +615,67 @@ def test_fxaa_smooths_a_green_step_and_stays_in_range():
     assert not (changed & ~edge).any()
 
 
+def _fxaa_green(green):
+    """FXAA output (green channel, alpha) of a grey image given by its green
+    values, through the oracle's post chain with no rough contribution."""
+    H, W = green.shape
+    base = np.zeros((H, W, 4), np.float32)
+    base[..., :3] = green[..., None]
+    z = _f16(np.zeros_like(base))
+    out = oracle_py.postprocess(_ubo(), _f16(base), z, z, use_fxaa=True)
+    o = half(out["final"])  # the FXAA result after the base/final swap
+    assert np.array_equal(o[..., 0], o[..., 1]) and np.array_equal(o[..., 2], o[..., 1])
+    return o[..., 1], o[..., 3]
+
+
+def test_fxaa_known_answers_straight_edge():
+    """A straight vertical edge (columns >= 4 bright) worked by hand from the
+    published algorithm: contrast 1 passes the gate; the column second
+    differences (4) beat the row ones (0), so the edge runs vertically; the
+    walk along the half-pixel line x = 4 reads 0.5 everywhere, never leaves
+    the reference luma 0.5 by the quarter step 0.25, so both ends sit at the
+    preset's full reach (26.5 texels) and the span shift is 0.5 - 26.5/53 = 0;
+    what moves the pixel is the subpixel term: the 3x3 mean is 1/3 (dark
+    side) or 2/3 (bright side), contrast 1/3, smoothstep 7/27, squared
+    49/729 = 0.0672 of a texel toward the edge, i.e. a bilinear weight of
+    rint(0.0672 * 256) = 17/256."""
+    g = np.zeros((8, 8), np.float32)
+    g[:, 4:] = 1.0
+    got, alpha = _fxaa_green(g)
+    assert np.all(got[:, 3] == np.float32(17 / 256))
+    assert np.all(got[:, 4] == np.float32(239 / 256))
+    assert np.all(got[:, :3] == 0.0) and np.all(got[:, 5:] == 1.0)  # no contrast: left as they are
+    assert np.all(alpha[:, 3] == 0.0) and np.all(alpha[:, 4] == 1.0)  # alpha carries the pixel's luma
+
+
+def test_fxaa_known_answers_edge_with_a_step():
+    """A horizontal edge with a one-row jog at x = 8 (bright below row 4 for
+    x >= 8, below row 5 for x < 8), worked by hand. Each pixel here has a
+    horizontal edge; the walk runs along the half-pixel row toward the
+    brighter (or darker) neighbour with the larger luma step:
+      (6,4): dark; walk +x hits the jog at reach 2 (luma 1 - 0.5 >= 0.25),
+             -x never stops (26.5); the nearer end turns away from the pixel,
+             shift = 0.5 - 2/28.5 = 0.4298 -> weight rint(110.03) = 110/256;
+      (7,4): the jog at reach 1: 0.5 - 1/27.5 = 0.4636 -> 119/256 (the 3x3
+             subpixel term, 0.25, is smaller);
+      (8,4): bright, side above: walk -x finds the dark row at reach 1,
+             shift 0.4636 upward -> 1 - 0.4636 of row 4 -> 137/256;
+      (7,5): bright, the nearer end (reach 1, luma 1) turns the same way as
+             the pixel, so no span shift; only the subpixel term: 3x3 mean
+             0.75, contrast 0.25, smoothstep 0.15625, squared 0.0244 ->
+             weight rint(249.75) = 250/256."""
+    H, W = 8, 16
+    yy, xx = np.mgrid[:H, :W]
+    g = (((xx >= 8) & (yy >= 4)) | ((xx < 8) & (yy >= 5))).astype(np.float32)
+    got, alpha = _fxaa_green(g)
+    assert got[4, 6] == np.float32(110 / 256)
+    assert got[4, 7] == np.float32(119 / 256)
+    assert got[4, 8] == np.float32(137 / 256)
+    assert got[5, 7] == np.float32(250 / 256)
+    assert alpha[4, 6] == 0.0 and alpha[5, 7] == 1.0
+    assert got[1, 1] == 0.0 and got[7, 1] == 1.0  # far from the edge: gate keeps the pixel
+
+
 def test_composite_clamps_and_writes_luma():
     W, H = 8, 4
     rng = np.random.default_rng(3)
