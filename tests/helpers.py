@@ -1,4 +1,6 @@
 """Shared test helpers: one scene set up in the HIP core and in the oracle."""
+import math
+
 import numpy as np
 
 import oracle_py
@@ -76,3 +78,29 @@ def compare_hits(h_gpu, h_ref, tie):
     """Indices where the closest-hit identity differs and the oracle saw no tie."""
     same_id = (h_gpu["instance"] == h_ref["instance"]) & ((h_gpu["primitive"] == h_ref["primitive"]) | (h_ref["instance"] == NO_HIT))
     return np.nonzero(~same_id & ~tie)[0]
+
+
+def sky64(d, L):
+    """miss.rmiss:38-74 evaluated in float64 (a second, independent reading);
+    a zero direction reads rayDir = 0 (the oracle's definition for the
+    zero-direction ray of total internal reflection)."""
+    d = np.asarray(d, np.float64)
+    L = np.asarray(L, np.float64)
+    nd = np.linalg.norm(d)
+    rd = d / nd if nd > 0 else np.zeros(3)
+    y = abs(d[1] + 1.5) / 3.0
+    sun = 1.0 - np.linalg.norm(rd - (-L / np.linalg.norm(L)))
+    sun = min(max(sun, 0.0), 2.0)
+    glow = min(max(sun, 0.0), 1.0)
+    sun = min(max(sun ** 80 * 1000.0, 0.0), 16.0)
+    glow = min(max((glow ** 6) ** y, 0.0), 1.0)
+    sun *= (y * y) ** (1 / 1.65)
+    glow *= (y * y) ** 0.5
+    sun += glow
+    sun_color = np.array([1.0, 0.6, 0.05]) * sun
+    atmosphere = math.sqrt(1.0 - y)
+    scatter = 1.0 - min(max((4.0 - L[1]) ** (1 / 15), 0.8), 1.0)
+    scatter_color = np.ones(3) * (1 - scatter) + np.array([1.0, 0.3, 0.0]) * 1.5 * scatter
+    a = atmosphere / 1.3
+    sky = np.array([0.2, 0.4, 0.8]) * (1 - a) + scatter_color * a
+    return sun_color + sky

@@ -14,7 +14,7 @@ import numpy as np
 import pytest
 
 import oracle_py
-from helpers import NO_HIT, half, oracle_scene, random_rays
+from helpers import NO_HIT, half, oracle_scene, random_rays, sky64
 from raygun_amd import procgen
 from raygun_amd.records import INSTANCE, RAY, VERTEX, default_material
 from raygun_amd.scene import Camera, Entity, Mesh, Model, Scene, Transform, camera_ubo
@@ -419,29 +419,6 @@ def test_ssaa_offsets_are_the_reference_table():
 # ---------------------------------------------------------------------------
 # shading closed forms
 # ---------------------------------------------------------------------------
-def _sky64(d, L):
-    """miss.rmiss:38-74 evaluated in float64 (a second, independent reading)."""
-    d = np.asarray(d, np.float64)
-    L = np.asarray(L, np.float64)
-    rd = d / np.linalg.norm(d)
-    y = abs(d[1] + 1.5) / 3.0
-    sun = 1.0 - np.linalg.norm(rd - (-L / np.linalg.norm(L)))
-    sun = min(max(sun, 0.0), 2.0)
-    glow = min(max(sun, 0.0), 1.0)
-    sun = min(max(sun ** 80 * 1000.0, 0.0), 16.0)
-    glow = min(max((glow ** 6) ** y, 0.0), 1.0)
-    sun *= (y * y) ** (1 / 1.65)
-    glow *= (y * y) ** 0.5
-    sun += glow
-    sun_color = np.array([1.0, 0.6, 0.05]) * sun
-    atmosphere = math.sqrt(1.0 - y)
-    scatter = 1.0 - min(max((4.0 - L[1]) ** (1 / 15), 0.8), 1.0)
-    scatter_color = np.ones(3) * (1 - scatter) + np.array([1.0, 0.3, 0.0]) * 1.5 * scatter
-    a = atmosphere / 1.3
-    sky = np.array([0.2, 0.4, 0.8]) * (1 - a) + scatter_color * a
-    return sun_color + sky
-
-
 def test_sky_matches_an_independent_float64_reading():
     L = np.array([0.4, -0.6, -0.8]) / np.linalg.norm([0.4, -0.6, -0.8])
     rng = np.random.default_rng(4)
@@ -450,7 +427,7 @@ def test_sky_matches_an_independent_float64_reading():
     dirs = list(g) + [-L, -L * 0.999 + 0.001, np.array([0, -1.0, 0]), np.array([0, 1.0, 0])]
     for d in dirs:
         ours = oracle_py.sky(d, L)
-        assert np.allclose(ours, _sky64(d, L), rtol=2e-5, atol=2e-5), d
+        assert np.allclose(ours, sky64(d, L), rtol=2e-5, atol=2e-5), d
     # at the sun's direction the sun term saturates at 16 (miss.rmiss:51)
     y = abs(-L[1] + 1.5) / 3
     assert oracle_py.sky(-L, L)[0] > 16 * (y * y) ** (1 / 1.65)
