@@ -325,6 +325,12 @@ rgrt_status rgrt_denoiser_set_params(rgrt_denoiser* den, const float* params, ui
  * (n*h*w*16 bf16, the correction added to the noisy RGB). Synchronous. */
 rgrt_status rgrt_denoiser_loss_and_grads(rgrt_denoiser* den, const uint16_t* x0, const float* target, uint32_t n,
                                          uint32_t height, uint32_t width, float* loss, float* grads, uint16_t* output);
+/* The network's forward pass alone on host inputs: x0 = n*h*w*16 bf16 bit
+ * patterns (the packed input), output = n*h*w*16 bf16 (the correction added
+ * to the noisy RGB; channels 3..15 zero). h and w multiples of 4, at most
+ * 2^22 pixels. Synchronous. No counterpart in the reference. */
+rgrt_status rgrt_denoiser_infer(rgrt_denoiser* den, const uint16_t* x0, uint32_t n, uint32_t height, uint32_t width,
+                                uint16_t* output);
 /* Training set in device memory: `count` frames of height x width. add_input
  * packs the context's current stochastic frame into a slot; add_target
  * (accumulate != 0: adds) weight * BASE.rgb of the current frame to the slot's

@@ -29,6 +29,8 @@ void orc_scene_destroy(orc_scene* s);
 int32_t orc_build_blas(orc_scene* s, uint32_t firstVertex, uint32_t indexOffsetBytes, uint32_t numFaces);
 /* Re-reads vertex positions (after orc_update_vertices) and recomputes boxes. */
 int32_t orc_refit_blas(orc_scene* s, int32_t blas);
+/* Full LBVH rebuild of BLAS `blas` from the current vertices, keeping its id. */
+int32_t orc_rebuild_blas(orc_scene* s, int32_t blas);
 int32_t orc_update_vertices(orc_scene* s, uint64_t firstVertex, const void* vertices, uint64_t count);
 uint32_t orc_blas_num_nodes(orc_scene* s, int32_t blas);
 /* Canonical nodes (64 B each, rgrt_bvh_node layout) + leaf primitive ids. */

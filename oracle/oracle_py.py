@@ -33,6 +33,8 @@ def lib():
         L.orc_build_blas.argtypes = [P, u32, u32, u32]
         L.orc_refit_blas.restype = i32
         L.orc_refit_blas.argtypes = [P, i32]
+        L.orc_rebuild_blas.restype = i32
+        L.orc_rebuild_blas.argtypes = [P, i32]
         L.orc_update_vertices.restype = i32
         L.orc_update_vertices.argtypes = [P, u64, P, u64]
         L.orc_blas_num_nodes.restype = u32
@@ -117,6 +119,10 @@ class OracleScene:
 
     def refit_blas(self, b):
         assert lib().orc_refit_blas(self.h, b) == 0
+
+    def rebuild_blas(self, b):
+        """Full LBVH rebuild of BLAS b from the current vertices (same id)."""
+        assert lib().orc_rebuild_blas(self.h, b) == 0
 
     def update_vertices(self, first, verts):
         from raygun_amd.records import VERTEX

@@ -1847,6 +1847,18 @@ int32_t orc_refit_blas(orc_scene* s, int32_t id)
     return 0;
 }
 
+// Full rebuild of an existing BLAS from the current vertices, keeping its id
+// (what a deforming mesh gets each frame: new Morton order, new hierarchy).
+int32_t orc_rebuild_blas(orc_scene* s, int32_t id)
+{
+    if(id < 0 || id >= (int32_t)s->blas.size()) return -1;
+    Blas& b = s->blas[id];
+    b.bvh = BVH{};
+    load_tris(*s, b);
+    build_lbvh(b.bvh, g_hierarchy);
+    return 0;
+}
+
 uint32_t orc_blas_num_nodes(orc_scene* s, int32_t id) { return (uint32_t)s->blas[id].bvh.nodes.size(); }
 
 uint32_t orc_blas_num_wide(orc_scene* s, int32_t id) { return (uint32_t)s->blas[id].bvh.wide.size(); }

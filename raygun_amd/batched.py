@@ -35,7 +35,7 @@ def scene_sizes(num_scenes, seed, lo=10_000, hi=4_000_000, sigma=1.2):
 class BatchScene:
     """One scene of the batch on its own context."""
 
-    def __init__(self, index, seed, tris, resolution, max_recursions=3):
+    def __init__(self, index, seed, tris, resolution, max_recursions=3, flags=0):
         rng = np.random.default_rng(seed)
         self.index = index
         self.width, self.height = resolution
@@ -48,10 +48,11 @@ class BatchScene:
             V = max(4, int(round(math.sqrt(tris / 8))))
             self.mesh = procgen.torus(4 * V, V, 1.0, 0.35)
         ground = procgen.plane(40.0, -1.5)
-        self.ctx = rgrt.Context(self.width, self.height, 0)
+        self.ctx = rgrt.Context(self.width, self.height, flags)
         models = [Model(self.mesh, np.array([default_material()])), Model(ground, np.array([default_material()]))]
         v, i, _ = pack_models(models)
         self.ctx.upload_geometry(v, i)
+        self.vertices, self.indices = v, i
         self.base_vertices = self.mesh.vertices.copy()
         n = int(rng.integers(1, 7))
         mats = [default_material(diffuse=(0.5, 0.5, 0.5), reflectivity=0.3, roughness=0.3, effectId=1)]
@@ -60,7 +61,8 @@ class BatchScene:
                                          reflectivity=float(rng.uniform(0.3, 1.0)) if rng.uniform() < 0.4 else 0.0,
                                          transparency=float(rng.uniform(0.5, 0.95)) if rng.uniform() < 0.25 else 0.0,
                                          ior=float(rng.uniform(1.0, 1.8))))
-        self.ctx.upload_materials(np.array(mats))
+        self.materials = np.array(mats)
+        self.ctx.upload_materials(self.materials)
         self.mesh_blas = self.ctx.build_blas(self.mesh.first_vertex, self.mesh.index_offset_bytes, self.mesh.num_faces(),
                                              len(self.mesh.vertices))
         self.ground_blas = self.ctx.build_blas(ground.first_vertex, ground.index_offset_bytes, ground.num_faces(),
@@ -109,6 +111,10 @@ class BatchScene:
         v["position"] = (p * w[:, None]).astype(np.float32)
         return v
 
+    def pose_of_frame(self, frame):
+        """The vertices deform(frame) puts in place."""
+        return self.pose(2.0 * math.pi * (frame % POSES) / POSES)
+
     def deform(self, frame):
         """Pose frame % POSES into the vertex buffer and a full BLAS rebuild,
         both queued on the scene's stream."""
@@ -125,11 +131,11 @@ class BatchScene:
 
 class BatchWorkload:
     def __init__(self, num_scenes=32, seed=3, lo=10_000, hi=4_000_000, resolutions=RESOLUTIONS,
-                 mode=rgrt.BATCH_GRAPH_PER_SCENE):
+                 mode=rgrt.BATCH_GRAPH_PER_SCENE, flags=0):
         rng = np.random.default_rng(seed + 17)
         sizes = scene_sizes(num_scenes, seed, lo, hi)
-        self.scenes = [BatchScene(k, seed * 1000 + k, sizes[k], resolutions[int(rng.integers(len(resolutions)))])
-                       for k in range(num_scenes)]
+        self.scenes = [BatchScene(k, seed * 1000 + k, sizes[k], resolutions[int(rng.integers(len(resolutions)))],
+                                  flags=flags) for k in range(num_scenes)]
         self.batch = rgrt.Batch([s.ctx for s in self.scenes], mode)
         self.frames_done = 0
         self.info = {"scenes": num_scenes, "triangles": [s.mesh.num_faces() for s in self.scenes],

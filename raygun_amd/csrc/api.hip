@@ -1615,6 +1615,19 @@ rgrt_status rgrt_denoiser_loss_and_grads(rgrt_denoiser* den, const uint16_t* x0,
     });
 }
 
+rgrt_status rgrt_denoiser_infer(rgrt_denoiser* den, const uint16_t* x0, uint32_t n, uint32_t h, uint32_t w,
+                                uint16_t* output)
+{
+    if(!den) return RGRT_ERR_INVALID_ARG;
+    rgrt_ctx* c = den->ctx;
+    return guarded(c, [&]() {
+        if(!x0 || !output || !n || !h || !w || h % 4 || w % 4 || (uint64_t)n * h * w > (1ull << 22))
+            return fail(c, RGRT_ERR_INVALID_ARG, "denoiser_infer: sizes (multiples of 4, at most 2^22 pixels)");
+        den->d->infer(c->stream, x0, (int)n, (int)h, (int)w, output);
+        return RGRT_OK;
+    });
+}
+
 rgrt_status rgrt_denoiser_dataset_reserve(rgrt_denoiser* den, uint32_t count, uint32_t h, uint32_t w)
 {
     if(!den) return RGRT_ERR_INVALID_ARG;

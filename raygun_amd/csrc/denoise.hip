@@ -759,6 +759,17 @@ float Denoiser::loss_and_grads(hipStream_t s, const uint16_t* x0Host, const floa
     }
 }
 
+void Denoiser::infer(hipStream_t s, const uint16_t* x0Host, int N, int H, int W, uint16_t* outHost)
+{
+    if(!ws || wsN != N || wsH != H || wsW != W) {
+        check(hipStreamSynchronize(s));
+        reserve(N, H, W, false);
+    }
+    check(hipMemcpyAsync(buf[X0], x0Host, (size_t)N * H * W * 32, hipMemcpyHostToDevice, s));
+    forward(s, N, H, W);
+    read_output(s, outHost);
+}
+
 void Denoiser::read_output(hipStream_t s, uint16_t* host)
 {
     check(hipStreamSynchronize(s));

@@ -168,6 +168,8 @@ class Denoiser {
     float loss_and_grads(hipStream_t s, const uint16_t* x0Host, const float* targetHost, int N, int H, int W,
                          float* gradHost);
     void read_output(hipStream_t s, uint16_t* host); // o of the last forward pass
+    // forward pass alone on host inputs (x0: N*H*W*16 bf16 bits); o to outHost
+    void infer(hipStream_t s, const uint16_t* x0Host, int N, int H, int W, uint16_t* outHost);
     // training set in device memory: packed inputs and fp32 targets
     void dataset_reserve(uint32_t count, int H, int W);
     void dataset_add_input(hipStream_t s, const Images& img, uint32_t slot);

@@ -42,7 +42,7 @@ EXPORTS = [
     "rgrt_read_image", "rgrt_read_image_rgba8", "rgrt_image_device_ptr", "rgrt_write_image", "rgrt_get_stats", "rgrt_get_pass_counts",
     "rgrt_set_tuning", "rgrt_get_tuning", "rgrt_set_sample_groups", "rgrt_set_sampling", "rgrt_denoiser_create", "rgrt_denoiser_destroy",
     "rgrt_denoiser_num_params", "rgrt_denoiser_get_params", "rgrt_denoiser_set_params", "rgrt_denoiser_loss_and_grads",
-    "rgrt_denoiser_dataset_reserve", "rgrt_denoiser_dataset_add_input", "rgrt_denoiser_dataset_add_target",
+    "rgrt_denoiser_infer", "rgrt_denoiser_dataset_reserve", "rgrt_denoiser_dataset_add_input", "rgrt_denoiser_dataset_add_target",
     "rgrt_denoiser_train", "rgrt_denoise", "rgrt_batch_create", "rgrt_batch_destroy", "rgrt_batch_last_error", "rgrt_batch_set_mode", "rgrt_batch_stream",
     "rgrt_batch_frame", "rgrt_export_blas_wide", "rgrt_export_tlas_wide", "rgrt_version",
 ]
@@ -123,6 +123,7 @@ def lib():
             "rgrt_denoiser_get_params": (i32, [P, P, u64]),
             "rgrt_denoiser_set_params": (i32, [P, P, u64]),
             "rgrt_denoiser_loss_and_grads": (i32, [P, P, P, u32, u32, u32, ctypes.POINTER(ctypes.c_float), P, P]),
+            "rgrt_denoiser_infer": (i32, [P, P, u32, u32, u32, P]),
             "rgrt_denoiser_dataset_reserve": (i32, [P, u32, u32, u32]),
             "rgrt_denoiser_dataset_add_input": (i32, [P, u32]),
             "rgrt_denoiser_dataset_add_target": (i32, [P, u32, ctypes.c_float, i32]),
@@ -434,6 +435,15 @@ class Denoiser:
         self.ctx._ok(self._L.rgrt_denoiser_loss_and_grads(self.h, _p(x), _p(t), n, hh, ww, ctypes.byref(loss), _p(g),
                                                           _p(out)))
         return (loss.value, g, out) if with_output else (loss.value, g)
+
+    def infer(self, x0_bits):
+        """Forward pass alone: x0_bits (n, h, w, 16) uint16 bf16 patterns ->
+        output (n, h, w, 16) uint16 bf16 patterns."""
+        x = np.ascontiguousarray(x0_bits, np.uint16)
+        n, hh, ww, _ = x.shape
+        out = np.zeros((n, hh, ww, 16), np.uint16)
+        self.ctx._ok(self._L.rgrt_denoiser_infer(self.h, _p(x), n, hh, ww, _p(out)))
+        return out
 
     def dataset_reserve(self, count, height, width):
         self.ctx._ok(self._L.rgrt_denoiser_dataset_reserve(self.h, count, height, width))

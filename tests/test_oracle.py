@@ -286,6 +286,30 @@ def test_refit_equals_exact_boxes_of_the_moved_triangles():
     assert np.mean(h["primitive"][both] == prim64[both]) > 0.99
 
 
+def test_rebuild_equals_a_fresh_build_of_the_moved_triangles():
+    """rebuild_blas (what a deforming mesh gets) on the moved vertices gives
+    the tree a fresh build of those vertices gives, and differs from the
+    refit (new Morton order), so it is a rebuild and not a refit."""
+    mesh, moved = _refit_case()
+    n = mesh.num_faces()
+    mats = np.array([default_material()])
+    osc = oracle_py.OracleScene(mesh.vertices, mesh.indices, mats)
+    b = osc.build_blas(0, 0, n)
+    osc.update_vertices(0, moved.vertices)
+    osc.rebuild_blas(b)
+    fresh = oracle_py.OracleScene(moved.vertices, moved.indices, mats)
+    fb = fresh.build_blas(0, 0, n)
+    (nodes, leaf), (fnodes, fleaf) = osc.export_blas(b), fresh.export_blas(fb)
+    assert np.array_equal(leaf, fleaf)
+    assert nodes.tobytes() == fnodes.tobytes()
+    assert osc.export_blas_wide(b, compressed=True).tobytes() == fresh.export_blas_wide(fb, compressed=True).tobytes()
+    refit = oracle_py.OracleScene(mesh.vertices, mesh.indices, mats)
+    rb = refit.build_blas(0, 0, n)
+    refit.update_vertices(0, moved.vertices)
+    refit.refit_blas(rb)
+    assert not np.array_equal(refit.export_blas(rb)[1], leaf)
+
+
 # ---------------------------------------------------------------------------
 # intersection
 # ---------------------------------------------------------------------------
