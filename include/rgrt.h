@@ -228,6 +228,15 @@ rgrt_status rgrt_refit_blas(rgrt_ctx* ctx, rgrt_blas blas);
  * (no host wait); contexts sharing the scene order their later work after
  * it. Rebuild the TLAS (or refit it) afterwards for the new root box. */
 rgrt_status rgrt_rebuild_blas(rgrt_ctx* ctx, rgrt_blas blas);
+/* No counterpart in the reference: for callers that cannot allocate (code
+ * under hipGraph capture, tight serving loops). rgrt_rebuild_blas with the
+ * builder's working arrays in a fixed per-GPU device array (48 MiB, held
+ * until the process ends) instead of memory the context allocates: it calls
+ * no allocation routine. At most 262144 triangles (RGRT_ERR_INVALID_ARG
+ * beyond). Same tree as rgrt_rebuild_blas. Every call on a GPU shares that
+ * GPU's array: calls that overlap on one GPU (other streams, other
+ * contexts) race and must be ordered by the caller. */
+rgrt_status rgrt_rebuild_blas_noalloc(rgrt_ctx* ctx, rgrt_blas blas);
 /* Number of internal nodes / leaves and the root box of a BLAS. */
 rgrt_status rgrt_blas_info(rgrt_ctx* ctx, rgrt_blas blas, uint32_t* numNodes, uint32_t* numLeaves, float rootLo[3],
                            float rootHi[3]);

@@ -258,6 +258,37 @@ void scene_write_end(rgrt_ctx* c)
         if(u != c) check(hipStreamWaitEvent(u->stream, c->sceneEv, 0));
 }
 
+// Scratch of rgrt_rebuild_blas_noalloc: the LBVH builder's working arrays for
+// up to kNoAllocMaxFaces triangles in a file-scope device array (48 MiB),
+// one copy per GPU, held until the process ends. Host code never passes the
+// array's name to a kernel or a copy: it uses the device address that
+// hipGetSymbolAddress returns for the current device (noAllocScratch).
+constexpr uint32_t kNoAllocMaxFaces = 1u << 18;
+constexpr size_t kNoAllocScratchBytes = lbvh_scratch_bytes(kNoAllocMaxFaces);
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+std::mutex noAllocMutex;
+std::vector<void*> noAllocScratch; // per device: the address of `scratch` there
+
+// Looks the array's address up once per device, with that device current
+// (rgrt_create sets it before calling this).
+void noalloc_scratch_setup(int device)
+{
+    std::lock_guard<std::mutex> g(noAllocMutex);
+    if((int)noAllocScratch.size() <= device) noAllocScratch.resize(device + 1, nullptr);
+    if(noAllocScratch[device]) return;
+    void* p = nullptr;
+    check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if(!p || (reinterpret_cast<uintptr_t>(p) & 255u)) throw std::runtime_error("no-alloc scratch: bad symbol address");
+    noAllocScratch[device] = p;
+}
+
+void* noalloc_scratch(int device)
+{
+    std::lock_guard<std::mutex> g(noAllocMutex);
+    return device < (int)noAllocScratch.size() ? noAllocScratch[device] : nullptr;
+}
+
 // Grows the LBVH builder's scratch for n primitives; earlier work queued on
 // the context may still use the old scratch, so it finishes first.
 void reserve_builder(rgrt_ctx* c, uint32_t n)
@@ -758,6 +789,7 @@ rgrt_status rgrt_create(rgrt_ctx** out, const rgrt_config* cfg)
     c->sd->add_user(c);
     rgrt_status st = guarded(c, [&]() {
         check(hipSetDevice(c->device));
+        noalloc_scratch_setup(c->device);
         check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         hipDeviceProp_t prop;
         check(hipGetDeviceProperties(&prop, c->device));
@@ -977,6 +1009,34 @@ rgrt_status rgrt_rebuild_blas(rgrt_ctx* c, rgrt_blas id)
         c->builder.build(s, n, b.nodes, b.leafPrim, c->sd->dBlasRoot + 2 * id);
         c->builder.collapse(s, n, b.nodes, b.wide, b.numWide);
         c->builder.quantize(s, std::max<uint32_t>(n, 2) - 1, b.wide, b.numWide, b.wideq);
+        launch_tri_gather(s, c->sd->vb, c->sd->ib, b.desc.firstVertex, io, b.leafPrim, n, b.tris);
+        scene_write_end(c);
+        return RGRT_OK;
+    });
+}
+
+rgrt_status rgrt_rebuild_blas_noalloc(rgrt_ctx* c, rgrt_blas id)
+{
+    return guarded(c, [&]() {
+        if(id >= c->sd->blas.size() || !c->sd->blas[id].valid)
+            return fail(c, RGRT_ERR_INVALID_ARG, "rebuild_blas_noalloc: bad handle");
+        BlasRec& b = c->sd->blas[id];
+        const uint32_t n = b.desc.numFaces;
+        if(n > kNoAllocMaxFaces)
+            return fail(c, RGRT_ERR_INVALID_ARG,
+                        "rebuild_blas_noalloc: " + std::to_string(n) + " triangles; its scratch holds at most " +
+                            std::to_string(kNoAllocMaxFaces) + " (use rgrt_rebuild_blas)");
+        void* base = noalloc_scratch(c->device);
+        if(!base) return fail(c, RGRT_ERR_NOT_READY, "rebuild_blas_noalloc: no scratch on this device");
+        LbvhBuilder bld; // views into the device's scratch array; owns nothing
+        bld.bind(base, kNoAllocMaxFaces);
+        hipStream_t s = c->stream;
+        const uint32_t io = b.desc.indexOffsetBytes / 4;
+        scene_write_begin(c);
+        launch_tri_boxes(s, c->sd->vb, c->sd->ib, b.desc.firstVertex, io, n, bld.boxLo, bld.boxHi);
+        bld.build(s, n, b.nodes, b.leafPrim, c->sd->dBlasRoot + 2 * id);
+        bld.collapse(s, n, b.nodes, b.wide, b.numWide);
+        bld.quantize(s, std::max<uint32_t>(n, 2) - 1, b.wide, b.numWide, b.wideq);
         launch_tri_gather(s, c->sd->vb, c->sd->ib, b.desc.firstVertex, io, b.leafPrim, n, b.tris);
         scene_write_end(c);
         return RGRT_OK;

@@ -22,6 +22,7 @@ namespace {
 constexpr int kSortThreads = 256;
 constexpr int kSortItems = 8;
 constexpr int kSortTile = kSortThreads * kSortItems;
+static_assert(kSortTile == (int)kLbvhSortTile, "the scratch layout assumes this sort tile");
 constexpr int kWaves = kSortThreads / 64;
 
 __device__ __forceinline__ uint32_t float_to_ordered(float f)
@@ -1247,6 +1248,33 @@ void LbvhBuilder::reserve(uint32_t n)
     check(hipMalloc(&sceneBox, 64));
     check(hipMalloc(&nodeRange, cap * 8ull));
     check(hipMalloc(&wideTmp, cap * sizeof(Node4)));
+    capacity = cap;
+}
+
+void LbvhBuilder::bind(void* base, uint32_t cap)
+{
+    char* p = static_cast<char*>(base);
+    auto take = [&](size_t bytes) {
+        char* q = p;
+        p += lbvh_scratch_align(bytes);
+        return q;
+    };
+    const uint32_t nb = blocks_for(cap, kSortTile);
+    keysA = reinterpret_cast<uint32_t*>(take(cap * 4ull));
+    keysB = reinterpret_cast<uint32_t*>(take(cap * 4ull));
+    valsA = reinterpret_cast<uint32_t*>(take(cap * 4ull));
+    valsB = reinterpret_cast<uint32_t*>(take(cap * 4ull));
+    hist = reinterpret_cast<uint32_t*>(take(256ull * nb * 4ull));
+    boxLo = reinterpret_cast<float4*>(take(cap * 16ull));
+    boxHi = reinterpret_cast<float4*>(take(cap * 16ull));
+    leafParent = reinterpret_cast<uint32_t*>(take(cap * 4ull));
+    counters = reinterpret_cast<uint32_t*>(take(cap * 4ull));
+    sceneBox = reinterpret_cast<uint32_t*>(take(64));
+    nodeRange = reinterpret_cast<uint2*>(take(cap * 8ull));
+    wideTmp = reinterpret_cast<Node4*>(take(cap * sizeof(Node4)));
+    static_assert(sizeof(Node4) == 128, "the scratch layout assumes 128-byte wide nodes");
+    if((size_t)(p - static_cast<char*>(base)) != lbvh_scratch_bytes(cap))
+        throw std::logic_error("LBVH scratch layout differs from lbvh_scratch_bytes");
     capacity = cap;
 }
 

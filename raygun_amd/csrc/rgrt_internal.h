@@ -23,6 +23,21 @@ inline void check(hipError_t e)
 }
 
 // Reusable scratch of the LBVH builder (device memory).
+// Primitives per radix-sort workgroup of the LBVH builder (256 threads x 8).
+constexpr uint32_t kLbvhSortTile = 2048;
+
+// Bytes of the LBVH builder's working arrays for `cap` primitives (cap >=
+// 1024) when they sit in one block (LbvhBuilder::bind): four key/value
+// arrays, the sort histogram, two box arrays, leaf parents, counters, the
+// scene box, leaf ranges and the candidate wide nodes, each 256-B aligned.
+constexpr size_t lbvh_scratch_align(size_t b) { return (b + 255) & ~(size_t)255; }
+constexpr size_t lbvh_scratch_bytes(uint32_t cap)
+{
+    return 4 * lbvh_scratch_align(cap * 4ull) + lbvh_scratch_align(256ull * ((cap + kLbvhSortTile - 1) / kLbvhSortTile) * 4) +
+           2 * lbvh_scratch_align(cap * 16ull) + 2 * lbvh_scratch_align(cap * 4ull) + lbvh_scratch_align(64) +
+           lbvh_scratch_align(cap * 8ull) + lbvh_scratch_align(cap * 128ull);
+}
+
 struct LbvhBuilder {
     uint32_t capacity = 0;
     uint32_t *keysA = nullptr, *keysB = nullptr, *valsA = nullptr, *valsB = nullptr;
@@ -36,6 +51,10 @@ struct LbvhBuilder {
 
     void reserve(uint32_t n);
     void release();
+    // Points the working arrays into `base` (device memory of
+    // lbvh_scratch_bytes(cap) bytes, 256-B aligned, owned by the caller);
+    // release() must not be called on such a builder.
+    void bind(void* base, uint32_t cap);
     void build(hipStream_t s, uint32_t n, Node* nodes, uint32_t* leafPrimOut, float4* rootBox);
     void refit(hipStream_t s, uint32_t n, Node* nodes, const uint32_t* leafPrim, float4* rootBox);
     void recompute_leaf_parents(hipStream_t s, uint32_t n, const Node* nodes);

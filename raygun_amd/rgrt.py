@@ -36,7 +36,7 @@ STATUS = {0: "OK", 1: "INVALID_ARG", 2: "OUT_OF_MEMORY", 3: "HIP", 4: "NOT_READY
 EXPORTS = [
     "rgrt_create", "rgrt_create_shared", "rgrt_destroy", "rgrt_last_error", "rgrt_stream", "rgrt_synchronize",
     "rgrt_upload_geometry", "rgrt_upload_materials", "rgrt_update_vertices", "rgrt_update_vertices_device",
-    "rgrt_build_blas", "rgrt_refit_blas", "rgrt_rebuild_blas", "rgrt_blas_info", "rgrt_export_blas",
+    "rgrt_build_blas", "rgrt_refit_blas", "rgrt_rebuild_blas", "rgrt_rebuild_blas_noalloc", "rgrt_blas_info", "rgrt_export_blas",
     "rgrt_build_tlas", "rgrt_refit_tlas", "rgrt_export_tlas", "rgrt_set_frame", "rgrt_resize",
     "rgrt_trace_rays", "rgrt_trace_rays_device", "rgrt_render", "rgrt_frame", "rgrt_trace", "rgrt_postprocess",
     "rgrt_read_image", "rgrt_read_image_rgba8", "rgrt_image_device_ptr", "rgrt_write_image", "rgrt_get_stats", "rgrt_get_pass_counts",
@@ -89,6 +89,7 @@ def lib():
             "rgrt_update_vertices": (i32, [P, u64, P, u64]),
             "rgrt_update_vertices_device": (i32, [P, u64, P, u64]),
             "rgrt_rebuild_blas": (i32, [P, u32]),
+            "rgrt_rebuild_blas_noalloc": (i32, [P, u32]),
             "rgrt_build_blas": (i32, [P, ctypes.POINTER(_BlasDesc), ctypes.POINTER(u32)]),
             "rgrt_refit_blas": (i32, [P, u32]),
             "rgrt_blas_info": (i32, [P, u32, ctypes.POINTER(u32), ctypes.POINTER(u32), P, P]),
@@ -228,6 +229,11 @@ class Context:
     def rebuild_blas(self, b):
         """Full LBVH rebuild of BLAS b from the current vertices; stream-ordered."""
         self._ok(self._L.rgrt_rebuild_blas(self.h, b))
+
+    def rebuild_blas_noalloc(self, b):
+        """rebuild_blas with the builder's arrays in the library's fixed
+        per-GPU scratch (no allocation; at most 262144 triangles)."""
+        self._ok(self._L.rgrt_rebuild_blas_noalloc(self.h, b))
 
     def blas_info(self, b):
         nn, nl = ctypes.c_uint32(), ctypes.c_uint32()
