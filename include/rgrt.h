@@ -347,6 +347,13 @@ rgrt_status rgrt_denoiser_infer(rgrt_denoiser* den, const uint16_t* x0, uint32_t
 rgrt_status rgrt_denoiser_dataset_reserve(rgrt_denoiser* den, uint32_t count, uint32_t height, uint32_t width);
 rgrt_status rgrt_denoiser_dataset_add_input(rgrt_denoiser* den, uint32_t slot);
 rgrt_status rgrt_denoiser_dataset_add_target(rgrt_denoiser* den, uint32_t slot, float weight, int32_t accumulate);
+/* Copies slots [first, first + count) of the training set to host memory:
+ * x0 = count*h*w*16 bf16 bit patterns (the packed inputs, as rgrt_denoise
+ * packs a frame), target = count*h*w*3 floats. height and width must be the
+ * set's (RGRT_ERR_INVALID_ARG otherwise). Either pointer may be null.
+ * Synchronous. */
+rgrt_status rgrt_denoiser_dataset_read(rgrt_denoiser* den, uint32_t first, uint32_t count, uint32_t height, uint32_t width,
+                                       uint16_t* x0, float* target);
 /* `steps` Adam steps (learning rate lr) on random batches of `batch` of the
  * first `used` slots; *loss = mean loss of the last steps. Synchronous. */
 rgrt_status rgrt_denoiser_train(rgrt_denoiser* den, uint32_t steps, uint32_t batch, float lr, uint32_t seed, uint32_t used,

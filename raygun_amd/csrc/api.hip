@@ -1720,6 +1720,17 @@ rgrt_status rgrt_denoiser_dataset_add_target(rgrt_denoiser* den, uint32_t slot, 
     });
 }
 
+rgrt_status rgrt_denoiser_dataset_read(rgrt_denoiser* den, uint32_t first, uint32_t count, uint32_t h, uint32_t w,
+                                       uint16_t* x0, float* target)
+{
+    if(!den) return RGRT_ERR_INVALID_ARG;
+    rgrt_ctx* c = den->ctx;
+    return guarded(c, [&]() {
+        den->d->dataset_read(c->stream, first, count, (int)h, (int)w, x0, target);
+        return RGRT_OK;
+    });
+}
+
 rgrt_status rgrt_denoiser_train(rgrt_denoiser* den, uint32_t steps, uint32_t batch, float lr, uint32_t seed, uint32_t used,
                                 float* loss)
 {

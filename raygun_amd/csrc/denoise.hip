@@ -380,10 +380,12 @@ __global__ void k_target_add(const uint16_t* __restrict__ base, long P, float we
     }
 }
 
-// Per 16x16 tile: the denoiser's change o against the frame's noise level.
-// A tile whose mean squared change exceeds a * (mean variance of the pixel
-// means) + b keeps the traced frame.
-__global__ void k_tile_check(const uint16_t* __restrict__ o, const uint16_t* __restrict__ feat, int H, int W, float a, float b,
+// Per 16x16 tile: the denoiser's change of the displayed colour (the noisy
+// RGB and the denoised RGB both clamped to [0, 1], the range the post pass
+// keeps) against the frame's noise level. A tile whose mean squared change
+// exceeds a * (mean variance of the pixel means) + b keeps the traced frame.
+__global__ void k_tile_check(const uint16_t* __restrict__ o, const uint16_t* __restrict__ base,
+                             const uint16_t* __restrict__ feat, int H, int W, float a, float b,
                              uint8_t* __restrict__ fallback)
 {
     __shared__ float se[kThreads], sv[kThreads];
@@ -394,7 +396,9 @@ __global__ void k_tile_check(const uint16_t* __restrict__ o, const uint16_t* __r
     if(x < W && y < H) {
         const long p = (long)y * W + x;
         for(int c = 0; c < 3; ++c) {
-            const float d = bf2f(o[p * 16 + c]);
+            const float x = h2f(base[4 * p + c]);
+            const float y = x + bf2f(o[p * 16 + c]);
+            const float d = fminf(fmaxf(y, 0.0f), 1.0f) - fminf(fmaxf(x, 0.0f), 1.0f);
             e += d * d;
         }
         e /= 3.0f;
@@ -810,6 +814,16 @@ void Denoiser::dataset_add_target(hipStream_t s, const Images& img, uint32_t slo
                                                                            dsT + (size_t)slot * dsH * dsW * 3);
 }
 
+void Denoiser::dataset_read(hipStream_t s, uint32_t first, uint32_t count, int H, int W, uint16_t* x0Host, float* targetHost)
+{
+    if(!ds || first > dsCount || count > dsCount - first) throw std::runtime_error("denoiser: dataset slots out of range");
+    if(H != dsH || W != dsW) throw std::runtime_error("denoiser: dataset frame size differs");
+    const size_t per = (size_t)dsH * dsW;
+    check(hipStreamSynchronize(s));
+    if(x0Host) check(hipMemcpy(x0Host, dsX + first * per * 16, count * per * 32, hipMemcpyDeviceToHost));
+    if(targetHost) check(hipMemcpy(targetHost, dsT + first * per * 3, count * per * 12, hipMemcpyDeviceToHost));
+}
+
 float Denoiser::train(hipStream_t s, uint32_t steps, uint32_t batch, float lr, uint32_t seed, uint32_t used)
 {
     if(!ds || used == 0 || used > dsCount) throw std::runtime_error("denoiser: no training data");
@@ -858,7 +872,7 @@ uint32_t Denoiser::denoise(hipStream_t s, Images& img, float fallbackA, float fa
     k_pack<<<grid_for((long)H * W, kThreads), kThreads, 0, s>>>(img.base, img.normal, img.features, H, W, buf[X0]);
     forward(s, 1, H, W);
     const dim3 tiles((W + 15) / 16, (H + 15) / 16);
-    k_tile_check<<<tiles, kThreads, 0, s>>>(buf[OUT], img.features, H, W, fallbackA, fallbackB, fallback);
+    k_tile_check<<<tiles, kThreads, 0, s>>>(buf[OUT], img.base, img.features, H, W, fallbackA, fallbackB, fallback);
     k_denoise_out<<<grid_for((long)H * W, kThreads), kThreads, 0, s>>>(buf[OUT], fallback, H, W, img.base);
     return tiles.x * tiles.y;
 }

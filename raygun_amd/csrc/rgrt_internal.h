@@ -193,6 +193,7 @@ class Denoiser {
     void dataset_reserve(uint32_t count, int H, int W);
     void dataset_add_input(hipStream_t s, const Images& img, uint32_t slot);
     void dataset_add_target(hipStream_t s, const Images& img, uint32_t slot, float weight, bool accumulate);
+    void dataset_read(hipStream_t s, uint32_t first, uint32_t count, int H, int W, uint16_t* x0Host, float* targetHost);
     // steps of Adam on random batches of the first `used` samples; returns the mean loss of the last steps
     float train(hipStream_t s, uint32_t steps, uint32_t batch, float lr, uint32_t seed, uint32_t used);
     // denoises img.base in place (rgb; fallback tiles keep the traced frame); returns the tile count

@@ -42,7 +42,7 @@ EXPORTS = [
     "rgrt_read_image", "rgrt_read_image_rgba8", "rgrt_image_device_ptr", "rgrt_write_image", "rgrt_get_stats", "rgrt_get_pass_counts",
     "rgrt_set_tuning", "rgrt_get_tuning", "rgrt_set_sample_groups", "rgrt_set_sampling", "rgrt_denoiser_create", "rgrt_denoiser_destroy",
     "rgrt_denoiser_num_params", "rgrt_denoiser_get_params", "rgrt_denoiser_set_params", "rgrt_denoiser_loss_and_grads",
-    "rgrt_denoiser_infer", "rgrt_denoiser_dataset_reserve", "rgrt_denoiser_dataset_add_input", "rgrt_denoiser_dataset_add_target",
+    "rgrt_denoiser_infer", "rgrt_denoiser_dataset_reserve", "rgrt_denoiser_dataset_add_input", "rgrt_denoiser_dataset_add_target", "rgrt_denoiser_dataset_read",
     "rgrt_denoiser_train", "rgrt_denoise", "rgrt_batch_create", "rgrt_batch_destroy", "rgrt_batch_last_error", "rgrt_batch_set_mode", "rgrt_batch_stream",
     "rgrt_batch_frame", "rgrt_export_blas_wide", "rgrt_export_tlas_wide", "rgrt_version",
 ]
@@ -128,6 +128,7 @@ def lib():
             "rgrt_denoiser_dataset_reserve": (i32, [P, u32, u32, u32]),
             "rgrt_denoiser_dataset_add_input": (i32, [P, u32]),
             "rgrt_denoiser_dataset_add_target": (i32, [P, u32, ctypes.c_float, i32]),
+            "rgrt_denoiser_dataset_read": (i32, [P, u32, u32, u32, u32, P, P]),
             "rgrt_denoiser_train": (i32, [P, u32, u32, ctypes.c_float, u32, u32, ctypes.POINTER(ctypes.c_float)]),
             "rgrt_denoise": (i32, [P, ctypes.c_float, ctypes.c_float, ctypes.POINTER(u32), ctypes.POINTER(u32)]),
             "rgrt_batch_create": (i32, [P, u32, ctypes.POINTER(P)]),
@@ -459,6 +460,14 @@ class Denoiser:
 
     def dataset_add_target(self, slot, weight, accumulate):
         self.ctx._ok(self._L.rgrt_denoiser_dataset_add_target(self.h, slot, weight, int(bool(accumulate))))
+
+    def dataset_read(self, first, count, height, width):
+        """Slots [first, first + count) of the training set: packed inputs
+        (count, h, w, 16) uint16 bf16 patterns and targets (count, h, w, 3)."""
+        x = np.zeros((count, height, width, 16), np.uint16)
+        t = np.zeros((count, height, width, 3), np.float32)
+        self.ctx._ok(self._L.rgrt_denoiser_dataset_read(self.h, first, count, height, width, _p(x), _p(t)))
+        return x, t
 
     def train(self, steps, batch, lr, seed, used):
         loss = ctypes.c_float()
