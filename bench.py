@@ -116,7 +116,7 @@ def main():
                     help="skip the config-5 measurement (3840x2160, 16M unique triangles, 4 spp, depth 6) after the headline")
     ap.add_argument("--config5-in-flight", type=int, choices=(1, 2, 3, 4, 5, 6), default=4,
                     help="frames in flight for the config-5 measurement")
-    ap.add_argument("--profile-workload", choices=("config5", "batched"), default="",
+    ap.add_argument("--profile-workload", choices=("config5", "batched", "denoise"), default="",
                     help="run only that secondary workload and print its JSON object (for profiler runs; "
                          "no headline line)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -130,6 +130,8 @@ def main():
     if args.profile_workload:
         if args.profile_workload == "config5":
             out = bench_config5(max(2, args.steps), args.config5_in_flight)
+        elif args.profile_workload == "denoise":
+            out = bench_denoise_inference(max(2, args.steps))
         else:
             out = bench_batched(args.batch_scenes or 32, max(2, args.steps))
         print(json.dumps(out), flush=True)
@@ -421,6 +423,30 @@ def bench_learned(train_count=1000, train_steps=3000, train_res=256, crop=128, e
     }
     log(f"[bench] learned vs plain {out['value']}x, PSNR {out['psnr_learned_db']} (noisy {out['psnr_noisy_4spp_db']})")
     return out
+
+
+def bench_denoise_inference(reps):
+    """The denoiser's inference alone on a 1920x1080 4-spp stochastic frame
+    of the config-1 family, `reps` times (for kernel profiles: the network's
+    cost does not depend on its parameter values, so the library's own
+    initialisation is loaded as the parameters)."""
+    from raygun_amd import denoise as D
+    from raygun_amd import rgrt
+    from raygun_amd.hiptime import time_ms
+    W, H = 1920, 1080
+    ctx = rgrt.Context(W, H, 0)
+    den = rgrt.Denoiser(ctx, seed=1)
+    den.set_params(den.get_params())
+    sc = D.GlossyScenes(ctx, W, H)
+    ubo = sc.scene(60000, 4)
+    for _ in D.trace_mean(ctx, ubo, 4, 1, 1):
+        pass
+    ms = time_ms(ctx, lambda: den.denoise(1e30, 1e30), reps=reps, warmup=1)
+    flops = 2.0 * D.MACS_PER_PIXEL * W * H
+    den.close()
+    ctx.close()
+    return {"workload": "denoiser inference, 1920x1080", "reps": reps, "ms": round(ms, 3), "gflop": round(flops / 1e9, 1),
+            "achieved_tflops": round(flops / (ms * 1e-3) / 1e12, 1)}
 
 
 def bench_config5(frames, in_flight=4):

@@ -42,67 +42,95 @@ __device__ __forceinline__ float h2f(uint16_t b) { return (float)__builtin_bit_c
 __device__ __forceinline__ uint16_t f2h(float x) { return __builtin_bit_cast(uint16_t, (_Float16)x); }
 
 constexpr int kThreads = 256;
+constexpr int kConvStrips = 4; // 16-pixel strips a convolution wave computes (64 pixels of one row)
 
 inline uint32_t grid_for(uint64_t n, uint32_t t) { return (uint32_t)((n + t - 1) / t); }
 
 // ---------------------------------------------------------------------------
 // Convolution 3x3, stride 1, zero padding 1: out = act(conv(in, wt) + bias)
 // in [N][H][W][KIN], wt [9][NOUT][KIN], out [N][H][W][NOUT]
+// With C0 > 0 the input is the concatenation the decoder convolves (skip
+// connection): channels [0, C0) are the nearest 2x upsampling of
+// low [N][H/2][W/2][C0], channels [C0, KIN) come from in [N][H][W][KIN - C0];
+// the concatenated tensor is never stored.
 // ---------------------------------------------------------------------------
-template <int KIN, int NOUT>
+template <int KIN, int NOUT, int C0 = 0>
 __global__ __launch_bounds__(kThreads) void k_conv(const uint16_t* __restrict__ in, const uint16_t* __restrict__ wt,
                                                    const float* __restrict__ bias, uint16_t* __restrict__ out, int N, int H,
-                                                   int W, int relu)
+                                                   int W, int relu, const uint16_t* __restrict__ low)
 {
     static_assert(KIN % 16 == 0 && NOUT % 16 == 0, "channel counts are multiples of 16");
+    static_assert(C0 % 8 == 0 && C0 < KIN, "a load of 8 channels comes from one source");
+    constexpr int C1 = KIN - C0;
     constexpr int KT = 9 * KIN;
     constexpr int KSTEPS = (KT + 31) / 32;
     constexpr int NT = NOUT / 16;
+    constexpr int SPW = kConvStrips; // 16-pixel strips of one row per wave, sharing the weight operands
     const int lane = threadIdx.x & 63;
     const long wave = (long)blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
     const int stripsPerRow = (W + 15) / 16;
-    if(wave >= (long)N * H * stripsPerRow) return; // whole waves only
-    const int sx = (int)(wave % stripsPerRow);
-    const long row = wave / stripsPerRow;
+    const int groupsPerRow = (stripsPerRow + SPW - 1) / SPW;
+    if(wave >= (long)N * H * groupsPerRow) return; // whole waves only
+    const int sg = (int)(wave % groupsPerRow);
+    const long row = wave / groupsPerRow;
     const int y = (int)(row % H);
     const int n = (int)(row / H);
     const int m = lane & 15, g = lane >> 4;
-    const int x = sx * 16 + m;
-    f32x4 acc[NT];
+    const int x0 = sg * SPW * 16 + m; // pixel of this lane in the group's first strip
+    f32x4 acc[SPW][NT];
 #pragma unroll
-    for(int j = 0; j < NT; ++j) acc[j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    for(int q = 0; q < SPW; ++q)
+#pragma unroll
+        for(int j = 0; j < NT; ++j) acc[q][j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
     const bf16x8 zero8 = __builtin_bit_cast(bf16x8, u32x4{0u, 0u, 0u, 0u});
-#pragma unroll 2
     for(int s = 0; s < KSTEPS; ++s) {
         const int k0 = 32 * s + 8 * g;
         const int tap = k0 / KIN;
         const int c0 = k0 - tap * KIN;
         const bool kin = k0 < KT;
-        bf16x8 a = zero8;
-        const int yy = y + tap / 3 - 1, xx = x + tap % 3 - 1;
-        if(kin && yy >= 0 && yy < H && xx >= 0 && xx < W)
-            a = __builtin_bit_cast(bf16x8, ld_global16(in + (((long)n * H + yy) * W + xx) * KIN + c0));
+        const int yy = y + tap / 3 - 1;
+        const bool rowIn = kin && yy >= 0 && yy < H;
+        bf16x8 b[NT];
 #pragma unroll
         for(int j = 0; j < NT; ++j) {
-            bf16x8 b = zero8;
-            if(kin) b = __builtin_bit_cast(bf16x8, ld_global16(wt + ((long)tap * NOUT + 16 * j + m) * KIN + c0));
-            acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[j], 0, 0, 0);
+            b[j] = zero8;
+            if(kin) b[j] = __builtin_bit_cast(bf16x8, ld_global16(wt + ((long)tap * NOUT + 16 * j + m) * KIN + c0));
         }
+        bf16x8 a[SPW];
+#pragma unroll
+        for(int q = 0; q < SPW; ++q) {
+            a[q] = zero8;
+            const int xx = x0 + 16 * q + tap % 3 - 1;
+            if(rowIn && xx >= 0 && xx < W) {
+                if(C0 > 0 && c0 < C0)
+                    a[q] = __builtin_bit_cast(bf16x8,
+                                              ld_global16(low + (((long)n * (H / 2) + yy / 2) * (W / 2) + xx / 2) * C0 + c0));
+                else
+                    a[q] = __builtin_bit_cast(bf16x8, ld_global16(in + (((long)n * H + yy) * W + xx) * C1 + (c0 - C0)));
+            }
+        }
+#pragma unroll
+        for(int q = 0; q < SPW; ++q)
+#pragma unroll
+            for(int j = 0; j < NT; ++j) acc[q][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[q], b[j], acc[q][j], 0, 0, 0);
     }
-    // lane holds D[row 4g + i][column m]: pixel sx*16 + 4g + i, channel 16j + m
+    // lane holds D[row 4g + i][column m] of each strip: pixel 4g + i of the
+    // strip, channel 16j + m
 #pragma unroll
     for(int j = 0; j < NT; ++j) {
         const int co = 16 * j + m;
         const float bv = bias ? bias[co] : 0.0f;
 #pragma unroll
-        for(int i = 0; i < 4; ++i) {
-            const int px = sx * 16 + 4 * g + i;
-            if(px < W) {
-                float v = acc[j][i] + bv;
-                if(relu) v = fmaxf(v, 0.0f);
-                out[(((long)n * H + y) * W + px) * NOUT + co] = f2bf(v);
+        for(int q = 0; q < SPW; ++q)
+#pragma unroll
+            for(int i = 0; i < 4; ++i) {
+                const int px = (sg * SPW + q) * 16 + 4 * g + i;
+                if(px < W) {
+                    float v = acc[q][j][i] + bv;
+                    if(relu) v = fmaxf(v, 0.0f);
+                    out[(((long)n * H + y) * W + px) * NOUT + co] = f2bf(v);
+                }
             }
-        }
     }
 }
 
@@ -440,12 +468,12 @@ __global__ void k_denoise_out(const uint16_t* __restrict__ o, const uint8_t* __r
 // ---------------------------------------------------------------------------
 // Launch helpers
 // ---------------------------------------------------------------------------
-template <int KIN, int NOUT>
+template <int KIN, int NOUT, int C0 = 0>
 void conv_t(hipStream_t s, const uint16_t* in, const uint16_t* wt, const float* bias, uint16_t* out, int N, int H, int W,
-            int relu)
+            int relu, const uint16_t* low = nullptr)
 {
-    const long waves = (long)N * H * ((W + 15) / 16);
-    k_conv<KIN, NOUT><<<grid_for(waves, kThreads / 64), kThreads, 0, s>>>(in, wt, bias, out, N, H, W, relu);
+    const long waves = (long)N * H * (((W + 15) / 16 + kConvStrips - 1) / kConvStrips);
+    k_conv<KIN, NOUT, C0><<<grid_for(waves, kThreads / 64), kThreads, 0, s>>>(in, wt, bias, out, N, H, W, relu, low);
 }
 
 void conv(hipStream_t s, int kin, int nout, const uint16_t* in, const uint16_t* wt, const float* bias, uint16_t* out, int N,
@@ -637,7 +665,7 @@ void Denoiser::reserve(int N, int H, int W, bool training)
     wsTraining = training;
 }
 
-void Denoiser::forward(hipStream_t s, int N, int H, int W)
+void Denoiser::forward(hipStream_t s, int N, int H, int W, bool keepConcat)
 {
     if(!weightsReady) prep_weights(s);
     const int H1 = H / 2, W1 = W / 2, H2 = H / 4, W2 = W / 4;
@@ -651,11 +679,15 @@ void Denoiser::forward(hipStream_t s, int N, int H, int W)
     k_pool<<<grid_for((long)N * H2 * W2 * 32, kThreads), kThreads, 0, s>>>(buf[S2], buf[P2], N, H1, W1, 32);
     conv(s, 32, 64, buf[P2], w(4), b(4), buf[A3], N, H2, W2, 1);
     conv(s, 64, 64, buf[A3], w(5), b(5), buf[B3], N, H2, W2, 1);
-    k_up_concat<<<grid_for((long)N * H1 * W1 * 96, kThreads), kThreads, 0, s>>>(buf[B3], buf[S2], buf[C2], N, H1, W1, 64, 32);
-    conv(s, 96, 32, buf[C2], w(6), b(6), buf[A4], N, H1, W1, 1);
+    // the decoder convolutions read up(b3) ++ s2 and up(u2) ++ s1 directly;
+    // the concatenations are stored only for the backward pass
+    if(keepConcat)
+        k_up_concat<<<grid_for((long)N * H1 * W1 * 96, kThreads), kThreads, 0, s>>>(buf[B3], buf[S2], buf[C2], N, H1, W1, 64, 32);
+    conv_t<96, 32, 64>(s, buf[S2], w(6), b(6), buf[A4], N, H1, W1, 1, buf[B3]);
     conv(s, 32, 32, buf[A4], w(7), b(7), buf[U2], N, H1, W1, 1);
-    k_up_concat<<<grid_for((long)N * H * W * 48, kThreads), kThreads, 0, s>>>(buf[U2], buf[S1], buf[C1], N, H, W, 32, 16);
-    conv(s, 48, 16, buf[C1], w(8), b(8), buf[A5], N, H, W, 1);
+    if(keepConcat)
+        k_up_concat<<<grid_for((long)N * H * W * 48, kThreads), kThreads, 0, s>>>(buf[U2], buf[S1], buf[C1], N, H, W, 32, 16);
+    conv_t<48, 16, 32>(s, buf[S1], w(8), b(8), buf[A5], N, H, W, 1, buf[U2]);
     conv(s, 16, 16, buf[A5], w(9), b(9), buf[U1], N, H, W, 1);
     conv(s, 16, 16, buf[U1], w(10), b(10), buf[OUT], N, H, W, 0);
 }
@@ -748,7 +780,7 @@ float Denoiser::loss_and_grads(hipStream_t s, const uint16_t* x0Host, const floa
     try {
         check(hipMemcpyAsync(buf[X0], x0Host, P * 32, hipMemcpyHostToDevice, s));
         check(hipMemcpyAsync(tgt, targetHost, P * 12, hipMemcpyHostToDevice, s));
-        forward(s, N, H, W);
+        forward(s, N, H, W, true);
         backward(s, tgt, N, H, W);
         float loss = 0.0f;
         check(hipMemcpyAsync(&loss, lossPart + 4095, 4, hipMemcpyDeviceToHost, s));
@@ -770,7 +802,7 @@ void Denoiser::infer(hipStream_t s, const uint16_t* x0Host, int N, int H, int W,
         reserve(N, H, W, false);
     }
     check(hipMemcpyAsync(buf[X0], x0Host, (size_t)N * H * W * 32, hipMemcpyHostToDevice, s));
-    forward(s, N, H, W);
+    forward(s, N, H, W, false);
     read_output(s, outHost);
 }
 
@@ -843,7 +875,7 @@ float Denoiser::train(hipStream_t s, uint32_t steps, uint32_t batch, float lr, u
                 check(hipMemcpyAsync(buf[X0] + b * per * 16, dsX + k * per * 16, per * 32, hipMemcpyDeviceToDevice, s));
                 check(hipMemcpyAsync(tgt + b * per * 3, dsT + k * per * 3, per * 12, hipMemcpyDeviceToDevice, s));
             }
-            forward(s, (int)batch, dsH, dsW);
+            forward(s, (int)batch, dsH, dsW, true);
             backward(s, tgt, (int)batch, dsH, dsW);
             adam_step(s, lr);
             if(it + 16 >= steps) { // mean loss of the last steps
@@ -870,7 +902,7 @@ uint32_t Denoiser::denoise(hipStream_t s, Images& img, float fallbackA, float fa
     const int H = (int)img.h, W = (int)img.w;
     if(!ws || wsN != 1 || wsH != H || wsW != W) reserve(1, H, W, false);
     k_pack<<<grid_for((long)H * W, kThreads), kThreads, 0, s>>>(img.base, img.normal, img.features, H, W, buf[X0]);
-    forward(s, 1, H, W);
+    forward(s, 1, H, W, false);
     const dim3 tiles((W + 15) / 16, (H + 15) / 16);
     k_tile_check<<<tiles, kThreads, 0, s>>>(buf[OUT], img.base, img.features, H, W, fallbackA, fallbackB, fallback);
     k_denoise_out<<<grid_for((long)H * W, kThreads), kThreads, 0, s>>>(buf[OUT], fallback, H, W, img.base);

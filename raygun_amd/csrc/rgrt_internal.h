@@ -203,7 +203,8 @@ class Denoiser {
   private:
     void reserve(int N, int H, int W, bool training);
     void prep_weights(hipStream_t s);
-    void forward(hipStream_t s, int N, int H, int W);
+    // keepConcat: also store the decoder's concatenated inputs (the backward pass reads them)
+    void forward(hipStream_t s, int N, int H, int W, bool keepConcat);
     void backward(hipStream_t s, const float* target, int N, int H, int W);
     void adam_step(hipStream_t s, float lr);
 
