@@ -21,6 +21,17 @@ constexpr int kSpill = 368; // with the 16 LDS entries: the 381-entry worst case
 #ifndef RGRT_TRACE_WAVES
 #define RGRT_TRACE_WAVES 6
 #endif
+// Iterations between the wave's instance entries / triangle tests (the
+// lanes of a wave start their rays together in both kernels here).
+#ifndef RGRT_INST_EVERY
+#define RGRT_INST_EVERY 4
+#endif
+#ifndef RGRT_TRI_EVERY
+#define RGRT_TRI_EVERY 1
+#endif
+#ifndef RGRT_SENT_EVERY
+#define RGRT_SENT_EVERY 1
+#endif
 
 __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(RGRT_TRACE_WAVES))) void k_trace_queue(TraceScene sc, const float* __restrict__ rays,
                                                               float* __restrict__ hits, uint32_t npix,
@@ -60,7 +71,7 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(RGR
             r.dz = rays[6 * (size_t)npix + i];
             r.tmax = rays[7 * (size_t)npix + i];
             const uint32_t flags = __float_as_uint(rays[9 * (size_t)npix + i]);
-            const HitRec h = traverse(sc, r, (flags >> kMaskShift) & 0xffu, stack);
+            const HitRec h = traverse<RGRT_INST_EVERY, RGRT_TRI_EVERY, RGRT_SENT_EVERY>(sc, r, (flags >> kMaskShift) & 0xffu, stack);
             hits[0 * (size_t)npix + i] = h.t;
             hits[1 * (size_t)npix + i] = h.u;
             hits[2 * (size_t)npix + i] = h.v;
@@ -86,7 +97,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_rays(TraceScene sc, const
     for(uint32_t i = gtid; i < n; i += gridDim.x * kTraceBlock) {
         const rgrt_ray rr = rays[i];
         RayIn r{rr.origin[0], rr.origin[1], rr.origin[2], rr.tmin, rr.dir[0], rr.dir[1], rr.dir[2], rr.tmax};
-        const HitRec h = traverse(sc, r, cullMask & 0xffu, stack);
+        const HitRec h = traverse<RGRT_INST_EVERY, RGRT_TRI_EVERY, RGRT_SENT_EVERY>(sc, r, cullMask & 0xffu, stack);
         rgrt_hit o;
         o.t = h.t;
         o.u = h.u;

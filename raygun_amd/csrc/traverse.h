@@ -297,8 +297,18 @@ __device__ __forceinline__ bool trav_begin(Traversal& T, const TraceScene& sc, c
 
 // One step: a wide node, a sentinel, an instance leaf or a triangle.
 // Returns false once the ray is finished (T.h holds the closest hit).
-template<typename Stack>
-__device__ __forceinline__ bool trav_step(Traversal& T, const TraceScene& sc, Stack& stack)
+//
+// A wave runs every kind of step its lanes need in each iteration, so the
+// rare kinds (instance entry, triangle test) cost their instructions in
+// nearly every iteration. With kInstEvery / kTriEvery > 1 those steps are
+// taken only in iterations `iter` that are multiples of it; a lane that
+// reaches one in another iteration waits (its state is unchanged), which
+// delays that ray by a few iterations but leaves the wave's other
+// iterations without the path. Results do not change: every ray takes the
+// same steps in the same order. `iter` must count the wave's iterations
+// (all lanes start the ray together).
+template<int kInstEvery = 1, int kTriEvery = 1, int kSentEvery = 1, typename Stack>
+__device__ __forceinline__ bool trav_step(Traversal& T, const TraceScene& sc, Stack& stack, uint32_t iter = 0)
 {
     if(T.ref >= 0) {
         if((uint32_t)T.ref >= T.nodeCap) {
@@ -313,12 +323,14 @@ __device__ __forceinline__ bool trav_step(Traversal& T, const TraceScene& sc, St
         }
     }
     else if(T.ref == kSentinel) {
+        if(kSentEvery > 1 && iter % kSentEvery != 0u) return true; // taken in a later iteration
         T.inBlas = false;
         T.nodes = sc.tlasNodes;
         T.nodeCap = sc.tlasNodeCap;
         box_ray_setup(T.br, T.ray.ox, T.ray.oy, T.ray.oz, T.ray.dx, T.ray.dy, T.ray.dz);
     }
     else if(!T.inBlas) {
+        if(kInstEvery > 1 && iter % kInstEvery != 0u) return true; // taken in a later iteration
         const int32_t leaf = ~T.ref;
         if((uint32_t)leaf >= sc.numInstances) {
             trav_error(sc, kTravErrInstance);
@@ -360,6 +372,7 @@ __device__ __forceinline__ bool trav_step(Traversal& T, const TraceScene& sc, St
         }
     }
     else {
+        if(kTriEvery > 1 && iter % kTriEvery != 0u) return true; // taken in a later iteration
         const int32_t leaf = ~T.ref;
         if((uint32_t)leaf >= T.numFaces) {
             trav_error(sc, kTravErrTriangle);
@@ -397,13 +410,13 @@ __device__ __forceinline__ bool trav_step(Traversal& T, const TraceScene& sc, St
 // Whole traversal of one ray. Each node and leaf is visited at most once
 // per level, so a well-formed two-level tree ends far below the step cap;
 // the cap only bounds the loop.
-template<typename Stack>
+template<int kInstEvery = 1, int kTriEvery = 1, int kSentEvery = 1, typename Stack>
 __device__ HitRec traverse(const TraceScene& sc, const RayIn& ray, uint32_t cullMask, Stack& stack)
 {
     Traversal T;
     if(trav_begin(T, sc, ray, cullMask, stack)) {
         for(uint32_t guard = 0; guard < (1u << 26); ++guard)
-            if(!trav_step(T, sc, stack)) break;
+            if(!trav_step<kInstEvery, kTriEvery, kSentEvery>(T, sc, stack, guard)) break;
     }
     return T.h;
 }
