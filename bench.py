@@ -129,7 +129,7 @@ def main():
         return
     if args.profile_workload:
         if args.profile_workload == "config5":
-            out = bench_config5(max(2, args.steps), args.config5_in_flight)
+            out = bench_config5(max(2, args.steps), args.config5_in_flight, learned=False)
         elif args.profile_workload == "denoise":
             out = bench_denoise_inference(max(2, args.steps))
         else:
@@ -353,7 +353,7 @@ def main():
             result["learned_vs_plain"]["error"] = repr(e)
     if not args.no_config5:
         try:
-            result["config5"] = bench_config5(max(4, args.steps // 6), args.config5_in_flight)
+            result["config5"] = bench_config5(max(4, args.steps // 6), args.config5_in_flight, learned=not args.no_learned)
         except Exception as e:  # the headline stands without it
             log(f"[bench] config5 failed: {e!r}")
     if args.batch_scenes:
@@ -449,7 +449,48 @@ def bench_denoise_inference(reps):
             "achieved_tflops": round(flops / (ms * 1e-3) / 1e12, 1)}
 
 
-def bench_config5(frames, in_flight=4):
+def bench_config5_learned(ctx, setup, train_views=300, train_steps=2000, res=128):
+    """Config 5's learned frame: 4 spp (stochastic sampling) + the denoiser
+    on config 5's own 3840x2160 view, against the plain tracer at equal PSNR,
+    PSNR against the 256-spp plain frame. The network is trained for this
+    scene family: seeded views of another config-5 scene (seed 11, 150k
+    triangles per mesh) at res^2, 4-spp inputs and 128-spp targets, on the
+    GPU in a child process (PyTorch fp32); inference on the library's MFMA
+    kernels."""
+    from raygun_amd import denoise as D
+    from raygun_amd import procgen, rgrt
+    t0 = time.time()
+    tctx = rgrt.Context(res, res, 0)
+    tden = rgrt.Denoiser(tctx, seed=1)
+    views = D.SceneViews(tctx, procgen.config5(res, res, seed=11, tris_per_mesh=150_000), res, res)
+    first, last, t_render, t_train = D.train_network(tctx, tden, views, train_views, first_seed=5000, spp_ref=128,
+                                                     steps=train_steps, batch=16, lr=1e-3)
+    params = tden.get_params()
+    tden.close()
+    tctx.close()
+    log(f"[bench] config5 learned: trained on {train_views} views ({t_render:.1f}s rendering, {t_train:.1f}s training, "
+        f"loss {first:.5f} -> {last:.5f})")
+    den = rgrt.Denoiser(ctx, seed=1)
+    try:
+        den.set_params(params)
+        ev = D.SceneViews(ctx, setup, setup.width, setup.height, fixed_camera=True, upload=False)
+        r = D.learned_vs_plain(ctx, den, ev, [5], sweep=(4, 8, 16, 32, 64))[0]
+    finally:
+        den.close()
+        ctx.set_sampling(rgrt.SAMPLING_WHITTED)
+    flops = 2.0 * D.MACS_PER_PIXEL * setup.width * setup.height
+    r["inference"] = {"gflop": round(flops / 1e9, 1), "achieved_tflops": round(flops / (r["denoise_ms"] * 1e-3) / 1e12, 1),
+                      "frac": round(flops / (r["denoise_ms"] * 1e-3) / 2.5e15, 4)}
+    r["training"] = {"views": train_views, "resolution": res, "steps": train_steps, "target_spp": 128,
+                     "loss_first": round(first, 6), "loss_last": round(last, 6), "render_s": round(t_render, 1),
+                     "train_s": round(t_train, 1)}
+    r["seconds"] = round(time.time() - t0, 1)
+    log(f"[bench] config5 learned: PSNR {r['psnr_learned']} (noisy {r['psnr_noisy']}), {r['learned_ms']} ms, "
+        f"{r['ratio_at_equal_psnr']}x at equal PSNR")
+    return r
+
+
+def bench_config5(frames, in_flight=4, learned=True):
     """Config 5's plain tracer: 3840x2160, 8 seeded displaced meshes of ~2M
     triangles (16M unique), 256 instances, one in five refractive,
     maxRecursions 6, 4 samples per pixel (the SSAA table), FXAA; frames as
@@ -490,15 +531,77 @@ def bench_config5(frames, in_flight=4):
         ms = frames_ms(ctxs, fns, frames, 2, ev)
         rays = ctx.stats()["rays"]
         out = {"workload": "config5: 3840x2160, 8 displaced meshes (16M unique triangles), 256 instances, 20% refractive "
-                           "(IOR U[1.3,1.7]), maxRecursions 6, 4 spp (SSAA table), FXAA; plain tracer only",
+                           "(IOR U[1.3,1.7]), maxRecursions 6, 4 spp (SSAA table), FXAA",
                "unique_triangles": setup.info["unique_triangles"], "instanced_triangles": setup.info["instanced_triangles"],
                "blas_build_s": round(build_s, 3), "frames": frames, "frames_in_flight": in_flight, "tuning": best,
                "rays_per_frame": rays, "ms_per_frame": round(ms, 3), "mrays_per_s": round(rays / (ms * 1e-3) / 1e6, 1)}
         log(f"[bench] config5: {out}")
+        try:
+            out["with_8_scenes"] = bench_config5_batch(ctx, ubo, inst, max(3, frames // 2), ev)
+        except Exception as e:  # the config-5 figure stands without it
+            log(f"[bench] config5 batch failed: {e!r}")
+        if learned:
+            try:
+                out["learned"] = bench_config5_learned(ctx, setup)
+            except Exception as e:  # the config-5 figure stands without it
+                log(f"[bench] config5 learned path failed: {e!r}")
         return out
     finally:
         for c in ctxs:
             c.close()
+
+
+def bench_config5_batch(ctx, ubo, inst, frames, ev):
+    """Config 5's frame submitted together with 8 lower-resolution scenes of
+    the batched family (320x180 .. 960x540, 10k-1M triangles, animated, one
+    deforming) in one rgrt_batch_frame (a hipGraph per scene on its own
+    stream): ms per batch frame against the config-5 frame alone (one frame
+    at a time), and the batch's rays per second."""
+    from raygun_amd import batched, rgrt
+    w = batched.BatchWorkload(num_scenes=8, seed=7, lo=10_000, hi=1_000_000,
+                              resolutions=[(320, 180), (480, 270), (640, 360), (960, 540)])
+    batch = rgrt.Batch([ctx] + [s.ctx for s in w.scenes], rgrt.BATCH_GRAPH_PER_SCENE)
+    try:
+        stream = batch.stream()
+
+        def frame(f, t):
+            for s in w.scenes:
+                if s.deforms and f:
+                    s.deform(f)
+            batch.frame([(ubo, inst, True, True)] + [(s.ubo, s.instances(t), True, f > 0) for s in w.scenes])
+
+        frame(0, 0.0)
+        frame(1, 0.05)
+        batch.synchronize()
+
+        def timed(fn, n):
+            a, b = ev.create(), ev.create()
+            ctx.synchronize()
+            ev.record(a, stream)
+            for k in range(n):
+                fn(k)
+            ev.record(b, stream)
+            ms = ev.elapsed_ms(a, b) / n
+            batch.synchronize()
+            ev.destroy(a)
+            ev.destroy(b)
+            return ms
+
+        ms_batch = timed(lambda k: frame(2 + k, 0.05 * (2 + k)), frames)
+        rays5 = ctx.stats()["rays"]
+        rays8 = sum(s.ctx.stats()["rays"] for s in w.scenes)
+        from raygun_amd.hiptime import time_ms
+        ms_alone = time_ms(ctx, lambda: ctx.frame(ubo, inst, fxaa=True, refit=True, graph=True), reps=frames, warmup=1,
+                           events=ev)
+        out = {"scenes": [[s.width, s.height, s.mesh.num_faces()] for s in w.scenes],
+               "rays_per_batch_frame": rays5 + rays8, "ms_per_batch_frame": round(ms_batch, 3),
+               "ms_config5_frame_alone": round(ms_alone, 3),
+               "mrays_per_s_batch": round((rays5 + rays8) / (ms_batch * 1e-3) / 1e6, 1)}
+        log(f"[bench] config5 with 8 scenes: {out}")
+        return out
+    finally:
+        batch.close()
+        w.close()
 
 
 def bench_batched(num_scenes, frames):

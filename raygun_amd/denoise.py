@@ -153,27 +153,27 @@ def reference_frame(ctx, ubo, spp_total, seed, spp_per_frame=32):
     return acc / frames
 
 
-def train_network(ctx, den, scenes, count, first_seed=1000, spp_in=4, spp_ref=256, steps=3000, batch=16, lr=1e-3, crop=None,
-                  seed=0, workdir=None, timeout=900, frame=None, clamp_target=False):
-    """Renders `count` training pairs with the library, trains the network on
-    the GPU in a child process (denoise_train; PyTorch has a HIP runtime of
-    its own) from the library's own initialisation and loads the result
-    into `den`. Returns (first loss, last loss, seconds rendering, seconds
-    training)."""
+def render_dataset(ctx, den, scenes, count, first_seed=1000, spp_in=4, spp_ref=256, frame=None):
+    """`count` training pairs rendered and packed by the library (the
+    denoiser's dataset on `ctx`), read back: (x0 bf16 bits N x H x W x 16,
+    targets N x H x W x 3)."""
+    build_dataset(ctx, den, scenes, count, first_seed=first_seed, spp_in=spp_in, spp_ref=spp_ref, frame=frame)
+    return den.dataset_read(0, count, scenes.height, scenes.width)
+
+
+def train_params(x, t, init, steps=3000, batch=16, lr=1e-3, crop=None, seed=0, clamp_target=False, workdir=None,
+                 timeout=900):
+    """Trains the network on the GPU in a child process (denoise_train;
+    PyTorch has a HIP runtime of its own) from `init`. Returns (parameters,
+    first loss, last loss)."""
     import json
     import os
     import subprocess
     import sys
     import tempfile
-    import time
-    t0 = time.perf_counter()
-    build_dataset(ctx, den, scenes, count, first_seed=first_seed, spp_in=spp_in, spp_ref=spp_ref, frame=frame)
-    x, t = den.dataset_read(0, count, scenes.height, scenes.width)
-    t1 = time.perf_counter()
     with tempfile.TemporaryDirectory(dir=workdir) as tmp:
         data, out = os.path.join(tmp, "train.npz"), os.path.join(tmp, "params.npy")
-        np.savez(data, x0=x, target=t, init=den.get_params())
-        del x, t
+        np.savez(data, x0=x, target=t, init=init)
         cmd = [sys.executable, "-m", "raygun_amd.denoise_train", data, out, "--steps", str(steps), "--batch", str(batch),
                "--lr", str(lr), "--seed", str(seed), "--crop", str(crop or 0)] + (["--clamp-target"] if clamp_target else [])
         root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -182,9 +182,24 @@ def train_network(ctx, den, scenes, count, first_seed=1000, spp_in=4, spp_ref=25
             raise RuntimeError(f"denoiser training failed ({r.returncode}):\n{r.stderr[-2000:]}")
         stats = json.loads(r.stdout.strip().splitlines()[-1])
         params = np.load(out, allow_pickle=False)
+    return params, stats["loss_first"], stats["loss_last"]
+
+
+def train_network(ctx, den, scenes, count, first_seed=1000, spp_in=4, spp_ref=256, steps=3000, batch=16, lr=1e-3, crop=None,
+                  seed=0, workdir=None, timeout=900, frame=None, clamp_target=False):
+    """Renders `count` training pairs with the library, trains the network on
+    the GPU from the library's own initialisation and loads the result into
+    `den`. Returns (first loss, last loss, seconds rendering, seconds
+    training)."""
+    import time
+    t0 = time.perf_counter()
+    x, t = render_dataset(ctx, den, scenes, count, first_seed=first_seed, spp_in=spp_in, spp_ref=spp_ref, frame=frame)
+    t1 = time.perf_counter()
+    params, first, last = train_params(x, t, den.get_params(), steps=steps, batch=batch, lr=lr, crop=crop, seed=seed,
+                                       clamp_target=clamp_target, workdir=workdir, timeout=timeout)
     t2 = time.perf_counter()
     den.set_params(params)
-    return stats["loss_first"], stats["loss_last"], t1 - t0, t2 - t1
+    return first, last, t1 - t0, t2 - t1
 
 
 # Multiply-accumulates per pixel of one inference as executed (input padded
@@ -281,3 +296,42 @@ def learned_vs_plain(ctx, den, scenes, seeds, spp_learned=4, spp_ref=256, sweep=
     ev.destroy(a)
     ev.destroy(b)
     return out
+
+
+class SceneViews:
+    """A procgen scene (SceneSetup) uploaded once into a context: geometry,
+    materials, BLASes and TLAS. scene(seed, spp) returns frame constants of a
+    seeded view of it (camera on a sphere of radius U[r0, r1] around the
+    origin, looking at a point in the central box of half-size `spread`,
+    vertical FOV U[40, 70]), or, with fixed_camera, the setup's own camera.
+    The config-5 counterpart of GlossyScenes."""
+
+    def __init__(self, ctx, setup, width, height, fixed_camera=False, r0=60.0, r1=170.0, spread=60.0, upload=True):
+        self.ctx, self.setup, self.width, self.height = ctx, setup, width, height
+        self.fixed_camera, self.r0, self.r1, self.spread = fixed_camera, r0, r1, spread
+        if not upload:  # the context holds the scene and its TLAS already
+            return
+        models = setup.scene.models()
+        v, i, m = pack_models(models)
+        ctx.upload_geometry(v, i)
+        ctx.upload_materials(m)
+        blas_of = {}
+        for mo in models:
+            k = id(mo.mesh)
+            if k not in blas_of:
+                blas_of[k] = ctx.build_blas(mo.mesh.first_vertex, mo.mesh.index_offset_bytes, mo.mesh.num_faces(),
+                                            len(mo.mesh.vertices))
+            mo.blas = blas_of[k]
+        ctx.build_tlas(setup.scene.collect_instances())
+
+    def scene(self, seed, spp, max_recursions=None):
+        rec = self.setup.max_recursions if max_recursions is None else max_recursions
+        if self.fixed_camera:
+            return camera_ubo(self.setup.scene.camera, self.width, self.height, num_samples=spp, max_recursions=rec)
+        rng = np.random.default_rng(seed)
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        eye = d * rng.uniform(self.r0, self.r1)
+        cam = Camera(fov_deg=float(rng.uniform(40, 70)))
+        cam.look_at(tuple(eye), tuple(rng.uniform(-self.spread, self.spread, 3)))
+        return camera_ubo(cam, self.width, self.height, num_samples=spp, max_recursions=rec)
