@@ -647,6 +647,41 @@ def bench_batched(num_scenes, frames):
         ms_per_scene = timed_batched(rgrt.BATCH_GRAPH_PER_SCENE, 0.0)
         rays = w.rays()
         ms_one_graph = timed_batched(rgrt.BATCH_ONE_GRAPH, 1.0)
+        # one graph tracing all scenes in shared launches: the batch's launch
+        # shape is the first scene's tuning, picked here for the batch
+        s0 = w.scenes[0]
+        own = s0.ctx.tuning()
+        persistent = {}
+        for passes in (4, 6, 8, 10):
+            for blocks in (6, 8):
+                s0.ctx.set_tuning(wavefrontPasses=passes, traceBlocksPerCU=blocks)
+                persistent[(passes, blocks)] = timed_batched(rgrt.BATCH_PERSISTENT, 3.0 + 0.1 * passes + blocks)
+        best_shape = min(persistent, key=persistent.get)
+        s0.ctx.set_tuning(wavefrontPasses=best_shape[0], traceBlocksPerCU=best_shape[1])
+        ms_persistent = timed_batched(rgrt.BATCH_PERSISTENT, 5.0)
+
+        def latency(mode, t_first):
+            """ms from submission to completion of one batch frame at a time."""
+            w.batch.set_mode(mode)
+            w.frame_batched(t_first)
+            w.synchronize()
+            a, b = ev.create(), ev.create()
+            total = 0.0
+            for f in range(frames):
+                w.synchronize()
+                ev.record(a, bstream)
+                w.frame_batched(t_first + 0.05 * (f + 1))
+                ev.record(b, bstream)
+                total += ev.elapsed_ms(a, b)
+            w.synchronize()
+            ev.destroy(a)
+            ev.destroy(b)
+            return total / frames
+
+        lat = {"persistent": latency(rgrt.BATCH_PERSISTENT, 6.0)}
+        s0.ctx.set_tuning(**own)
+        lat["graph_per_scene"] = latency(rgrt.BATCH_GRAPH_PER_SCENE, 7.0)
+        lat["one_graph"] = latency(rgrt.BATCH_ONE_GRAPH, 8.0)
         w.batch.set_mode(rgrt.BATCH_GRAPH_PER_SCENE)
         t2 = time.perf_counter()
         for f in range(frames):
@@ -654,7 +689,8 @@ def bench_batched(num_scenes, frames):
         w.synchronize()
         ms_seq_wall = (time.perf_counter() - t2) * 1e3 / frames
         log(f"[bench] batched ms per frame: graph per scene {ms_per_scene:.3f}, one graph {ms_one_graph:.3f}, "
-            f"scene by scene (eager, wall) {ms_seq_wall:.3f}")
+            f"persistent {ms_persistent:.3f} (passes, blocks per CU {best_shape}), scene by scene (eager, wall) "
+            f"{ms_seq_wall:.3f}; one batch frame at a time {lat}")
         return {"workload": f"config3: {num_scenes} scenes, triangles log-normal in [10k, 4M], resolutions 320x180.."
                             "1920x1080, animated instances (TLAS refit), one scene in four deforming (device-resident "
                             "poses, full BLAS rebuild)",
@@ -664,6 +700,10 @@ def bench_batched(num_scenes, frames):
                 "ms_per_batched_frame": round(ms_per_scene, 3),
                 "mrays_per_s_batched": round(rays / (ms_per_scene * 1e-3) / 1e6, 1),
                 "ms_per_batched_frame_one_graph": round(ms_one_graph, 3),
+                "ms_per_batched_frame_persistent": round(ms_persistent, 3),
+                "persistent_shape": {"wavefrontPasses": best_shape[0], "traceBlocksPerCU": best_shape[1]},
+                "persistent_sweep_ms": {f"{p}x{b}": round(m, 3) for (p, b), m in persistent.items()},
+                "ms_one_batch_frame_at_a_time": {k: round(v, 3) for k, v in lat.items()},
                 "ms_per_sequential_frame_wall": round(ms_seq_wall, 3),
                 "autotuned": tunings}
     finally:

@@ -399,6 +399,14 @@ rgrt_status rgrt_frame(rgrt_ctx* ctx, const rgrt_ubo* ubo, const rgrt_instance* 
 /*  RGRT_BATCH_ONE_GRAPH: one hipGraph with a branch per scene, launched on  */
 /*   the first context's stream after the work queued on every scene's      */
 /*   stream; work issued later on any of the contexts runs after the batch. */
+/*  RGRT_BATCH_PERSISTENT: one hipGraph as in RGRT_BATCH_ONE_GRAPH, but the  */
+/*   scenes are traced together: each wavefront pass of every scene is one  */
+/*   traversal launch and one shading launch over all scenes' ray queues,   */
+/*   and one drain launch finishes every scene's recursion tail (waves move */
+/*   on to the next scene as they run out of work). The instance uploads,   */
+/*   TLAS builds, ray generation and post passes stay one branch per scene.  */
+/*   Launch shape and pass count: the first context's tuning. At most 64    */
+/*   scenes. Same images as the other modes.                                 */
 /* In both modes the batch's own stream (rgrt_batch_stream) is ordered after */
 /* every scene of every batch frame submitted so far.                        */
 /* How far the scenes overlap depends on the hardware queues the HIP runtime */
@@ -417,6 +425,7 @@ void rgrt_batch_destroy(rgrt_batch* batch);
 const char* rgrt_batch_last_error(const rgrt_batch* batch);
 #define RGRT_BATCH_GRAPH_PER_SCENE 0u
 #define RGRT_BATCH_ONE_GRAPH 1u
+#define RGRT_BATCH_PERSISTENT 2u
 rgrt_status rgrt_batch_set_mode(rgrt_batch* batch, uint32_t mode);
 /* descs[i] is the frame of ctxs[i]. Asynchronous. */
 rgrt_status rgrt_batch_frame(rgrt_batch* batch, const rgrt_frame_desc* descs);

@@ -1387,6 +1387,18 @@ struct rgrt_batch {
     uint32_t mode = RGRT_BATCH_GRAPH_PER_SCENE;
     hipStream_t stream = nullptr; // ordered after every scene of each batch frame
     std::string err;
+    // RGRT_BATCH_PERSISTENT: the scenes' kernel arguments in device memory
+    // (two slots: a frame's slot is rewritten only after that frame ran),
+    // the pass counters of the shared traversal launches, and the graph
+    hipGraphExec_t pexec = nullptr;
+    ShadeParams* dParams[2] = {nullptr, nullptr};
+    TraceScene* dScenes[2] = {nullptr, nullptr};
+    uint32_t* dStarts[2] = {nullptr, nullptr}; // block starts: raygen, rough/composite, FXAA (n + 1 each)
+    hipEvent_t slotFree[2] = {nullptr, nullptr};
+    int slot = 0;
+    uint32_t* dCounters = nullptr;
+    hipEvent_t fork2 = nullptr;
+    std::vector<hipEvent_t> join2;
 };
 
 extern "C" {
@@ -1411,7 +1423,20 @@ rgrt_status rgrt_batch_create(rgrt_ctx* const* ctxs, uint32_t n, rgrt_batch** ou
         for(auto& e: b->join) check(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         b->pre.resize(n, nullptr);
         for(auto& e: b->pre) check(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        b->join2.resize(n, nullptr);
+        for(auto& e: b->join2) check(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        check(hipEventCreateWithFlags(&b->fork2, hipEventDisableTiming));
         check(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+        if(n <= (uint32_t)kMaxBatchScenes) {
+            for(int k = 0; k < 2; ++k) {
+                check(hipMalloc((void**)&b->dParams[k], n * sizeof(ShadeParams)));
+                check(hipMalloc((void**)&b->dScenes[k], n * sizeof(TraceScene)));
+                check(hipMalloc((void**)&b->dStarts[k], 3 * (n + 1) * sizeof(uint32_t)));
+                check(hipEventCreateWithFlags(&b->slotFree[k], hipEventDisableTiming));
+            }
+            check(hipMalloc((void**)&b->dCounters, 2 * sizeof(uint32_t)));
+            check(hipMemset(b->dCounters, 0, 2 * sizeof(uint32_t)));
+        }
     }
     catch(...) {
         rgrt_batch_destroy(b);
@@ -1427,6 +1452,17 @@ void rgrt_batch_destroy(rgrt_batch* b)
     for(rgrt_ctx* c: b->ctxs)
         if(c && c->stream) (void)hipStreamSynchronize(c->stream);
     if(b->exec) (void)hipGraphExecDestroy(b->exec);
+    if(b->pexec) (void)hipGraphExecDestroy(b->pexec);
+    for(int k = 0; k < 2; ++k) {
+        if(b->dParams[k]) (void)hipFree(b->dParams[k]);
+        if(b->dScenes[k]) (void)hipFree(b->dScenes[k]);
+        if(b->dStarts[k]) (void)hipFree(b->dStarts[k]);
+        if(b->slotFree[k]) (void)hipEventDestroy(b->slotFree[k]);
+    }
+    if(b->dCounters) (void)hipFree(b->dCounters);
+    if(b->fork2) (void)hipEventDestroy(b->fork2);
+    for(hipEvent_t e: b->join2)
+        if(e) (void)hipEventDestroy(e);
     for(hipEvent_t e: b->join)
         if(e) (void)hipEventDestroy(e);
     for(hipEvent_t e: b->pre)
@@ -1446,10 +1482,172 @@ void* rgrt_batch_stream(rgrt_batch* b) { return b ? (void*)b->stream : nullptr; 
 
 rgrt_status rgrt_batch_set_mode(rgrt_batch* b, uint32_t mode)
 {
-    if(!b || (mode != RGRT_BATCH_GRAPH_PER_SCENE && mode != RGRT_BATCH_ONE_GRAPH)) return RGRT_ERR_INVALID_ARG;
+    if(!b || (mode != RGRT_BATCH_GRAPH_PER_SCENE && mode != RGRT_BATCH_ONE_GRAPH && mode != RGRT_BATCH_PERSISTENT))
+        return RGRT_ERR_INVALID_ARG;
+    if(mode == RGRT_BATCH_PERSISTENT && b->ctxs.size() > (size_t)kMaxBatchScenes) {
+        b->err = "persistent batches hold at most " + std::to_string(kMaxBatchScenes) + " scenes";
+        return RGRT_ERR_INVALID_ARG;
+    }
     b->mode = mode;
     return RGRT_OK;
 }
+
+} // extern "C"
+
+namespace {
+
+// RGRT_BATCH_PERSISTENT: one graph on the first context's stream. Per scene
+// (a branch on its own stream): instance upload and TLAS; then every scene
+// together: ray generation, the passes (one traversal and one shading launch
+// each, over every scene's queue), one drain launch, and the post passes
+// (rough + composite, FXAA); per scene again only the sums of side-by-side
+// samples and, with showAlpha, the post passes. The scenes' kernel
+// arguments are in device memory (b->dParams / dScenes / dStarts, the slot of
+// this frame).
+rgrt_status batch_frame_persistent(rgrt_batch* b, const rgrt_frame_desc* descs)
+{
+    const uint32_t n = (uint32_t)b->ctxs.size();
+    if(n > (uint32_t)kMaxBatchScenes || !b->dCounters) {
+        b->err = "persistent batches hold at most " + std::to_string(kMaxBatchScenes) + " scenes";
+        return RGRT_ERR_INVALID_ARG;
+    }
+    rgrt_ctx* c0 = b->ctxs[0];
+    hipStream_t s0 = c0->stream;
+    std::vector<uint32_t> prevN(n);
+    std::vector<bool> prevHave(n);
+    for(uint32_t i = 0; i < n; ++i) {
+        prevN[i] = b->ctxs[i]->numInstances;
+        prevHave[i] = b->ctxs[i]->haveTlas;
+        b->ctxs[i]->numInstances = descs[i].numInstances;
+        b->ctxs[i]->haveTlas = true;
+    }
+    try {
+        check(hipSetDevice(c0->device));
+        std::vector<ShadeParams> ps(n);
+        std::vector<TraceScene> scs(n);
+        uint32_t chain = 1;
+        for(uint32_t i = 0; i < n; ++i) {
+            ps[i] = shade_params(b->ctxs[i]);
+            scs[i] = b->ctxs[i]->scene();
+            chain = std::max(chain, chain_iters(b->ctxs[i]));
+        }
+        const uint32_t passes = std::min<uint32_t>(c0->tuning.wavefrontPasses, chain);
+        // block starts of the merged launches
+        std::vector<uint32_t> starts(3 * (n + 1), 0u);
+        bool showAlpha = false, sampleSums = false;
+        for(uint32_t i = 0; i < n; ++i) {
+            const rgrt_ctx* c = b->ctxs[i];
+            starts[i + 1] = starts[i] + (ps[i].st.npix + 255u) / 256u;
+            starts[(n + 1) + i + 1] = starts[(n + 1) + i] + post_tiles(c->w, c->h);
+            starts[2 * (n + 1) + i + 1] =
+                starts[2 * (n + 1) + i] + ((descs[i].flags & RGRT_FRAME_FXAA) ? fxaa_tiles(c->w, c->h) : 0u);
+            showAlpha = showAlpha || c->ubo.showAlpha;
+            sampleSums = sampleSums || ps[i].st.groups > 1u;
+        }
+        const int slot = b->slot;
+        b->slot ^= 1;
+        check(hipEventSynchronize(b->slotFree[slot])); // the frame that used this slot has run
+        check(hipMemcpy(b->dParams[slot], ps.data(), n * sizeof(ShadeParams), hipMemcpyHostToDevice));
+        check(hipMemcpy(b->dScenes[slot], scs.data(), n * sizeof(TraceScene), hipMemcpyHostToDevice));
+        check(hipMemcpy(b->dStarts[slot], starts.data(), starts.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+        // work the scenes' own streams already hold comes before the batch
+        for(uint32_t i = 1; i < n; ++i) {
+            check(hipEventRecord(b->pre[i], b->ctxs[i]->stream));
+            check(hipStreamWaitEvent(s0, b->pre[i], 0));
+        }
+        hipGraph_t g = nullptr;
+        check(hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal));
+        uint32_t joined = 0, joined2 = 0;
+        try {
+            launch_zero_u32(s0, b->dCounters, 2);
+            check(hipEventRecord(b->fork, s0));
+            for(uint32_t i = 0; i < n; ++i) {
+                rgrt_ctx* c = b->ctxs[i];
+                hipStream_t s = c->stream;
+                if(i) check(hipStreamWaitEvent(s, b->fork, 0));
+                const uint32_t ni = descs[i].numInstances;
+                if(ni)
+                    check(hipMemcpyAsync(c->dInstIn, c->hInstStage[c->stageIdx], ni * sizeof(rgrt_instance),
+                                         hipMemcpyHostToDevice, s));
+                enqueue_tlas(c, ni, (descs[i].flags & RGRT_FRAME_REFIT_TLAS) != 0);
+                if(i) {
+                    check(hipEventRecord(b->join[i], s));
+                    check(hipStreamWaitEvent(s0, b->join[i], 0));
+                }
+                joined = i + 1;
+            }
+            launch_raygen_multi(s0, b->dParams[slot], b->dStarts[slot], (int)n, starts[n]);
+            int cur = 0;
+            for(uint32_t it = 0; it < passes; ++it) {
+                launch_trace_multi(s0, b->dParams[slot], b->dScenes[slot], (int)n, cur, c0->traceGrid, c0->spill, b->dCounters);
+                launch_shade_multi(s0, b->dParams[slot], (int)n, cur, c0->shadeGrid);
+                cur ^= 1;
+            }
+            launch_drain_multi(s0, b->dParams[slot], b->dScenes[slot], (int)n, cur, c0->drainGrid, c0->spill);
+            if(sampleSums || showAlpha) {
+                check(hipEventRecord(b->fork2, s0));
+                for(uint32_t i = 0; i < n; ++i) {
+                    rgrt_ctx* c = b->ctxs[i];
+                    hipStream_t s = c->stream;
+                    if(i) check(hipStreamWaitEvent(s, b->fork2, 0));
+                    launch_finalize_samples(s, ps[i]);
+                    if(showAlpha) launch_post(s, c->img, c->ubo, (descs[i].flags & RGRT_FRAME_FXAA) ? 1 : 0, nullptr);
+                    if(i) {
+                        check(hipEventRecord(b->join2[i], s));
+                        check(hipStreamWaitEvent(s0, b->join2[i], 0));
+                    }
+                    joined2 = i + 1;
+                }
+            }
+            if(!showAlpha)
+                launch_post_multi(s0, b->dParams[slot], b->dStarts[slot] + (n + 1), (int)n, starts[2 * (n + 1) - 1],
+                                  starts[3 * (n + 1) - 1]);
+        }
+        catch(...) {
+            // join whatever branches were opened, then end the capture
+            for(uint32_t i = 1; i < n && i <= joined; ++i) {
+                (void)hipEventRecord(b->join[i], b->ctxs[i]->stream);
+                (void)hipStreamWaitEvent(s0, b->join[i], 0);
+            }
+            for(uint32_t i = 1; i < n && i <= joined2; ++i) {
+                (void)hipEventRecord(b->join2[i], b->ctxs[i]->stream);
+                (void)hipStreamWaitEvent(s0, b->join2[i], 0);
+            }
+            (void)hipStreamEndCapture(s0, &g);
+            if(g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        check(hipStreamEndCapture(s0, &g));
+        update_exec(&b->pexec, g);
+        check(hipGraphLaunch(b->pexec, s0));
+        check(hipEventRecord(b->slotFree[slot], s0));
+        check(hipEventRecord(b->done, s0));
+        check(hipStreamWaitEvent(b->stream, b->done, 0));
+        for(uint32_t i = 0; i < n; ++i) {
+            rgrt_ctx* c = b->ctxs[i];
+            check(hipEventRecord(c->stageDone[c->stageIdx], s0));
+            if(i) check(hipStreamWaitEvent(c->stream, b->done, 0));
+            frame_finish(c, descs[i].instances, descs[i].numInstances, descs[i].flags);
+            c->timedFrame = false;
+            c->lastTraceLaunches = passes;
+            c->lastDrained = true;
+            c->lastWaves = passes;
+        }
+    }
+    catch(const std::exception& e) {
+        for(uint32_t i = 0; i < n; ++i) {
+            b->ctxs[i]->numInstances = prevN[i];
+            b->ctxs[i]->haveTlas = prevHave[i];
+        }
+        b->err = e.what();
+        return RGRT_ERR_HIP;
+    }
+    return RGRT_OK;
+}
+
+} // namespace
+
+extern "C" {
 
 rgrt_status rgrt_batch_frame(rgrt_batch* b, const rgrt_frame_desc* descs)
 {
@@ -1489,6 +1687,7 @@ rgrt_status rgrt_batch_frame(rgrt_batch* b, const rgrt_frame_desc* descs)
         }
         return RGRT_OK;
     }
+    if(b->mode == RGRT_BATCH_PERSISTENT) return batch_frame_persistent(b, descs);
     std::vector<uint32_t> prevN(n);
     std::vector<bool> prevHave(n);
     for(uint32_t i = 0; i < n; ++i) {

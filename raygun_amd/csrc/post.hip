@@ -52,13 +52,11 @@ __device__ __forceinline__ int8_t to_snorm8(float f)
 }
 __device__ __forceinline__ float from_snorm8(int8_t v) { return fmaxf((float)v / 127.0f, -1.0f); }
 
-__global__ __launch_bounds__(kPostThreads) void k_rough_post(Images img, float4 fade)
+// One 32x32 output tile at (tx0, ty0) of rough prepare, the 20 blur passes
+// and the composite; A, B, T are the block's LDS region arrays.
+__device__ __forceinline__ void rough_post_tile(const Images& img, float4 fade, int tx0, int ty0, ushort4* A, ushort4* B, float* T)
 {
-    __shared__ ushort4 A[kRegionPix];
-    __shared__ ushort4 B[kRegionPix];
-    __shared__ float T[kRegionPix];
     const int W = (int)img.w, H = (int)img.h;
-    const int tx0 = blockIdx.x * kTile, ty0 = blockIdx.y * kTile;
     const int rx0 = tx0 - kHalo, ry0 = ty0 - kHalo;
 
     // rough_prepare over the region
@@ -153,6 +151,30 @@ __global__ __launch_bounds__(kPostThreads) void k_rough_post(Images img, float4 
         f.w = f2h(luma);
         reinterpret_cast<ushort4*>(img.final_)[pix] = f;
     }
+}
+
+__global__ __launch_bounds__(kPostThreads) void k_rough_post(Images img, float4 fade)
+{
+    __shared__ ushort4 A[kRegionPix];
+    __shared__ ushort4 B[kRegionPix];
+    __shared__ float T[kRegionPix];
+    rough_post_tile(img, fade, blockIdx.x * kTile, blockIdx.y * kTile, A, B, T);
+}
+
+// The post passes of every scene of a batch in one launch: block b is tile
+// b - start[s] of scene s (row-major over its 32x32 tiles).
+__global__ __launch_bounds__(kPostThreads) void k_rough_post_multi(const ShadeParams* __restrict__ ps,
+                                                                   const uint32_t* __restrict__ start, int numScenes)
+{
+    __shared__ ushort4 A[kRegionPix];
+    __shared__ ushort4 B[kRegionPix];
+    __shared__ float T[kRegionPix];
+    const int s = block_scene(start, numScenes, blockIdx.x);
+    const ShadeParams& p = ps[s];
+    const uint32_t t = blockIdx.x - start[s];
+    const uint32_t tilesX = (p.img.w + kTile - 1) / kTile;
+    const float4 fade = make_float4(p.ubo.fadeColor[0], p.ubo.fadeColor[1], p.ubo.fadeColor[2], p.ubo.fadeColor[3]);
+    rough_post_tile(p.img, fade, (int)(t % tilesX) * kTile, (int)(t / tilesX) * kTile, A, B, T);
 }
 
 // postprocess.comp:41-49 (ubo.showAlpha): every image shows its alpha.
@@ -257,10 +279,8 @@ __device__ __forceinline__ EdgeEnd walk_edge(const Tex& t, float su, float sv, f
 
 __device__ __forceinline__ float second_diff(float a, float b, float c) { return fabsf((a + c) - 2.0f * b); }
 
-__global__ __launch_bounds__(256) void k_fxaa(const uint16_t* __restrict__ src, uint16_t* __restrict__ out, int W, int H)
+__device__ __forceinline__ void fxaa_pixel(const uint16_t* __restrict__ src, uint16_t* __restrict__ out, int W, int H, int x, int y)
 {
-    const int x = blockIdx.x * 16 + (threadIdx.x & 15);
-    const int y = blockIdx.y * 16 + (threadIdx.x >> 4);
     if(x >= W || y >= H) return;
     const Tex t{src, W, H};
     const float texelU = 1.0f / (float)W, texelV = 1.0f / (float)H;
@@ -324,6 +344,24 @@ __global__ __launch_bounds__(256) void k_fxaa(const uint16_t* __restrict__ src, 
     *dst = make_ushort4(f2h(c.x), f2h(c.y), f2h(c.z), f2h(lC));
 }
 
+__global__ __launch_bounds__(256) void k_fxaa(const uint16_t* __restrict__ src, uint16_t* __restrict__ out, int W, int H)
+{
+    fxaa_pixel(src, out, W, H, blockIdx.x * 16 + (threadIdx.x & 15), blockIdx.y * 16 + (threadIdx.x >> 4));
+}
+
+// FXAA of every scene of a batch that asks for it in one launch: block b is
+// 16x16 tile b - start[s] of scene s (scenes without FXAA have no tiles).
+__global__ __launch_bounds__(256) void k_fxaa_multi(const ShadeParams* __restrict__ ps, const uint32_t* __restrict__ start,
+                                                    int numScenes)
+{
+    const int s = block_scene(start, numScenes, blockIdx.x);
+    const ShadeParams& p = ps[s];
+    const uint32_t t = blockIdx.x - start[s];
+    const uint32_t tilesX = (p.img.w + 15) / 16;
+    fxaa_pixel(p.img.final_, p.img.base, (int)p.img.w, (int)p.img.h, (int)(t % tilesX) * 16 + (threadIdx.x & 15),
+               (int)(t / tilesX) * 16 + (threadIdx.x >> 4));
+}
+
 // ---------------------------------------------------------------------------
 // Readback in a display format (the swapchain blit of
 // raygun/render/render_system.cpp:118-147 converts the rgba16f result to the
@@ -371,6 +409,19 @@ void launch_to_rgba8(hipStream_t s, const uint16_t* src, uint8_t* dst, size_t np
 // Post passes: rough + composite into `final_`, optional showAlpha, and
 // FXAA from `final_` into `base`. `roughEnd` (may be null) is recorded
 // after the fused rough/composite kernel.
+// Post passes of a batch: `start` (device) holds numScenes + 1 block starts
+// for the rough/composite tiles, then numScenes + 1 for the FXAA tiles;
+// blocks are the totals (host copies of the last entries).
+void launch_post_multi(hipStream_t s, const ShadeParams* ps, const uint32_t* start, int numScenes, uint32_t roughBlocks,
+                       uint32_t fxaaBlocks)
+{
+    if(roughBlocks) k_rough_post_multi<<<roughBlocks, kPostThreads, 0, s>>>(ps, start, numScenes);
+    if(fxaaBlocks) k_fxaa_multi<<<fxaaBlocks, 256, 0, s>>>(ps, start + numScenes + 1, numScenes);
+}
+
+uint32_t post_tiles(uint32_t w, uint32_t h) { return ((w + kTile - 1) / kTile) * ((h + kTile - 1) / kTile); }
+uint32_t fxaa_tiles(uint32_t w, uint32_t h) { return ((w + 15) / 16) * ((h + 15) / 16); }
+
 void launch_post(hipStream_t s, const Images& img, const rgrt_ubo& ubo, int useFxaa, hipEvent_t roughEnd)
 {
     const dim3 grid((img.w + kTile - 1) / kTile, (img.h + kTile - 1) / kTile);

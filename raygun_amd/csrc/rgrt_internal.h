@@ -230,7 +230,36 @@ class Denoiser {
     int dsH = 0, dsW = 0;
 };
 
+// Scenes a batch traces in one launch per pass at most (RGRT_BATCH_PERSISTENT).
+constexpr int kMaxBatchScenes = 64;
+
+// Scene of block b of a batch launch from the scenes' block starts
+// (start[s] <= b < start[s + 1]); uniform over the block.
+__device__ __forceinline__ int block_scene(const uint32_t* start, int numScenes, uint32_t b)
+{
+    int s = 0;
+    while(s + 1 < numScenes && start[s + 1] <= b) ++s;
+    return s;
+}
+
 void launch_raygen(hipStream_t s, const ShadeParams& p);
+// One pass / the drain of every scene of a batch in one launch; ps and scs
+// are device arrays of numScenes entries (1..kMaxBatchScenes).
+void launch_trace_multi(hipStream_t s, const ShadeParams* ps, const TraceScene* scs, int numScenes, int cur, uint32_t gridBlocks,
+                        int32_t* spill, uint32_t* batchCounters);
+void launch_shade_multi(hipStream_t s, const ShadeParams* ps, int numScenes, int cur, uint32_t gridBlocks);
+void launch_zero_u32(hipStream_t s, uint32_t* p, uint32_t n);
+// rough + composite and FXAA of every scene of a batch, one launch each
+// (start: block starts, see post.hip); tile counts per scene for the starts
+void launch_post_multi(hipStream_t s, const ShadeParams* ps, const uint32_t* start, int numScenes, uint32_t roughBlocks,
+                       uint32_t fxaaBlocks);
+uint32_t post_tiles(uint32_t w, uint32_t h);
+uint32_t fxaa_tiles(uint32_t w, uint32_t h);
+// Ray generation of every scene of a batch in one launch (start: numScenes + 1
+// block starts of 256 slots each)
+void launch_raygen_multi(hipStream_t s, const ShadeParams* ps, const uint32_t* start, int numScenes, uint32_t blocks);
+void launch_drain_multi(hipStream_t s, const ShadeParams* ps, const TraceScene* scs, int numScenes, int cur, uint32_t gridBlocks,
+                        int32_t* spill);
 void launch_trace_queue(hipStream_t s, const TraceScene& sc, const WavefrontState& st, int cur, uint32_t gridBlocks);
 void launch_shade(hipStream_t s, const ShadeParams& p, int cur, uint32_t gridBlocks);
 void launch_finalize_samples(hipStream_t s, const ShadeParams& p);

@@ -649,9 +649,8 @@ __device__ __forceinline__ void write_ray(float* rays, size_t n, uint32_t slot, 
     rays[9 * n + slot] = __uint_as_float(flags);
 }
 
-__global__ __launch_bounds__(kShadeBlock) void k_raygen(ShadeParams p)
+__device__ __forceinline__ void raygen_slot(const ShadeParams& p, uint32_t pix)
 {
-    const uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x;
     if(pix == 0) {
         uint32_t* c = p.st.counters;
         c[0] = p.st.npix; // queue 0 holds every pixel's primary ray
@@ -679,6 +678,20 @@ __global__ __launch_bounds__(kShadeBlock) void k_raygen(ShadeParams p)
     p.st.meta[pix] = g << 8;
     // only sample 0's primary ray carries the primary-hit flag (AOV, features)
     write_ray(p.st.rays[0], p.st.npix, pix, r, pix, (g == 0u ? kFlagPrimary : 0u) | (0xffu << kMaskShift));
+}
+
+__global__ __launch_bounds__(kShadeBlock) void k_raygen(ShadeParams p)
+{
+    raygen_slot(p, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// Ray generation of every scene of a batch: block b covers slots
+// [256 (b - start[s]), +256) of scene s.
+__global__ __launch_bounds__(kShadeBlock) void k_raygen_multi(const ShadeParams* __restrict__ ps, const uint32_t* __restrict__ start,
+                                                              int numScenes)
+{
+    const int s = block_scene(start, numScenes, blockIdx.x);
+    raygen_slot(ps[s], (blockIdx.x - start[s]) * kShadeBlock + threadIdx.x);
 }
 
 // Sample groups > 1: every pixel's sample records summed in sample order
@@ -824,20 +837,17 @@ __device__ __forceinline__ HitRec load_hit(const float* hits, size_t n, uint32_t
     return h;
 }
 
-// Wavefront shading pass over the rays traced by the preceding traversal pass.
-__global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(RGRT_SHADE_WAVES))) void k_shade(ShadeParams p, int cur)
+// Shades queue entries [base, base + 64) of queue `cur` (one wave, one lane
+// per entry), appending the emitted rays to the other queue with one atomic
+// per wave.
+__device__ __forceinline__ void shade_wave(const ShadeParams& p, const Ctx& c, int cur, uint32_t count, uint32_t base, int lane,
+                                           uint64_t ltMask)
 {
-    const uint32_t count = p.st.counters[cur];
     const size_t n = p.st.npix;
     const float* __restrict__ rays = p.st.rays[cur];
     float* __restrict__ outRays = p.st.rays[1 - cur];
     uint32_t* outCount = p.st.counters + (1 - cur);
-    const int lane = threadIdx.x & 63;
-    const uint64_t ltMask = lane == 0 ? 0ull : ((~0ull) >> (64 - lane));
-    const Ctx c{p, f3(p.ubo.lightDir[0], p.ubo.lightDir[1], p.ubo.lightDir[2]), p.ubo.maxRecursions, 0u};
-    const uint32_t stride = gridDim.x * blockDim.x;
-
-    for(uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < count; base += stride) {
+    {
         const uint32_t i = base + lane;
         bool emit = false;
         OutRay out;
@@ -870,6 +880,63 @@ __global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(RGR
     }
 }
 
+// Wavefront shading pass over the rays traced by the preceding traversal pass.
+__global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(RGRT_SHADE_WAVES))) void k_shade(ShadeParams p, int cur)
+{
+    const uint32_t count = p.st.counters[cur];
+    const int lane = threadIdx.x & 63;
+    const uint64_t ltMask = lane == 0 ? 0ull : ((~0ull) >> (64 - lane));
+    const Ctx c{p, f3(p.ubo.lightDir[0], p.ubo.lightDir[1], p.ubo.lightDir[2]), p.ubo.maxRecursions, 0u};
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for(uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < count; base += stride)
+        shade_wave(p, c, cur, count, base, lane, ltMask);
+}
+
+// Scene of a batch that chunk k (64 queue entries) belongs to, from the
+// inclusive prefix of the scenes' chunk counts in LDS; wave-uniform.
+__device__ __forceinline__ int chunk_scene(const uint32_t* chunkEnd, int numScenes, uint32_t k, int lane)
+{
+    return __popcll(__ballot(lane < numScenes && chunkEnd[lane < numScenes ? lane : 0] <= k));
+}
+
+// Inclusive prefix over the batch's scenes (lane s < numScenes <= 64) of the
+// 64-entry chunks in queue `cur`, written to chunkEnd by the block's first
+// wave; the other waves wait at the barrier.
+__device__ __forceinline__ void batch_chunk_prefix(const ShadeParams* ps, int numScenes, int cur, uint32_t* chunkEnd)
+{
+    if(threadIdx.x < 64) {
+        const int lane = threadIdx.x;
+        uint32_t v = lane < numScenes ? (ps[lane].st.counters[cur] + 63u) / 64u : 0u;
+        for(int off = 1; off < 64; off <<= 1) {
+            const uint32_t u = __shfl_up(v, off);
+            if(lane >= off) v += u;
+        }
+        if(lane < numScenes) chunkEnd[lane] = v;
+    }
+    __syncthreads();
+}
+
+// Shading pass of a whole batch: the queues `cur` of every scene in one
+// launch, 64-entry chunks handed to waves in order (a chunk belongs to one
+// scene, so a wave shades one scene at a time).
+__global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(RGRT_SHADE_WAVES))) void k_shade_multi(
+    const ShadeParams* __restrict__ ps, int numScenes, int cur)
+{
+    __shared__ uint32_t chunkEnd[kMaxBatchScenes];
+    batch_chunk_prefix(ps, numScenes, cur, chunkEnd);
+    const uint32_t chunks = chunkEnd[numScenes - 1];
+    const int lane = threadIdx.x & 63;
+    const uint64_t ltMask = lane == 0 ? 0ull : ((~0ull) >> (64 - lane));
+    const uint32_t waves = gridDim.x * (kShadeBlock / 64);
+    for(uint32_t k = blockIdx.x * (kShadeBlock / 64) + (threadIdx.x >> 6); k < chunks; k += waves) {
+        const int s = __builtin_amdgcn_readfirstlane(chunk_scene(chunkEnd, numScenes, k, lane));
+        const ShadeParams& p = ps[s];
+        const uint32_t first = s ? chunkEnd[s - 1] : 0u;
+        const Ctx c{p, f3(p.ubo.lightDir[0], p.ubo.lightDir[1], p.ubo.lightDir[2]), p.ubo.maxRecursions, 0u};
+        shade_wave(p, c, cur, p.st.counters[cur], (k - first) * 64u, lane, ltMask);
+    }
+}
+
 constexpr int kDrainBlock = 256;
 constexpr int kDrainLdsStack = 16;
 constexpr int kDrainSpill = 368; // see traverse.h: 16 + 368 >= 381
@@ -880,9 +947,9 @@ constexpr int kDrainSpill = 368; // see traverse.h: 16 + 368 >= 381
 // the queue (one atomic per wave, a ballot to hand out the entries). The
 // recursion tail thus costs the longest remaining chain of one pixel rather
 // than one launch per recursion step.
-__global__ __launch_bounds__(kDrainBlock) __attribute__((amdgpu_waves_per_eu(RGRT_DRAIN_WAVES))) void k_drain(ShadeParams p, TraceScene sc, int cur, int32_t* __restrict__ spillBase)
+template<typename Stack>
+__device__ __forceinline__ void drain_scene(const ShadeParams& p, const TraceScene& sc, int cur, Stack& stack)
 {
-    __shared__ int32_t ldsStack[kDrainLdsStack * kDrainBlock];
     const uint32_t count = p.st.counters[cur];
     const size_t n = p.st.npix;
     const float* __restrict__ rays = p.st.rays[cur];
@@ -890,9 +957,6 @@ __global__ __launch_bounds__(kDrainBlock) __attribute__((amdgpu_waves_per_eu(RGR
     const int lane = threadIdx.x & 63;
     const uint64_t ltMask = lane == 0 ? 0ull : ((~0ull) >> (64 - lane));
     const Ctx c{p, f3(p.ubo.lightDir[0], p.ubo.lightDir[1], p.ubo.lightDir[2]), p.ubo.maxRecursions, 0u};
-    const uint32_t gtid = blockIdx.x * kDrainBlock + threadIdx.x;
-    TraversalStack<kDrainLdsStack, kDrainBlock, kDrainSpill> stack{ldsStack, spillBase + (size_t)gtid * kDrainSpill,
-                                                                   (int)threadIdx.x};
     bool active = false;
     bool exhausted = false;
     uint32_t pix = 0, flags = 0;
@@ -956,12 +1020,44 @@ __global__ __launch_bounds__(kDrainBlock) __attribute__((amdgpu_waves_per_eu(RGR
     if(lane == 0 && traced) atomicAdd(reinterpret_cast<unsigned long long*>(p.st.counters + 4), (unsigned long long)traced);
 }
 
+__global__ __launch_bounds__(kDrainBlock) __attribute__((amdgpu_waves_per_eu(RGRT_DRAIN_WAVES))) void k_drain(ShadeParams p, TraceScene sc, int cur, int32_t* __restrict__ spillBase)
+{
+    __shared__ int32_t ldsStack[kDrainLdsStack * kDrainBlock];
+    const uint32_t gtid = blockIdx.x * kDrainBlock + threadIdx.x;
+    TraversalStack<kDrainLdsStack, kDrainBlock, kDrainSpill> stack{ldsStack, spillBase + (size_t)gtid * kDrainSpill,
+                                                                   (int)threadIdx.x};
+    drain_scene(p, sc, cur, stack);
+}
+
+// Drain of a whole batch: every wave drains the scenes in order, moving to
+// the next scene once the current one has no queued ray left for it and its
+// lanes are idle, so the waves that finish a scene early work on the next
+// one while the last chains of the earlier scenes are still traced.
+__global__ __launch_bounds__(kDrainBlock) __attribute__((amdgpu_waves_per_eu(RGRT_DRAIN_WAVES))) void k_drain_multi(
+    const ShadeParams* __restrict__ ps, const TraceScene* __restrict__ scs, int numScenes, int cur,
+    int32_t* __restrict__ spillBase)
+{
+    __shared__ int32_t ldsStack[kDrainLdsStack * kDrainBlock];
+    const uint32_t gtid = blockIdx.x * kDrainBlock + threadIdx.x;
+    TraversalStack<kDrainLdsStack, kDrainBlock, kDrainSpill> stack{ldsStack, spillBase + (size_t)gtid * kDrainSpill,
+                                                                   (int)threadIdx.x};
+    for(int s = 0; s < numScenes; ++s) {
+        const TraceScene sc = scs[s];
+        drain_scene(ps[s], sc, cur, stack);
+    }
+}
+
 } // namespace
 
 void launch_raygen(hipStream_t s, const ShadeParams& p)
 {
     const uint32_t blocks = (p.st.npix + kShadeBlock - 1) / kShadeBlock;
     k_raygen<<<blocks, kShadeBlock, 0, s>>>(p);
+}
+
+void launch_raygen_multi(hipStream_t s, const ShadeParams* ps, const uint32_t* start, int numScenes, uint32_t blocks)
+{
+    if(blocks) k_raygen_multi<<<blocks, kShadeBlock, 0, s>>>(ps, start, numScenes);
 }
 
 void launch_finalize_samples(hipStream_t s, const ShadeParams& p)
@@ -980,6 +1076,17 @@ uint32_t drain_block_size() { return kDrainBlock; }
 void launch_drain(hipStream_t s, const ShadeParams& p, const TraceScene& sc, int cur, uint32_t gridBlocks, int32_t* spill)
 {
     k_drain<<<gridBlocks, kDrainBlock, 0, s>>>(p, sc, cur, spill);
+}
+
+void launch_shade_multi(hipStream_t s, const ShadeParams* ps, int numScenes, int cur, uint32_t gridBlocks)
+{
+    k_shade_multi<<<gridBlocks, kShadeBlock, 0, s>>>(ps, numScenes, cur);
+}
+
+void launch_drain_multi(hipStream_t s, const ShadeParams* ps, const TraceScene* scs, int numScenes, int cur, uint32_t gridBlocks,
+                        int32_t* spill)
+{
+    k_drain_multi<<<gridBlocks, kDrainBlock, 0, s>>>(ps, scs, numScenes, cur, spill);
 }
 
 } // namespace rgrt

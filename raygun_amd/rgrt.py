@@ -28,7 +28,7 @@ FORMAT_BGRA8_SRGB = 3
 IMAGE_FINAL, IMAGE_BASE, IMAGE_NORMAL, IMAGE_ROUGH, IMAGE_ROUGH_TRANSITIONS, IMAGE_ROUGH_COLORS_A, IMAGE_ROUGH_COLORS_B, \
     IMAGE_BASE_F32, IMAGE_NORMAL_F32, IMAGE_ROUGH_F32, IMAGE_PRIMARY_HIT, IMAGE_FEATURES = range(12)
 SAMPLING_WHITTED, SAMPLING_STOCHASTIC = 0, 1
-BATCH_GRAPH_PER_SCENE, BATCH_ONE_GRAPH = 0, 1
+BATCH_GRAPH_PER_SCENE, BATCH_ONE_GRAPH, BATCH_PERSISTENT = 0, 1, 2
 
 STATUS = {0: "OK", 1: "INVALID_ARG", 2: "OUT_OF_MEMORY", 3: "HIP", 4: "NOT_READY", 5: "NO_DEVICE"}
 
@@ -492,8 +492,9 @@ class _FrameDesc(ctypes.Structure):
 
 class Batch:
     """Frames of several contexts submitted together (rgrt_batch_*): a
-    hipGraph per scene on its own stream (BATCH_GRAPH_PER_SCENE) or one
-    hipGraph with a branch per scene (BATCH_ONE_GRAPH)."""
+    hipGraph per scene on its own stream (BATCH_GRAPH_PER_SCENE), one
+    hipGraph with a branch per scene (BATCH_ONE_GRAPH), or one hipGraph
+    tracing all scenes in shared launches (BATCH_PERSISTENT)."""
 
     def __init__(self, contexts, mode=BATCH_GRAPH_PER_SCENE):
         self._L = lib()

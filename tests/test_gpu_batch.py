@@ -1,6 +1,7 @@
 """Batched scenes (rgrt_batch_frame: a hipGraph per scene on its own stream,
-or one hipGraph with a branch per scene) against the same scenes rendered one
-by one with the eager calls: identical
+one hipGraph with a branch per scene, or one hipGraph tracing every scene in
+shared launches) against the same scenes rendered one by one with the eager
+calls: identical
 final images per scene over animated frames (TLAS refits, a deforming mesh
 refitted), a refused frame naming its scene, and the autotuner changing only
 timing, never a frame."""
@@ -18,7 +19,7 @@ def _workload(mode=rgrt.BATCH_GRAPH_PER_SCENE):
     return batched.BatchWorkload(num_scenes=4, seed=5, lo=2_000, hi=30_000, resolutions=RES, mode=mode)
 
 
-@pytest.mark.parametrize("mode", [rgrt.BATCH_GRAPH_PER_SCENE, rgrt.BATCH_ONE_GRAPH])
+@pytest.mark.parametrize("mode", [rgrt.BATCH_GRAPH_PER_SCENE, rgrt.BATCH_ONE_GRAPH, rgrt.BATCH_PERSISTENT])
 def test_batched_frames_equal_sequential_frames(mode):
     print(f"[case] batch 4 scenes x 3 frames mode {mode}", flush=True)
     a, b = _workload(mode), _workload()
@@ -37,7 +38,7 @@ def test_batched_frames_equal_sequential_frames(mode):
         b.close()
 
 
-@pytest.mark.parametrize("mode", [rgrt.BATCH_GRAPH_PER_SCENE, rgrt.BATCH_ONE_GRAPH])
+@pytest.mark.parametrize("mode", [rgrt.BATCH_GRAPH_PER_SCENE, rgrt.BATCH_ONE_GRAPH, rgrt.BATCH_PERSISTENT])
 def test_batch_refuses_a_bad_scene_and_recovers(mode):
     print(f"[case] batch error path mode {mode}", flush=True)
     w = _workload(mode)

@@ -44,7 +44,7 @@ class _Twin:
             self.osc.refit_tlas(inst)
 
 
-@pytest.mark.parametrize("mode", [rgrt.BATCH_GRAPH_PER_SCENE, rgrt.BATCH_ONE_GRAPH])
+@pytest.mark.parametrize("mode", [rgrt.BATCH_GRAPH_PER_SCENE, rgrt.BATCH_ONE_GRAPH, rgrt.BATCH_PERSISTENT])
 def test_batched_frames_match_oracle(mode):
     flags = rgrt.CFG_FP32_RADIANCE | rgrt.CFG_PRIMARY_AOV
     w = batched.BatchWorkload(num_scenes=8, seed=21, lo=3_000, hi=80_000, resolutions=RES, mode=mode, flags=flags)
