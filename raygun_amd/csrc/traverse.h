@@ -74,26 +74,31 @@ __device__ __forceinline__ void cswap(float& ta, int32_t& ra, float& tb, int32_t
 
 // Entry distance of child c of a compressed node, +inf on a miss. A plane at
 // origin + q * 2^e lies at t = q * (2^e / d) + (origin - o) / d: one fused
-// multiply-add per plane (the box test only culls). The far distance is
-// widened by 1 + 2*gamma(3) (Ize 2013) plus twice the reciprocal's 1-ulp
-// error (near and far may come from different axes): 1 + 14 * 2^-24.
-__device__ __forceinline__ float qbox_enter(uint32_t wx, uint32_t hxw, uint32_t wy, uint32_t hyw, uint32_t wz, uint32_t hzw, int c,
+// multiply-add per plane (the box test only culls). The words nx..fz hold
+// the byte offsets of the near and far plane of each axis: for a direction
+// component >= 0 the low bound is nearer, otherwise the high bound, and
+// fma(q, a, b) is monotonic in q, so these are exactly the min and max of
+// the two planes' distances. The far distance is widened by 1 + 2*gamma(3)
+// (Ize 2013) plus twice the reciprocal's 1-ulp error (near and far may come
+// from different axes): 1 + 14 * 2^-24.
+__device__ __forceinline__ float qbox_enter(uint32_t nx, uint32_t fx, uint32_t ny, uint32_t fy, uint32_t nz, uint32_t fz, int c,
                                             float ax, float bx, float ay, float by, float az, float bz, float tmin, float tmax)
 {
     const int sh = 8 * c;
-    const float x0 = __builtin_fmaf((float)((wx >> sh) & 0xffu), ax, bx), x1 = __builtin_fmaf((float)((hxw >> sh) & 0xffu), ax, bx);
-    const float y0 = __builtin_fmaf((float)((wy >> sh) & 0xffu), ay, by), y1 = __builtin_fmaf((float)((hyw >> sh) & 0xffu), ay, by);
-    const float z0 = __builtin_fmaf((float)((wz >> sh) & 0xffu), az, bz), z1 = __builtin_fmaf((float)((hzw >> sh) & 0xffu), az, bz);
-    const float n = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), tmin));
-    const float f = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tmax));
+    const float xn = __builtin_fmaf((float)((nx >> sh) & 0xffu), ax, bx), xf = __builtin_fmaf((float)((fx >> sh) & 0xffu), ax, bx);
+    const float yn = __builtin_fmaf((float)((ny >> sh) & 0xffu), ay, by), yf = __builtin_fmaf((float)((fy >> sh) & 0xffu), ay, by);
+    const float zn = __builtin_fmaf((float)((nz >> sh) & 0xffu), az, bz), zf = __builtin_fmaf((float)((fz >> sh) & 0xffu), az, bz);
+    const float n = fmaxf(fmaxf(xn, yn), fmaxf(zn, tmin));
+    const float f = fminf(fminf(xf, yf), fminf(zf, tmax));
     return n <= f * 1.0000008f ? n : INFINITY;
 }
 
 // Visits a compressed wide node (64 bytes, four 16-byte loads): tests the
-// four child boxes, orders the hit children by entry distance with the
-// comparator network (0,1) (2,3) (0,2) (1,3) (1,2) (a swap only on a strictly
-// smaller distance), pushes all but the nearest farthest first and returns
-// the nearest (kEmptyChild if none was hit).
+// four child boxes, orders them by entry distance with the comparator
+// network (0,1) (2,3) (0,2) (1,3) (1,2) (a swap only on a strictly smaller
+// distance; missed and empty children sort last at +inf), pushes the hit
+// children but the nearest farthest first and returns the nearest
+// (kEmptyChild if none was hit).
 template<typename Stack>
 __device__ __forceinline__ int32_t visit_wide(const Node4q* nodes, int32_t ref, const BoxRay& br, float tmin, float tbest, Stack& stack)
 {
@@ -105,29 +110,30 @@ __device__ __forceinline__ int32_t visit_wide(const Node4q* nodes, int32_t ref, 
     const float bx = (__uint_as_float(v0.x) - br.ox) * br.ix;
     const float by = (__uint_as_float(v0.y) - br.oy) * br.iy;
     const float bz = (__uint_as_float(v0.z) - br.oz) * br.iz;
+    const bool sx = br.ix < 0.0f, sy = br.iy < 0.0f, sz = br.iz < 0.0f;
+    const uint32_t nx = sx ? v1.y : v1.x, fx = sx ? v1.x : v1.y;
+    const uint32_t ny = sy ? v1.w : v1.z, fy = sy ? v1.z : v1.w;
+    const uint32_t nz = sz ? v2.y : v2.x, fz = sz ? v2.x : v2.y;
     // empty slots hold zero offsets, so their (discarded) tests are harmless
     // and all four run without branches
-    const float e0 = qbox_enter(v1.x, v1.y, v1.z, v1.w, v2.x, v2.y, 0, ax, bx, ay, by, az, bz, tmin, tbest);
-    const float e1 = qbox_enter(v1.x, v1.y, v1.z, v1.w, v2.x, v2.y, 1, ax, bx, ay, by, az, bz, tmin, tbest);
-    const float e2 = qbox_enter(v1.x, v1.y, v1.z, v1.w, v2.x, v2.y, 2, ax, bx, ay, by, az, bz, tmin, tbest);
-    const float e3 = qbox_enter(v1.x, v1.y, v1.z, v1.w, v2.x, v2.y, 3, ax, bx, ay, by, az, bz, tmin, tbest);
+    const float e0 = qbox_enter(nx, fx, ny, fy, nz, fz, 0, ax, bx, ay, by, az, bz, tmin, tbest);
+    const float e1 = qbox_enter(nx, fx, ny, fy, nz, fz, 1, ax, bx, ay, by, az, bz, tmin, tbest);
+    const float e2 = qbox_enter(nx, fx, ny, fy, nz, fz, 2, ax, bx, ay, by, az, bz, tmin, tbest);
+    const float e3 = qbox_enter(nx, fx, ny, fy, nz, fz, 3, ax, bx, ay, by, az, bz, tmin, tbest);
     float t0 = ch.x != kEmptyChild ? e0 : INFINITY;
     float t1 = ch.y != kEmptyChild ? e1 : INFINITY;
     float t2 = ch.z != kEmptyChild ? e2 : INFINITY;
     float t3 = ch.w != kEmptyChild ? e3 : INFINITY;
-    int32_t r0 = t0 != INFINITY ? ch.x : kEmptyChild;
-    int32_t r1 = t1 != INFINITY ? ch.y : kEmptyChild;
-    int32_t r2 = t2 != INFINITY ? ch.z : kEmptyChild;
-    int32_t r3 = t3 != INFINITY ? ch.w : kEmptyChild;
+    int32_t r0 = ch.x, r1 = ch.y, r2 = ch.z, r3 = ch.w;
     cswap(t0, r0, t1, r1);
     cswap(t2, r2, t3, r3);
     cswap(t0, r0, t2, r2);
     cswap(t1, r1, t3, r3);
     cswap(t1, r1, t2, r2);
-    if(r3 != kEmptyChild) stack.push(r3);
-    if(r2 != kEmptyChild) stack.push(r2);
-    if(r1 != kEmptyChild) stack.push(r1);
-    return r0;
+    if(t3 != INFINITY) stack.push(r3);
+    if(t2 != INFINITY) stack.push(r2);
+    if(t1 != INFINITY) stack.push(r1);
+    return t0 != INFINITY ? r0 : kEmptyChild;
 }
 
 // Per-ray setup of the watertight test (Woop, Benthin, Wald 2013).
