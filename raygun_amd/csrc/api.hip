@@ -353,7 +353,6 @@ void free_images(rgrt_ctx* c)
     dev_free(s.lum2);
     dev_free(s.payload);
     dev_free(s.accum);
-    dev_free(s.meta);
     dev_free(s.frames);
     dev_free(s.rays[0]);
     dev_free(s.rays[1]);
@@ -400,7 +399,6 @@ void alloc_images(rgrt_ctx* c, uint32_t w, uint32_t h)
     c->slotCap = npix;
     dev_alloc(s.payload, npix * kPayloadWords);
     dev_alloc(s.accum, npix * kAccumWords);
-    dev_alloc(s.meta, npix);
     dev_alloc(s.lum2, npix);
     dev_alloc(s.frames, npix * kFrameStride * kMaxFrames);
     dev_alloc(s.rays[0], npix * kRayWords);
@@ -508,7 +506,7 @@ ShadeParams shade_params(const rgrt_ctx* c)
     return p;
 }
 
-// Wavefront slots at most (≈ 1.26 KB of state each: payload, frames, queues).
+// Wavefront slots at most (≈ 0.8 KB of state each: payload, frames, queues).
 constexpr uint64_t kMaxSlots = 1ull << 25;
 
 // Sample groups for the frame constants in c->ubo: with several samples per
@@ -530,7 +528,6 @@ void prepare_sample_groups(rgrt_ctx* c)
     if(slots > c->slotCap) {
         check(hipStreamSynchronize(c->stream)); // queued frames still use the old arrays
         dev_alloc(s.payload, slots * kPayloadWords);
-        dev_alloc(s.meta, slots);
         dev_alloc(s.frames, slots * kFrameStride * kMaxFrames);
         dev_alloc(s.rays[0], slots * kRayWords);
         dev_alloc(s.rays[1], slots * kRayWords);
@@ -921,6 +918,7 @@ rgrt_status rgrt_upload_materials(rgrt_ctx* c, const rgrt_material* materials, u
 {
     return guarded(c, [&]() {
         if(n && !materials) return fail(c, RGRT_ERR_INVALID_ARG, "upload_materials: null");
+        if(n > kMaxMaterials) return fail(c, RGRT_ERR_INVALID_ARG, "upload_materials: more than 2^28 materials");
         scene_quiesce(c);
         dev_alloc(c->sd->mats, n);
         if(n) check(hipMemcpy(c->sd->mats, materials, n * sizeof(rgrt_material), hipMemcpyHostToDevice));

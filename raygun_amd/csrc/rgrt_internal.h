@@ -86,10 +86,17 @@ bool launch_tlas_fused(hipStream_t s, const rgrt_instance* in, uint32_t n, const
 // ---------------------------------------------------------------------------
 // Wavefront state (device memory, structure of arrays, one slot per pixel)
 // ---------------------------------------------------------------------------
-constexpr int kPayloadWords = 16;
+// A pixel's state between passes: the payload fields a later pass reads
+// (hit value, depth, IOR, reference depth, recursion depth) and the frame
+// stack depth, ray type and sample number, 8 words.
+constexpr int kPayloadWords = 8;
 constexpr int kAccumWords = 12;
-constexpr int kFrameWords = 34;
-constexpr int kFrameStride = 36; // words per stored frame: 34 + padding to 16-byte vectors
+// A suspended closest-hit frame: stage, material index, hit distance,
+// origin, direction, normal, the reference depth at the hit, shadow and
+// reflection colours and reflection depth (the material's values and the
+// grid effect are recomputed from these on resume), 19 words + 1 of padding.
+constexpr int kFrameStride = 20;
+constexpr uint32_t kMaxMaterials = 1u << 28; // the material index shares a frame word with the stage
 constexpr int kMaxFrames = 8; // maxRecursions <= 7 keeps at most 7 frames waiting
 constexpr int kRayWords = 10; // origin, tmin, dir, tmax, pixel, flags
 constexpr int kHitWords = 5;
@@ -114,7 +121,6 @@ struct WavefrontState {
     // 16-byte loads, and consecutive lanes read consecutive records).
     float* payload = nullptr; // [npix][kPayloadWords]
     float* accum = nullptr;   // [npix][kAccumWords]
-    uint32_t* meta = nullptr; // [npix]: sp | sample << 8
     float* frames = nullptr;  // [kMaxFrames][npix][kFrameStride]
     float* rays[2] = {nullptr, nullptr}; // [kRayWords][npix]
     float* hits = nullptr;    // [kHitWords][npix]

@@ -21,17 +21,6 @@ constexpr int kSpill = 368; // with the 16 LDS entries: the 381-entry worst case
 #ifndef RGRT_TRACE_WAVES
 #define RGRT_TRACE_WAVES 6
 #endif
-// Iterations between the wave's instance entries / triangle tests (the
-// lanes of a wave start their rays together in both kernels here).
-#ifndef RGRT_INST_EVERY
-#define RGRT_INST_EVERY 4
-#endif
-#ifndef RGRT_TRI_EVERY
-#define RGRT_TRI_EVERY 1
-#endif
-#ifndef RGRT_SENT_EVERY
-#define RGRT_SENT_EVERY 1
-#endif
 
 __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(RGRT_TRACE_WAVES))) void k_trace_queue(TraceScene sc, const float* __restrict__ rays,
                                                               float* __restrict__ hits, uint32_t npix,

@@ -413,6 +413,20 @@ __device__ __forceinline__ bool trav_step(Traversal& T, const TraceScene& sc, St
     return true;
 }
 
+// Iterations between a wave's instance entries / triangle tests / returns
+// to the TLAS (trav_step): the schedule of the traversal kernels, whose
+// lanes start their rays together (the queue pass, the ray queries and
+// each round of the drain).
+#ifndef RGRT_INST_EVERY
+#define RGRT_INST_EVERY 4
+#endif
+#ifndef RGRT_TRI_EVERY
+#define RGRT_TRI_EVERY 1
+#endif
+#ifndef RGRT_SENT_EVERY
+#define RGRT_SENT_EVERY 1
+#endif
+
 // Whole traversal of one ray. Each node and leaf is visited at most once
 // per level, so a well-formed two-level tree ends far below the step cap;
 // the cap only bounds the loop.

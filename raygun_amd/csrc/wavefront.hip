@@ -72,8 +72,14 @@ struct Payload {
     int rayType, recDepth;
 };
 
+// A closest-hit invocation in registers. Stored while suspended are only
+// stage, frontFacing, mat, t, origin, dir, vn, gridRef, shadowColor,
+// reflectColor and reflectDepth; the material's fields, the grid effect
+// and baseColor are recomputed from those when it resumes (load_frame).
 struct Frame {
     int stage, frontFacing;
+    uint32_t mat;  // material index (clamped to the table)
+    float gridRef; // the payload's refDepth at the hit (grid effect input)
     F3 origin, dir, vn;
     float t;
     F3 diffuse;
@@ -85,7 +91,6 @@ struct Frame {
     F3 baseColor, shadowColor, reflectColor;
     float reflectDepth;
 };
-static_assert(sizeof(Frame) == kFrameWords * 4, "frame words");
 
 struct OutRay {
     F3 o;
@@ -106,44 +111,51 @@ struct Ctx {
 // ---------------------------------------------------------------------------
 // State in HBM
 // ---------------------------------------------------------------------------
-// The payload record is the Payload struct's 16 words in field order.
-static_assert(sizeof(Payload) == kPayloadWords * 4, "payload words");
-
-__device__ __forceinline__ void load_payload(const ShadeParams& p, uint32_t pix, Payload& P)
+// Between passes a pixel keeps only the payload fields that a later pass
+// reads: hitValue (the input of a shadow ray's hit), depth (read after a
+// nested trace that may not set it), curIOR, refDepth, recDepth and
+// rayType. normal, rv and reflectContribution are written by the outermost
+// closest-hit (or the primary ray's miss) in the same pass as the sample's
+// accumulation, after any earlier write, so they start at their initial
+// zeros when a pass loads the state. Record: hitValue, depth | curIOR,
+// refDepth, recDepth, sp | rayType << 8 | sample << 10.
+__device__ __forceinline__ void load_state(const ShadeParams& p, uint32_t pix, Payload& P, int& sp, int& sample)
 {
-    const float4* b = reinterpret_cast<const float4*>(p.st.payload) + (size_t)pix * 4;
-    float4 v[4];
-#pragma unroll
-    for(int k = 0; k < 4; ++k) v[k] = b[k];
-    __builtin_memcpy(&P, v, sizeof(Payload));
+    const float4* b = reinterpret_cast<const float4*>(p.st.payload) + (size_t)pix * 2;
+    const float4 a = b[0], c = b[1];
+    P.hitValue = f3(a.x, a.y, a.z);
+    P.depth = a.w;
+    P.curIOR = c.x;
+    P.refDepth = c.y;
+    P.recDepth = __float_as_int(c.z);
+    const uint32_t m = __float_as_uint(c.w);
+    sp = (int)(m & 0xffu);
+    P.rayType = (int)((m >> 8) & 3u);
+    sample = (int)(m >> 10);
+    P.normal = f3(0.0f);
+    P.rv[0] = P.rv[1] = P.rv[2] = P.rv[3] = 0.0f;
+    P.reflectContribution = 0.0f;
 }
 
-__device__ __forceinline__ void store_payload(const ShadeParams& p, uint32_t pix, const Payload& P)
+__device__ __forceinline__ void store_state(const ShadeParams& p, uint32_t pix, const Payload& P, int sp, int sample)
 {
-    float4* b = reinterpret_cast<float4*>(p.st.payload) + (size_t)pix * 4;
-    float4 v[4];
-    __builtin_memcpy(v, &P, sizeof(Payload));
-#pragma unroll
-    for(int k = 0; k < 4; ++k) b[k] = v[k];
+    float4* b = reinterpret_cast<float4*>(p.st.payload) + (size_t)pix * 2;
+    const uint32_t m = (uint32_t)sp | ((uint32_t)P.rayType << 8) | ((uint32_t)sample << 10);
+    b[0] = make_float4(P.hitValue.x, P.hitValue.y, P.hitValue.z, P.depth);
+    b[1] = make_float4(P.curIOR, P.refDepth, __int_as_float(P.recDepth), __uint_as_float(m));
 }
 
+// Frame record (kFrameStride words): stage | frontFacing << 3 | mat << 4,
+// t, origin, dir, vn, gridRef, shadowColor, reflectDepth, reflectColor, 0.
 __device__ __forceinline__ void store_frame(const ShadeParams& p, uint32_t pix, int level, const Frame& F)
 {
     float4* b = reinterpret_cast<float4*>(p.st.frames + ((size_t)level * p.st.npix + pix) * kFrameStride);
-    float4 v[kFrameStride / 4];
-    v[kFrameStride / 4 - 1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    __builtin_memcpy(v, &F, sizeof(Frame));
-#pragma unroll
-    for(int k = 0; k < kFrameStride / 4; ++k) b[k] = v[k];
-}
-
-__device__ __forceinline__ void load_frame(const ShadeParams& p, uint32_t pix, int level, Frame& F)
-{
-    const float4* b = reinterpret_cast<const float4*>(p.st.frames + ((size_t)level * p.st.npix + pix) * kFrameStride);
-    float4 v[kFrameStride / 4];
-#pragma unroll
-    for(int k = 0; k < kFrameStride / 4; ++k) v[k] = b[k];
-    __builtin_memcpy(&F, v, sizeof(Frame));
+    const uint32_t w0 = ((uint32_t)F.stage & 7u) | (((uint32_t)F.frontFacing & 1u) << 3) | (F.mat << 4);
+    b[0] = make_float4(__uint_as_float(w0), F.t, F.origin.x, F.origin.y);
+    b[1] = make_float4(F.origin.z, F.dir.x, F.dir.y, F.dir.z);
+    b[2] = make_float4(F.vn.x, F.vn.y, F.vn.z, F.gridRef);
+    b[3] = make_float4(F.shadowColor.x, F.shadowColor.y, F.shadowColor.z, F.reflectDepth);
+    b[4] = make_float4(F.reflectColor.x, F.reflectColor.y, F.reflectColor.z, 0.0f);
 }
 
 __device__ __forceinline__ void init_payload(Payload& P) // raygen.h:92-100
@@ -341,6 +353,48 @@ __device__ void grid_effect(float refDepth, float hitT, Frame& F, F3 pos) // clo
     }
 }
 
+// The material's fields of a frame (closesthit.rchit:103-110), then the grid
+// effect (:113) for materials with effectId 1.
+__device__ __forceinline__ void frame_material(const ShadeParams& p, Frame& F)
+{
+    const rgrt_material& m = p.mats[F.mat];
+    F.diffuse = f3(m.diffuse[0], m.diffuse[1], m.diffuse[2]);
+    F.transparency = m.transparency;
+    F.specular = f3(m.specular[0], m.specular[1], m.specular[2]);
+    F.reflectivity = m.reflectivity;
+    F.roughness = m.roughness;
+    F.ior = m.ior;
+    F.rayConsumption = (int)m.rayConsumption;
+    F.emission = m.emission;
+    if(m.effectId == 1u) grid_effect(F.gridRef, F.t, F, F.origin);
+}
+
+// A suspended frame back in registers: the stored fields, then what the
+// closest-hit stage had derived from them (material, grid effect and
+// baseColor = max(dot(-L, vn), 0.2) * diffuse, closesthit.rchit:174-175),
+// with the same operations on the same values.
+__device__ __forceinline__ void load_frame(const Ctx& c, uint32_t pix, int level, Frame& F)
+{
+    const ShadeParams& p = c.p;
+    const float4* b = reinterpret_cast<const float4*>(p.st.frames + ((size_t)level * p.st.npix + pix) * kFrameStride);
+    const float4 a0 = b[0], a1 = b[1], a2 = b[2], a3 = b[3], a4 = b[4];
+    const uint32_t w0 = __float_as_uint(a0.x);
+    F.stage = (int)(w0 & 7u);
+    F.frontFacing = (int)((w0 >> 3) & 1u);
+    F.mat = min(w0 >> 4, p.numMaterials ? p.numMaterials - 1 : 0u); // as stored; the clamp only guards the table
+    F.t = a0.y;
+    F.origin = f3(a0.z, a0.w, a1.x);
+    F.dir = f3(a1.y, a1.z, a1.w);
+    F.vn = f3(a2.x, a2.y, a2.z);
+    F.gridRef = a2.w;
+    F.shadowColor = f3(a3.x, a3.y, a3.z);
+    F.reflectDepth = a3.w;
+    F.reflectColor = f3(a4.x, a4.y, a4.z);
+    frame_material(p, F);
+    const float dp = fmaxf(dot(-c.L, F.vn), 0.2f);
+    F.baseColor = dp * F.diffuse;
+}
+
 // closesthit.rchit:93-204 up to the first nested trace.
 // Material of a hit (closesthit.rchit:95-110: offset table of the instance's
 // customIndex, vertex 0's matIndex), with the gathers clamped to the buffers.
@@ -378,16 +432,8 @@ __device__ int closest_hit(const Ctx& c, Payload& P, uint32_t instPos, uint32_t 
     const rgrt_vertex& v1 = p.vb[min((uint64_t)table.vertexBufferOffset + i1, lastVertex)];
     const rgrt_vertex& v2 = p.vb[min((uint64_t)table.vertexBufferOffset + i2, lastVertex)];
     const uint32_t mi = table.materialBufferOffset + v0.matIndex;
-    const rgrt_material& m = p.mats[p.numMaterials ? min(mi, p.numMaterials - 1) : 0u];
-
-    F.diffuse = f3(m.diffuse[0], m.diffuse[1], m.diffuse[2]);
-    F.transparency = m.transparency;
-    F.specular = f3(m.specular[0], m.specular[1], m.specular[2]);
-    F.reflectivity = m.reflectivity;
-    F.roughness = m.roughness;
-    F.ior = m.ior;
-    F.rayConsumption = (int)m.rayConsumption;
-    F.emission = m.emission;
+    F.mat = p.numMaterials ? min(mi, p.numMaterials - 1) : 0u;
+    F.frontFacing = 0; // set below for a generic hit
 
     const float tmin = 0.01f, tmax = 1000.0f;
     const F3 origin = O + D * hitT;
@@ -398,11 +444,11 @@ __device__ int closest_hit(const Ctx& c, Payload& P, uint32_t instPos, uint32_t 
                      (w[8] * vn.x + w[9] * vn.y) + w[10] * vn.z);
     F3 vnW = normalize(wn);
 
-    if(m.effectId == 1u) grid_effect(P.refDepth, hitT, F, origin);
-
     F.origin = origin;
     F.dir = D;
     F.t = hitT;
+    F.gridRef = P.refDepth;
+    frame_material(p, F); // the material's fields and the grid effect
 
     if(P.rayType == RT_SHADOW_INTERNAL) { // :125-152
         const float thicknessModulation = clampf((hitT * (1.0f - F.transparency)) * 10.0f, 0.0f, 1.0f);
@@ -669,13 +715,12 @@ __device__ __forceinline__ void raygen_slot(const ShadeParams& p, uint32_t pix)
     primary_ray(p.ubo, px, py, p.width, p.height, (int)g, p.sampling, p.seed, r);
     Payload P;
     init_payload(P);
-    store_payload(p, pix, P);
+    store_state(p, pix, P, 0, (int)g);
     if(g == 0u) {
         float4* a = reinterpret_cast<float4*>(p.st.accum) + (size_t)i * 3;
         for(int k = 0; k < 3; ++k) a[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         if(p.sampling == RGRT_SAMPLING_STOCHASTIC) p.st.lum2[i] = 0.0f;
     }
-    p.st.meta[pix] = g << 8;
     // only sample 0's primary ray carries the primary-hit flag (AOV, features)
     write_ray(p.st.rays[0], p.st.npix, pix, r, pix, (g == 0u ? kFlagPrimary : 0u) | (0xffu << kMaskShift));
 }
@@ -821,7 +866,7 @@ __device__ bool advance_pixel(const Ctx& c0, uint32_t pix, uint32_t flags, F3 O,
             return false;
         }
         sp--;
-        if(sp < kMaxFrames) load_frame(p, pix, sp, F);
+        if(sp < kMaxFrames) load_frame(c, pix, sp, F);
         step = resume(c, F, P, out);
     }
 }
@@ -859,15 +904,10 @@ __device__ __forceinline__ void shade_wave(const ShadeParams& p, const Ctx& c, i
             const F3 D = f3(rays[4 * n + i], rays[5 * n + i], rays[6 * n + i]);
             const HitRec h = load_hit(p.st.hits, n, i);
             Payload P;
-            load_payload(p, pix, P);
-            const uint32_t meta = p.st.meta[pix];
-            int sp = (int)(meta & 0xffu);
-            int sample = (int)(meta >> 8);
+            int sp, sample;
+            load_state(p, pix, P, sp, sample);
             emit = advance_pixel(c, pix, flags, O, D, h, P, sp, sample, out);
-            if(emit) {
-                store_payload(p, pix, P);
-                p.st.meta[pix] = (uint32_t)sp | ((uint32_t)sample << 8);
-            }
+            if(emit) store_state(p, pix, P, sp, sample);
         }
         // append the emitted rays to the next queue, one atomic per wave
         const uint64_t m = __ballot(emit);
@@ -985,10 +1025,7 @@ __device__ __forceinline__ void drain_scene(const ShadeParams& p, const TraceSce
                         r.dy = rays[5 * n + i];
                         r.dz = rays[6 * n + i];
                         r.tmax = rays[7 * n + i];
-                        load_payload(p, pix, P);
-                        const uint32_t meta = p.st.meta[pix];
-                        sp = (int)(meta & 0xffu);
-                        sample = (int)(meta >> 8);
+                        load_state(p, pix, P, sp, sample);
                         active = true;
                     }
                 }
@@ -996,7 +1033,9 @@ __device__ __forceinline__ void drain_scene(const ShadeParams& p, const TraceSce
         }
         if(!__ballot(active)) break;
         if(active) {
-            const HitRec h = traverse(sc, r, (flags >> kMaskShift) & 0xffu, stack);
+            // the active lanes start this traversal together, so the
+            // wave-wide step schedule applies
+            const HitRec h = traverse<RGRT_INST_EVERY, RGRT_TRI_EVERY, RGRT_SENT_EVERY>(sc, r, (flags >> kMaskShift) & 0xffu, stack);
             ++traced;
             OutRay out;
             if(advance_pixel(c, pix, flags, f3(r.ox, r.oy, r.oz), f3(r.dx, r.dy, r.dz), h, P, sp, sample, out)) {
