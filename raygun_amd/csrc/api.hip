@@ -954,7 +954,7 @@ rgrt_status rgrt_build_blas(rgrt_ctx* c, const rgrt_blas_desc* d, rgrt_blas* out
         dev_alloc(b.wide, std::max<uint32_t>(n, 2) - 1);
         dev_alloc(b.wideq, std::max<uint32_t>(n, 2) - 1);
         dev_alloc(b.numWide, 1);
-        dev_alloc(b.tris, n);
+        dev_alloc(b.tris, 3ull * n); // three axis orders, see Tri
         dev_alloc(b.leafPrim, n);
         reserve_builder(c, std::max<uint32_t>(n, 1));
         launch_tri_boxes(s, c->sd->vb, c->sd->ib, d->firstVertex, (uint32_t)io, n, c->builder.boxLo, c->builder.boxHi);

@@ -993,20 +993,24 @@ __global__ void k_tri_gather(const rgrt_vertex* __restrict__ vb, const uint32_t*
     const rgrt_vertex& a = vb[firstVertex + ix[0]];
     const rgrt_vertex& b = vb[firstVertex + ix[1]];
     const rgrt_vertex& c = vb[firstVertex + ix[2]];
-    Tri t;
-    t.v0x = a.position[0];
-    t.v0y = a.position[1];
-    t.v0z = a.position[2];
-    t.prim = prim;
-    t.v1x = b.position[0];
-    t.v1y = b.position[1];
-    t.v1z = b.position[2];
-    t.pad0 = 0;
-    t.v2x = c.position[0];
-    t.v2y = c.position[1];
-    t.v2z = c.position[2];
-    t.pad1 = 0;
-    tris[k] = t;
+#pragma unroll
+    for(int r = 0; r < 3; ++r) {
+        const int x = r == 2 ? 0 : r + 1, y = r == 0 ? 2 : r - 1; // (r + 1, r + 2) mod 3
+        Tri t;
+        t.v0x = a.position[x];
+        t.v0y = a.position[y];
+        t.v0z = a.position[r];
+        t.prim = prim;
+        t.v1x = b.position[x];
+        t.v1y = b.position[y];
+        t.v1z = b.position[r];
+        t.pad0 = 0;
+        t.v2x = c.position[x];
+        t.v2y = c.position[y];
+        t.v2z = c.position[r];
+        t.pad1 = 0;
+        tris[(size_t)r * n + k] = t;
+    }
 }
 
 __global__ void k_inst_gather(const float* __restrict__ w2o, const rgrt_instance* __restrict__ in,

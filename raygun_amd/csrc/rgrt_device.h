@@ -73,7 +73,11 @@ struct __align__(16) Node4q {
 static_assert(sizeof(Node4q) == 64, "compressed wide node is 64 bytes");
 
 // Leaf triangle in Morton order: the three vertices and the original
-// primitive index (gl_PrimitiveID), 48 bytes.
+// primitive index (gl_PrimitiveID), 48 bytes. A BLAS stores its triangles
+// three times, [3][numFaces]: copy r holds every vertex's coordinates in the
+// order (r + 1, r + 2, r) mod 3, the axis order of the watertight test for
+// a ray whose largest direction component is r, so the test reads its
+// permuted coordinates directly (no per-lane selects).
 struct __align__(16) Tri {
     float v0x, v0y, v0z;
     uint32_t prim;

@@ -136,11 +136,16 @@ __device__ __forceinline__ int32_t visit_wide(const Node4q* nodes, int32_t ref, 
     return t0 != INFINITY ? r0 : kEmptyChild;
 }
 
-// Per-ray setup of the watertight test (Woop, Benthin, Wald 2013).
+// Per-ray setup of the watertight test (Woop, Benthin, Wald 2013). kz is
+// the axis of the largest direction component, kx = kz + 1 and ky = kz + 2
+// (mod 3); the paper swaps kx and ky when d[kz] < 0 to keep the triangles'
+// winding, which only negates U, V, W, det and T exactly (the roundings are
+// symmetric), so t, u, v and the accept decision are the same without it.
+// The triangle copy `rot` = kz holds the vertices in (kx, ky, kz) order.
 struct WtRay {
-    int kx, ky, kz;
+    int rot;
     float Sx, Sy, Sz;
-    float okx, oky, okz; // origin permuted
+    float okx, oky, okz; // origin in (kx, ky, kz) order
 };
 
 __device__ __forceinline__ float sel3(float x, float y, float z, int k) { return k == 0 ? x : (k == 1 ? y : z); }
@@ -154,17 +159,10 @@ __device__ __forceinline__ void wt_setup(WtRay& w, float ox, float oy, float oz,
         m = fabsf(dy);
     }
     if(fabsf(dz) > m) kz = 2;
-    int kx = kz + 1 == 3 ? 0 : kz + 1;
-    int ky = kx + 1 == 3 ? 0 : kx + 1;
+    const int kx = kz + 1 == 3 ? 0 : kz + 1;
+    const int ky = kx + 1 == 3 ? 0 : kx + 1;
+    w.rot = kz;
     const float dkz = sel3(dx, dy, dz, kz);
-    if(dkz < 0.0f) {
-        const int t = kx;
-        kx = ky;
-        ky = t;
-    }
-    w.kx = kx;
-    w.ky = ky;
-    w.kz = kz;
     const float dkx = sel3(dx, dy, dz, kx), dky = sel3(dx, dy, dz, ky);
     w.Sz = 1.0f / dkz; // one division; the shears reuse it
     w.Sx = dkx * w.Sz;
@@ -174,20 +172,21 @@ __device__ __forceinline__ void wt_setup(WtRay& w, float ox, float oy, float oz,
     w.okz = sel3(ox, oy, oz, kz);
 }
 
-// Watertight ray/triangle test; on a hit with tmin <= t < tbest returns
-// true with t and the weights u (vertex 1), v (vertex 2).
+// Watertight ray/triangle test on a triangle copy in the ray's axis order;
+// on a hit with tmin <= t < tbest returns true with t and the weights u
+// (vertex 1), v (vertex 2).
 __device__ __forceinline__ bool wt_intersect(const WtRay& w, const Tri& tri, float tmin, float tbest, float& t, float& u,
                                              float& v)
 {
-    const float Akx = sel3(tri.v0x, tri.v0y, tri.v0z, w.kx) - w.okx;
-    const float Aky = sel3(tri.v0x, tri.v0y, tri.v0z, w.ky) - w.oky;
-    const float Akz = sel3(tri.v0x, tri.v0y, tri.v0z, w.kz) - w.okz;
-    const float Bkx = sel3(tri.v1x, tri.v1y, tri.v1z, w.kx) - w.okx;
-    const float Bky = sel3(tri.v1x, tri.v1y, tri.v1z, w.ky) - w.oky;
-    const float Bkz = sel3(tri.v1x, tri.v1y, tri.v1z, w.kz) - w.okz;
-    const float Ckx = sel3(tri.v2x, tri.v2y, tri.v2z, w.kx) - w.okx;
-    const float Cky = sel3(tri.v2x, tri.v2y, tri.v2z, w.ky) - w.oky;
-    const float Ckz = sel3(tri.v2x, tri.v2y, tri.v2z, w.kz) - w.okz;
+    const float Akx = tri.v0x - w.okx;
+    const float Aky = tri.v0y - w.oky;
+    const float Akz = tri.v0z - w.okz;
+    const float Bkx = tri.v1x - w.okx;
+    const float Bky = tri.v1y - w.oky;
+    const float Bkz = tri.v1z - w.okz;
+    const float Ckx = tri.v2x - w.okx;
+    const float Cky = tri.v2y - w.oky;
+    const float Ckz = tri.v2z - w.okz;
     const float Ax = Akx - w.Sx * Akz;
     const float Ay = Aky - w.Sy * Akz;
     const float Bx = Bkx - w.Sx * Bkz;
@@ -368,7 +367,8 @@ __device__ __forceinline__ bool trav_step(Traversal& T, const TraceScene& sc, St
             wt_setup(T.wt, rox, roy, roz, rdx, rdy, rdz);
             stack.push(kSentinel);
             T.nodes = reinterpret_cast<const Node4q*>((uint64_t)b0.x | ((uint64_t)b0.y << 32));
-            T.tris = reinterpret_cast<const Tri*>((uint64_t)b0.z | ((uint64_t)b0.w << 32));
+            // the copy of the triangles in this ray's axis order
+            T.tris = reinterpret_cast<const Tri*>((uint64_t)b0.z | ((uint64_t)b0.w << 32)) + (size_t)T.wt.rot * numFaces;
             T.inBlas = true;
             T.curInst = i3.z;
             T.numFaces = numFaces;
