@@ -39,6 +39,9 @@ CONFIG5_CANDIDATES = {"wavefrontPasses": [24, 32, 48, 64], "traceBlocksPerCU": [
 # launch shapes the autotuner tries for the headline pipeline (coordinate descent)
 TUNE_CANDIDATES = {"wavefrontPasses": [6, 8, 10, 12, 14], "traceBlocksPerCU": [4, 6, 8],
                    "drainBlocksPerCU": [2, 4, 8], "shadeBlocksPerCU": [4, 8, 12]}
+# One frame at a time the recursion tail is not hidden by the next frame, so
+# fewer wavefront passes and a larger drain grid pay off.
+ONE_FRAME_CANDIDATES = {"wavefrontPasses": [4, 5, 6, 8, 10], "drainBlocksPerCU": [2, 3, 4]}
 
 
 def log(msg):
@@ -225,6 +228,21 @@ def main():
     value = rays / (ms_per_step * 1e-3) / 1e6
     log(f"[bench] {args.steps} frames: {ms_per_step:.3f} ms/frame, {rays} rays/frame, {value:.1f} Mrays/s")
 
+    # one frame at a time (each frame's graph starts when the previous frame
+    # has finished, as the reference waits for its fence), with a launch
+    # shape picked for that submission; the headline context gets its own
+    # shape back afterwards
+    one = None
+    if not args.no_autotune:
+        t_main = dict(ctx.tuning())
+        one_fn = [graph_on(ctx)]
+        b1, b1_ms, _ = autotune.tune(ctx, None, candidates=ONE_FRAME_CANDIDATES, contexts=[ctx],
+                                     timer=lambda: frames_ms([ctx], one_fn, 12, 3, ev))
+        one_ms = frames_ms([ctx], one_fn, args.steps, args.warmup, ev)
+        one = {"ms_per_frame": round(one_ms, 4), "mrays_per_s": round(rays / (one_ms * 1e-3) / 1e6, 1), "tuning": b1}
+        ctx.set_tuning(**t_main)
+        log(f"[bench] one frame at a time: {one}")
+
     # per-stage and per-kernel timing of two more frames, alone on the GPU,
     # on a context with the timing events switched on (sharing the scene)
     tctx = ctx.shared(flags=rgrt.CFG_TIMING)
@@ -339,7 +357,7 @@ def main():
                                                              "traceKernelMs", "shadeKernelMs", "drainKernelMs")},
                    "tuning": ctx.tuning(), "wavefront_pass_rays": passes.tolist(),
                    "submission": chosen, "frames_in_flight": args.in_flight, "frame_ms_by_submission": frame_ms,
-                   "autotune": tuned},
+                   "autotune": tuned, "one_frame_at_a_time": one},
         "roofline": roof,
         "cpu_baseline": cpu,
         "learned_vs_plain": {"value": None, "unit": "x"},
