@@ -61,6 +61,11 @@ int32_t orc_trace_rays(orc_scene* s, const float* rays, uint32_t n, uint32_t cul
 /* Same queries by brute force in FP64 over every triangle of every instance. */
 int32_t orc_trace_rays_fp64(orc_scene* s, const float* rays, uint32_t n, uint32_t cullMask, double* t_out,
                             uint32_t* inst_out, uint32_t* prim_out, double* uv_out, int32_t threads);
+/* Same queries by brute force in FP32: the instance mapping, ray setup and
+ * watertight test of orc_trace_rays over every triangle (no tree), so the
+ * closest t must equal orc_trace_rays' bit for bit. */
+int32_t orc_trace_rays_brute32(orc_scene* s, const float* rays, uint32_t n, uint32_t cullMask, float* t_out,
+                               uint32_t* inst_out, uint32_t* prim_out, int32_t threads);
 
 /* Whole traceRaysKHR restatement: raygen + closest-hit/miss recursion.
  * ubo: 192-byte block. Outputs (any may be null):

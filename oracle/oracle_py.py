@@ -67,6 +67,8 @@ def lib():
         L.orc_trace_rays.argtypes = [P, P, u32, u32, P, P, P, P, i32]
         L.orc_trace_rays_fp64.restype = i32
         L.orc_trace_rays_fp64.argtypes = [P, P, u32, u32, P, P, P, P, i32]
+        L.orc_trace_rays_brute32.restype = i32
+        L.orc_trace_rays_brute32.argtypes = [P, P, u32, u32, P, P, P, i32]
         L.orc_render.restype = i32
         L.orc_render.argtypes = [P, P, u32, u32, P, P, P, P, P, P, P, P, P, i32, i32, i32]
         L.orc_render_counts.restype = i32
@@ -196,6 +198,18 @@ class OracleScene:
         uv = np.zeros((n, 2))
         lib().orc_trace_rays_fp64(self.h, _p(rays), n, cull_mask, _p(t), _p(inst), _p(prim), _p(uv), threads or default_threads())
         return t, inst, prim, uv
+
+    def trace_rays_brute32(self, rays, cull_mask=0xFF, threads=None):
+        """Closest hits by FP32 brute force (no tree): t, instance
+        customIndex (NO_HIT on a miss), primitive."""
+        from raygun_amd.records import RAY
+        rays = np.ascontiguousarray(rays, RAY)
+        n = len(rays)
+        t = np.zeros(n, np.float32)
+        inst = np.zeros(n, np.uint32)
+        prim = np.zeros(n, np.uint32)
+        lib().orc_trace_rays_brute32(self.h, _p(rays), n, cull_mask, _p(t), _p(inst), _p(prim), threads or default_threads())
+        return t, inst, prim
 
     def render(self, ubo, width, height, threads=None, rows=None):
         """Returns dict of images (fp16 as uint16 arrays HxWx4, fp32 HxWx4),

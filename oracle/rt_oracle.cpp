@@ -837,22 +837,6 @@ void ray_pre(RayPre& r, const float o[3], const float d[3])
     r.Sy = d[ky] * r.Sz;
 }
 
-// Slab test in the fused form t = fma(bound, 1/d, -o/d); the far distance
-// is widened by 1 + 2*gamma(3) (Ize 2013) so the box test never rejects what
-// the watertight triangle test would accept.
-inline bool ray_box(const RayPre& r, const float lo[3], const float hi[3], float tmin, float tmax, float& tnear)
-{
-    float t0[3], t1[3];
-    for(int k = 0; k < 3; ++k) {
-        t0[k] = std::fma(lo[k], r.inv[k], r.no[k]);
-        t1[k] = std::fma(hi[k], r.inv[k], r.no[k]);
-    }
-    float tn = std::fmax(std::fmax(std::fmin(t0[0], t1[0]), std::fmin(t0[1], t1[1])), std::fmax(std::fmin(t0[2], t1[2]), tmin));
-    float tf = std::fmin(std::fmin(std::fmax(t0[0], t1[0]), std::fmax(t0[1], t1[1])), std::fmin(std::fmax(t0[2], t1[2]), tmax));
-    tnear = tn;
-    return tn <= tf * 1.0000004f;
-}
-
 // Watertight ray/triangle (Woop, Benthin, Wald 2013). Accepts t in
 // [tmin, tlimit]. u, v are the weights of vertex 1 and vertex 2.
 inline bool ray_tri(const RayPre& r, const Tri& tri, float tmin, float tlimit, float& t, float& u, float& v)
@@ -932,12 +916,18 @@ bool trace_closest(const orc_scene& s, const float o[3], const float d[3], float
     auto widen = [](float b) { return b * (1.0f + 8.0f * TIE_REL); };
 
     // Child boxes of a compressed node: plane distance t = q * (2^e / d) +
-    // (origin - o) / d, one fused multiply-add per plane.
+    // (origin - o) / d, one fused multiply-add per plane. b = (origin - o) / d
+    // carries two roundings, an absolute error up to |b| * 2^-23 that does not
+    // shrink with t (a ray starting far from the node's corner), so besides
+    // the relative widening each axis's near distance is lowered and its far
+    // distance raised by |b| * 2^-21 (per axis: an axis with a tiny direction
+    // component has a huge b, and its planes only bind when that far away).
     auto visit = [&](const std::vector<Node4q>& nodes, const RayPre& r, int32_t ref, int32_t* stack, int& sp) {
         const Node4q& q = nodes[ref];
         h.nodes++;
         const float ax = pow2f(q.ex) * r.inv[0], ay = pow2f(q.ey) * r.inv[1], az = pow2f(q.ez) * r.inv[2];
         const float bx = (q.ox - r.o[0]) * r.inv[0], by = (q.oy - r.o[1]) * r.inv[1], bz = (q.oz - r.o[2]) * r.inv[2];
+        const float mx = std::fabs(bx) * 0x1p-21f, my = std::fabs(by) * 0x1p-21f, mz = std::fabs(bz) * 0x1p-21f;
         const float lim = widen(best);
         float tn[4];
         int32_t rf[4];
@@ -945,8 +935,8 @@ bool trace_closest(const orc_scene& s, const float o[3], const float d[3], float
             const float x0 = std::fma((float)q.qlox[c], ax, bx), x1 = std::fma((float)q.qhix[c], ax, bx);
             const float y0 = std::fma((float)q.qloy[c], ay, by), y1 = std::fma((float)q.qhiy[c], ay, by);
             const float z0 = std::fma((float)q.qloz[c], az, bz), z1 = std::fma((float)q.qhiz[c], az, bz);
-            const float n = std::fmax(std::fmax(std::fmin(x0, x1), std::fmin(y0, y1)), std::fmax(std::fmin(z0, z1), tmin));
-            const float f = std::fmin(std::fmin(std::fmax(x0, x1), std::fmax(y0, y1)), std::fmin(std::fmax(z0, z1), lim));
+            const float n = std::fmax(std::fmax(std::fmin(x0, x1) - mx, std::fmin(y0, y1) - my), std::fmax(std::fmin(z0, z1) - mz, tmin));
+            const float f = std::fmin(std::fmin(std::fmax(x0, x1) + mx, std::fmax(y0, y1) + my), std::fmin(std::fmax(z0, z1) + mz, lim));
             const bool hit = q.child[c] != EMPTY_CHILD && n <= f * 1.0000004f;
             tn[c] = hit ? n : std::numeric_limits<float>::infinity();
             rf[c] = hit ? q.child[c] : EMPTY_CHILD;
@@ -1022,6 +1012,39 @@ bool trace_closest(const orc_scene& s, const float o[3], const float d[3], float
         for(const Cand& c: cands) {
             if(c.inst == h.inst && c.prim == h.prim) continue;
             if(std::fabs(c.t - h.t) <= TIE_REL * h.t) h.tie = true;
+        }
+    }
+    return found;
+}
+
+// FP32 brute force: the same instance mapping, ray setup and watertight test
+// as trace_closest, over every triangle of every instance whose mask passes
+// (no tree). A box test that culls a child holding the closest hit makes
+// trace_closest differ from this.
+bool brute_fp32(const orc_scene& s, const float o[3], const float d[3], float tmin, float tmax, uint32_t cullMask, float& bt,
+                uint32_t& binst, uint32_t& bprim)
+{
+    bool found = false;
+    bt = tmax;
+    for(size_t ii = 0; ii < s.inst.size(); ++ii) {
+        const InstanceX& ix = s.inst[ii];
+        if((ix.rec.mask & cullMask & 0xffu) == 0) continue;
+        const Blas& b = s.blas[ix.rec.blas];
+        const float* w = ix.w2o;
+        float oo[3], od[3];
+        for(int row = 0; row < 3; ++row) {
+            oo[row] = ((w[row * 4 + 0] * o[0] + w[row * 4 + 1] * o[1]) + w[row * 4 + 2] * o[2]) + w[row * 4 + 3];
+            od[row] = (w[row * 4 + 0] * d[0] + w[row * 4 + 1] * d[1]) + w[row * 4 + 2] * d[2];
+        }
+        RayPre r;
+        ray_pre(r, oo, od);
+        for(uint32_t p = 0; p < b.numFaces; ++p) {
+            float t, u, v;
+            if(!ray_tri(r, b.tris[p], tmin, bt, t, u, v) || !(t < bt)) continue;
+            bt = t;
+            binst = s.inst[ii].rec.customIndex;
+            bprim = p;
+            found = true;
         }
     }
     return found;
@@ -1978,6 +2001,21 @@ int32_t orc_trace_rays_fp64(orc_scene* s, const float* rays, uint32_t n, uint32_
             uv_out[2 * i] = u;
             uv_out[2 * i + 1] = v;
         }
+    });
+    return 0;
+}
+
+int32_t orc_trace_rays_brute32(orc_scene* s, const float* rays, uint32_t n, uint32_t cullMask, float* t_out, uint32_t* inst_out,
+                               uint32_t* prim_out, int32_t threads)
+{
+    parallel_rows(0, (int)n, threads, [&](int i, int) {
+        const float* r = rays + 8 * (size_t)i;
+        float t;
+        uint32_t inst = NO_HIT, prim = 0;
+        const bool hit = brute_fp32(*s, r, r + 4, r[3], r[7], cullMask, t, inst, prim);
+        t_out[i] = hit ? t : 0.0f;
+        inst_out[i] = hit ? inst : NO_HIT;
+        prim_out[i] = prim;
     });
     return 0;
 }

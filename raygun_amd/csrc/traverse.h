@@ -77,17 +77,23 @@ __device__ __forceinline__ void cswap(float& ta, int32_t& ra, float& tb, int32_t
 // multiply-add per plane (the box test only culls). The words nx..fz hold
 // the byte offsets of the near and far plane of each axis: for a direction
 // component >= 0 the low bound is nearer, otherwise the high bound, and
-// fma(q, a, b) is monotonic in q, so these are exactly the min and max of
-// the two planes' distances. The far distance is widened by 1 + 2*gamma(3)
+// fma(q, a, b) is monotonic in q, so these are the min and max of the two
+// planes' distances. Error bounds: b = (origin - o) / d carries two
+// roundings, an absolute error up to |b| * 2^-23 that does not shrink with t
+// (a ray starting far from the node's corner), so near planes use
+// bn = b - |b| * 2^-21 and far planes bf = b + |b| * 2^-21 (per axis: an
+// axis with a tiny direction component has a huge b, and its planes only
+// bind when they are as far away); the relative part is 1 + 2*gamma(3)
 // (Ize 2013) plus twice the reciprocal's 1-ulp error (near and far may come
 // from different axes): 1 + 14 * 2^-24.
 __device__ __forceinline__ float qbox_enter(uint32_t nx, uint32_t fx, uint32_t ny, uint32_t fy, uint32_t nz, uint32_t fz, int c,
-                                            float ax, float bx, float ay, float by, float az, float bz, float tmin, float tmax)
+                                            float ax, float bxn, float bxf, float ay, float byn, float byf, float az, float bzn,
+                                            float bzf, float tmin, float tmax)
 {
     const int sh = 8 * c;
-    const float xn = __builtin_fmaf((float)((nx >> sh) & 0xffu), ax, bx), xf = __builtin_fmaf((float)((fx >> sh) & 0xffu), ax, bx);
-    const float yn = __builtin_fmaf((float)((ny >> sh) & 0xffu), ay, by), yf = __builtin_fmaf((float)((fy >> sh) & 0xffu), ay, by);
-    const float zn = __builtin_fmaf((float)((nz >> sh) & 0xffu), az, bz), zf = __builtin_fmaf((float)((fz >> sh) & 0xffu), az, bz);
+    const float xn = __builtin_fmaf((float)((nx >> sh) & 0xffu), ax, bxn), xf = __builtin_fmaf((float)((fx >> sh) & 0xffu), ax, bxf);
+    const float yn = __builtin_fmaf((float)((ny >> sh) & 0xffu), ay, byn), yf = __builtin_fmaf((float)((fy >> sh) & 0xffu), ay, byf);
+    const float zn = __builtin_fmaf((float)((nz >> sh) & 0xffu), az, bzn), zf = __builtin_fmaf((float)((fz >> sh) & 0xffu), az, bzf);
     const float n = fmaxf(fmaxf(xn, yn), fmaxf(zn, tmin));
     const float f = fminf(fminf(xf, yf), fminf(zf, tmax));
     return n <= f * 1.0000008f ? n : INFINITY;
@@ -110,16 +116,19 @@ __device__ __forceinline__ int32_t visit_wide(const Node4q* nodes, int32_t ref, 
     const float bx = (__uint_as_float(v0.x) - br.ox) * br.ix;
     const float by = (__uint_as_float(v0.y) - br.oy) * br.iy;
     const float bz = (__uint_as_float(v0.z) - br.oz) * br.iz;
+    const float bxn = __builtin_fmaf(-fabsf(bx), 0x1p-21f, bx), bxf = __builtin_fmaf(fabsf(bx), 0x1p-21f, bx);
+    const float byn = __builtin_fmaf(-fabsf(by), 0x1p-21f, by), byf = __builtin_fmaf(fabsf(by), 0x1p-21f, by);
+    const float bzn = __builtin_fmaf(-fabsf(bz), 0x1p-21f, bz), bzf = __builtin_fmaf(fabsf(bz), 0x1p-21f, bz);
     const bool sx = br.ix < 0.0f, sy = br.iy < 0.0f, sz = br.iz < 0.0f;
     const uint32_t nx = sx ? v1.y : v1.x, fx = sx ? v1.x : v1.y;
     const uint32_t ny = sy ? v1.w : v1.z, fy = sy ? v1.z : v1.w;
     const uint32_t nz = sz ? v2.y : v2.x, fz = sz ? v2.x : v2.y;
     // empty slots hold zero offsets, so their (discarded) tests are harmless
     // and all four run without branches
-    const float e0 = qbox_enter(nx, fx, ny, fy, nz, fz, 0, ax, bx, ay, by, az, bz, tmin, tbest);
-    const float e1 = qbox_enter(nx, fx, ny, fy, nz, fz, 1, ax, bx, ay, by, az, bz, tmin, tbest);
-    const float e2 = qbox_enter(nx, fx, ny, fy, nz, fz, 2, ax, bx, ay, by, az, bz, tmin, tbest);
-    const float e3 = qbox_enter(nx, fx, ny, fy, nz, fz, 3, ax, bx, ay, by, az, bz, tmin, tbest);
+    const float e0 = qbox_enter(nx, fx, ny, fy, nz, fz, 0, ax, bxn, bxf, ay, byn, byf, az, bzn, bzf, tmin, tbest);
+    const float e1 = qbox_enter(nx, fx, ny, fy, nz, fz, 1, ax, bxn, bxf, ay, byn, byf, az, bzn, bzf, tmin, tbest);
+    const float e2 = qbox_enter(nx, fx, ny, fy, nz, fz, 2, ax, bxn, bxf, ay, byn, byf, az, bzn, bzf, tmin, tbest);
+    const float e3 = qbox_enter(nx, fx, ny, fy, nz, fz, 3, ax, bxn, bxf, ay, byn, byf, az, bzn, bzf, tmin, tbest);
     float t0 = ch.x != kEmptyChild ? e0 : INFINITY;
     float t1 = ch.y != kEmptyChild ? e1 : INFINITY;
     float t2 = ch.z != kEmptyChild ? e2 : INFINITY;

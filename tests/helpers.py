@@ -70,6 +70,48 @@ def random_rays(n, seed, center=(0, 0, 0), radius=3.0, tmin=0.001, tmax=1e4):
     return r
 
 
+def seam_scene(n=64, rays=50_000, seed=1, x0=3000.0, z0=100.0):
+    """A ground grid of n x n unit quads (2 triangles each) at z = z0 with its
+    corner at (x0, x0), and seeded steep rays that hit it on the lines where
+    the LBVH splits the grid (x or y = x0 + n/2, n/4, 3n/4, n/8), from 0.05 to
+    3 units above. The child boxes of those splits meet there, the hit is
+    close to the ray origin, and the origin is far from the nodes' corners
+    relative to 1/|d|: the box test's plane distances then carry absolute
+    rounding errors (of (corner - o)/d) far above its relative widening.
+    Returns vertices, indices, one instance record and the rays."""
+    from raygun_amd.records import INSTANCE
+    xs = x0 + np.arange(n + 1, dtype=np.float64)
+    gx, gy = np.meshgrid(xs, xs, indexing="ij")
+    v = np.zeros((n + 1) * (n + 1), VERTEX)
+    v["position"][:, 0] = gx.ravel()
+    v["position"][:, 1] = gy.ravel()
+    v["position"][:, 2] = z0
+    v["normal"][:, 2] = 1.0
+    a = (np.arange(n)[:, None] * (n + 1) + np.arange(n)[None, :]).ravel()
+    quads = np.stack([a, a + n + 1, a + 1, a + 1, a + n + 1, a + n + 2], 1)
+    idx = quads.astype(np.uint32).ravel()
+    inst = np.zeros(1, INSTANCE)
+    inst["transform"][0] = [1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0]
+    inst["mask"] = 0xFF
+    inst["flags"] = 1
+    rng = np.random.default_rng(seed)
+    seams = x0 + np.array([n // 2, n // 4, 3 * n // 4, n // 8], np.float64)
+    p = np.zeros((rays, 3))
+    p[:, 0] = rng.choice(seams, rays)
+    p[:, 1] = x0 + rng.uniform(0.5, n - 0.5, rays)
+    p[:, 2] = z0
+    swap = rng.random(rays) < 0.5
+    p[swap, 0], p[swap, 1] = p[swap, 1].copy(), p[swap, 0].copy()
+    d = np.stack([rng.normal(0, 0.02, rays), rng.normal(0, 0.02, rays), -np.ones(rays)], 1)
+    h = rng.uniform(0.05, 3.0, rays)
+    r = np.zeros(rays, RAY)
+    r["origin"] = (p - d * h[:, None]).astype(np.float32)
+    r["dir"] = d.astype(np.float32)
+    r["tmin"] = 0.0
+    r["tmax"] = 1e4
+    return v, idx, inst, r
+
+
 def small_config2(width=320, height=180, tris_per_mesh=20_000, seed=2):
     return procgen.config2(width, height, seed=seed, tris_per_mesh=tris_per_mesh)
 

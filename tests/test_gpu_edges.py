@@ -128,3 +128,24 @@ def test_mirrored_instance_transform():
         assert len(compare_hits(hg, ho, tie)) == 0
     finally:
         ctx.close()
+
+
+def test_box_test_keeps_hits_where_child_boxes_meet_far_from_the_node_corner():
+    """rgrt_trace_rays finds exactly what FP32 brute force over every
+    triangle finds (the oracle's watertight test, no tree) on rays that hit a
+    ground grid at x, y ~ 3000 on the seams between child boxes, close to
+    their origins, where the box test's plane distances carry an absolute
+    rounding error that only the absolute part of its far margin covers."""
+    from helpers import seam_scene
+    v, idx, inst, rays = seam_scene()
+    inst["blas"] = 0
+    mats = np.array([default_material()])
+    ctx, osc = _both(v, idx, mats, [(0, 0, len(idx) // 3)], list(inst))
+    try:
+        h = ctx.trace_rays(rays)
+        t_b, inst_b, _ = osc.trace_rays_brute32(rays)
+        assert (inst_b != NO_HIT).all()
+        assert np.array_equal(h["instance"] != NO_HIT, inst_b != NO_HIT)
+        assert np.array_equal(h["t"].view(np.uint32), t_b.view(np.uint32))
+    finally:
+        ctx.close()
