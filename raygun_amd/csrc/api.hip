@@ -1575,13 +1575,15 @@ rgrt_status batch_frame_persistent(rgrt_batch* b, const rgrt_frame_desc* descs)
                 joined = i + 1;
             }
             launch_raygen_multi(s0, b->dParams[slot], b->dStarts[slot], (int)n, starts[n]);
+            bool oneSample = true;
+            for(uint32_t i = 0; i < n; ++i) oneSample = oneSample && run_one_sample(ps[i]);
             int cur = 0;
             for(uint32_t it = 0; it < passes; ++it) {
                 launch_trace_multi(s0, b->dParams[slot], b->dScenes[slot], (int)n, cur, c0->traceGrid, c0->spill, b->dCounters);
-                launch_shade_multi(s0, b->dParams[slot], (int)n, cur, c0->shadeGrid);
+                launch_shade_multi(s0, b->dParams[slot], (int)n, cur, c0->shadeGrid, oneSample);
                 cur ^= 1;
             }
-            launch_drain_multi(s0, b->dParams[slot], b->dScenes[slot], (int)n, cur, c0->drainGrid, c0->spill);
+            launch_drain_multi(s0, b->dParams[slot], b->dScenes[slot], (int)n, cur, c0->drainGrid, c0->spill, oneSample);
             if(sampleSums || showAlpha) {
                 check(hipEventRecord(b->fork2, s0));
                 for(uint32_t i = 0; i < n; ++i) {

@@ -253,7 +253,8 @@ void launch_raygen(hipStream_t s, const ShadeParams& p);
 // are device arrays of numScenes entries (1..kMaxBatchScenes).
 void launch_trace_multi(hipStream_t s, const ShadeParams* ps, const TraceScene* scs, int numScenes, int cur, uint32_t gridBlocks,
                         int32_t* spill, uint32_t* batchCounters);
-void launch_shade_multi(hipStream_t s, const ShadeParams* ps, int numScenes, int cur, uint32_t gridBlocks);
+// oneSample: every scene runs one sample per wavefront slot (run_one_sample)
+void launch_shade_multi(hipStream_t s, const ShadeParams* ps, int numScenes, int cur, uint32_t gridBlocks, bool oneSample);
 void launch_zero_u32(hipStream_t s, uint32_t* p, uint32_t n);
 // rough + composite and FXAA of every scene of a batch, one launch each
 // (start: block starts, see post.hip); tile counts per scene for the starts
@@ -265,7 +266,11 @@ uint32_t fxaa_tiles(uint32_t w, uint32_t h);
 // block starts of 256 slots each)
 void launch_raygen_multi(hipStream_t s, const ShadeParams* ps, const uint32_t* start, int numScenes, uint32_t blocks);
 void launch_drain_multi(hipStream_t s, const ShadeParams* ps, const TraceScene* scs, int numScenes, int cur, uint32_t gridBlocks,
-                        int32_t* spill);
+                        int32_t* spill, bool oneSample);
+// True when every wavefront slot of the frame runs exactly one sample of its
+// pixel (sample groups >= samples): the shading kernels then leave out the
+// start of a next sample.
+bool run_one_sample(const ShadeParams& p);
 void launch_trace_queue(hipStream_t s, const TraceScene& sc, const WavefrontState& st, int cur, uint32_t gridBlocks);
 void launch_shade(hipStream_t s, const ShadeParams& p, int cur, uint32_t gridBlocks);
 void launch_finalize_samples(hipStream_t s, const ShadeParams& p);

@@ -43,6 +43,12 @@ constexpr int kShadeBlock = 256;
 #ifndef RGRT_DRAIN_WAVES
 #define RGRT_DRAIN_WAVES 1
 #endif
+// The drain variant for one sample per slot (no next-sample start) fits in
+// 143 VGPRs (3 waves); compiled for 4 waves it spills 12 VGPRs and runs
+// faster (config 2 at 1080p: 1.58-1.60 against 1.69-1.82 ms).
+#ifndef RGRT_DRAIN1_WAVES
+#define RGRT_DRAIN1_WAVES 4
+#endif
 
 enum Stage : int { G_SHADOW = 1, G_REFL = 2, G_REFR = 3, SI_TRACE = 4, SI_SKY = 5, ST_INTERNAL = 6 };
 enum Step : int { RET = 0, EMIT_FRAME = 1, EMIT = 2, DONE = 3 };
@@ -777,7 +783,11 @@ __global__ __launch_bounds__(kShadeBlock) void k_finalize_samples(ShadeParams p)
 // One closest-hit or miss stage for a traced ray of pixel `pix`, then the
 // continuations it returns into. Returns true with `out` set when the pixel
 // issues its next ray; false once its last sample is accumulated and the
-// outputs are written.
+// outputs are written. kNextSample = false compiles out the start of a
+// slot's next sample (its camera ray needs the frame's two matrices, ~22
+// VGPRs at the kernels' peak): valid when every slot runs one sample, i.e.
+// sample groups >= samples (the launchers pick; see run_one_sample).
+template<bool kNextSample>
 __device__ bool advance_pixel(const Ctx& c0, uint32_t pix, uint32_t flags, F3 O, F3 D, const HitRec& h, Payload& P, int& sp,
                               int& sample, OutRay& out)
 {
@@ -853,7 +863,7 @@ __device__ bool advance_pixel(const Ctx& c0, uint32_t pix, uint32_t flags, F3 O,
             else
                 store_sample(p, tile, sample, P);
             sample += (int)p.st.groups;
-            if(sample < p.ubo.numSamples) {
+            if(kNextSample && sample < p.ubo.numSamples) {
                 init_payload(P);
                 uint32_t px, py;
                 slot_pixel(tile, p.width, p.height, px, py);
@@ -885,6 +895,7 @@ __device__ __forceinline__ HitRec load_hit(const float* hits, size_t n, uint32_t
 // Shades queue entries [base, base + 64) of queue `cur` (one wave, one lane
 // per entry), appending the emitted rays to the other queue with one atomic
 // per wave.
+template<bool kNextSample>
 __device__ __forceinline__ void shade_wave(const ShadeParams& p, const Ctx& c, int cur, uint32_t count, uint32_t base, int lane,
                                            uint64_t ltMask)
 {
@@ -906,7 +917,7 @@ __device__ __forceinline__ void shade_wave(const ShadeParams& p, const Ctx& c, i
             Payload P;
             int sp, sample;
             load_state(p, pix, P, sp, sample);
-            emit = advance_pixel(c, pix, flags, O, D, h, P, sp, sample, out);
+            emit = advance_pixel<kNextSample>(c, pix, flags, O, D, h, P, sp, sample, out);
             if(emit) store_state(p, pix, P, sp, sample);
         }
         // append the emitted rays to the next queue, one atomic per wave
@@ -921,6 +932,7 @@ __device__ __forceinline__ void shade_wave(const ShadeParams& p, const Ctx& c, i
 }
 
 // Wavefront shading pass over the rays traced by the preceding traversal pass.
+template<bool kNextSample>
 __global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(RGRT_SHADE_WAVES))) void k_shade(ShadeParams p, int cur)
 {
     const uint32_t count = p.st.counters[cur];
@@ -929,7 +941,7 @@ __global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(RGR
     const Ctx c{p, f3(p.ubo.lightDir[0], p.ubo.lightDir[1], p.ubo.lightDir[2]), p.ubo.maxRecursions, 0u};
     const uint32_t stride = gridDim.x * blockDim.x;
     for(uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < count; base += stride)
-        shade_wave(p, c, cur, count, base, lane, ltMask);
+        shade_wave<kNextSample>(p, c, cur, count, base, lane, ltMask);
 }
 
 // Scene of a batch that chunk k (64 queue entries) belongs to, from the
@@ -959,6 +971,7 @@ __device__ __forceinline__ void batch_chunk_prefix(const ShadeParams* ps, int nu
 // Shading pass of a whole batch: the queues `cur` of every scene in one
 // launch, 64-entry chunks handed to waves in order (a chunk belongs to one
 // scene, so a wave shades one scene at a time).
+template<bool kNextSample>
 __global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(RGRT_SHADE_WAVES))) void k_shade_multi(
     const ShadeParams* __restrict__ ps, int numScenes, int cur)
 {
@@ -973,7 +986,7 @@ __global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(RGR
         const ShadeParams& p = ps[s];
         const uint32_t first = s ? chunkEnd[s - 1] : 0u;
         const Ctx c{p, f3(p.ubo.lightDir[0], p.ubo.lightDir[1], p.ubo.lightDir[2]), p.ubo.maxRecursions, 0u};
-        shade_wave(p, c, cur, p.st.counters[cur], (k - first) * 64u, lane, ltMask);
+        shade_wave<kNextSample>(p, c, cur, p.st.counters[cur], (k - first) * 64u, lane, ltMask);
     }
 }
 
@@ -987,7 +1000,7 @@ constexpr int kDrainSpill = 368; // see traverse.h: 16 + 368 >= 381
 // the queue (one atomic per wave, a ballot to hand out the entries). The
 // recursion tail thus costs the longest remaining chain of one pixel rather
 // than one launch per recursion step.
-template<typename Stack>
+template<bool kNextSample, typename Stack>
 __device__ __forceinline__ void drain_scene(const ShadeParams& p, const TraceScene& sc, int cur, Stack& stack)
 {
     const uint32_t count = p.st.counters[cur];
@@ -1038,7 +1051,7 @@ __device__ __forceinline__ void drain_scene(const ShadeParams& p, const TraceSce
             const HitRec h = traverse<RGRT_INST_EVERY, RGRT_TRI_EVERY, RGRT_SENT_EVERY>(sc, r, (flags >> kMaskShift) & 0xffu, stack);
             ++traced;
             OutRay out;
-            if(advance_pixel(c, pix, flags, f3(r.ox, r.oy, r.oz), f3(r.dx, r.dy, r.dz), h, P, sp, sample, out)) {
+            if(advance_pixel<kNextSample>(c, pix, flags, f3(r.ox, r.oy, r.oz), f3(r.dx, r.dy, r.dz), h, P, sp, sample, out)) {
                 r.ox = out.o.x;
                 r.oy = out.o.y;
                 r.oz = out.o.z;
@@ -1059,20 +1072,22 @@ __device__ __forceinline__ void drain_scene(const ShadeParams& p, const TraceSce
     if(lane == 0 && traced) atomicAdd(reinterpret_cast<unsigned long long*>(p.st.counters + 4), (unsigned long long)traced);
 }
 
-__global__ __launch_bounds__(kDrainBlock) __attribute__((amdgpu_waves_per_eu(RGRT_DRAIN_WAVES))) void k_drain(ShadeParams p, TraceScene sc, int cur, int32_t* __restrict__ spillBase)
+template<bool kNextSample>
+__global__ __launch_bounds__(kDrainBlock) __attribute__((amdgpu_waves_per_eu(kNextSample ? RGRT_DRAIN_WAVES : RGRT_DRAIN1_WAVES))) void k_drain(ShadeParams p, TraceScene sc, int cur, int32_t* __restrict__ spillBase)
 {
     __shared__ int32_t ldsStack[kDrainLdsStack * kDrainBlock];
     const uint32_t gtid = blockIdx.x * kDrainBlock + threadIdx.x;
     TraversalStack<kDrainLdsStack, kDrainBlock, kDrainSpill> stack{ldsStack, spillBase + (size_t)gtid * kDrainSpill,
                                                                    (int)threadIdx.x};
-    drain_scene(p, sc, cur, stack);
+    drain_scene<kNextSample>(p, sc, cur, stack);
 }
 
 // Drain of a whole batch: every wave drains the scenes in order, moving to
 // the next scene once the current one has no queued ray left for it and its
 // lanes are idle, so the waves that finish a scene early work on the next
 // one while the last chains of the earlier scenes are still traced.
-__global__ __launch_bounds__(kDrainBlock) __attribute__((amdgpu_waves_per_eu(RGRT_DRAIN_WAVES))) void k_drain_multi(
+template<bool kNextSample>
+__global__ __launch_bounds__(kDrainBlock) __attribute__((amdgpu_waves_per_eu(kNextSample ? RGRT_DRAIN_WAVES : RGRT_DRAIN1_WAVES))) void k_drain_multi(
     const ShadeParams* __restrict__ ps, const TraceScene* __restrict__ scs, int numScenes, int cur,
     int32_t* __restrict__ spillBase)
 {
@@ -1082,7 +1097,7 @@ __global__ __launch_bounds__(kDrainBlock) __attribute__((amdgpu_waves_per_eu(RGR
                                                                    (int)threadIdx.x};
     for(int s = 0; s < numScenes; ++s) {
         const TraceScene sc = scs[s];
-        drain_scene(ps[s], sc, cur, stack);
+        drain_scene<kNextSample>(ps[s], sc, cur, stack);
     }
 }
 
@@ -1104,9 +1119,19 @@ void launch_finalize_samples(hipStream_t s, const ShadeParams& p)
     if(p.st.groups > 1u) k_finalize_samples<<<(p.st.nimg + kShadeBlock - 1) / kShadeBlock, kShadeBlock, 0, s>>>(p);
 }
 
+// True when every wavefront slot runs exactly one sample of its pixel
+// (sample groups >= samples), so no slot starts a next sample.
+bool run_one_sample(const ShadeParams& p)
+{
+    return p.st.groups >= (uint32_t)std::max(1, p.ubo.numSamples);
+}
+
 void launch_shade(hipStream_t s, const ShadeParams& p, int cur, uint32_t gridBlocks)
 {
-    k_shade<<<gridBlocks, kShadeBlock, 0, s>>>(p, cur);
+    if(run_one_sample(p))
+        k_shade<false><<<gridBlocks, kShadeBlock, 0, s>>>(p, cur);
+    else
+        k_shade<true><<<gridBlocks, kShadeBlock, 0, s>>>(p, cur);
 }
 
 uint32_t drain_spill_words_per_thread() { return kDrainSpill; }
@@ -1114,18 +1139,27 @@ uint32_t drain_block_size() { return kDrainBlock; }
 
 void launch_drain(hipStream_t s, const ShadeParams& p, const TraceScene& sc, int cur, uint32_t gridBlocks, int32_t* spill)
 {
-    k_drain<<<gridBlocks, kDrainBlock, 0, s>>>(p, sc, cur, spill);
+    if(run_one_sample(p))
+        k_drain<false><<<gridBlocks, kDrainBlock, 0, s>>>(p, sc, cur, spill);
+    else
+        k_drain<true><<<gridBlocks, kDrainBlock, 0, s>>>(p, sc, cur, spill);
 }
 
-void launch_shade_multi(hipStream_t s, const ShadeParams* ps, int numScenes, int cur, uint32_t gridBlocks)
+void launch_shade_multi(hipStream_t s, const ShadeParams* ps, int numScenes, int cur, uint32_t gridBlocks, bool oneSample)
 {
-    k_shade_multi<<<gridBlocks, kShadeBlock, 0, s>>>(ps, numScenes, cur);
+    if(oneSample)
+        k_shade_multi<false><<<gridBlocks, kShadeBlock, 0, s>>>(ps, numScenes, cur);
+    else
+        k_shade_multi<true><<<gridBlocks, kShadeBlock, 0, s>>>(ps, numScenes, cur);
 }
 
 void launch_drain_multi(hipStream_t s, const ShadeParams* ps, const TraceScene* scs, int numScenes, int cur, uint32_t gridBlocks,
-                        int32_t* spill)
+                        int32_t* spill, bool oneSample)
 {
-    k_drain_multi<<<gridBlocks, kDrainBlock, 0, s>>>(ps, scs, numScenes, cur, spill);
+    if(oneSample)
+        k_drain_multi<false><<<gridBlocks, kDrainBlock, 0, s>>>(ps, scs, numScenes, cur, spill);
+    else
+        k_drain_multi<true><<<gridBlocks, kDrainBlock, 0, s>>>(ps, scs, numScenes, cur, spill);
 }
 
 } // namespace rgrt
