@@ -103,24 +103,28 @@ def trace_mean(ctx, ubo, spp, frames, seed):
 
 
 def build_dataset(ctx, den, scenes, count, first_seed=1000, spp_in=4, spp_ref=256, ref_spp_per_frame=32, frame=None):
-    """count training pairs: slot k = the 4-spp frame of scene first_seed + k
-    (packed features) and the mean of spp_ref samples (fp32 target). With
-    frame = (W, H) each slot is a seeded window of a W x H frame of its scene
-    (the pixel footprint of that frame), else the whole scene at the
-    context's size."""
+    """count training pairs: slot k = the spp_in-sample frame of scene
+    first_seed + k (packed features) and the mean of spp_ref samples (fp32
+    target); with a sequence of sample counts, slot k takes
+    spp_in[k % len(spp_in)] (the features carry each pixel's variance, so one
+    network learns the noise levels together). With frame = (W, H) each slot
+    is a seeded window of a W x H frame of its scene (the pixel footprint of
+    that frame), else the whole scene at the context's size."""
     den.dataset_reserve(count, scenes.height, scenes.width)
     frames = max(1, spp_ref // ref_spp_per_frame)
+    counts = tuple(spp_in) if isinstance(spp_in, (tuple, list)) else (spp_in,)
     for k in range(count):
         seed = first_seed + k
+        spp = counts[k % len(counts)]
         if frame is None:
-            ubo = scenes.scene(seed, spp_in)
+            ubo = scenes.scene(seed, spp)
         else:
             r = np.random.default_rng(seed + 7_000_000)
             tile = (int(r.integers(0, frame[0] - scenes.width + 1)), int(r.integers(0, frame[1] - scenes.height + 1)))
-            ubo = scenes.scene(seed, spp_in, frame=frame, tile=tile)
+            ubo = scenes.scene(seed, spp, frame=frame, tile=tile)
         for f in trace_mean(ctx, ubo, ref_spp_per_frame, frames, seed * 1000 + 1):
             den.dataset_add_target(k, 1.0 / frames, f > 0)
-        for _ in trace_mean(ctx, ubo, spp_in, 1, seed * 1000):
+        for _ in trace_mean(ctx, ubo, spp, 1, seed * 1000):
             den.dataset_add_input(k)
     ctx.synchronize()
 
@@ -233,14 +237,16 @@ def _equal_psnr_ms(points, target):
 
 
 def learned_vs_plain(ctx, den, scenes, seeds, spp_learned=4, spp_ref=256, sweep=(4, 8, 16, 32, 64, 128),
-                     fallback=(8.0, 2e-4), events=None):
+                     fallback=(8.0, 2e-4), events=None, spp_more=()):
     """For each seeded scene at the context's size: the plain tracer's
     spp_ref frame (the reference), plain frames of each `sweep` sample count
     and the learned frame (spp_learned samples + the denoiser, fallback
     tiles traced), PSNR against the reference, and GPU milliseconds per frame
     (hipEvents on the context's stream: the traces, the denoiser and the
     post pass). The learned frame's cost is compared with the plain frame's
-    at the same PSNR (interpolated over the sweep)."""
+    at the same PSNR (interpolated over the sweep). Each count in spp_more
+    adds the same figures for a learned frame of that many samples
+    (`learned_more`)."""
     from .hiptime import HipEvents
     ev = events or HipEvents()
     stream = rgrt.lib().rgrt_stream(ctx.h)
@@ -259,8 +265,8 @@ def learned_vs_plain(ctx, den, scenes, seeds, spp_learned=4, spp_ref=256, sweep=
             pass
         ctx.postprocess(False)
 
-    def learned_run(ubo, seed):
-        for _ in trace_mean(ctx, ubo, spp_learned, 1, seed):
+    def learned_run(ubo, seed, spp=spp_learned):
+        for _ in trace_mean(ctx, ubo, spp, 1, seed):
             pass
         den.denoise(*fallback)
         ctx.postprocess(False)
@@ -285,6 +291,18 @@ def learned_vs_plain(ctx, den, scenes, seeds, spp_learned=4, spp_ref=256, sweep=
         den_ms = min(gpu_ms(lambda: den.denoise(1e9, 1e9)) for _ in range(3))
         p_l = psnr(img_l, ref)
         eq = _equal_psnr_ms(points, p_l)
+        more = []
+        for spp in spp_more:
+            seed_m = sd * 7919 + 60 + spp
+            learned_run(ubo, seed_m, spp)  # warm
+            ms_m = min(gpu_ms(lambda: learned_run(ubo, seed_m, spp)) for _ in range(3))
+            img_m, (fb_m, _) = denoised_frame(ctx, den, ubo, spp, seed_m, fallback=fallback)
+            p_m = psnr(img_m, ref)
+            eq_m = _equal_psnr_ms(points, p_m)
+            more.append({"spp": spp, "psnr_learned": round(p_m, 2), "psnr_noisy": round(psnr(noisy_frame(ctx, ubo, spp, seed_m), ref), 2),
+                         "learned_ms": round(ms_m, 3), "fallback_tiles": fb_m,
+                         "plain_ms_at_equal_psnr": None if eq_m is None else round(eq_m, 3),
+                         "ratio_at_equal_psnr": None if eq_m is None else round(eq_m / ms_m, 2)})
         out.append({"seed": sd, "psnr_learned": round(p_l, 2), "psnr_noisy": round(psnr(noisy, ref), 2),
                     "psnr_learned_every_tile": round(psnr(img_all, ref), 2),
                     "learned_ms": round(ms_l, 3), "denoise_ms": round(den_ms, 3), "fallback_tiles": fb, "tiles": tiles,
@@ -292,7 +310,7 @@ def learned_vs_plain(ctx, den, scenes, seeds, spp_learned=4, spp_ref=256, sweep=
                     "reference_spp": spp_ref, "reference_ms": round(ref_ms, 3),
                     "plain_ms_at_equal_psnr": None if eq is None else round(eq, 3),
                     "ratio_at_equal_psnr": None if eq is None else round(eq / ms_l, 2),
-                    "ratio_vs_reference": round(ref_ms / ms_l, 2)})
+                    "ratio_vs_reference": round(ref_ms / ms_l, 2), "learned_more": more})
     ev.destroy(a)
     ev.destroy(b)
     return out
