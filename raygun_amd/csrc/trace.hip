@@ -38,7 +38,8 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(RGR
     }
     const int lane = threadIdx.x & 63;
     const uint32_t gtid = blockIdx.x * kTraceBlock + threadIdx.x;
-    TraversalStack<kLdsStack, kTraceBlock, kSpill> stack{ldsStack, spillBase + (size_t)gtid * kSpill, (int)threadIdx.x};
+    __shared__ float boxRay[kBoxRayWords * kTraceBlock];
+    TraversalStack<kLdsStack, kTraceBlock, kSpill> stack{ldsStack, spillBase + (size_t)gtid * kSpill, (int)threadIdx.x, boxRay};
     uint32_t* fetch = counters + 2 + cur;
     // Wave w first takes rays [64w, 64w + 64); rays beyond the grid's static
     // share are fetched 64 at a time from a counter, so a pass with fewer
@@ -112,7 +113,8 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(RGR
     __syncthreads();
     const uint32_t chunks = chunkEnd[numScenes - 1];
     const uint32_t gtid = blockIdx.x * kTraceBlock + threadIdx.x;
-    TraversalStack<kLdsStack, kTraceBlock, kSpill> stack{ldsStack, spillBase + (size_t)gtid * kSpill, (int)threadIdx.x};
+    __shared__ float boxRay[kBoxRayWords * kTraceBlock];
+    TraversalStack<kLdsStack, kTraceBlock, kSpill> stack{ldsStack, spillBase + (size_t)gtid * kSpill, (int)threadIdx.x, boxRay};
     const uint32_t wavesInGrid = gridDim.x * (kTraceBlock / 64);
     uint32_t k = blockIdx.x * (kTraceBlock / 64) + (threadIdx.x >> 6);
     for(;;) {
@@ -159,7 +161,8 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_rays(TraceScene sc, const
 {
     __shared__ int32_t ldsStack[kLdsStack * kTraceBlock];
     const uint32_t gtid = blockIdx.x * kTraceBlock + threadIdx.x;
-    TraversalStack<kLdsStack, kTraceBlock, kSpill> stack{ldsStack, spillBase + (size_t)gtid * kSpill, (int)threadIdx.x};
+    __shared__ float boxRay[kBoxRayWords * kTraceBlock];
+    TraversalStack<kLdsStack, kTraceBlock, kSpill> stack{ldsStack, spillBase + (size_t)gtid * kSpill, (int)threadIdx.x, boxRay};
     for(uint32_t i = gtid; i < n; i += gridDim.x * kTraceBlock) {
         const rgrt_ray rr = rays[i];
         RayIn r{rr.origin[0], rr.origin[1], rr.origin[2], rr.tmin, rr.dir[0], rr.dir[1], rr.dir[2], rr.tmax};

@@ -44,6 +44,8 @@ __device__ __forceinline__ float safe_rcp(float d)
     return __builtin_amdgcn_rcpf(a);
 }
 
+constexpr int kBoxRayWords = 6; // origin, 1/d
+
 struct BoxRay {
     float ox, oy, oz; // origin
     float ix, iy, iz; // 1/d (zero components made finite)
@@ -231,7 +233,33 @@ struct TraversalStack {
     int32_t* lds;   // base of this block's [kLdsStack][kBlock] array
     int32_t* spill; // this thread's global overflow region (kSpillWords)
     int tid;
+    // base of this block's [kBoxRayWords][kBlock] array: the box test's ray
+    // (origin, 1/d) of the tree being walked. It changes only at instance
+    // entries and exits; held here it is no loop-carried register state,
+    // which a wave's divergent step kinds would otherwise copy between
+    // registers in every iteration.
+    float* brl;
     int sp = 0;
+    __device__ __forceinline__ void set_box_ray(float ox, float oy, float oz, float ix, float iy, float iz)
+    {
+        brl[0 * kBlock + tid] = ox;
+        brl[1 * kBlock + tid] = oy;
+        brl[2 * kBlock + tid] = oz;
+        brl[3 * kBlock + tid] = ix;
+        brl[4 * kBlock + tid] = iy;
+        brl[5 * kBlock + tid] = iz;
+    }
+    __device__ __forceinline__ BoxRay box_ray() const
+    {
+        BoxRay b;
+        b.ox = brl[0 * kBlock + tid];
+        b.oy = brl[1 * kBlock + tid];
+        b.oz = brl[2 * kBlock + tid];
+        b.ix = brl[3 * kBlock + tid];
+        b.iy = brl[4 * kBlock + tid];
+        b.iz = brl[5 * kBlock + tid];
+        return b;
+    }
     // kLdsStack + kSpillWords >= 384 covers the deepest two-level tree, so
     // no entry is ever dropped; the bound check only keeps every access
     // inside the allocation.
@@ -263,10 +291,11 @@ __device__ __forceinline__ void xform_ray(const float* w, float ox, float oy, fl
     rdz = (w[8] * dx + w[9] * dy) + w[10] * dz;
 }
 
-// Traversal state of one ray between steps.
+// Traversal state of one ray between steps (the box test's ray is in the
+// stack's LDS, see TraversalStack::brl; h.t is tbest once h.inst is set).
 struct Traversal {
     RayIn ray;
-    BoxRay br;
+    float wix, wiy, wiz; // 1/d of the world ray (safe_rcp)
     WtRay wt;
     const Node4q* nodes;
     const Tri* tris;
@@ -290,7 +319,6 @@ template<typename Stack>
 __device__ __forceinline__ bool trav_begin(Traversal& T, const TraceScene& sc, const RayIn& ray, uint32_t cullMask, Stack& stack)
 {
     T.ray = ray;
-    T.h.t = 0.0f;
     T.h.u = 0.0f;
     T.h.v = 0.0f;
     T.h.inst = kNoHit;
@@ -305,7 +333,10 @@ __device__ __forceinline__ bool trav_begin(Traversal& T, const TraceScene& sc, c
     T.nodeCap = sc.tlasNodeCap;
     T.numFaces = 0;
     stack.sp = 0;
-    box_ray_setup(T.br, ray.ox, ray.oy, ray.oz, ray.dx, ray.dy, ray.dz);
+    T.wix = safe_rcp(ray.dx);
+    T.wiy = safe_rcp(ray.dy);
+    T.wiz = safe_rcp(ray.dz);
+    stack.set_box_ray(ray.ox, ray.oy, ray.oz, T.wix, T.wiy, T.wiz);
     return sc.numInstances != 0;
 }
 
@@ -330,7 +361,7 @@ __device__ __forceinline__ bool trav_step(Traversal& T, const TraceScene& sc, St
             T.h.inst = kNoHit;
             return false;
         }
-        const int32_t nearest = visit_wide(T.nodes, T.ref, T.br, T.ray.tmin, T.tbest, stack);
+        const int32_t nearest = visit_wide(T.nodes, T.ref, stack.box_ray(), T.ray.tmin, T.tbest, stack);
         if(nearest != kEmptyChild) {
             T.ref = nearest;
             return true;
@@ -341,7 +372,7 @@ __device__ __forceinline__ bool trav_step(Traversal& T, const TraceScene& sc, St
         T.inBlas = false;
         T.nodes = sc.tlasNodes;
         T.nodeCap = sc.tlasNodeCap;
-        box_ray_setup(T.br, T.ray.ox, T.ray.oy, T.ray.oz, T.ray.dx, T.ray.dy, T.ray.dz);
+        stack.set_box_ray(T.ray.ox, T.ray.oy, T.ray.oz, T.wix, T.wiy, T.wiz);
     }
     else if(!T.inBlas) {
         if(kInstEvery > 1 && iter % kInstEvery != 0u) return true; // taken in a later iteration
@@ -372,7 +403,7 @@ __device__ __forceinline__ bool trav_step(Traversal& T, const TraceScene& sc, St
             }
             float rox, roy, roz, rdx, rdy, rdz;
             xform_ray(w, T.ray.ox, T.ray.oy, T.ray.oz, T.ray.dx, T.ray.dy, T.ray.dz, rox, roy, roz, rdx, rdy, rdz);
-            box_ray_setup(T.br, rox, roy, roz, rdx, rdy, rdz);
+            stack.set_box_ray(rox, roy, roz, safe_rcp(rdx), safe_rcp(rdy), safe_rcp(rdz));
             wt_setup(T.wt, rox, roy, roz, rdx, rdy, rdz);
             stack.push(kSentinel);
             T.nodes = reinterpret_cast<const Node4q*>((uint64_t)b0.x | ((uint64_t)b0.y << 32));
@@ -410,7 +441,6 @@ __device__ __forceinline__ bool trav_step(Traversal& T, const TraceScene& sc, St
         float t, u, v;
         if(wt_intersect(T.wt, tri, T.ray.tmin, T.tbest, t, u, v)) {
             T.tbest = t;
-            T.h.t = t;
             T.h.u = u;
             T.h.v = v;
             T.h.inst = T.curInst;
@@ -447,6 +477,7 @@ __device__ HitRec traverse(const TraceScene& sc, const RayIn& ray, uint32_t cull
         for(uint32_t guard = 0; guard < (1u << 26); ++guard)
             if(!trav_step<kInstEvery, kTriEvery, kSentEvery>(T, sc, stack, guard)) break;
     }
+    T.h.t = T.h.inst != kNoHit ? T.tbest : 0.0f;
     return T.h;
 }
 

@@ -1077,8 +1077,9 @@ __global__ __launch_bounds__(kDrainBlock) __attribute__((amdgpu_waves_per_eu(kNe
 {
     __shared__ int32_t ldsStack[kDrainLdsStack * kDrainBlock];
     const uint32_t gtid = blockIdx.x * kDrainBlock + threadIdx.x;
+    __shared__ float boxRay[kBoxRayWords * kDrainBlock];
     TraversalStack<kDrainLdsStack, kDrainBlock, kDrainSpill> stack{ldsStack, spillBase + (size_t)gtid * kDrainSpill,
-                                                                   (int)threadIdx.x};
+                                                                   (int)threadIdx.x, boxRay};
     drain_scene<kNextSample>(p, sc, cur, stack);
 }
 
@@ -1093,8 +1094,9 @@ __global__ __launch_bounds__(kDrainBlock) __attribute__((amdgpu_waves_per_eu(kNe
 {
     __shared__ int32_t ldsStack[kDrainLdsStack * kDrainBlock];
     const uint32_t gtid = blockIdx.x * kDrainBlock + threadIdx.x;
+    __shared__ float boxRay[kBoxRayWords * kDrainBlock];
     TraversalStack<kDrainLdsStack, kDrainBlock, kDrainSpill> stack{ldsStack, spillBase + (size_t)gtid * kDrainSpill,
-                                                                   (int)threadIdx.x};
+                                                                   (int)threadIdx.x, boxRay};
     for(int s = 0; s < numScenes; ++s) {
         const TraceScene sc = scs[s];
         drain_scene<kNextSample>(ps[s], sc, cur, stack);
