@@ -41,7 +41,8 @@ TUNE_CANDIDATES = {"wavefrontPasses": [6, 8, 10, 12, 14], "traceBlocksPerCU": [4
                    "drainBlocksPerCU": [2, 4, 8], "shadeBlocksPerCU": [4, 8, 12]}
 # One frame at a time the recursion tail is not hidden by the next frame, so
 # fewer wavefront passes and a larger drain grid pay off.
-ONE_FRAME_CANDIDATES = {"wavefrontPasses": [4, 5, 6, 8, 10], "drainBlocksPerCU": [2, 3, 4]}
+ONE_FRAME_CANDIDATES = {"wavefrontPasses": [3, 4, 5, 6, 8, 10], "drainBlocksPerCU": [2, 3, 4],
+                        "traceBlocksPerCU": [4, 6, 7], "shadeBlocksPerCU": [4, 8, 12]}
 
 
 def log(msg):
