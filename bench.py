@@ -107,7 +107,7 @@ def main():
                     help="launch shape as JSON (rgrt_tuning fields), used instead of autotuning (for profiling runs)")
     ap.add_argument("--only-chosen", action="store_true",
                     help="time only the chosen submission mode (for profiling runs)")
-    ap.add_argument("--in-flight", type=int, choices=(1, 2, 3, 4), default=3,
+    ap.add_argument("--in-flight", type=int, choices=(1, 2, 3, 4, 5, 6), default=4,
                     help="frames in flight: N > 1 rotates frames over N contexts sharing the scene "
                          "(rgrt_create_shared), so one frame's tail overlaps the next frame's start")
     ap.add_argument("--batch-scenes", type=int, default=32,
