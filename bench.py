@@ -27,6 +27,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# wave64 VALU instructions the GPU can issue per second without packed FP32:
+# 256 CUs x 4 SIMDs x 2.4 GHz / 4 cycles per instruction
+VALU_PEAK_PER_S = 256 * 4 * 2.4e9 / 4
 NODE_BYTES = 64   # compressed 4-wide node (rgrt::Node4q)
 INST_BYTES = 64   # TLAS instance leaf (rgrt::DevInstance)
 TRI_BYTES = 48    # rgrt::Tri
@@ -47,6 +50,22 @@ ONE_FRAME_CANDIDATES = {"wavefrontPasses": [3, 4, 5, 6, 8, 10], "drainBlocksPerC
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
+
+
+def measured_valu(kernel, rays_per_launch, launch_ms):
+    """The VALU-issue side of a kernel's roofline from the committed counter
+    summary (SQ_INSTS_VALU per traced ray) at this run's rays and launch time:
+    instructions per launch / (launch time x the issue peak); None without
+    one."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_valu.json")
+    try:
+        with open(path) as f:
+            per_ray = json.load(f)["valu_instructions_per_ray"]
+    except (OSError, KeyError, ValueError):
+        return None
+    per_launch = per_ray * rays_per_launch
+    return {"instructions_per_launch": round(per_launch), "peak_per_s": VALU_PEAK_PER_S,
+            "frac": round(per_launch / (launch_ms * 1e-3) / VALU_PEAK_PER_S, 4), "source": f"profiles/{kernel}_valu.json"}
 
 
 def measured_traffic(kernel, rays_per_launch):
@@ -309,6 +328,9 @@ def main():
                    "algorithmic_bytes_per_launch": round(alg_per_launch)}
             if tsrc:
                 obj["traffic_source"] = tsrc
+            valu = measured_valu(name, kk["rays"] / kk["launches"], avg_ms)
+            if valu:
+                obj["valu_issue"] = valu
             if roof is None:
                 roof = obj
                 roof.update({"node_bytes": NODE_BYTES, "inst_bytes": INST_BYTES, "tri_bytes": TRI_BYTES,
