@@ -133,8 +133,8 @@ def main():
                     help="scenes of the batched workload measured after the headline (0 = skip)")
     ap.add_argument("--no-learned", action="store_true",
                     help="skip the learned path (training on traced frames, then learned vs plain at 1080p)")
-    ap.add_argument("--learned-train-count", type=int, default=1000)
-    ap.add_argument("--learned-train-steps", type=int, default=3000)
+    ap.add_argument("--learned-train-count", type=int, default=2000)
+    ap.add_argument("--learned-train-steps", type=int, default=6000)
     ap.add_argument("--no-config5", action="store_true",
                     help="skip the config-5 measurement (3840x2160, 16M unique triangles, 4 spp, depth 6) after the headline")
     ap.add_argument("--config5-in-flight", type=int, choices=(1, 2, 3, 4, 5, 6), default=4,
@@ -409,7 +409,7 @@ LEARNED_SPP = (4, 8, 16)  # sample counts of the learned path's input
 PSNR_TARGET_DB = 38.0
 
 
-def bench_learned(train_count=1000, train_steps=3000, train_res=256, crop=128, eval_seeds=(60000, 60001, 60002)):
+def bench_learned(train_count=2000, train_steps=6000, train_res=256, crop=128, eval_seeds=(60000, 60001, 60002)):
     """The metric's second half: frame ms of the learned path (stochastic
     frame + the denoiser) against the plain tracer's at equal PSNR, at
     1920x1080 on held-out scenes of the config-1 family, PSNR against each
