@@ -51,16 +51,6 @@ struct BoxRay {
     float ix, iy, iz; // 1/d (zero components made finite)
 };
 
-__device__ __forceinline__ void box_ray_setup(BoxRay& b, float ox, float oy, float oz, float dx, float dy, float dz)
-{
-    b.ox = ox;
-    b.oy = oy;
-    b.oz = oz;
-    b.ix = safe_rcp(dx);
-    b.iy = safe_rcp(dy);
-    b.iz = safe_rcp(dz);
-}
-
 __device__ __forceinline__ float pow2i(int e) { return __uint_as_float((uint32_t)(e + 127) << 23); }
 
 __device__ __forceinline__ void cswap(float& ta, int32_t& ra, float& tb, int32_t& rb)
