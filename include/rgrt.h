@@ -205,6 +205,8 @@ rgrt_status rgrt_synchronize(rgrt_ctx* ctx);
  * BLAS built on earlier geometry. */
 rgrt_status rgrt_upload_geometry(rgrt_ctx* ctx, const rgrt_vertex* vertices, uint64_t numVertices, const uint32_t* indices,
                                  uint64_t numIndices);
+/* At most 2^28 materials (a suspended closest-hit frame keeps the material
+ * index in 28 bits); more returns RGRT_ERR_INVALID_ARG. */
 rgrt_status rgrt_upload_materials(rgrt_ctx* ctx, const rgrt_material* materials, uint32_t numMaterials);
 /* Rewrites vertex positions/normals of a range in place (deforming meshes);
  * follow with rgrt_refit_blas or rgrt_build_blas. */
