@@ -41,11 +41,13 @@ CONFIG5_CANDIDATES = {"wavefrontPasses": [24, 32, 48, 64], "traceBlocksPerCU": [
                       "drainBlocksPerCU": [2, 4, 8]}
 # launch shapes the autotuner tries for the headline pipeline (coordinate descent)
 TUNE_CANDIDATES = {"wavefrontPasses": [6, 8, 10, 12, 14], "traceBlocksPerCU": [4, 6, 8],
-                   "drainBlocksPerCU": [2, 4, 8], "shadeBlocksPerCU": [4, 8, 12]}
-# One frame at a time the recursion tail is not hidden by the next frame, so
-# fewer wavefront passes and a larger drain grid pay off.
-ONE_FRAME_CANDIDATES = {"wavefrontPasses": [3, 4, 5, 6, 8, 10], "drainBlocksPerCU": [2, 3, 4],
-                        "traceBlocksPerCU": [4, 6, 7], "shadeBlocksPerCU": [4, 8, 12]}
+                   "drainBlocksPerCU": [2, 4, 8], "shadeBlocksPerCU": [4, 8, 12], "frameBands": [1, 2]}
+# One frame at a time the recursion tail is not hidden by the next frame:
+# the frame traced in bands on separate streams (so one band's sparse last
+# passes overlap another's work), fewer wavefront passes and a larger drain
+# grid pay off.
+ONE_FRAME_CANDIDATES = {"frameBands": [1, 2, 3, 4], "wavefrontPasses": [3, 4, 5, 6, 8, 10],
+                        "drainBlocksPerCU": [2, 3, 4], "traceBlocksPerCU": [4, 6, 7], "shadeBlocksPerCU": [4, 8, 12]}
 
 
 def log(msg):
@@ -256,7 +258,7 @@ def main():
     if not args.no_autotune:
         t_main = dict(ctx.tuning())
         one_fn = [graph_on(ctx)]
-        b1, b1_ms, _ = autotune.tune(ctx, None, candidates=ONE_FRAME_CANDIDATES, contexts=[ctx],
+        b1, b1_ms, _ = autotune.tune(ctx, None, candidates=ONE_FRAME_CANDIDATES, contexts=[ctx], rounds=2,
                                      timer=lambda: frames_ms([ctx], one_fn, 12, 3, ev))
         one_ms = frames_ms([ctx], one_fn, args.steps, args.warmup, ev)
         one = {"ms_per_frame": round(one_ms, 4), "mrays_per_s": round(rays / (one_ms * 1e-3) / 1e6, 1), "tuning": b1}

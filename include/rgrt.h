@@ -519,10 +519,16 @@ typedef struct rgrt_tuning {
     uint32_t hostLag;           /* wavefront passes the GPU may run ahead of the host's check
                                    for an empty queue (1..16); the passes queued after the
                                    last ray are empty launches                               */
+    uint32_t frameBands;        /* graph frames (RGRT_FRAME_GRAPH): the frame split into this
+                                   many strips of whole 8x8 tiles, each traced on a stream of
+                                   its own, side by side (1..4; 0 = 1); the images are those
+                                   of the frame in one piece. The strips need hardware queues
+                                   of their own (GPU_MAX_HW_QUEUES, 4 by default) to overlap  */
 } rgrt_tuning;
 
 /* Sets / reads the launch shape (defaults tuned for MI355X). Values outside
- * 1..32 blocks per CU or a hostLag outside 1..16 are rejected. */
+ * 1..32 blocks per CU, a hostLag outside 1..16 or more than 4 frameBands
+ * are rejected. */
 rgrt_status rgrt_set_tuning(rgrt_ctx* ctx, const rgrt_tuning* t);
 rgrt_status rgrt_get_tuning(rgrt_ctx* ctx, rgrt_tuning* t);
 /* No counterpart in the reference (its raygen loops over a pixel's samples,

@@ -151,21 +151,36 @@ struct rgrt_ctx {
     int32_t* spill = nullptr;
     size_t spillBytes = 0;
     uint32_t traceGrid = 0, shadeGrid = 0, drainGrid = 0;
-    rgrt_tuning tuning{6, 6, 8, 4, 2}; // measured on config 2 (1080p, 2M triangles)
+    rgrt_tuning tuning{6, 6, 8, 4, 2, 1}; // measured on config 2 (1080p, 2M triangles)
     rgrt_sampling sampling{RGRT_SAMPLING_WHITTED, 0u, {0u, 0u}};
     hipEvent_t evDrainStart = nullptr, evDrainEnd = nullptr;
     bool lastDrained = false;
 
+    // Graph frames in bands: the streams bands 1.. run on (band 0 runs on
+    // `stream`), their fork and join events, and the frame's executable
+    // graphs (instance upload and TLAS; each band's trace; the post passes).
+    uint32_t bands = 1;
+    hipStream_t bandStream[kMaxBands] = {};
+    hipEvent_t bandFork = nullptr;
+    hipEvent_t bandJoin[kMaxBands] = {};
+    hipGraphExec_t preExec = nullptr, postExec = nullptr;
+    hipGraphExec_t bandExec[kMaxBands] = {};
+    uint32_t lastBands = 1;   // bands of the last traced frame (their counter blocks hold its statistics)
+    uint32_t passStride = 0;  // per-band stride of the pass counts in hostCounts
+    size_t spillWords = 0;    // spill words of one band
+
     // Grids from the tuning; the traversal spill region covers the larger of
-    // the persistent traversal and drain grids (they never run at once).
+    // the persistent traversal and drain grids (they never run at once), once
+    // per band (the bands' launches run side by side).
     void apply_tuning()
     {
         traceGrid = numCUs * tuning.traceBlocksPerCU;
         shadeGrid = numCUs * tuning.shadeBlocksPerCU;
         drainGrid = numCUs * tuning.drainBlocksPerCU;
-        const size_t need = std::max((size_t)traceGrid * trace_block_size() * trace_spill_words_per_thread(),
-                                     (size_t)drainGrid * drain_block_size() * drain_spill_words_per_thread()) *
-                            4;
+        bands = std::max<uint32_t>(1u, std::min(tuning.frameBands, kMaxBands));
+        spillWords = std::max((size_t)traceGrid * trace_block_size() * trace_spill_words_per_thread(),
+                              (size_t)drainGrid * drain_block_size() * drain_spill_words_per_thread());
+        const size_t need = spillWords * bands * 4;
         if(need > spillBytes) {
             if(spill) {
                 check(hipStreamSynchronize(stream));
@@ -541,6 +556,8 @@ void prepare_sample_groups(rgrt_ctx* c)
         c->sampleCap = (size_t)n * s.nimg;
     }
     s.npix = (uint32_t)slots;
+    s.chunkFirst = 0;
+    s.frameSlots = (uint32_t)slots;
     s.groups = g;
     s.numSamples = n;
 }
@@ -639,6 +656,7 @@ rgrt_status do_trace(rgrt_ctx* c)
     c->lastTraceLaunches = traceLaunches;
     c->lastDrained = drained;
     c->lastWaves = it;
+    c->lastBands = 1;
     c->timedRt = timing;
     return RGRT_OK;
 }
@@ -666,6 +684,63 @@ void enqueue_trace_fixed(rgrt_ctx* c)
     c->lastTraceLaunches = passes;
     c->lastDrained = true;
     c->lastWaves = passes;
+    c->lastBands = 1;
+}
+
+// Band b of `bands`: a run of the frame's 64-slot chunks (whole 8x8 tiles
+// in tile order, so a strip of the image). Strips rather than tiles dealt
+// out in turn: the strips' work differs (sky against geometry), so they
+// reach their sparse last passes at different times and overlap them with
+// each other's work; dealt-out tiles give every band the same tail at the
+// same time (measured: 7.6-7.7 against 8.0 ms per config-2 frame).
+void band_chunks(uint32_t npix, uint32_t b, uint32_t bands, uint32_t& first, uint32_t& slots, uint32_t& queueFirst)
+{
+    const uint32_t chunks = (npix + 63u) / 64u;
+    const uint32_t lo = (uint32_t)((uint64_t)chunks * b / bands), hi = (uint32_t)((uint64_t)chunks * (b + 1) / bands);
+    first = lo;
+    slots = std::min(npix, hi * 64u) - std::min(npix, lo * 64u);
+    queueFirst = std::min(npix, lo * 64u);
+}
+
+// Band b of `bands` with queues, hit records and a counter block of its own,
+// carved from the whole frame's arrays, so the bands together use what one
+// frame does; the per-slot arrays are shared, each slot belonging to one
+// band.
+WavefrontState band_state(const WavefrontState& st, uint32_t b, uint32_t bands)
+{
+    uint32_t first = 0, slots = 0, q0 = 0;
+    band_chunks(st.npix, b, bands, first, slots, q0);
+    WavefrontState w = st;
+    w.npix = slots;
+    w.chunkFirst = first;
+    w.rays[0] = st.rays[0] + (size_t)kRayWords * q0;
+    w.rays[1] = st.rays[1] + (size_t)kRayWords * q0;
+    w.hits = st.hits + (size_t)kHitWords * q0;
+    w.counters = st.counters + (size_t)kCounterBlock * b;
+    return w;
+}
+
+// The trace of band b as a fixed launch sequence on stream s (capturable):
+// raygen of the band's slots, the wavefront passes and the drain over its
+// own queues, with its own spill region and pass-count slots. An empty band
+// (a frame of fewer than bands x 64 slots) launches empty passes.
+void enqueue_band(rgrt_ctx* c, uint32_t b, uint32_t bands, hipStream_t s)
+{
+    ShadeParams p = shade_params(c);
+    p.st = band_state(c->st, b, bands);
+    const uint32_t maxIters = chain_iters(c);
+    const uint32_t passes = std::min<uint32_t>(c->tuning.wavefrontPasses, maxIters);
+    const TraceScene sc = c->scene();
+    int32_t* spill = c->spill + c->spillWords * b;
+    uint32_t* counts = c->hostCountsDev + (size_t)c->passStride * b;
+    launch_raygen(s, p);
+    int cur = 0;
+    for(uint32_t it = 0; it < passes; ++it) {
+        launch_trace_queue_impl(s, sc, p.st, cur, c->traceGrid, spill, counts + it);
+        launch_shade(s, p, cur, c->shadeGrid);
+        cur ^= 1;
+    }
+    launch_drain(s, p, sc, cur, c->drainGrid, spill);
 }
 
 rgrt_status do_post(rgrt_ctx* c, int32_t useFxaa)
@@ -851,6 +926,17 @@ void rgrt_destroy(rgrt_ctx* c)
     for(hipEvent_t e: c->traceEvents) (void)hipEventDestroy(e);
     for(hipEvent_t e: c->shadeEvents) (void)hipEventDestroy(e);
     if(c->frameExec) (void)hipGraphExecDestroy(c->frameExec);
+    for(hipGraphExec_t g: {c->preExec, c->postExec, c->bandExec[0], c->bandExec[1], c->bandExec[2], c->bandExec[3]})
+        if(g) (void)hipGraphExecDestroy(g);
+    static_assert(kMaxBands == 4, "band graphs destroyed above");
+    for(uint32_t b = 1; b < kMaxBands; ++b) {
+        if(c->bandStream[b]) {
+            (void)hipStreamSynchronize(c->bandStream[b]);
+            (void)hipStreamDestroy(c->bandStream[b]);
+        }
+        if(c->bandJoin[b]) (void)hipEventDestroy(c->bandJoin[b]);
+    }
+    if(c->bandFork) (void)hipEventDestroy(c->bandFork);
     for(hipEvent_t e: {c->evAsStart, c->evAsEnd, c->evRtStart, c->evRtOnlyEnd, c->evRoughEnd, c->evPostEnd, c->stageDone[0],
                        c->stageDone[1], c->sceneEv,
                        c->evDrainStart, c->evDrainEnd, c->evFrameStart, c->evFrameEnd})
@@ -1268,18 +1354,32 @@ rgrt_status frame_prepare(rgrt_ctx* c, const rgrt_ubo* ubo, const rgrt_instance*
     stage_to_host(c, in, n);
     reserve_builder(c, std::max<uint32_t>(n, 1));
     prepare_sample_groups(c);
-    ensure_host_counts(c, chain_iters(c) + 2);
+    c->passStride = chain_iters(c) + 2;
+    ensure_host_counts(c, c->passStride * c->bands);
+    if(c->bands > 1) {
+        if(!c->bandFork) check(hipEventCreateWithFlags(&c->bandFork, hipEventDisableTiming));
+        for(uint32_t b = 1; b < c->bands; ++b) {
+            if(!c->bandStream[b]) check(hipStreamCreateWithFlags(&c->bandStream[b], hipStreamNonBlocking));
+            if(!c->bandJoin[b]) check(hipEventCreateWithFlags(&c->bandJoin[b], hipEventDisableTiming));
+        }
+    }
     return RGRT_OK;
+}
+
+// Instance upload and TLAS build or refit of a graph frame.
+void frame_enqueue_tlas(rgrt_ctx* c, uint32_t n, uint32_t flags)
+{
+    if(n)
+        check(hipMemcpyAsync(c->dInstIn, c->hInstStage[c->stageIdx], n * sizeof(rgrt_instance), hipMemcpyHostToDevice,
+                             c->stream));
+    enqueue_tlas(c, n, (flags & RGRT_FRAME_REFIT_TLAS) != 0);
 }
 
 void frame_enqueue(rgrt_ctx* c, uint32_t n, uint32_t flags)
 {
-    hipStream_t s = c->stream;
-    if(n)
-        check(hipMemcpyAsync(c->dInstIn, c->hInstStage[c->stageIdx], n * sizeof(rgrt_instance), hipMemcpyHostToDevice, s));
-    enqueue_tlas(c, n, (flags & RGRT_FRAME_REFIT_TLAS) != 0);
+    frame_enqueue_tlas(c, n, flags);
     enqueue_trace_fixed(c);
-    launch_post(s, c->img, c->ubo, (flags & RGRT_FRAME_FXAA) ? 1 : 0, nullptr);
+    launch_post(c->stream, c->img, c->ubo, (flags & RGRT_FRAME_FXAA) ? 1 : 0, nullptr);
 }
 
 void frame_finish(rgrt_ctx* c, const rgrt_instance* in, uint32_t n, uint32_t flags)
@@ -1313,8 +1413,34 @@ void update_exec(hipGraphExec_t* exec, hipGraph_t g)
     (void)hipGraphDestroy(g);
 }
 
+// Captures what f enqueues on stream s into *exec (update_exec).
+template<typename F>
+void capture_into(hipGraphExec_t* exec, hipStream_t s, F f)
+{
+    hipGraph_t g = nullptr;
+    check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    try {
+        f();
+    }
+    catch(...) {
+        (void)hipStreamEndCapture(s, &g);
+        if(g) (void)hipGraphDestroy(g);
+        throw;
+    }
+    check(hipStreamEndCapture(s, &g));
+    update_exec(exec, g);
+}
+
 // A prepared frame (frame_prepare) captured on the context's stream and
-// launched there as one hipGraph.
+// launched there as one hipGraph. With frameBands > 1 the frame is four
+// kinds of graph instead: instance upload and TLAS on the context's stream,
+// then every band's trace on a stream of its own (band 0 on the context's
+// stream), then the post passes once every band has joined. The bands'
+// passes run side by side, so one band's last, sparse passes overlap
+// another's work instead of leaving the GPU idle; the images are those of
+// the frame in one piece (each slot is traced by the same code either way).
+// Separate graphs rather than one graph with a branch per band: the runtime
+// was measured to run such branches one after another.
 void launch_frame_graph(rgrt_ctx* c, const rgrt_instance* in, uint32_t n, uint32_t flags)
 {
     hipStream_t s = c->stream;
@@ -1328,22 +1454,49 @@ void launch_frame_graph(rgrt_ctx* c, const rgrt_instance* in, uint32_t n, uint32
     const bool prevHave = c->haveTlas;
     c->numInstances = n;
     c->haveTlas = true;
-    hipGraph_t g = nullptr;
-    check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    const uint32_t bands = c->bands;
     try {
-        frame_enqueue(c, n, flags);
+        if(bands == 1) {
+            capture_into(&c->frameExec, s, [&]() { frame_enqueue(c, n, flags); });
+        }
+        else {
+            capture_into(&c->preExec, s, [&]() { frame_enqueue_tlas(c, n, flags); });
+            for(uint32_t b = 0; b < bands; ++b) {
+                hipStream_t bs = b ? c->bandStream[b] : s;
+                capture_into(&c->bandExec[b], bs, [&]() { enqueue_band(c, b, bands, bs); });
+            }
+            capture_into(&c->postExec, s, [&]() {
+                launch_finalize_samples(s, shade_params(c));
+                launch_post(s, c->img, c->ubo, (flags & RGRT_FRAME_FXAA) ? 1 : 0, nullptr);
+            });
+        }
     }
     catch(...) {
-        (void)hipStreamEndCapture(s, &g);
-        if(g) (void)hipGraphDestroy(g);
         c->numInstances = prevInstances;
         c->haveTlas = prevHave;
         throw;
     }
-    check(hipStreamEndCapture(s, &g));
-    update_exec(&c->frameExec, g);
     if(timing) check(hipEventRecord(c->evFrameStart, s));
-    check(hipGraphLaunch(c->frameExec, s));
+    if(bands == 1) {
+        check(hipGraphLaunch(c->frameExec, s));
+    }
+    else {
+        check(hipGraphLaunch(c->preExec, s));
+        check(hipEventRecord(c->bandFork, s));
+        for(uint32_t b = 1; b < bands; ++b) {
+            check(hipStreamWaitEvent(c->bandStream[b], c->bandFork, 0));
+            check(hipGraphLaunch(c->bandExec[b], c->bandStream[b]));
+            check(hipEventRecord(c->bandJoin[b], c->bandStream[b]));
+        }
+        check(hipGraphLaunch(c->bandExec[0], s));
+        for(uint32_t b = 1; b < bands; ++b) check(hipStreamWaitEvent(s, c->bandJoin[b], 0));
+        check(hipGraphLaunch(c->postExec, s));
+        const uint32_t passes = std::min<uint32_t>(c->tuning.wavefrontPasses, chain_iters(c));
+        c->lastTraceLaunches = passes;
+        c->lastDrained = true;
+        c->lastWaves = passes;
+        c->lastBands = bands;
+    }
     if(timing) check(hipEventRecord(c->evFrameEnd, s));
     check(hipEventRecord(c->stageDone[c->stageIdx], s)); // this staging buffer is free after this frame
     frame_finish(c, in, n, flags);
@@ -2022,11 +2175,14 @@ rgrt_status rgrt_get_stats(rgrt_ctx* c, rgrt_stats* out)
         check(hipStreamSynchronize(c->stream));
         std::memset(out, 0, sizeof(*out));
         if(c->st.counters) {
-            uint64_t cnt[6];
-            check(hipMemcpy(cnt, c->st.counters + 4, sizeof(cnt), hipMemcpyDeviceToHost));
-            out->rays = cnt[0];
-            out->skyLookups = cnt[1];
-            out->zeroDirRays = cnt[2];
+            // the counter blocks of the last frame's bands
+            for(uint32_t b = 0; b < c->lastBands; ++b) {
+                uint64_t cnt[3];
+                check(hipMemcpy(cnt, c->st.counters + (size_t)kCounterBlock * b + 4, sizeof(cnt), hipMemcpyDeviceToHost));
+                out->rays += cnt[0];
+                out->skyLookups += cnt[1];
+                out->zeroDirRays += cnt[2];
+            }
         }
         out->waves = c->lastWaves;
         out->traceLaunches = c->lastTraceLaunches;
@@ -2061,6 +2217,7 @@ rgrt_status rgrt_set_tuning(rgrt_ctx* c, const rgrt_tuning* t)
         for(uint32_t v: {t->traceBlocksPerCU, t->shadeBlocksPerCU, t->drainBlocksPerCU})
             if(v < 1 || v > 32) return fail(c, RGRT_ERR_INVALID_ARG, "set_tuning: blocks per CU outside 1..32");
         if(t->hostLag < 1 || t->hostLag > 16) return fail(c, RGRT_ERR_INVALID_ARG, "set_tuning: hostLag outside 1..16");
+        if(t->frameBands > kMaxBands) return fail(c, RGRT_ERR_INVALID_ARG, "set_tuning: frameBands outside 0..4");
         c->tuning = *t;
         c->apply_tuning();
         return RGRT_OK;
@@ -2089,10 +2246,15 @@ rgrt_status rgrt_get_pass_counts(rgrt_ctx* c, uint32_t* counts, uint32_t capacit
 {
     return guarded(c, [&]() {
         check(hipStreamSynchronize(c->stream));
+        const uint32_t stride = c->lastBands > 1 ? c->passStride : 0u;
         const uint32_t n = std::min(c->lastTraceLaunches, c->hostCountsCap);
         if(count) *count = n;
         if(counts)
-            for(uint32_t i = 0; i < std::min(n, capacity); ++i) counts[i] = c->hostCounts[i];
+            for(uint32_t i = 0; i < std::min(n, capacity); ++i) {
+                uint32_t v = 0;
+                for(uint32_t b = 0; b < c->lastBands; ++b) v += c->hostCounts[(size_t)stride * b + i];
+                counts[i] = v;
+            }
         return RGRT_OK;
     });
 }

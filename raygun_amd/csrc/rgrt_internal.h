@@ -101,6 +101,8 @@ constexpr int kMaxFrames = 8; // maxRecursions <= 7 keeps at most 7 frames waiti
 constexpr int kRayWords = 10; // origin, tmin, dir, tmax, pixel, flags
 constexpr int kHitWords = 5;
 constexpr int kTraversalErrorWord = 12; // in WavefrontState::counters, sticky (cleared at allocation)
+constexpr int kCounterBlock = 16;       // counter words per band (the error word is band 0's)
+constexpr uint32_t kMaxBands = 4;
 
 // Ray flags in a queue entry
 constexpr uint32_t kFlagMiss1 = 1u;     // miss shader index 1 (shadow miss)
@@ -108,8 +110,16 @@ constexpr uint32_t kFlagPrimary = 2u;   // primary ray of sample 0 (AOV record)
 constexpr uint32_t kMaskShift = 8u;     // cull mask in bits 8..15
 
 struct WavefrontState {
-    uint32_t npix = 0; // wavefront slots: image pixels x sample groups
+    uint32_t npix = 0; // wavefront slots: image pixels x sample groups (of the band, see chunkFirst)
     uint32_t nimg = 0; // image pixels
+    // A frame traced in bands: a band takes the npix slots from the frame's
+    // 64-slot chunk chunkFirst on (its slot q is frame slot chunkFirst * 64
+    // + q), with its own queues, hit records and counter block; the per-slot
+    // arrays (payload, accum, frames, lum2) are the whole frame's, frames
+    // with frameSlots slots per level. The frame in one piece: chunkFirst 0,
+    // frameSlots = npix.
+    uint32_t chunkFirst = 0;
+    uint32_t frameSlots = 0;
     // Samples of a pixel traced side by side: slot g * nimg + i runs samples
     // g, g + groups, ... of the pixel in tile order i and stores each
     // sample's record; k_finalize_samples sums them in sample order.

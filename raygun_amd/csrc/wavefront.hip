@@ -155,7 +155,7 @@ __device__ __forceinline__ void store_state(const ShadeParams& p, uint32_t pix, 
 // t, origin, dir, vn, gridRef, shadowColor, reflectDepth, reflectColor, 0.
 __device__ __forceinline__ void store_frame(const ShadeParams& p, uint32_t pix, int level, const Frame& F)
 {
-    float4* b = reinterpret_cast<float4*>(p.st.frames + ((size_t)level * p.st.npix + pix) * kFrameStride);
+    float4* b = reinterpret_cast<float4*>(p.st.frames + ((size_t)level * p.st.frameSlots + pix) * kFrameStride);
     const uint32_t w0 = ((uint32_t)F.stage & 7u) | (((uint32_t)F.frontFacing & 1u) << 3) | (F.mat << 4);
     b[0] = make_float4(__uint_as_float(w0), F.t, F.origin.x, F.origin.y);
     b[1] = make_float4(F.origin.z, F.dir.x, F.dir.y, F.dir.z);
@@ -382,7 +382,7 @@ __device__ __forceinline__ void frame_material(const ShadeParams& p, Frame& F)
 __device__ __forceinline__ void load_frame(const Ctx& c, uint32_t pix, int level, Frame& F)
 {
     const ShadeParams& p = c.p;
-    const float4* b = reinterpret_cast<const float4*>(p.st.frames + ((size_t)level * p.st.npix + pix) * kFrameStride);
+    const float4* b = reinterpret_cast<const float4*>(p.st.frames + ((size_t)level * p.st.frameSlots + pix) * kFrameStride);
     const float4 a0 = b[0], a1 = b[1], a2 = b[2], a3 = b[3], a4 = b[4];
     const uint32_t w0 = __float_as_uint(a0.x);
     F.stage = (int)(w0 & 7u);
@@ -701,17 +701,19 @@ __device__ __forceinline__ void write_ray(float* rays, size_t n, uint32_t slot, 
     rays[9 * n + slot] = __uint_as_float(flags);
 }
 
-__device__ __forceinline__ void raygen_slot(const ShadeParams& p, uint32_t pix)
+// Primary ray of the band's slot `q`, at position q of the band's queue 0.
+__device__ __forceinline__ void raygen_slot(const ShadeParams& p, uint32_t q)
 {
-    if(pix == 0) {
+    if(q == 0) {
         uint32_t* c = p.st.counters;
-        c[0] = p.st.npix; // queue 0 holds every pixel's primary ray
+        c[0] = p.st.npix; // queue 0 holds every slot's primary ray
         c[1] = 0;
         c[2] = 0;
         c[3] = 0;
         for(int k = 4; k < 12; ++k) c[k] = 0; // ray / sky / zero-dir counters (u64)
     }
-    if(pix >= p.st.npix) return;
+    if(q >= p.st.npix) return;
+    const uint32_t pix = p.st.chunkFirst * 64u + q;
     // slot `pix` starts sample g (its group) of tile-order pixel i
     const uint32_t g = p.st.groups == 1u ? 0u : pix / p.st.nimg;
     const uint32_t i = pix - g * p.st.nimg;
@@ -728,7 +730,7 @@ __device__ __forceinline__ void raygen_slot(const ShadeParams& p, uint32_t pix)
         if(p.sampling == RGRT_SAMPLING_STOCHASTIC) p.st.lum2[i] = 0.0f;
     }
     // only sample 0's primary ray carries the primary-hit flag (AOV, features)
-    write_ray(p.st.rays[0], p.st.npix, pix, r, pix, (g == 0u ? kFlagPrimary : 0u) | (0xffu << kMaskShift));
+    write_ray(p.st.rays[0], p.st.npix, q, r, pix, (g == 0u ? kFlagPrimary : 0u) | (0xffu << kMaskShift));
 }
 
 __global__ __launch_bounds__(kShadeBlock) void k_raygen(ShadeParams p)
@@ -1107,7 +1109,8 @@ __global__ __launch_bounds__(kDrainBlock) __attribute__((amdgpu_waves_per_eu(kNe
 
 void launch_raygen(hipStream_t s, const ShadeParams& p)
 {
-    const uint32_t blocks = (p.st.npix + kShadeBlock - 1) / kShadeBlock;
+    // at least one block: slot 0's thread resets the counters of an empty band too
+    const uint32_t blocks = std::max(1u, (p.st.npix + kShadeBlock - 1) / kShadeBlock);
     k_raygen<<<blocks, kShadeBlock, 0, s>>>(p);
 }
 

@@ -52,7 +52,7 @@ STATS_FIELDS = [
 ]
 
 TUNING = np.dtype([("wavefrontPasses", "<u4"), ("traceBlocksPerCU", "<u4"), ("shadeBlocksPerCU", "<u4"),
-                   ("drainBlocksPerCU", "<u4"), ("hostLag", "<u4")])
+                   ("drainBlocksPerCU", "<u4"), ("hostLag", "<u4"), ("frameBands", "<u4")])
 
 WIDE_NODE = np.dtype([("lox", "<f4", 4), ("hix", "<f4", 4), ("loy", "<f4", 4), ("hiy", "<f4", 4), ("loz", "<f4", 4),
                       ("hiz", "<f4", 4), ("child", "<i4", 4), ("pad", "<i4", 4)])

@@ -116,3 +116,65 @@ def test_frame_rejects_bad_flags_and_refit_mismatch():
         ctx.frame(setup.ubo(), inst, refit=True, graph=True)  # still usable
     finally:
         ctx.close()
+
+
+@pytest.mark.parametrize("which,bands,spp", [("config2-small", 2, 1), ("config2-small", 3, 1), ("config2-small", 4, 1),
+                                             ("config5-small", 3, 4), ("config1-tiny", 4, 1)])
+def test_frames_in_bands_equal_the_frame_in_one_piece(which, bands, spp):
+    """frameBands: the graph frame traced as runs of tiles on separate
+    streams gives the images, ray statistics and per-pass ray counts of the
+    frame in one piece, over frames that move the camera and refit the TLAS;
+    with several samples per slot group and with bands left empty (a 16x8
+    image has two 64-slot chunks for four bands)."""
+    if which == "config2-small":
+        setup = small_config2(320, 180, tris_per_mesh=5_000)
+    elif which == "config5-small":
+        setup = procgen.config5(192, 108, tris_per_mesh=3_000)
+    else:
+        setup = procgen.config1(16, 8)
+    setup.num_samples = spp
+    print(f"[case] frame-bands {which} bands={bands} spp={spp}", flush=True)
+    one, _ = gpu_context(setup)
+    banded, _ = gpu_context(setup)
+    try:
+        banded.set_tuning(frameBands=bands, wavefrontPasses=3)
+        one.set_tuning(wavefrontPasses=3)
+        assert banded.tuning()["frameBands"] == bands
+        for f, (ubo, inst) in enumerate(_frames(setup, 3)):
+            refit = f > 0
+            one.frame(ubo, inst, fxaa=True, refit=refit, graph=True)
+            want = _read_all(one)
+            banded.frame(ubo, inst, fxaa=True, refit=refit, graph=True)
+            got = _read_all(banded)
+            for k, (a, b) in zip(IMAGES, zip(got, want)):
+                assert np.array_equal(a, b), (f, k)
+            s1, sb = one.stats(), banded.stats()
+            for key in ("rays", "skyLookups", "zeroDirRays"):
+                assert sb[key] == s1[key], (f, key)
+            assert np.array_equal(banded.pass_counts(), one.pass_counts()), f
+        # an eager frame (always in one piece) in between, then bands again
+        ubo, inst = _frames(setup, 1)[0]
+        for c in (one, banded):
+            c.set_frame(ubo)
+            c.render(True)
+        assert np.array_equal(banded.read_image(rgrt.IMAGE_FINAL), one.read_image(rgrt.IMAGE_FINAL))
+        assert banded.stats()["rays"] == one.stats()["rays"]
+        for c in (one, banded):
+            c.frame(ubo, inst, fxaa=True, refit=True, graph=True)
+        assert np.array_equal(banded.read_image(rgrt.IMAGE_FINAL), one.read_image(rgrt.IMAGE_FINAL))
+    finally:
+        one.close()
+        banded.close()
+
+
+def test_frame_bands_range_checked():
+    setup = procgen.config1(64, 36)
+    print("[case] frame-bands range", flush=True)
+    ctx, _ = gpu_context(setup)
+    try:
+        with pytest.raises(RuntimeError):
+            ctx.set_tuning(frameBands=5)
+        ctx.set_tuning(frameBands=0)  # one piece
+        assert ctx.tuning()["frameBands"] == 0
+    finally:
+        ctx.close()
