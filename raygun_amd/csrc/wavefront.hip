@@ -151,17 +151,28 @@ __device__ __forceinline__ void store_state(const ShadeParams& p, uint32_t pix, 
     b[1] = make_float4(P.curIOR, P.refDepth, __int_as_float(P.recDepth), __uint_as_float(m));
 }
 
-// Frame record (kFrameStride words): stage | frontFacing << 3 | mat << 4,
-// t, origin, dir, vn, gridRef, shadowColor, reflectDepth, reflectColor, 0.
-__device__ __forceinline__ void store_frame(const ShadeParams& p, uint32_t pix, int level, const Frame& F)
+// Frame record (kFrameStride words) in five 16-byte parts: [0] stage |
+// frontFacing << 3 | mat << 4, t, origin.xy; [1] origin.z, dir; [2] vn,
+// gridRef; [3] shadowColor, reflectDepth; [4] reflectColor, 0.
+// Only the parts a later resume reads and that changed are written: a frame
+// suspended by the closest-hit stage (`fresh`) writes [0..2]; [3] once a
+// stage after it reads shadowColor or reflectDepth (every stage but G_SHADOW,
+// whose resume sets shadowColor, reflectColor and reflectDepth before reading
+// them, and ST_INTERNAL, which reads nothing but the stage); [4] for G_REFR,
+// the only stage whose resume reads reflectColor. A resumed frame's [1], [2]
+// are unchanged in memory.
+__device__ __forceinline__ void store_frame(const ShadeParams& p, uint32_t pix, int level, const Frame& F, bool fresh)
 {
     float4* b = reinterpret_cast<float4*>(p.st.frames + ((size_t)level * p.st.frameSlots + pix) * kFrameStride);
     const uint32_t w0 = ((uint32_t)F.stage & 7u) | (((uint32_t)F.frontFacing & 1u) << 3) | (F.mat << 4);
     b[0] = make_float4(__uint_as_float(w0), F.t, F.origin.x, F.origin.y);
-    b[1] = make_float4(F.origin.z, F.dir.x, F.dir.y, F.dir.z);
-    b[2] = make_float4(F.vn.x, F.vn.y, F.vn.z, F.gridRef);
-    b[3] = make_float4(F.shadowColor.x, F.shadowColor.y, F.shadowColor.z, F.reflectDepth);
-    b[4] = make_float4(F.reflectColor.x, F.reflectColor.y, F.reflectColor.z, 0.0f);
+    if(fresh) {
+        b[1] = make_float4(F.origin.z, F.dir.x, F.dir.y, F.dir.z);
+        b[2] = make_float4(F.vn.x, F.vn.y, F.vn.z, F.gridRef);
+    }
+    if(F.stage != G_SHADOW && F.stage != ST_INTERNAL)
+        b[3] = make_float4(F.shadowColor.x, F.shadowColor.y, F.shadowColor.z, F.reflectDepth);
+    if(F.stage == G_REFR) b[4] = make_float4(F.reflectColor.x, F.reflectColor.y, F.reflectColor.z, 0.0f);
 }
 
 __device__ __forceinline__ void init_payload(Payload& P) // raygen.h:92-100
@@ -383,6 +394,9 @@ __device__ __forceinline__ void load_frame(const Ctx& c, uint32_t pix, int level
 {
     const ShadeParams& p = c.p;
     const float4* b = reinterpret_cast<const float4*>(p.st.frames + ((size_t)level * p.st.frameSlots + pix) * kFrameStride);
+    // all five parts in one go (reading only the written ones would wait for
+    // the stage first); parts store_frame skipped hold values that are set
+    // before they are read
     const float4 a0 = b[0], a1 = b[1], a2 = b[2], a3 = b[3], a4 = b[4];
     const uint32_t w0 = __float_as_uint(a0.x);
     F.stage = (int)(w0 & 7u);
@@ -830,6 +844,7 @@ __device__ bool advance_pixel(const Ctx& c0, uint32_t pix, uint32_t flags, F3 O,
     }
     Frame F;
     int step;
+    bool fresh = true; // F comes from the closest-hit stage, not from a resume
     if(h.inst != kNoHit) {
         step = closest_hit(c, P, h.inst, h.prim, h.t, h.u, h.v, O, D, F, out);
     }
@@ -839,7 +854,7 @@ __device__ bool advance_pixel(const Ctx& c0, uint32_t pix, uint32_t flags, F3 O,
     }
     for(;;) {
         if(step == EMIT_FRAME) {
-            if(sp < kMaxFrames) store_frame(p, pix, sp, F);
+            if(sp < kMaxFrames) store_frame(p, pix, sp, F, fresh);
             sp++;
             step = EMIT;
         }
@@ -880,6 +895,7 @@ __device__ bool advance_pixel(const Ctx& c0, uint32_t pix, uint32_t flags, F3 O,
         sp--;
         if(sp < kMaxFrames) load_frame(c, pix, sp, F);
         step = resume(c, F, P, out);
+        fresh = false;
     }
 }
 
