@@ -40,7 +40,7 @@ GATHER_BYTES = 184  # closest-hit gathers: 12 B indices + 96 B vertices + 64 B m
 CONFIG5_CANDIDATES = {"wavefrontPasses": [24, 32, 48, 64], "traceBlocksPerCU": [4, 6, 8],
                       "drainBlocksPerCU": [2, 4, 8]}
 # launch shapes the autotuner tries for the headline pipeline (coordinate descent)
-TUNE_CANDIDATES = {"wavefrontPasses": [6, 8, 10, 12, 14], "traceBlocksPerCU": [4, 6, 8],
+TUNE_CANDIDATES = {"wavefrontPasses": [6, 8, 10, 12, 14], "traceBlocksPerCU": [4, 6, 7, 8],
                    "drainBlocksPerCU": [2, 4, 8], "shadeBlocksPerCU": [4, 8, 12], "frameBands": [1, 2]}
 # One frame at a time the recursion tail is not hidden by the next frame:
 # the frame traced in bands on separate streams (so one band's sparse last

@@ -16,10 +16,10 @@ constexpr int kSpill = 368; // with the 16 LDS entries: the 381-entry worst case
 // Wavefront trace pass: rays [0, count) of queue `cur` -> hit records.
 // Waves fetch 64 rays at a time from a global counter (persistent grid).
 // Waves per SIMD the traversal pass is compiled for: the register budget is
-// 512 / waves VGPRs (6 fits its working set without spilling; more waves
-// hide more of the node-fetch latency).
+// 512 / waves VGPRs (7: 73, which its 71 fit without spilling; more waves
+// hide more of the node-fetch latency; 8 spills).
 #ifndef RGRT_TRACE_WAVES
-#define RGRT_TRACE_WAVES 6
+#define RGRT_TRACE_WAVES 7
 #endif
 
 __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(RGRT_TRACE_WAVES))) void k_trace_queue(TraceScene sc, const float* __restrict__ rays,
