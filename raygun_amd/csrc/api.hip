@@ -497,6 +497,7 @@ void make_event(hipEvent_t& e)
 }
 
 static_assert(sizeof(rgrt_stats) == 80, "rgrt_stats layout (Python binding: records.STATS_FIELDS)");
+static_assert(sizeof(rgrt_tuning) == 24, "rgrt_tuning layout (Python binding: records.TUNING)");
 
 ShadeParams shade_params(const rgrt_ctx* c)
 {

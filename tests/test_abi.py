@@ -41,3 +41,15 @@ def test_create_without_device_reports_status():
     with pytest.raises(rgrt.RgrtError) as e:
         rgrt.Context(16, 16)
     assert e.value.status == 5
+
+
+def test_tuning_record_matches_the_header():
+    """records.TUNING (what the binding passes to rgrt_set_tuning) has the
+    header's rgrt_tuning fields, in order, all 32-bit unsigned."""
+    from raygun_amd.records import TUNING
+    text = open(os.path.join(ROOT, "include", "rgrt.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    body = re.search(r"typedef struct rgrt_tuning\s*\{(.*?)\}\s*rgrt_tuning;", text, flags=re.S).group(1)
+    fields = re.findall(r"uint32_t\s+(\w+)\s*;", body)
+    assert fields == list(TUNING.names)
+    assert TUNING.itemsize == 4 * len(fields)
