@@ -202,14 +202,18 @@ rgrt_status rgrt_synchronize(rgrt_ctx* ctx);
 /* Replaces RenderSystem::setupModelBuffers / updateModelBuffers
  * (raygun/render/render_system.cpp:192-230, 270-330): the caller passes the
  * already concatenated vertex / index / material buffers. Invalidates every
- * BLAS built on earlier geometry. */
+ * BLAS built on earlier geometry. Device memory: the buffers plus a 48-byte
+ * shading record per triangle (numIndices / 3), which BLAS builds fill. */
 rgrt_status rgrt_upload_geometry(rgrt_ctx* ctx, const rgrt_vertex* vertices, uint64_t numVertices, const uint32_t* indices,
                                  uint64_t numIndices);
 /* At most 2^28 materials (a suspended closest-hit frame keeps the material
  * index in 28 bits); more returns RGRT_ERR_INVALID_ARG. */
 rgrt_status rgrt_upload_materials(rgrt_ctx* ctx, const rgrt_material* materials, uint32_t numMaterials);
 /* Rewrites vertex positions/normals of a range in place (deforming meshes);
- * follow with rgrt_refit_blas or rgrt_build_blas. */
+ * follow with rgrt_refit_blas or rgrt_build_blas when positions change.
+ * Normals and material indices take effect in the next frame without one
+ * (the per-triangle shading records of every BLAS over the range are
+ * gathered again on the context's stream). */
 rgrt_status rgrt_update_vertices(rgrt_ctx* ctx, uint64_t firstVertex, const rgrt_vertex* vertices, uint64_t count);
 /* No counterpart in the reference. The same from device memory (vertices
  * animated on the GPU): ordered on the context's stream, returns at once;

@@ -62,6 +62,7 @@ struct SceneData {
     uint64_t ni = 0;
     rgrt_material* mats = nullptr;
     uint32_t nm = 0;
+    ShadeTri* srec = nullptr; // shading record per triangle of the index buffer (ni / 3)
     std::vector<BlasRec> blas;
     DevBlas* dBlasTable = nullptr;
     float4* dBlasRoot = nullptr;
@@ -81,6 +82,7 @@ struct SceneData {
         }
         dev_free(vb);
         dev_free(ib);
+        dev_free(srec);
         dev_free(mats);
         dev_free(dBlasTable);
         dev_free(dBlasRoot);
@@ -314,6 +316,26 @@ void reserve_builder(rgrt_ctx* c, uint32_t n)
 }
 
 // New geometry invalidates every BLAS, so every sharing context's TLAS.
+// Shading records of the triangles of BLAS description d (launched on s).
+void shade_records_of(rgrt_ctx* c, hipStream_t s, const rgrt_blas_desc& d)
+{
+    launch_shade_records(s, c->sd->vb, c->sd->ib, c->sd->nv, c->sd->ni, d.firstVertex, d.indexOffsetBytes / 4, d.numFaces,
+                         c->sd->srec);
+}
+
+// After vertices [first, first + count) changed: the records of every BLAS
+// whose vertex range overlaps them are gathered again (on the context's
+// stream, after the vertex copy), so a record never holds an old normal or
+// material index.
+void refresh_shade_records(rgrt_ctx* c, uint64_t first, uint64_t count)
+{
+    for(const BlasRec& b: c->sd->blas) {
+        if(!b.valid) continue;
+        const uint64_t lo = b.desc.firstVertex, hi = lo + b.desc.maxVertex;
+        if(first < hi && lo < first + count) shade_records_of(c, c->stream, b.desc);
+    }
+}
+
 void scene_drop_tlases(rgrt_ctx* c)
 {
     std::lock_guard<std::mutex> g(c->sd->m);
@@ -506,6 +528,7 @@ ShadeParams shade_params(const rgrt_ctx* c)
     p.ib = c->sd->ib;
     p.mats = c->sd->mats;
     p.inst = c->dShade;
+    p.srec = c->sd->srec;
     p.numInstances = c->numInstances;
     p.numVertices = c->sd->nv;
     p.numIndices = c->sd->ni;
@@ -967,8 +990,11 @@ rgrt_status rgrt_upload_geometry(rgrt_ctx* c, const rgrt_vertex* vertices, uint6
         scene_quiesce(c);
         dev_alloc(c->sd->vb, nv);
         dev_alloc(c->sd->ib, ni);
+        dev_alloc(c->sd->srec, ni / 3);
         if(nv) check(hipMemcpy(c->sd->vb, vertices, nv * sizeof(rgrt_vertex), hipMemcpyHostToDevice));
         if(ni) check(hipMemcpy(c->sd->ib, indices, ni * 4, hipMemcpyHostToDevice));
+        // no triangle has a shading record until a BLAS over it is built
+        check(hipMemset(c->sd->srec, 0xff, std::max<uint64_t>(ni / 3, 1) * sizeof(ShadeTri)));
         c->sd->nv = nv;
         c->sd->ni = ni;
         for(auto& b: c->sd->blas) b.valid = false; // geometry changed: every BLAS must be rebuilt
@@ -984,6 +1010,7 @@ rgrt_status rgrt_update_vertices(rgrt_ctx* c, uint64_t first, const rgrt_vertex*
         scene_quiesce(c);
         if(count)
             check(hipMemcpyAsync(c->sd->vb + first, vertices, count * sizeof(rgrt_vertex), hipMemcpyHostToDevice, c->stream));
+        refresh_shade_records(c, first, count);
         check(hipStreamSynchronize(c->stream));
         return RGRT_OK;
     });
@@ -997,6 +1024,7 @@ rgrt_status rgrt_update_vertices_device(rgrt_ctx* c, uint64_t first, const rgrt_
         if(!count) return RGRT_OK;
         scene_write_begin(c);
         check(hipMemcpyAsync(c->sd->vb + first, vertices, count * sizeof(rgrt_vertex), hipMemcpyDeviceToDevice, c->stream));
+        refresh_shade_records(c, first, count);
         scene_write_end(c);
         return RGRT_OK;
     });
@@ -1050,6 +1078,7 @@ rgrt_status rgrt_build_blas(rgrt_ctx* c, const rgrt_blas_desc* d, rgrt_blas* out
         c->builder.collapse(s, n, b.nodes, b.wide, b.numWide);
         c->builder.quantize(s, std::max<uint32_t>(n, 2) - 1, b.wide, b.numWide, b.wideq);
         launch_tri_gather(s, c->sd->vb, c->sd->ib, d->firstVertex, (uint32_t)io, b.leafPrim, n, b.tris);
+        shade_records_of(c, s, *d);
         DevBlas db{b.wideq, b.tris, n, 0};
         check(hipMemcpyAsync(c->sd->dBlasTable + id, &db, sizeof(DevBlas), hipMemcpyHostToDevice, s));
         check(hipStreamSynchronize(s)); // the reference blocks on a fence (raytracer.cpp:71-73)
@@ -1075,6 +1104,7 @@ rgrt_status rgrt_refit_blas(rgrt_ctx* c, rgrt_blas id)
         c->builder.collapse(s, n, b.nodes, b.wide, b.numWide); // wide topology follows the new areas
         c->builder.quantize(s, std::max<uint32_t>(n, 2) - 1, b.wide, b.numWide, b.wideq);
         launch_tri_gather(s, c->sd->vb, c->sd->ib, b.desc.firstVertex, io, b.leafPrim, n, b.tris);
+        shade_records_of(c, s, b.desc);
         check(hipStreamSynchronize(s));
         return RGRT_OK;
     });
@@ -1096,6 +1126,7 @@ rgrt_status rgrt_rebuild_blas(rgrt_ctx* c, rgrt_blas id)
         c->builder.collapse(s, n, b.nodes, b.wide, b.numWide);
         c->builder.quantize(s, std::max<uint32_t>(n, 2) - 1, b.wide, b.numWide, b.wideq);
         launch_tri_gather(s, c->sd->vb, c->sd->ib, b.desc.firstVertex, io, b.leafPrim, n, b.tris);
+        shade_records_of(c, s, b.desc);
         scene_write_end(c);
         return RGRT_OK;
     });
@@ -1124,6 +1155,7 @@ rgrt_status rgrt_rebuild_blas_noalloc(rgrt_ctx* c, rgrt_blas id)
         bld.collapse(s, n, b.nodes, b.wide, b.numWide);
         bld.quantize(s, std::max<uint32_t>(n, 2) - 1, b.wide, b.numWide, b.wideq);
         launch_tri_gather(s, c->sd->vb, c->sd->ib, b.desc.firstVertex, io, b.leafPrim, n, b.tris);
+        shade_records_of(c, s, b.desc);
         scene_write_end(c);
         return RGRT_OK;
     });

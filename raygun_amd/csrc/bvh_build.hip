@@ -983,6 +983,36 @@ __global__ void k_quantize(const Node4* __restrict__ wide, const uint32_t* __res
 // ---------------------------------------------------------------------------
 // leaf payloads
 // ---------------------------------------------------------------------------
+// Shading records (ShadeTri) of triangles indexOffset / 3 + k, with the
+// gathers clamped to the buffers exactly as the closest-hit stage clamps
+// them, so a record equals what that stage would gather.
+__global__ void k_shade_records(const rgrt_vertex* __restrict__ vb, const uint32_t* __restrict__ ib, uint64_t nv, uint64_t ni,
+                                uint32_t firstVertex, uint32_t indexOffset, uint32_t n, ShadeTri* __restrict__ srec)
+{
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if(k >= n) return;
+    const uint64_t lastIndex = ni ? ni - 1 : 0;
+    const uint64_t lastVertex = nv ? nv - 1 : 0;
+    const uint64_t ib0 = (uint64_t)indexOffset + 3ull * k;
+    const uint32_t i0 = ib[ib0 + 0 < lastIndex ? ib0 + 0 : lastIndex];
+    const uint32_t i1 = ib[ib0 + 1 < lastIndex ? ib0 + 1 : lastIndex];
+    const uint32_t i2 = ib[ib0 + 2 < lastIndex ? ib0 + 2 : lastIndex];
+    const uint64_t a0 = (uint64_t)firstVertex + i0, a1 = (uint64_t)firstVertex + i1, a2 = (uint64_t)firstVertex + i2;
+    const rgrt_vertex& v0 = vb[a0 < lastVertex ? a0 : lastVertex];
+    const rgrt_vertex& v1 = vb[a1 < lastVertex ? a1 : lastVertex];
+    const rgrt_vertex& v2 = vb[a2 < lastVertex ? a2 : lastVertex];
+    ShadeTri r;
+    for(int j = 0; j < 3; ++j) {
+        r.n0[j] = v0.normal[j];
+        r.n1[j] = v1.normal[j];
+        r.n2[j] = v2.normal[j];
+    }
+    r.matIndex = v0.matIndex;
+    r.vertexOffset = firstVertex;
+    r.pad = 0;
+    srec[indexOffset / 3 + k] = r;
+}
+
 __global__ void k_tri_gather(const rgrt_vertex* __restrict__ vb, const uint32_t* __restrict__ ib, uint32_t firstVertex,
                              uint32_t indexOffset, const uint32_t* __restrict__ leafPrim, uint32_t n, Tri* __restrict__ tris)
 {
@@ -1383,6 +1413,13 @@ void launch_tri_boxes(hipStream_t s, const rgrt_vertex* vb, const uint32_t* ib, 
                       uint32_t n, float4* boxLo, float4* boxHi)
 {
     if(n) k_tri_boxes<<<blocks_for(n, 256), 256, 0, s>>>(vb, ib, firstVertex, indexOffset, n, boxLo, boxHi);
+}
+
+void launch_shade_records(hipStream_t s, const rgrt_vertex* vb, const uint32_t* ib, uint64_t nv, uint64_t ni,
+                          uint32_t firstVertex, uint32_t indexOffset, uint32_t n, ShadeTri* srec)
+{
+    if(n && srec && indexOffset % 3 == 0)
+        k_shade_records<<<blocks_for(n, 256), 256, 0, s>>>(vb, ib, nv, ni, firstVertex, indexOffset, n, srec);
 }
 
 void launch_tri_gather(hipStream_t s, const rgrt_vertex* vb, const uint32_t* ib, uint32_t firstVertex, uint32_t indexOffset,

@@ -88,6 +88,24 @@ struct __align__(16) Tri {
 };
 static_assert(sizeof(Tri) == 48, "triangle is 48 bytes");
 
+// Shading record of one triangle of the index buffer (triangle g = indices
+// 3g .. 3g + 2), 48 bytes: the three vertex normals and vertex 0's material
+// index as the closest-hit stage reads them from vertex offset
+// `vertexOffset` (closesthit.rchit:95-110), so a hit reads one record
+// instead of three indices and three vertices. Written by every BLAS build
+// over its index range with its first vertex; kNoShadeRecord marks a
+// triangle no BLAS has written since the geometry was uploaded.
+struct __align__(16) ShadeTri {
+    float n0[3];
+    uint32_t matIndex;
+    float n1[3];
+    uint32_t vertexOffset;
+    float n2[3];
+    uint32_t pad;
+};
+static_assert(sizeof(ShadeTri) == 48, "shading record is 48 bytes");
+constexpr uint32_t kNoShadeRecord = 0xffffffffu;
+
 // TLAS leaf record in Morton order, 64 bytes.
 struct __align__(16) DevInstance {
     float w2o[12];     // world -> object, row-major 3x4

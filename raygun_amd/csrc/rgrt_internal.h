@@ -69,6 +69,10 @@ struct LbvhBuilder {
 
 void launch_tri_boxes(hipStream_t s, const rgrt_vertex* vb, const uint32_t* ib, uint32_t firstVertex, uint32_t indexOffset,
                       uint32_t n, float4* boxLo, float4* boxHi);
+// Shading records of the n triangles from index offset `indexOffset` (a
+// multiple of 3) read with vertex offset firstVertex, into srec[indexOffset / 3 ..].
+void launch_shade_records(hipStream_t s, const rgrt_vertex* vb, const uint32_t* ib, uint64_t nv, uint64_t ni,
+                          uint32_t firstVertex, uint32_t indexOffset, uint32_t n, ShadeTri* srec);
 void launch_tri_gather(hipStream_t s, const rgrt_vertex* vb, const uint32_t* ib, uint32_t firstVertex, uint32_t indexOffset,
                        const uint32_t* leafPrim, uint32_t n, Tri* tris);
 void launch_inst_prep(hipStream_t s, const rgrt_instance* in, uint32_t n, const float4* blasRoot, float4* boxLo,
@@ -160,6 +164,7 @@ struct ShadeParams {
     const uint32_t* ib;
     const rgrt_material* mats;
     const InstanceShade* inst; // by instance position
+    const ShadeTri* srec;      // by triangle of the index buffer (numIndices / 3)
     uint32_t numInstances;
     uint64_t numVertices, numIndices;
     uint32_t numMaterials;

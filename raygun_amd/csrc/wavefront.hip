@@ -415,6 +415,25 @@ __device__ __forceinline__ void load_frame(const Ctx& c, uint32_t pix, int level
     F.baseColor = dp * F.diffuse;
 }
 
+// The shading record of triangle (indexBufferOffset + 3 prim) / 3 when one
+// gathered with this table entry's vertex offset is there (false otherwise:
+// the caller gathers the vertices itself).
+__device__ __forceinline__ bool shade_record(const ShadeParams& p, const rgrt_instance_offsets& table, uint32_t prim, F3& n0,
+                                             F3& n1, F3& n2, uint32_t& matIndex)
+{
+    if(table.indexBufferOffset % 3u != 0u || table.vertexBufferOffset == kNoShadeRecord) return false;
+    const uint64_t g = (uint64_t)(table.indexBufferOffset / 3u) + prim;
+    if(g >= p.numIndices / 3) return false;
+    const float4* q = reinterpret_cast<const float4*>(p.srec + g);
+    const float4 a = q[0], b = q[1], c = q[2];
+    if(__float_as_uint(b.w) != table.vertexBufferOffset) return false;
+    n0 = f3(a.x, a.y, a.z);
+    n1 = f3(b.x, b.y, b.z);
+    n2 = f3(c.x, c.y, c.z);
+    matIndex = __float_as_uint(a.w);
+    return true;
+}
+
 // closesthit.rchit:93-204 up to the first nested trace.
 // Material of a hit (closesthit.rchit:95-110: offset table of the instance's
 // customIndex, vertex 0's matIndex), with the gathers clamped to the buffers.
@@ -422,13 +441,17 @@ __device__ const rgrt_material& hit_material(const ShadeParams& p, uint32_t inst
 {
     const InstanceShade& is = p.inst[min(instPos, p.numInstances - 1)];
     const rgrt_instance_offsets table = p.inst[min(is.customIndex, p.numInstances - 1)].offsets;
-    const uint64_t lastIndex = p.numIndices ? p.numIndices - 1 : 0;
-    const uint64_t lastVertex = p.numVertices ? p.numVertices - 1 : 0;
-    const uint64_t ib0 = (uint64_t)table.indexBufferOffset + 3ull * prim;
-    const uint32_t i0 = p.ib[min(ib0, lastIndex)];
-    const uint64_t vi0 = (uint64_t)table.vertexBufferOffset + i0;
-    const rgrt_vertex& v0 = p.vb[min(vi0, lastVertex)];
-    const uint32_t mi = table.materialBufferOffset + v0.matIndex;
+    uint32_t matIndex;
+    F3 n0, n1, n2;
+    if(!shade_record(p, table, prim, n0, n1, n2, matIndex)) {
+        const uint64_t lastIndex = p.numIndices ? p.numIndices - 1 : 0;
+        const uint64_t lastVertex = p.numVertices ? p.numVertices - 1 : 0;
+        const uint64_t ib0 = (uint64_t)table.indexBufferOffset + 3ull * prim;
+        const uint32_t i0 = p.ib[min(ib0, lastIndex)];
+        const uint64_t vi0 = (uint64_t)table.vertexBufferOffset + i0;
+        matIndex = p.vb[min(vi0, lastVertex)].matIndex;
+    }
+    const uint32_t mi = table.materialBufferOffset + matIndex;
     return p.mats[p.numMaterials ? min(mi, p.numMaterials - 1) : 0u];
 }
 
@@ -442,23 +465,32 @@ __device__ int closest_hit(const Ctx& c, Payload& P, uint32_t instPos, uint32_t 
     const rgrt_instance_offsets table = p.inst[min(is.customIndex, p.numInstances - 1)].offsets; // offset table[customIndex]
 
     const float b0 = (1.0f - u) - v, b1 = u, b2 = v;
-    const uint64_t lastIndex = p.numIndices ? p.numIndices - 1 : 0;
-    const uint64_t lastVertex = p.numVertices ? p.numVertices - 1 : 0;
-    const uint64_t ib0 = (uint64_t)table.indexBufferOffset + 3ull * prim;
-    const uint32_t i0 = p.ib[min(ib0 + 0, lastIndex)];
-    const uint32_t i1 = p.ib[min(ib0 + 1, lastIndex)];
-    const uint32_t i2 = p.ib[min(ib0 + 2, lastIndex)];
-    const rgrt_vertex& v0 = p.vb[min((uint64_t)table.vertexBufferOffset + i0, lastVertex)];
-    const rgrt_vertex& v1 = p.vb[min((uint64_t)table.vertexBufferOffset + i1, lastVertex)];
-    const rgrt_vertex& v2 = p.vb[min((uint64_t)table.vertexBufferOffset + i2, lastVertex)];
-    const uint32_t mi = table.materialBufferOffset + v0.matIndex;
+    // the three vertex normals and vertex 0's material index: from the
+    // triangle's shading record, or gathered through the index buffer
+    F3 n0, n1, n2;
+    uint32_t matIndex;
+    if(!shade_record(p, table, prim, n0, n1, n2, matIndex)) {
+        const uint64_t lastIndex = p.numIndices ? p.numIndices - 1 : 0;
+        const uint64_t lastVertex = p.numVertices ? p.numVertices - 1 : 0;
+        const uint64_t ib0 = (uint64_t)table.indexBufferOffset + 3ull * prim;
+        const uint32_t i0 = p.ib[min(ib0 + 0, lastIndex)];
+        const uint32_t i1 = p.ib[min(ib0 + 1, lastIndex)];
+        const uint32_t i2 = p.ib[min(ib0 + 2, lastIndex)];
+        const rgrt_vertex& v0 = p.vb[min((uint64_t)table.vertexBufferOffset + i0, lastVertex)];
+        const rgrt_vertex& v1 = p.vb[min((uint64_t)table.vertexBufferOffset + i1, lastVertex)];
+        const rgrt_vertex& v2 = p.vb[min((uint64_t)table.vertexBufferOffset + i2, lastVertex)];
+        n0 = f3(v0.normal[0], v0.normal[1], v0.normal[2]);
+        n1 = f3(v1.normal[0], v1.normal[1], v1.normal[2]);
+        n2 = f3(v2.normal[0], v2.normal[1], v2.normal[2]);
+        matIndex = v0.matIndex;
+    }
+    const uint32_t mi = table.materialBufferOffset + matIndex;
     F.mat = p.numMaterials ? min(mi, p.numMaterials - 1) : 0u;
     F.frontFacing = 0; // set below for a generic hit
 
     const float tmin = 0.01f, tmax = 1000.0f;
     const F3 origin = O + D * hitT;
-    const F3 vn = (f3(v0.normal[0], v0.normal[1], v0.normal[2]) * b0 + f3(v1.normal[0], v1.normal[1], v1.normal[2]) * b1) +
-                  f3(v2.normal[0], v2.normal[1], v2.normal[2]) * b2;
+    const F3 vn = (n0 * b0 + n1 * b1) + n2 * b2;
     const float* w = is.o2w;
     const F3 wn = f3((w[0] * vn.x + w[1] * vn.y) + w[2] * vn.z, (w[4] * vn.x + w[5] * vn.y) + w[6] * vn.z,
                      (w[8] * vn.x + w[9] * vn.y) + w[10] * vn.z);

@@ -149,3 +149,73 @@ def test_box_test_keeps_hits_where_child_boxes_meet_far_from_the_node_corner():
         assert np.array_equal(h["t"].view(np.uint32), t_b.view(np.uint32))
     finally:
         ctx.close()
+
+
+def _fan(n=6, seed=3):
+    """n triangles (n / 2 quads side by side) tiling [-0.6, 0.6] x [-0.5,
+    0.5], each with vertices of its own, distinct vertex normals and vertex 0
+    material indices 0 / 1 in turn, then one more triangle (three more
+    vertices and indices) that no BLAS covers, so that shifted offset tables
+    still address the buffers."""
+    rng = np.random.default_rng(seed)
+    v = np.zeros(3 * n + 3, VERTEX)
+    q = n // 2
+    for k in range(n):
+        x0 = -0.6 + 1.2 * (k // 2) / q
+        x1 = -0.6 + 1.2 * (k // 2 + 1) / q
+        tri = [(x0, -0.5, 0.0), (x1, -0.5, 0.0), (x1, 0.5, 0.0)] if k % 2 == 0 else \
+              [(x0, -0.5, 0.0), (x1, 0.5, 0.0), (x0, 0.5, 0.0)]
+        v["position"][3 * k:3 * k + 3] = tri
+        nrm = rng.normal(size=(3, 3)) * 0.3 + (0, 0, 1)
+        v["normal"][3 * k:3 * k + 3] = nrm / np.linalg.norm(nrm, axis=1)[:, None]
+        v["matIndex"][3 * k] = k % 2
+    v["position"][3 * n:] = [(5.0, 5.0, 5.0), (5.1, 5.0, 5.0), (5.0, 5.1, 5.0)]
+    v["normal"][3 * n:] = (0.0, 0.0, 1.0)
+    return v, np.arange(3 * n + 3, dtype=np.uint32)
+
+
+def test_shading_records_fall_back_and_follow_vertex_updates():
+    """Hits read a per-triangle shading record only when the instance's offset
+    table entry matches the vertex offset it was gathered with; otherwise
+    (another vertex offset, an index offset that is not a whole triangle)
+    the vertices are gathered as the closest-hit stage describes. Normals and
+    material indices changed after the build, without a refit, from the host
+    and from device memory, are seen by the next frame."""
+    print("[case] edge shading records", flush=True)
+    from raygun_amd.hiptime import DeviceBuffer
+    v, i = _fan()
+    red, green, blue = default_material(), default_material(), default_material()
+    red["diffuse"] = (0.9, 0.1, 0.1)
+    green["diffuse"] = (0.1, 0.8, 0.2)
+    blue["diffuse"] = (0.1, 0.2, 0.9)
+    mats = np.array([red, green, blue])
+    move = lambda x: np.array([[1, 0, 0, x], [0, 1, 0, 0], [0, 0, 1, 0]], np.float32)
+    instances = [_instance(0, move(-1.3), custom=0, offsets=(0, 0, 0)),   # the record's vertex offset
+                 _instance(0, move(0.0), custom=1, offsets=(3, 0, 1)),    # another vertex offset: gathered
+                 _instance(0, move(1.3), custom=2, offsets=(0, 1, 0))]    # index offset 1: gathered
+    ctx, osc = _both(v, i, mats, [(0, 0, len(i) // 3 - 1)], instances, W=48, H=24)
+    buf = None
+    try:
+        gp = _render_both(ctx, osc, W=48, H=24)
+        assert set(np.unique(gp["instance"])) >= {0, 1, 2}
+        # normals and materials rewritten from the host, positions kept
+        v2 = v.copy()
+        v2["normal"] = v["normal"][::-1]
+        v2["matIndex"] = 1 - v["matIndex"]
+        ctx.update_vertices(0, v2)
+        osc.update_vertices(0, v2)
+        _render_both(ctx, osc, W=48, H=24)
+        # and from device memory
+        v3 = v2.copy()
+        v3["normal"][:9] = v["normal"][:9]
+        v3["matIndex"][:9] = v["matIndex"][:9]
+        buf = DeviceBuffer(9 * 32)
+        buf.upload(v3[:9])
+        ctx.update_vertices_device(0, buf.address, 9)
+        ctx.synchronize()
+        osc.update_vertices(0, v3[:9])
+        _render_both(ctx, osc, W=48, H=24)
+    finally:
+        ctx.close()
+        if buf is not None:
+            buf.close()
