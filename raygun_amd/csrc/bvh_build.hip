@@ -772,26 +772,29 @@ __global__ void k_wide_candidates(const Node* __restrict__ nodes, int numInterna
     cand[r] = w;
 }
 
-constexpr int kCollapseItems = 4; // frontier entries per thread and chunk (consecutive)
+constexpr int kCollapseItems = 4; // frontier entries per thread and chunk (consecutive; 8 and 16 measured slower)
 
+// Breadth-first numbering of the wide tree (one workgroup): `order` receives
+// the binary node of every wide node in wide-node order (the frontiers of
+// all levels one after another, level L's at [base, base + count)), and
+// each wide node's child refs, renumbered, are written to its slot in `out`
+// (16 bytes; k_wide_boxes copies the rest of the node in parallel). Internal
+// children in entry and slot order get consecutive next-level numbers.
 __global__ __launch_bounds__(kCollapseThreads) void k_collapse_numbered(const Node4* __restrict__ cand, int numInternal,
-                                                                        Node4* __restrict__ out, int32_t* frontA,
-                                                                        int32_t* frontB, uint32_t* numWide)
+                                                                        Node4* __restrict__ out, int32_t* order,
+                                                                        uint32_t* numWide)
 {
     __shared__ int32_t waveTot[kCollapseThreads / 64];
     constexpr int kWavesPerBlock = kCollapseThreads / 64;
     constexpr int kChunk = kCollapseThreads * kCollapseItems;
-    // a chunk's candidates and renumbered child refs, for the coalesced copy
-    __shared__ int32_t chunkNode[kChunk];
-    __shared__ u32x4 chunkRefs[kChunk];
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
-    int32_t* front = frontA;
-    int32_t* next = frontB;
-    if(tid == 0) front[0] = 0;
+    if(tid == 0) order[0] = 0;
     __syncthreads();
     int fcount = 1, base = 0;
     for(int level = 0; level < 128 && fcount > 0; ++level) {
+        const int32_t* front = order + base;
+        int32_t* next = order + base + fcount;
         int nextCount = 0;
         for(int chunk = 0; chunk < fcount; chunk += kChunk) {
             // thread tid takes frontier entries k0 .. k0 + kCollapseItems - 1 (in order)
@@ -830,7 +833,6 @@ __global__ __launch_bounds__(kCollapseThreads) void k_collapse_numbered(const No
                 excl += wv < wave ? t : 0;
                 total += t;
             }
-            // internal children in entry and slot order get consecutive next-level numbers
             int at = nextCount + excl;
 #pragma unroll
             for(int j = 0; j < kCollapseItems; ++j) {
@@ -842,33 +844,35 @@ __global__ __launch_bounds__(kCollapseThreads) void k_collapse_numbered(const No
                 if((int)c1 >= 0) next[i1] = (int32_t)c1;
                 if((int)c2 >= 0) next[i2] = (int32_t)c2;
                 if((int)c3 >= 0) next[i3] = (int32_t)c3;
-                chunkNode[tid * kCollapseItems + j] = r[j];
-                chunkRefs[tid * kCollapseItems + j] = u32x4{(int)c0 >= 0 ? (uint32_t)(base + fcount + i0) : c0,
-                                                            (int)c1 >= 0 ? (uint32_t)(base + fcount + i1) : c1,
-                                                            (int)c2 >= 0 ? (uint32_t)(base + fcount + i2) : c2,
-                                                            (int)c3 >= 0 ? (uint32_t)(base + fcount + i3) : c3};
-            }
-            __syncthreads(); // chunkNode / chunkRefs complete
-            // copy the chunk's nodes to their places: consecutive lanes move
-            // consecutive 16-byte pieces (8 per node), so a wave writes 1 KB at once
-            const int items = min(kChunk, fcount - chunk);
-            for(int piece = tid; piece < items * 8; piece += kCollapseThreads) {
-                const int it = piece >> 3, q = piece & 7;
-                const u32x4 v = q == 6 ? chunkRefs[it] : ld_global16(reinterpret_cast<const u32x4*>(cand + chunkNode[it]) + q);
-                reinterpret_cast<u32x4*>(out + base + chunk + it)[q] = v;
+                reinterpret_cast<u32x4*>(out + base + k0 + j)[6] =
+                    u32x4{(int)c0 >= 0 ? (uint32_t)(base + fcount + i0) : c0, (int)c1 >= 0 ? (uint32_t)(base + fcount + i1) : c1,
+                          (int)c2 >= 0 ? (uint32_t)(base + fcount + i2) : c2, (int)c3 >= 0 ? (uint32_t)(base + fcount + i3) : c3};
             }
             nextCount += total;
-            __syncthreads(); // waveTot[], the chunk arrays and next[] settle before the next chunk / level
+            __syncthreads(); // waveTot[] and next[] settle before the next chunk / level
         }
         base += fcount;
         fcount = nextCount;
-        int32_t* t = front;
-        front = next;
-        next = t;
         __threadfence_block();
         __syncthreads();
     }
     if(tid == 0) *numWide = (uint32_t)base;
+}
+
+// The rest of every wide node (the child boxes and the padding, seven of its
+// eight 16-byte pieces) from its candidate, in parallel: piece q of wide node
+// i comes from candidate order[i].
+__global__ void k_wide_boxes(const Node4* __restrict__ cand, const int32_t* __restrict__ order,
+                             const uint32_t* __restrict__ numWide, int capacity, Node4* __restrict__ out)
+{
+    const long piece = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long i = piece >> 3;
+    const int q = (int)(piece & 7);
+    const long nw = min((long)*numWide, (long)capacity);
+    if(i >= nw || q == 6) return;
+    const int32_t r = order[i];
+    if(r < 0 || r >= capacity) return;
+    reinterpret_cast<u32x4*>(out + i)[q] = ld_global16(reinterpret_cast<const u32x4*>(cand + r) + q);
 }
 
 // ---------------------------------------------------------------------------
@@ -1391,12 +1395,13 @@ __global__ void k_leaf_parents(const Node* __restrict__ nodes, uint32_t numInter
 
 void LbvhBuilder::collapse(hipStream_t s, uint32_t n, const Node* nodes, Node4* wide, uint32_t* numWide)
 {
-    // the sort's second key/value buffers are free after the build: they
-    // hold the two breadth-first frontiers (at most n - 1 entries each)
+    // the sort's second key buffer is free after the build: it holds the
+    // breadth-first order of the wide nodes (at most n - 1 entries)
     const int numInternal = (int)std::max<uint32_t>(n, 2) - 1;
     k_wide_candidates<<<blocks_for(numInternal, 256), 256, 0, s>>>(nodes, numInternal, wideTmp);
-    k_collapse_numbered<<<1, kCollapseThreads, 0, s>>>(wideTmp, numInternal, wide, reinterpret_cast<int32_t*>(keysB),
-                                                       reinterpret_cast<int32_t*>(valsB), numWide);
+    k_collapse_numbered<<<1, kCollapseThreads, 0, s>>>(wideTmp, numInternal, wide, reinterpret_cast<int32_t*>(keysB), numWide);
+    k_wide_boxes<<<blocks_for(8u * (uint32_t)numInternal, 256), 256, 0, s>>>(wideTmp, reinterpret_cast<const int32_t*>(keysB),
+                                                                             numWide, numInternal, wide);
 }
 
 void LbvhBuilder::quantize(hipStream_t s, uint32_t capacity, const Node4* wide, const uint32_t* numWide, Node4q* out)
