@@ -648,20 +648,20 @@ __device__ __forceinline__ int wide_children(const Src& src, int r, WideEntry S[
     return ns;
 }
 
-// One workgroup of kCollapseThreads walks the binary tree breadth-first. A
+// One workgroup of kThreads walks the binary tree breadth-first. A
 // wide node starts from its binary node's two children and opens the
 // internal child of largest surface area (the first on equal areas) until it
 // has four children or only leaves; its internal children become the next
 // level's wide nodes, numbered in order (a block-wide prefix sum over the
 // per-node counts: a shuffle scan in each wave, then the wave totals). The
 // frontiers hold at most n - 1 entries each; `waveTot` is
-// kCollapseThreads / 64 ints of LDS. With outq set, every wide node is also
+// kThreads / 64 ints of LDS. With outq set, every wide node is also
 // written compressed. Returns the wide node count (in every thread).
-template <class Src>
+template <int kThreads, class Src>
 __device__ int collapse_bfs(const Src src, Node4* __restrict__ out, Node4q* __restrict__ outq, int32_t* frontA,
                             int32_t* frontB, int32_t* waveTot)
 {
-    constexpr int kWavesPerBlock = kCollapseThreads / 64;
+    constexpr int kWavesPerBlock = kThreads / 64;
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
     int32_t* front = frontA;
@@ -671,7 +671,7 @@ __device__ int collapse_bfs(const Src src, Node4* __restrict__ out, Node4q* __re
     int fcount = 1, base = 0;
     for(int level = 0; level < 128 && fcount > 0; ++level) {
         int nextCount = 0;
-        for(int chunk = 0; chunk < fcount; chunk += kCollapseThreads) {
+        for(int chunk = 0; chunk < fcount; chunk += kThreads) {
             const int k = chunk + tid;
             const bool valid = k < fcount;
             WideEntry S[4];
@@ -1073,7 +1073,11 @@ __global__ void k_inst_gather(const float* __restrict__ w2o, const rgrt_instance
 // produces; the refit combines child boxes in the same order as
 // k_refit_chunk / k_refit_top (fminf/fmaxf of slot 0 and slot 1).
 // The trees are therefore identical to the multi-kernel path's.
+// Instantiated for kMax = 256 and 1024 instances (one thread per instance):
+// the smaller form holds 28 KB of LDS instead of 116 KB, so with a frame's
+// other kernels resident it finds a CU sooner.
 constexpr int kFusedTlasMax = kCollapseThreads;
+constexpr int kFusedTlasSmall = 256;
 
 // Binary nodes held in LDS: box[node][12] in Node's float order, child refs.
 struct LdsNodes {
@@ -1093,20 +1097,21 @@ struct LdsNodes {
     }
 };
 
-__global__ __launch_bounds__(kCollapseThreads) void k_tlas_fused(
+template <int kMax>
+__global__ __launch_bounds__(kMax) void k_tlas_fused(
     const rgrt_instance* __restrict__ in, int n, const float4* __restrict__ blasRoot, float* __restrict__ w2oOut,
     InstanceShade* __restrict__ shade, Node* __restrict__ nodes, uint32_t* __restrict__ leafOut, float4* __restrict__ rootBox,
     Node4* __restrict__ wide, Node4q* __restrict__ wideq, uint32_t* __restrict__ numWide, DevInstance* __restrict__ instOut,
     int refit)
 {
-    __shared__ float ilo[3][kFusedTlasMax], ihi[3][kFusedTlasMax]; // instance world boxes
-    __shared__ uint32_t key0[kFusedTlasMax];                         // by instance
-    __shared__ uint32_t keys[kFusedTlasMax];                         // sorted
-    __shared__ uint32_t order[kFusedTlasMax];                        // sorted position -> instance
-    __shared__ int32_t child0[kFusedTlasMax], child1[kFusedTlasMax];
-    __shared__ uint32_t parent[kFusedTlasMax], leafParent[kFusedTlasMax], arrive[kFusedTlasMax];
-    __shared__ float box[12 * (kFusedTlasMax - 1)];
-    __shared__ int32_t frontA[kFusedTlasMax], frontB[kFusedTlasMax], waveTot[kCollapseThreads / 64];
+    __shared__ float ilo[3][kMax], ihi[3][kMax]; // instance world boxes
+    __shared__ uint32_t key0[kMax];              // by instance
+    __shared__ uint32_t keys[kMax];              // sorted
+    __shared__ uint32_t order[kMax];             // sorted position -> instance
+    __shared__ int32_t child0[kMax], child1[kMax];
+    __shared__ uint32_t parent[kMax], leafParent[kMax], arrive[kMax];
+    __shared__ float box[12 * (kMax - 1)];
+    __shared__ int32_t frontA[kMax], frontB[kMax], waveTot[kMax / 64];
     __shared__ uint32_t sb[6];
 
     const int tid = threadIdx.x;
@@ -1257,7 +1262,7 @@ __global__ __launch_bounds__(kCollapseThreads) void k_tlas_fused(
     if(mine) leafOut[tid] = order[tid];
 
     // wide + compressed nodes
-    const int nw = collapse_bfs(LdsNodes{box, child0, child1}, wide, wideq, frontA, frontB, waveTot);
+    const int nw = collapse_bfs<kMax>(LdsNodes{box, child0, child1}, wide, wideq, frontA, frontB, waveTot);
     if(tid == 0) *numWide = (uint32_t)nw;
 }
 
@@ -1450,7 +1455,12 @@ bool launch_tlas_fused(hipStream_t s, const rgrt_instance* in, uint32_t n, const
                        uint32_t* numWide, DevInstance* inst, bool refit)
 {
     if(n < 2 || n > (uint32_t)kFusedTlasMax) return false;
-    k_tlas_fused<<<1, kCollapseThreads, 0, s>>>(in, (int)n, blasRoot, w2o, shade, nodes, leaf, rootBox, wide, wideq, numWide,
+    if(n <= (uint32_t)kFusedTlasSmall) {
+        k_tlas_fused<kFusedTlasSmall><<<1, kFusedTlasSmall, 0, s>>>(in, (int)n, blasRoot, w2o, shade, nodes, leaf, rootBox, wide,
+                                                                    wideq, numWide, inst, refit ? 1 : 0);
+        return true;
+    }
+    k_tlas_fused<kFusedTlasMax><<<1, kFusedTlasMax, 0, s>>>(in, (int)n, blasRoot, w2o, shade, nodes, leaf, rootBox, wide, wideq, numWide,
                                                 inst, refit ? 1 : 0);
     return true;
 }
