@@ -993,8 +993,11 @@ rgrt_status rgrt_upload_geometry(rgrt_ctx* c, const rgrt_vertex* vertices, uint6
         dev_alloc(c->sd->srec, ni / 3);
         if(nv) check(hipMemcpy(c->sd->vb, vertices, nv * sizeof(rgrt_vertex), hipMemcpyHostToDevice));
         if(ni) check(hipMemcpy(c->sd->ib, indices, ni * 4, hipMemcpyHostToDevice));
-        // no triangle has a shading record until a BLAS over it is built
-        check(hipMemset(c->sd->srec, 0xff, std::max<uint64_t>(ni / 3, 1) * sizeof(ShadeTri)));
+        // no triangle has a shading record until a BLAS over it is built;
+        // complete before returning, so any context's later builds (on
+        // their own non-blocking streams) write after it
+        check(hipMemsetAsync(c->sd->srec, 0xff, std::max<uint64_t>(ni / 3, 1) * sizeof(ShadeTri), c->stream));
+        check(hipStreamSynchronize(c->stream));
         c->sd->nv = nv;
         c->sd->ni = ni;
         for(auto& b: c->sd->blas) b.valid = false; // geometry changed: every BLAS must be rebuilt
@@ -1620,7 +1623,8 @@ rgrt_status rgrt_batch_create(rgrt_ctx* const* ctxs, uint32_t n, rgrt_batch** ou
                 check(hipEventCreateWithFlags(&b->slotFree[k], hipEventDisableTiming));
             }
             check(hipMalloc((void**)&b->dCounters, 2 * sizeof(uint32_t)));
-            check(hipMemset(b->dCounters, 0, 2 * sizeof(uint32_t)));
+            check(hipMemsetAsync(b->dCounters, 0, 2 * sizeof(uint32_t), b->stream));
+            check(hipStreamSynchronize(b->stream));
         }
     }
     catch(...) {

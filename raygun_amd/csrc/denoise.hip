@@ -569,6 +569,7 @@ Denoiser::Denoiser(uint32_t seed)
     check(hipMalloc(&wBwd, bfOff * 2));
     check(hipMemset(adamM, 0, numParams * 4));
     check(hipMemset(adamV, 0, numParams * 4));
+    check(hipDeviceSynchronize()); // the fills are done before any stream's work reads them
     // He initialisation of the weights over the real fan-in (11 input
     // channels in the first layer), zero biases; padded output channels of
     // the last layer stay zero (their gradient is zero too).
