@@ -385,7 +385,11 @@ rgrt_status rgrt_denoise(rgrt_denoiser* den, float fallbackA, float fallbackB, u
  * executable graph updated in place (hipGraphExecUpdate), so frame
  * constants, instances and the render size may change between frames. In
  * graph form the pass count is fixed (min(wavefrontPasses, recursion bound))
- * and the drain always runs; the images equal the eager form's. The staging
+ * and the drain always runs; the images equal the eager form's. With
+ * rgrt_tuning.frameBands > 1 the graph form is several graphs instead: the
+ * instance upload and TLAS, then each band's trace on a stream of its own,
+ * then (after all bands) the post passes, all ordered after the context
+ * stream's earlier work and before its later work; same images. The staging
  * of a frame's instances waits until the previous frame's upload has run
  * (one frame in flight, like the reference's per-frame fence).
  * Asynchronous on the context stream. */
