@@ -36,11 +36,13 @@ def _frames(setup, k):
     return out
 
 
-@pytest.mark.parametrize("graph", [False, True])
+@pytest.mark.parametrize("graph,bands", [(False, 1), (True, 1), (True, 2)])
 @pytest.mark.parametrize("which", ["config1", "config2-small"])
-def test_alternating_shared_frames_equal_single_context(which, graph):
+def test_alternating_shared_frames_equal_single_context(which, graph, bands):
+    """bands > 1: the pair's graph frames are traced in bands on streams of
+    their own while the other context's frame runs (frameBands)."""
     setup = procgen.config1(320, 180) if which == "config1" else small_config2(320, 180, tris_per_mesh=5_000)
-    print(f"[case] shared-frames {which} graph={graph}", flush=True)
+    print(f"[case] shared-frames {which} graph={graph} bands={bands}", flush=True)
     # reference: two independent contexts, each frame read back before the
     # next starts (each context builds its TLAS on its first frame, then refits)
     own = [gpu_context(setup)[0], gpu_context(setup)[0]]
@@ -57,6 +59,8 @@ def test_alternating_shared_frames_equal_single_context(which, graph):
     b = a.shared()
     try:
         pair = (a, b)
+        for c in pair:
+            c.set_tuning(frameBands=bands)
         for f in range(0, len(frames), 2):
             # frame f on a and frame f + 1 on b are both queued before either is read
             for k in range(2):
