@@ -1,7 +1,7 @@
 """GPU parity of the TLAS build over instance counts on both sides of the
-one-workgroup build (2..1024 instances) and of the multi-kernel LBVH above
-it: binary nodes, leaf order, wide and compressed nodes identical to the
-oracle's, and closest hits identical on seeded rays. Every seventh instance
+one-workgroup builds (2..256 instances in 256 threads, 257..1024 in 1024)
+and of the multi-kernel LBVH above it: binary nodes, leaf order, wide and
+compressed nodes identical to the oracle's, and closest hits identical on seeded rays. Every seventh instance
 repeats an earlier position (equal Morton codes, so the sort's tie order
 matters) and the instances use two BLASes, random rotations, scales and
 customIndex permutations.
@@ -72,7 +72,7 @@ def scene():
     ctx.close()
 
 
-@pytest.mark.parametrize("n", [2, 3, 64, 257, 1024, 1025, 3000])
+@pytest.mark.parametrize("n", [2, 3, 64, 256, 257, 1024, 1025, 3000])
 def test_tlas_matches_oracle(scene, n):
     osc, ctx = scene
     print(f"[case] tlas n={n}", flush=True)
@@ -107,7 +107,7 @@ def _moved(inst, seed):
     return out
 
 
-@pytest.mark.parametrize("n", [1, 2, 64, 1024, 1500])
+@pytest.mark.parametrize("n", [1, 2, 64, 256, 257, 1024, 1500])
 def test_tlas_refit_matches_oracle(scene, n):
     """Build, move every instance, refit on both sides: the same binary, wide
     and compressed trees (hierarchy and leaf order of the build kept) and the
