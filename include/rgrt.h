@@ -527,11 +527,14 @@ typedef struct rgrt_tuning {
     uint32_t hostLag;           /* wavefront passes the GPU may run ahead of the host's check
                                    for an empty queue (1..16); the passes queued after the
                                    last ray are empty launches                               */
-    uint32_t frameBands;        /* graph frames (RGRT_FRAME_GRAPH): the frame split into this
-                                   many strips of whole 8x8 tiles, each traced on a stream of
-                                   its own, side by side (1..4; 0 = 1); the images are those
-                                   of the frame in one piece. The strips need hardware queues
-                                   of their own (GPU_MAX_HW_QUEUES, 4 by default) to overlap  */
+    uint32_t frameBands;        /* the frame split into this many strips of whole 8x8 tiles,
+                                   each traced on a stream of its own, side by side (1..4;
+                                   0 = 1), in graph frames and in eager traces (these then
+                                   run every pass, with no host check for an empty queue;
+                                   with RGRT_CFG_TIMING they stay in one piece); the images
+                                   are those of the frame in one piece. The strips need
+                                   hardware queues of their own (GPU_MAX_HW_QUEUES, 4 by
+                                   default) to overlap                                        */
 } rgrt_tuning;
 
 /* Sets / reads the launch shape (defaults tuned for MI355X). Values outside

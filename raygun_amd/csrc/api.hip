@@ -594,6 +594,9 @@ uint32_t chain_iters(const rgrt_ctx* c)
     return max_sequential_rays(c->ubo.maxRecursions) * perSlot + 2;
 }
 
+void ensure_band_streams(rgrt_ctx* c);
+void enqueue_band(rgrt_ctx* c, uint32_t b, uint32_t bands, hipStream_t s);
+
 rgrt_status do_trace(rgrt_ctx* c)
 {
     if(!c->haveTlas) return fail(c, RGRT_ERR_NOT_READY, "rgrt_trace: no TLAS built");
@@ -612,6 +615,30 @@ rgrt_status do_trace(rgrt_ctx* c)
         hipEvent_t e;
         check(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         c->iterEvents.push_back(e);
+    }
+    if(c->bands > 1 && !timing) {
+        // in bands (as the graph form): band 0 on the context stream, the
+        // others on theirs after its work so far, joined back before the
+        // sample records are finalized; no host waits (every pass runs)
+        ensure_band_streams(c);
+        c->passStride = maxIters + 2;
+        ensure_host_counts(c, c->passStride * c->bands);
+        check(hipEventRecord(c->bandFork, s));
+        for(uint32_t b = 1; b < c->bands; ++b) {
+            check(hipStreamWaitEvent(c->bandStream[b], c->bandFork, 0));
+            enqueue_band(c, b, c->bands, c->bandStream[b]);
+            check(hipEventRecord(c->bandJoin[b], c->bandStream[b]));
+        }
+        enqueue_band(c, 0, c->bands, s);
+        for(uint32_t b = 1; b < c->bands; ++b) check(hipStreamWaitEvent(s, c->bandJoin[b], 0));
+        launch_finalize_samples(s, p);
+        const uint32_t passes = std::min<uint32_t>(c->tuning.wavefrontPasses, maxIters);
+        c->lastTraceLaunches = passes;
+        c->lastDrained = true;
+        c->lastWaves = passes;
+        c->lastBands = c->bands;
+        c->timedRt = false;
+        return RGRT_OK;
     }
     if(timing) {
         make_event(c->evRtStart);
@@ -743,6 +770,17 @@ WavefrontState band_state(const WavefrontState& st, uint32_t b, uint32_t bands)
     w.hits = st.hits + (size_t)kHitWords * q0;
     w.counters = st.counters + (size_t)kCounterBlock * b;
     return w;
+}
+
+// Streams and fork/join events of bands 1.. (created once, default priority).
+void ensure_band_streams(rgrt_ctx* c)
+{
+    if(c->bands < 2) return;
+    if(!c->bandFork) check(hipEventCreateWithFlags(&c->bandFork, hipEventDisableTiming));
+    for(uint32_t b = 1; b < c->bands; ++b) {
+        if(!c->bandStream[b]) check(hipStreamCreateWithFlags(&c->bandStream[b], hipStreamNonBlocking));
+        if(!c->bandJoin[b]) check(hipEventCreateWithFlags(&c->bandJoin[b], hipEventDisableTiming));
+    }
 }
 
 // The trace of band b as a fixed launch sequence on stream s (capturable):
@@ -1393,13 +1431,7 @@ rgrt_status frame_prepare(rgrt_ctx* c, const rgrt_ubo* ubo, const rgrt_instance*
     prepare_sample_groups(c);
     c->passStride = chain_iters(c) + 2;
     ensure_host_counts(c, c->passStride * c->bands);
-    if(c->bands > 1) {
-        if(!c->bandFork) check(hipEventCreateWithFlags(&c->bandFork, hipEventDisableTiming));
-        for(uint32_t b = 1; b < c->bands; ++b) {
-            if(!c->bandStream[b]) check(hipStreamCreateWithFlags(&c->bandStream[b], hipStreamNonBlocking));
-            if(!c->bandJoin[b]) check(hipEventCreateWithFlags(&c->bandJoin[b], hipEventDisableTiming));
-        }
-    }
+    ensure_band_streams(c);
     return RGRT_OK;
 }
 

@@ -152,13 +152,17 @@ def test_frames_in_bands_equal_the_frame_in_one_piece(which, bands, spp):
             for key in ("rays", "skyLookups", "zeroDirRays"):
                 assert sb[key] == s1[key], (f, key)
             assert np.array_equal(banded.pass_counts(), one.pass_counts()), f
-        # an eager frame (always in one piece) in between, then bands again
+        # an eager frame in between (in bands on the banded context too), then
+        # graph frames in bands again
         ubo, inst = _frames(setup, 1)[0]
         for c in (one, banded):
             c.set_frame(ubo)
             c.render(True)
         assert np.array_equal(banded.read_image(rgrt.IMAGE_FINAL), one.read_image(rgrt.IMAGE_FINAL))
         assert banded.stats()["rays"] == one.stats()["rays"]
+        pb, po = banded.pass_counts(), one.pass_counts()
+        # the one-piece eager trace stops once its queue is empty; the bands run every pass
+        assert len(po) <= len(pb) and np.array_equal(pb[:len(po)], po) and pb[len(po):].sum() == 0
         for c in (one, banded):
             c.frame(ubo, inst, fxaa=True, refit=True, graph=True)
         assert np.array_equal(banded.read_image(rgrt.IMAGE_FINAL), one.read_image(rgrt.IMAGE_FINAL))
