@@ -36,11 +36,11 @@ def _frames(setup, k):
     return out
 
 
-@pytest.mark.parametrize("graph,bands", [(False, 1), (True, 1), (True, 2)])
+@pytest.mark.parametrize("graph,bands", [(False, 1), (True, 1), (True, 2), (False, 3)])
 @pytest.mark.parametrize("which", ["config1", "config2-small"])
 def test_alternating_shared_frames_equal_single_context(which, graph, bands):
-    """bands > 1: the pair's graph frames are traced in bands on streams of
-    their own while the other context's frame runs (frameBands)."""
+    """bands > 1: the pair's frames (graph or eager) are traced in bands on
+    streams of their own while the other context's frame runs (frameBands)."""
     setup = procgen.config1(320, 180) if which == "config1" else small_config2(320, 180, tris_per_mesh=5_000)
     print(f"[case] shared-frames {which} graph={graph} bands={bands}", flush=True)
     # reference: two independent contexts, each frame read back before the
