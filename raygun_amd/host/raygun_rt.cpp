@@ -580,6 +580,29 @@ RenderSystem::RenderSystem(uint32_t width, uint32_t height, uint32_t flags, int3
     }
     m_raytracer = std::make_unique<Raytracer>(m_ctx);
     m_last = m_ctx;
+    try {
+        ok(m_ctx, rgrt_get_tuning(m_ctx, &m_defaultTuning), "RenderSystem");
+        applyTuning();
+    }
+    catch(...) {
+        m_raytracer.reset();
+        rgrt_destroy(m_ctx);
+        throw;
+    }
+}
+
+void RenderSystem::applyTuning()
+{
+    rgrt_tuning t = m_defaultTuning;
+    if(m_extra.empty()) {
+        t.frameBands = 3;
+        t.wavefrontPasses = 5;
+        t.traceBlocksPerCU = 4;
+        t.shadeBlocksPerCU = 8;
+        t.drainBlocksPerCU = 2;
+    }
+    ok(m_ctx, rgrt_set_tuning(m_ctx, &t), "RenderSystem tuning");
+    for(rgrt_ctx* c: m_extra) ok(c, rgrt_set_tuning(c, &t), "RenderSystem tuning");
 }
 
 RenderSystem::~RenderSystem()
@@ -605,6 +628,7 @@ void RenderSystem::setFramesInFlight(uint32_t n)
         m_extra.push_back(c);
         m_extraRt.push_back(std::make_unique<Raytracer>(c));
     }
+    applyTuning();
 }
 
 void RenderSystem::setupModelBuffers(const Scene& scene)

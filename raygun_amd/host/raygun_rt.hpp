@@ -301,6 +301,12 @@ struct RenderSystem {
     // (rgrt_create_shared); renderFrame then rotates over the n contexts, so
     // a frame starts while the previous ones still run. readImage and
     // readImageRGBA8 read the most recent renderFrame's images.
+    // Launch shape (no counterpart in the reference): with one frame in
+    // flight every frame is traced in three strips side by side
+    // (rgrt_tuning.frameBands 3, 5 wavefront passes, 4/8/2 trace/shade/drain
+    // blocks per CU, measured on the 1080p 2M-triangle scene); with more,
+    // the contexts already run side by side and keep the library's one-band
+    // default. Both constructor and setFramesInFlight set it on every context.
     void setFramesInFlight(uint32_t n);
     uint32_t framesInFlight() const { return 1u + (uint32_t)m_extra.size(); }
     std::vector<uint16_t> readImage(rgrt_image which) const; // rgba16f bits
@@ -324,6 +330,8 @@ struct RenderSystem {
     rgrt_ctx* m_last = nullptr; // context of the most recent renderFrame / render
     int32_t m_maxRecursions = 5, m_numSamples = 1;
     vec3 m_lightDir = {0.4, -0.6, -0.8};
+    rgrt_tuning m_defaultTuning{}; // the library's launch shape, as rgrt_create left it
+    void applyTuning();
 };
 
 // packing without a context: fills the buffer refs, returns the buffers

@@ -69,6 +69,15 @@ class Raytracer:
         return rgrt.IMAGE_FINAL
 
 
+# Launch shape of the frame when one frame is in flight: traced in three
+# strips side by side (frameBands), 5 wavefront passes, 4/8/2 trace/shade/
+# drain blocks per CU (measured on the 1080p 2M-triangle scene). With frames
+# in flight the contexts already run side by side and keep the library's
+# one-band default.
+ONE_FRAME_TUNING = {"frameBands": 3, "wavefrontPasses": 5, "traceBlocksPerCU": 4, "shadeBlocksPerCU": 8,
+                    "drainBlocksPerCU": 2}
+
+
 class RenderSystem:
     def __init__(self, width, height, flags=0, device=0):
         self.ctx = rgrt.Context(width, height, flags, device)
@@ -77,6 +86,18 @@ class RenderSystem:
         self._extra = []  # (context, Raytracer) per frame in flight beyond the first
         self._frame = 0
         self._last = self.ctx
+        self._default_tuning = self.ctx.tuning()
+        self._apply_tuning()
+
+    def _apply_tuning(self):
+        """Sets the launch shape on every context: ONE_FRAME_TUNING with one
+        frame in flight, the library default with more (no counterpart in
+        the reference)."""
+        t = dict(self._default_tuning)
+        if not self._extra:
+            t.update(ONE_FRAME_TUNING)
+        for c in [self.ctx] + [c for c, _ in self._extra]:
+            c.set_tuning(**t)
 
     def set_frames_in_flight(self, n):
         """n - 1 more contexts sharing the scene (rgrt_create_shared):
@@ -91,6 +112,7 @@ class RenderSystem:
         for _ in range(n - 1):
             c = self.ctx.shared()
             self._extra.append((c, Raytracer(c)))
+        self._apply_tuning()
         self._frame = 0
         self._last = self.ctx
 

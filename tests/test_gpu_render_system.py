@@ -1,7 +1,8 @@
 """The Python mirror of the reference's RenderSystem / Raytracer
 (raygun_amd/raytracer.py): load_scene + updateUniformBuffer + render gives
-the image of the plain C-ABI calls, and render_frame with three frames in
-flight (contexts sharing the scene, TLAS refits) gives the frames of
+the image of the plain C-ABI calls (the mirror traces in three bands, the
+plain context in one piece), and render_frame with three frames in flight
+(contexts sharing the scene, TLAS refits, one band each) gives the frames of
 render_frame one at a time."""
 import numpy as np
 import pytest
@@ -51,6 +52,8 @@ def test_frames_in_flight_equal_frames_one_at_a_time():
         rs = RenderSystem(192, 108)
         try:
             rs.set_frames_in_flight(n)
+            # one frame in flight: traced in three bands; more: one band each
+            assert rs.ctx.tuning()["frameBands"] == (3 if n == 1 else 1)
             rs.raytracer.refit_tlas = True
             rs.load_scene(setup.scene)
             rs.updateUniformBuffer(setup.scene.camera, max_recursions=setup.max_recursions)
