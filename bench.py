@@ -232,7 +232,7 @@ def main():
         # launch shape for the measured pipeline, picked before the timed region
         t_tune = time.time()
         best, best_ms, table = autotune.tune(ctx, None, candidates=TUNE_CANDIDATES, contexts=ctxs,
-                                             timer=lambda: timed(variants[chosen], 16, 3))
+                                             timer=lambda: timed(variants[chosen], 16, 3), confirm=3)
         tuned = {"tuning": best, "ms": round(best_ms, 4), "candidates_timed": len(table),
                  "seconds": round(time.time() - t_tune, 1)}
         log(f"[bench] autotuned {chosen}: {tuned}")
@@ -259,7 +259,7 @@ def main():
         t_main = dict(ctx.tuning())
         one_fn = [graph_on(ctx)]
         b1, b1_ms, _ = autotune.tune(ctx, None, candidates=ONE_FRAME_CANDIDATES, contexts=[ctx], rounds=2,
-                                     timer=lambda: frames_ms([ctx], one_fn, 20, 3, ev))
+                                     timer=lambda: frames_ms([ctx], one_fn, 20, 3, ev), confirm=3)
         one_ms = frames_ms([ctx], one_fn, args.steps, args.warmup, ev)
         one = {"ms_per_frame": round(one_ms, 4), "mrays_per_s": round(rays / (one_ms * 1e-3) / 1e6, 1), "tuning": b1}
         ctx.set_tuning(**t_main)

@@ -20,14 +20,20 @@ CANDIDATES = {
 }
 
 
-def tune(ctx, frame, candidates=None, rounds=1, reps=2, log=None, contexts=None, timer=None):
+def tune(ctx, frame, candidates=None, rounds=1, reps=2, log=None, contexts=None, timer=None, confirm=0, confirm_reps=3):
     """frame(): enqueues one frame on ctx. Returns (best tuning, best ms,
     table of (tuning, ms)). Leaves ctx set to the best tuning.
 
     For frames in flight on several contexts sharing a scene: `contexts`
     lists every context the tuning is applied to, and timer() returns the
     ms per frame of the whole pipeline under the current tuning (frame is
-    then unused)."""
+    then unused).
+
+    confirm > 1: the descent keeps whichever shape had the luckiest single
+    timing, so afterwards the `confirm` fastest distinct shapes of the table
+    are timed again `confirm_reps` times each, in turn, and the lowest mean
+    wins (its mean is the returned ms; the table keeps the descent's
+    timings)."""
     cand = candidates or CANDIDATES
     ev = HipEvents()
     ctxs = contexts or [ctx]
@@ -57,4 +63,23 @@ def tune(ctx, frame, candidates=None, rounds=1, reps=2, log=None, contexts=None,
                 if ms < best_ms:
                     best, best_ms = t, ms
             apply(best)
+    if confirm > 1:
+        finalists, seen = [], set()
+        for t, _ in sorted(table, key=lambda e: e[1]):
+            key = tuple(sorted(t.items()))
+            if key not in seen:
+                seen.add(key)
+                finalists.append(t)
+            if len(finalists) == confirm:
+                break
+        sums = [0.0] * len(finalists)
+        for _ in range(confirm_reps):
+            for i, t in enumerate(finalists):
+                apply(t)
+                sums[i] += measure()
+        i = min(range(len(finalists)), key=lambda k: sums[k])
+        best, best_ms = finalists[i], sums[i] / confirm_reps
+        if log:
+            log(f"[autotune] confirmed {best} {best_ms:.3f} ms over {confirm_reps} timings of {len(finalists)} shapes")
+        apply(best)
     return best, best_ms, table

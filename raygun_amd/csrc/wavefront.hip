@@ -43,6 +43,13 @@ constexpr int kShadeBlock = 256;
 #ifndef RGRT_DRAIN_WAVES
 #define RGRT_DRAIN_WAVES 1
 #endif
+// The shading variant for one sample per slot fits in 128 VGPRs (4 waves);
+// compiled for 5 waves it spills 29 VGPRs and is slightly faster beside
+// other frames' traversal (config 2 at 1080p, four frames in flight:
+// 5.36 against 5.40-5.43 ms per frame).
+#ifndef RGRT_SHADE1_WAVES
+#define RGRT_SHADE1_WAVES 5
+#endif
 // The drain variant for one sample per slot (no next-sample start) fits in
 // 143 VGPRs (3 waves); compiled for 4 waves it spills 12 VGPRs and runs
 // faster (config 2 at 1080p: 1.58-1.60 against 1.69-1.82 ms).
@@ -983,7 +990,7 @@ __device__ __forceinline__ void shade_wave(const ShadeParams& p, const Ctx& c, i
 
 // Wavefront shading pass over the rays traced by the preceding traversal pass.
 template<bool kNextSample>
-__global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(RGRT_SHADE_WAVES))) void k_shade(ShadeParams p, int cur)
+__global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(kNextSample ? RGRT_SHADE_WAVES : RGRT_SHADE1_WAVES))) void k_shade(ShadeParams p, int cur)
 {
     const uint32_t count = p.st.counters[cur];
     const int lane = threadIdx.x & 63;
@@ -1022,7 +1029,7 @@ __device__ __forceinline__ void batch_chunk_prefix(const ShadeParams* ps, int nu
 // launch, 64-entry chunks handed to waves in order (a chunk belongs to one
 // scene, so a wave shades one scene at a time).
 template<bool kNextSample>
-__global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(RGRT_SHADE_WAVES))) void k_shade_multi(
+__global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(kNextSample ? RGRT_SHADE_WAVES : RGRT_SHADE1_WAVES))) void k_shade_multi(
     const ShadeParams* __restrict__ ps, int numScenes, int cur)
 {
     __shared__ uint32_t chunkEnd[kMaxBatchScenes];

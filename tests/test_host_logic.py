@@ -43,6 +43,29 @@ def test_autotune_finds_the_minimum_of_a_separable_cost():
     assert len(table) == 1 + 4 + 2 + 2  # the start, then each other candidate once
 
 
+def test_autotune_confirmation_discards_a_lucky_timing():
+    """The shape with traceBlocksPerCU 4 costs 6.0 but its first timing
+    reads 5.0 (noise); 8 costs 5.5 every time. The descent alone keeps 4,
+    the confirmation rounds keep 8."""
+    start = {"wavefrontPasses": 6, "traceBlocksPerCU": 6}
+    cand = {"traceBlocksPerCU": [4, 6, 8]}
+    for confirm, want, want_ms in ((0, 4, 5.0), (3, 8, 5.5)):
+        ctx = _FakeContext(start)
+        seen = {}
+
+        def cost():
+            b = ctx.t["traceBlocksPerCU"]
+            seen[b] = seen.get(b, 0) + 1
+            if b == 4:
+                return 5.0 if seen[b] == 1 else 6.0
+            return {6: 7.0, 8: 5.5}[b]
+
+        best, ms, table = autotune.tune(ctx, None, candidates=cand, timer=cost, confirm=confirm, confirm_reps=2)
+        assert best["traceBlocksPerCU"] == want and ms == want_ms and ctx.t == best
+        assert len(table) == 3
+        assert sum(seen.values()) == 3 + (3 * 2 if confirm else 0)
+
+
 def test_batched_scene_sizes_are_seeded_and_clipped():
     a = batched.scene_sizes(32, 3)
     assert a == batched.scene_sizes(32, 3)
