@@ -52,7 +52,9 @@ constexpr int kShadeBlock = 256;
 #endif
 // The drain variant for one sample per slot (no next-sample start) fits in
 // 143 VGPRs (3 waves); compiled for 4 waves it spills 12 VGPRs and runs
-// faster (config 2 at 1080p: 1.58-1.60 against 1.69-1.82 ms).
+// faster (config 2 at 1080p: 1.58-1.60 against 1.69-1.82 ms). At 5 waves it
+// spills 55 inside the traversal loop: 5.60 against 5.34-5.36 ms per
+// frame with four frames in flight.
 #ifndef RGRT_DRAIN1_WAVES
 #define RGRT_DRAIN1_WAVES 4
 #endif
