@@ -593,7 +593,7 @@ def bench_config5(frames, in_flight=4, learned=True):
         ev = HipEvents()
         t2 = time.time()
         best, best_ms, table = autotune.tune(ctx, None, candidates=CONFIG5_CANDIDATES, contexts=ctxs,
-                                             timer=lambda: frames_ms(ctxs, fns, 3, 1, ev))
+                                             timer=lambda: frames_ms(ctxs, fns, 3, 1, ev), confirm=3)
         log(f"[bench] config5 autotuned in {time.time() - t2:.1f}s: {best} {best_ms:.2f} ms")
         ms = frames_ms(ctxs, fns, frames, 2, ev)
         rays = ctx.stats()["rays"]
